@@ -1,0 +1,904 @@
+// CPU ORACLE — test infrastructure only.
+//
+// A from-scratch C++ restatement of the reference's δ-3D pressure projection
+// (`Volume3::project_mobile3` with its multigrid preconditioner). It exists to
+// judge the HIP library: only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it, and only as the checker or as the timed CPU
+// baseline. The product path never calls it.
+//
+// Template parameter T:
+//   float  — "f32-faithful": the reference's loop order, summation order and
+//            chunk-64 ordered f32 reductions; build with -ffp-contract=off.
+//   double — the FP64 oracle used to state the error of mixed-precision runs.
+//
+// Every function cites the reference file:line it restates (paths relative to
+// the reference checkout, code/water-core/src/).
+//
+// Pinning (what checks each part, see tests/test_oracle_*.py):
+//   - project/v_cycle/operator: the reference's published iteration counts of
+//     its S385 bench (MULTIGRILLE-3D-S385.md:57-58: 9/10/11 with multigrid,
+//     102/191/365 with Jacobi, nx = 32/64/128) reproduced exactly;
+//   - operator/assembly/correction/divergence: dense brute-force solves on
+//     small random geometries, DCT closed forms on box domains, symmetry,
+//     positivity and prec == 1/diag bitwise (tests_delta3d.rs:352, :507);
+//   - hierarchy: level_count3 known answers (tests_delta3d.rs:1814-1817);
+//   - fingerprint: FNV-1a 64 published offset/prime known answers;
+//   - V-cycle: symmetry and positivity (tests_delta3d.rs:1840-1872).
+// Extensions that are not in the reference (RBGS smoother, level cap, coarse
+// sweep count, pure-Neumann gauge) are marked EXTENSION; with their default
+// settings the oracle is the reference algorithm.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace orc {
+
+enum Status : int32_t { OK = 0, ERR_DOMAIN = 1, ERR_NOT_FINITE = 2, ERR_SHAPE = 3, ERR_CONVERGENCE = 4 };
+
+// delta_projection.rs:164-188
+struct Report {
+  uint32_t refinements;
+  uint32_t iterations;
+  int32_t degraded;
+  int32_t floor;
+  double residual;
+  double divergence;
+  double backward_error;
+  double divergence_plain;
+};
+
+struct Options {
+  int32_t multigrid;      // 0: Jacobi-PCG (delta3d_mobile.rs:435-461), 1: V-cycle (delta3d_multigrid.rs:393-465)
+  int32_t smoother;       // 0: damped Jacobi ω=6/7 (delta3d_multigrid.rs:28); 1: EXTENSION symmetric red-black Gauss-Seidel
+  int32_t max_levels;     // EXTENSION: cap on coarse levels; <0 = the reference rule (delta3d_multigrid.rs:73-87)
+  int32_t pre_sweeps;     // delta3d_multigrid.rs:32 (2)
+  int32_t post_sweeps;    // delta3d_multigrid.rs:33 (2)
+  int32_t coarse_sweeps;  // delta3d_multigrid.rs:34 (8)
+  int32_t gauge;          // EXTENSION: 1 = remove the mean of b and of p on a pure-Neumann problem
+};
+
+inline Options default_options() { return Options{1, 0, -1, 2, 2, 8, 1}; }
+
+// Geometry: the subset of Volume3's state that the projection reads.
+struct Geometry {
+  int32_t nx, ny, nz;
+  float dx;
+  float rho, g_eff, rest;
+  int32_t homogeneous_ghost;   // delta3d_mobile.rs:130, :140
+  int32_t closed_lid;          // EXTENSION of the cut apertures: aperture 0 on every top face
+  const float* eta;            // nx*ny column heights
+  const float* eta_roundoff;   // nx*ny or null (zeros)
+  const uint8_t* solid;        // cells, nonzero = solid (cut frac == 0), or null
+  const float* open_u;         // apertures, or null (derived)
+  const float* open_v;
+  const float* open_w;
+};
+
+// delta3d.rs:69-72 and delta_projection.rs:200-203
+template <class T> inline T roundoff_3d() { const T u = T(1) / T(16777216); return T(10) * u / (T(1) - T(10) * u); }
+template <class T> inline T roundoff_2d() { const T u = T(1) / T(16777216); return T(8) * u / (T(1) - T(8) * u); }
+
+// delta3d.rs:1360-1369 — FNV-1a over the little-endian bytes of each f32.
+inline uint64_t fingerprint_f32(const float* v, size_t n) {
+  uint64_t h = 0xcbf29ce484222325ull;
+  for (size_t i = 0; i < n; ++i) {
+    uint32_t bits;
+    std::memcpy(&bits, &v[i], 4);
+    for (int b = 0; b < 4; ++b) {
+      h ^= (uint64_t)((bits >> (8 * b)) & 0xffu);
+      h *= 0x100000001b3ull;
+    }
+  }
+  return h;
+}
+
+template <class T> inline uint64_t fingerprint(const std::vector<T>& v) {
+  // The f64 oracle fingerprints the f32 image of p (the cycle test is about
+  // a state that returns bit for bit; the f64 state rarely cycles).
+  std::vector<float> f(v.size());
+  for (size_t i = 0; i < v.size(); ++i) f[i] = (float)v[i];
+  return fingerprint_f32(f.data(), f.size());
+}
+
+// delta3d_multigrid.rs:73-87
+inline bool coarsens3(size_t nx, size_t ny, size_t nz) {
+  return nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0 && nx >= 4 && ny >= 4 && nz >= 4;
+}
+inline size_t level_count3(size_t nx, size_t ny, size_t nz) {
+  size_t n = 0;
+  while (coarsens3(nx, ny, nz)) { nx /= 2; ny /= 2; nz /= 2; ++n; }
+  return n;
+}
+// delta3d_multigrid.rs:91-105 (no overflow concerns at oracle sizes)
+inline size_t hierarchy_floats3(size_t nx, size_t ny, size_t nz) {
+  size_t total = 2 * nx * ny * nz;
+  while (coarsens3(nx, ny, nz)) {
+    size_t cx = nx / 2, cy = ny / 2, cz = nz / 2;
+    size_t cells = cx * cy * cz;
+    size_t faces = (cx + 1) * cy * cz + cx * (cy + 1) * cz + cx * cy * (cz + 1);
+    total += cells * 5 + faces;
+    nx = cx; ny = cy; nz = cz;
+  }
+  return total;
+}
+
+// delta3d_multigrid.rs:37-52
+template <class T> struct Level3 {
+  size_t nx, ny, nz;
+  T inv;
+  std::vector<T> open_u, open_v, open_w, kind, diag, x, r, t;
+  size_t cells() const { return nx * ny * nz; }
+  size_t c(size_t i, size_t j, size_t k) const { return (k * ny + j) * nx + i; }
+};
+
+// delta3d_multigrid.rs:209-211
+static const int CHILDREN[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+static const int QUARTER[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
+
+struct RowEntry {
+  int64_t n;  // neighbour cell, -1 = none
+  float a_f;
+  double a_d, it_d, val_d;
+};
+
+template <class T> class Volume {
+ public:
+  // ---------------------------------------------------------------- state
+  size_t nx, ny, nz;
+  T dx, rho, g, rest;
+  bool homogeneous;
+  bool cut;  // cut semantics: aperture-weighted divergence (delta3d.rs:806-855)
+  std::vector<T> eta, eta_roundoff;
+  std::vector<uint8_t> solid;
+  std::vector<T> open_u, open_v, open_w;
+  std::vector<T> us, vs, ws, u, v, w;
+  std::vector<T> p, rhs, res, dir, tmp, prec;
+  // multigrid (delta3d_multigrid.rs:65-69)
+  bool mg_on = false;
+  std::vector<Level3<T>> levels;
+  std::vector<T> mz, mt;
+  Options opt;
+  bool pure_neumann = false;
+
+  size_t cells() const { return nx * ny * nz; }
+  size_t c(size_t i, size_t j, size_t k) const { return (k * ny + j) * nx + i; }            // delta3d.rs:188
+  size_t fu(size_t i, size_t j, size_t k) const { return (k * ny + j) * (nx + 1) + i; }     // delta3d.rs:192
+  size_t fv(size_t i, size_t j, size_t k) const { return (k * (ny + 1) + j) * nx + i; }     // delta3d.rs:197
+  size_t fw(size_t i, size_t j, size_t k) const { return (k * ny + j) * nx + i; }           // delta3d.rs:201
+  size_t col(size_t i, size_t j) const { return j * nx + i; }                               // delta3d.rs:205
+  size_t nu() const { return (nx + 1) * ny * nz; }
+  size_t nv() const { return nx * (ny + 1) * nz; }
+  size_t nw() const { return nx * ny * (nz + 1); }
+
+  Status configure(const Geometry& g_, const Options& o) {
+    if (g_.nx <= 0 || g_.ny <= 0 || g_.nz <= 0 || !(g_.dx > 0.f) || !std::isfinite(g_.dx)) return ERR_DOMAIN;
+    if (!std::isfinite(g_.rho) || !(g_.rho > 0.f) || !std::isfinite(g_.g_eff)) return ERR_NOT_FINITE;
+    nx = g_.nx; ny = g_.ny; nz = g_.nz;
+    dx = (T)g_.dx; rho = (T)g_.rho; g = (T)g_.g_eff; rest = (T)g_.rest;
+    homogeneous = g_.homogeneous_ghost != 0;
+    opt = o;
+    size_t cols = nx * ny;
+    eta.assign(cols, T(0)); eta_roundoff.assign(cols, T(0));
+    for (size_t q = 0; q < cols; ++q) {
+      if (!std::isfinite(g_.eta[q])) return ERR_NOT_FINITE;
+      eta[q] = (T)g_.eta[q];
+      if (g_.eta_roundoff) eta_roundoff[q] = (T)g_.eta_roundoff[q];
+    }
+    solid.assign(cells(), 0);
+    if (g_.solid) for (size_t q = 0; q < cells(); ++q) solid[q] = g_.solid[q] ? 1 : 0;
+    cut = g_.solid || g_.open_u || g_.open_v || g_.open_w || g_.closed_lid;
+    open_u.assign(nu(), T(1)); open_v.assign(nv(), T(1)); open_w.assign(nw(), T(1));
+    if (cut) derive_apertures(g_);
+    us.assign(nu(), T(0)); vs.assign(nv(), T(0)); ws.assign(nw(), T(0));
+    u = us; v = vs; w = ws;
+    p.assign(cells(), T(0)); rhs = p; res = p; dir = p; tmp = p; prec = p;
+    mg_on = o.multigrid != 0;
+    if (mg_on) build_hierarchy();
+    return OK;
+  }
+
+  // Apertures when none are given: a face touching a solid cell or the box
+  // walls is closed; the lid (top faces) is open unless closed_lid. With the
+  // reference's cut geometry (delta3d_cut.rs) a face covered by solid has
+  // aperture 0 in the same way.
+  void derive_apertures(const Geometry& g_) {
+    auto S = [&](size_t i, size_t j, size_t k) { return solid[c(i, j, k)] != 0; };
+    for (size_t k = 0; k < nz; ++k)
+      for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i <= nx; ++i) {
+          float a = (i == 0 || i == nx || S(i - 1, j, k) || S(i, j, k)) ? 0.f : 1.f;
+          open_u[fu(i, j, k)] = g_.open_u ? (T)g_.open_u[fu(i, j, k)] : (T)a;
+        }
+    for (size_t k = 0; k < nz; ++k)
+      for (size_t j = 0; j <= ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+          float a = (j == 0 || j == ny || S(i, j - 1, k) || S(i, j, k)) ? 0.f : 1.f;
+          open_v[fv(i, j, k)] = g_.open_v ? (T)g_.open_v[fv(i, j, k)] : (T)a;
+        }
+    for (size_t k = 0; k <= nz; ++k)
+      for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+          float a;
+          if (k == 0) a = 0.f;
+          else if (k == nz) a = (S(i, j, nz - 1) || g_.closed_lid) ? 0.f : 1.f;
+          else a = (S(i, j, k - 1) || S(i, j, k)) ? 0.f : 1.f;
+          T val = g_.open_w ? (T)g_.open_w[fw(i, j, k)] : (T)a;
+          if (k == nz && g_.closed_lid) val = T(0);
+          open_w[fw(i, j, k)] = val;
+        }
+  }
+
+  // ---------------------------------------------------------------- classification
+  T height3(size_t i, size_t j) const { return eta[col(i, j)]; }                         // delta3d_mobile.rs:92-95
+  bool solid3(size_t cc) const { return solid[cc] != 0; }                               // delta3d_mobile.rs:122-124
+  bool wet3(size_t i, size_t j, size_t k) const {                                       // delta3d_mobile.rs:97-103
+    if (solid3(c(i, j, k))) return false;
+    return ((T)k + T(0.5)) * dx < height3(i, j);
+  }
+  T open3(int axis, size_t f) const {                                                   // delta3d_mobile.rs:107-119
+    if (!cut) return T(1);
+    return axis == 0 ? open_u[f] : axis == 1 ? open_v[f] : open_w[f];
+  }
+  static T theta_min() { return (T)1e-3; }                                               // delta_mobile.rs:14
+  static T tmax(T a, T b) { return (std::isnan(a) || a < b) ? b : a; }                  // Rust f32::max
+
+  // delta3d_mobile.rs:126-134
+  void ghost_up3(size_t i, size_t j, size_t k, T& it, T& value) const {
+    size_t cc = col(i, j);
+    T theta = tmax((height3(i, j) - ((T)k + T(0.5)) * dx) / dx, theta_min());
+    it = T(1) / theta;
+    if (homogeneous) { value = T(0); return; }
+    value = rho * g * ((eta[cc] - rest) - eta_roundoff[cc]);
+  }
+  // delta3d_mobile.rs:136-147
+  void ghost_side3(size_t i, size_t j, size_t k, size_t x, size_t y, T& it, T& value) const {
+    T zc = ((T)k + T(0.5)) * dx;
+    T h = height3(i, j);
+    T theta = tmax((h - zc) / (h - height3(x, y)), theta_min());
+    it = T(1) / theta;
+    if (homogeneous) { value = T(0); return; }
+    value = rho * g * (zc - rest);
+  }
+
+  struct Row { int64_t n[6]; T a[6], it[6], val[6]; };
+
+  // delta3d_mobile.rs:152-188 — order x−, x+, y−, y+, z−, z+.
+  Row mobile_row(size_t i, size_t j, size_t k) const {
+    Row row;
+    for (int f = 0; f < 6; ++f) { row.n[f] = -1; row.a[f] = T(0); row.it[f] = T(0); row.val[f] = T(0); }
+    struct Nb { bool has; size_t x, y; T a; };
+    Nb nb[4] = {
+        {i > 0, i > 0 ? i - 1 : 0, j, open3(0, fu(i, j, k))},
+        {i + 1 < nx, i + 1, j, open3(0, fu(i + 1, j, k))},
+        {j > 0, i, j > 0 ? j - 1 : 0, open3(1, fv(i, j, k))},
+        {j + 1 < ny, i, j + 1, open3(1, fv(i, j + 1, k))},
+    };
+    for (int f = 0; f < 4; ++f) {
+      if (!nb[f].has) continue;
+      size_t x = nb[f].x, y = nb[f].y;
+      T a = nb[f].a;
+      if (a == T(0) || solid3(c(x, y, k))) continue;
+      if (wet3(x, y, k)) {
+        row.n[f] = (int64_t)c(x, y, k); row.a[f] = a;
+      } else {
+        T it, b;
+        ghost_side3(i, j, k, x, y, it, b);
+        row.a[f] = a; row.it[f] = it; row.val[f] = b;
+      }
+    }
+    T down = open3(2, fw(i, j, k));
+    if (k > 0 && down > T(0) && !solid3(c(i, j, k - 1))) { row.n[4] = (int64_t)c(i, j, k - 1); row.a[4] = down; }
+    T up = open3(2, fw(i, j, k + 1));
+    if (up > T(0)) {
+      if (k + 1 < nz && wet3(i, j, k + 1)) {
+        row.n[5] = (int64_t)c(i, j, k + 1); row.a[5] = up;
+      } else {
+        T it, b;
+        ghost_up3(i, j, k, it, b);
+        row.a[5] = up; row.it[5] = it; row.val[5] = b;
+      }
+    }
+    return row;
+  }
+
+  T inv() const { return T(1) / (dx * dx); }
+
+  // delta3d_mobile.rs:190-213
+  void apply_mobile3(const std::vector<T>& x, std::vector<T>& out) const {
+    const T iv = inv();
+    for (size_t k = 0; k < nz; ++k)
+      for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+          size_t cc = c(i, j, k);
+          if (!wet3(i, j, k)) { out[cc] = T(0); continue; }
+          Row row = mobile_row(i, j, k);
+          T acc = T(0);
+          for (int f = 0; f < 6; ++f) {
+            if (row.n[f] >= 0) acc += row.a[f] * (x[cc] - x[row.n[f]]);
+            else if (row.a[f] > T(0)) acc += row.a[f] * x[cc] * row.it[f];
+          }
+          out[cc] = acc * iv;
+        }
+  }
+
+  // delta3d_mobile.rs:215-241 (rhs holds the divergence on entry)
+  void rhs_mobile3(T scale) {
+    const T iv = inv();
+    for (size_t k = 0; k < nz; ++k)
+      for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+          size_t cc = c(i, j, k);
+          if (!wet3(i, j, k)) { rhs[cc] = T(0); prec[cc] = T(0); continue; }
+          T b = scale * rhs[cc], diag = T(0);
+          Row row = mobile_row(i, j, k);
+          for (int f = 0; f < 6; ++f) {
+            if (row.n[f] >= 0) diag += row.a[f];
+            else if (row.a[f] > T(0)) {
+              diag += row.a[f] * row.it[f];
+              b += row.a[f] * row.val[f] * row.it[f] * iv;
+            }
+          }
+          rhs[cc] = b;
+          prec[cc] = diag > T(0) ? T(1) / (diag * iv) : T(0);
+        }
+  }
+
+  // delta3d.rs:747-763 (plain) and :806-823 (cut, fixed solids)
+  void divergence(const std::vector<T>& uu, const std::vector<T>& vv, const std::vector<T>& ww, std::vector<T>& out) const {
+    for (size_t k = 0; k < nz; ++k)
+      for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+          size_t cc = c(i, j, k);
+          if (cut) {
+            if (solid3(cc)) { out[cc] = T(0); continue; }
+            T fl = open_u[fu(i, j, k)] * uu[fu(i, j, k)], fr = open_u[fu(i + 1, j, k)] * uu[fu(i + 1, j, k)];
+            T ff = open_v[fv(i, j, k)] * vv[fv(i, j, k)], fk = open_v[fv(i, j + 1, k)] * vv[fv(i, j + 1, k)];
+            T fb = open_w[fw(i, j, k)] * ww[fw(i, j, k)], ft = open_w[fw(i, j, k + 1)] * ww[fw(i, j, k + 1)];
+            out[cc] = ((fr - fl + ft - fb) + (fk - ff)) / dx;
+          } else {
+            T fl = uu[fu(i, j, k)], fr = uu[fu(i + 1, j, k)];
+            T ff = vv[fv(i, j, k)], fk = vv[fv(i, j + 1, k)];
+            T fb = ww[fw(i, j, k)], ft = ww[fw(i, j, k + 1)];
+            out[cc] = ((fr - fl + ft - fb) + (fk - ff)) / dx;
+          }
+        }
+  }
+
+  // delta3d.rs:860-868 with SequentialJobs (host_impl.rs:76-93): f32 sums of
+  // 64-element chunks, merged in index order in f32.
+  T dot(const std::vector<T>& a, const std::vector<T>& b) const {
+    T accum = T(0);
+    size_t n = a.size();
+    for (size_t s = 0; s < n; s += 64) {
+      size_t e = s + 64 < n ? s + 64 : n;
+      T chunk = T(0);
+      for (size_t q = s; q < e; ++q) chunk += a[q] * b[q];
+      accum = accum + chunk;
+    }
+    return accum;
+  }
+  // delta3d.rs:871-876
+  Status norm2(const std::vector<T>& a, T& out) const {
+    T n = dot(a, a);
+    if (!std::isfinite(n)) return ERR_NOT_FINITE;
+    if (n == T(0)) for (T x : a) if (x != T(0)) return ERR_NOT_FINITE;
+    out = n;
+    return OK;
+  }
+  // delta3d_mobile.rs:447-461
+  Status dot_prec3(T& out) const {
+    T accum = T(0);
+    size_t n = p.size();
+    for (size_t s = 0; s < n; s += 64) {
+      size_t e = s + 64 < n ? s + 64 : n;
+      T chunk = T(0);
+      for (size_t q = s; q < e; ++q) chunk += res[q] * prec[q] * res[q];
+      accum = accum + chunk;
+    }
+    if (!std::isfinite(accum)) return ERR_NOT_FINITE;
+    out = accum;
+    return OK;
+  }
+  // delta3d_mobile.rs:435-445
+  Status prime_mobile3(T beta, T& rz) {
+    for (size_t q = 0; q < dir.size(); ++q) {
+      T z = prec[q] * res[q];
+      dir[q] = beta == T(0) ? z : z + beta * dir[q];
+    }
+    return dot_prec3(rz);
+  }
+
+  // delta3d_mobile.rs:243-271
+  T backward_error_mobile3() const {
+    const T iv = inv();
+    T worst = T(0);
+    for (size_t k = 0; k < nz; ++k)
+      for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+          if (!wet3(i, j, k)) continue;
+          size_t cc = c(i, j, k);
+          T pc = std::fabs(p[cc]);
+          T acc = T(0);
+          Row row = mobile_row(i, j, k);
+          for (int f = 0; f < 6; ++f) {
+            if (row.n[f] >= 0) acc += row.a[f] * (pc + std::fabs(p[row.n[f]]));
+            else if (row.a[f] > T(0)) acc += row.a[f] * pc * row.it[f];
+          }
+          T scale = std::fabs(rhs[cc]) + acc * iv;
+          if (scale > T(0)) worst = tmax(worst, std::fabs(res[cc]) / scale);
+        }
+    return worst;
+  }
+  // delta3d_mobile.rs:273-280
+  T mobile_roundoff() const { return ny == 1 ? roundoff_2d<T>() : roundoff_3d<T>(); }
+
+  // delta3d_mobile.rs:284-343
+  void correct_mobile3(T k1) {
+    u = us; v = vs; w = ws;
+    for (size_t k = 0; k < nz; ++k)
+      for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i)
+          for (int axis = 0; axis < 2; ++axis) {
+            if ((axis == 0 && i == 0) || (axis == 1 && j == 0)) continue;
+            size_t x = axis == 0 ? i - 1 : i, y = axis == 0 ? j : j - 1;
+            size_t l = c(x, y, k), r = c(i, j, k);
+            size_t face = axis == 0 ? fu(i, j, k) : fv(i, j, k);
+            if (open3(axis, face) == T(0) || solid3(l) || solid3(r)) continue;
+            bool wl = wet3(x, y, k), wr = wet3(i, j, k);
+            T change;
+            if (wl && wr) change = k1 * (p[r] - p[l]) / dx;
+            else if (wl && !wr) { T a, b; ghost_side3(x, y, k, i, j, a, b); change = k1 * (b - p[l]) * a / dx; }
+            else if (!wl && wr) { T a, b; ghost_side3(i, j, k, x, y, a, b); change = k1 * (p[r] - b) * a / dx; }
+            else continue;
+            if (axis == 0) u[face] -= change; else v[face] -= change;
+          }
+    for (size_t k = 1; k <= nz; ++k)
+      for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+          size_t f = fw(i, j, k);
+          if (open3(2, f) == T(0) || !wet3(i, j, k - 1)) continue;
+          size_t below = c(i, j, k - 1);
+          if (k < nz && wet3(i, j, k)) {
+            w[f] -= k1 * (p[c(i, j, k)] - p[below]) / dx;
+          } else {
+            T a, b;
+            ghost_up3(i, j, k - 1, a, b);
+            w[f] -= k1 * (b - p[below]) * a / dx;
+          }
+        }
+  }
+
+  // delta3d_mobile.rs:398-433 (PROJECTION_VELOCITY_FLOOR = 1e-4, delta3d.rs:41)
+  void divergence_mobile3(double& all_out, double& plain_out, T* all_raw = nullptr, T* plain_raw = nullptr) {
+    divergence(u, v, w, tmp);
+    T all = T(0), plain = T(0);
+    for (size_t k = 0; k < nz; ++k)
+      for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+          if (!wet3(i, j, k)) continue;
+          T d = std::fabs(tmp[c(i, j, k)]);
+          all = tmax(all, d);
+          Row row = mobile_row(i, j, k);
+          bool ghost = false;
+          for (int f = 0; f < 6; ++f) if (row.n[f] < 0 && row.a[f] > T(0)) ghost = true;
+          if (!ghost) plain = tmax(plain, d);
+        }
+    T mx = T(0);
+    for (T x : u) mx = tmax(mx, std::fabs(x));
+    for (T x : v) mx = tmax(mx, std::fabs(x));
+    for (T x : w) mx = tmax(mx, std::fabs(x));
+    T echelle = tmax(mx, (T)1e-4);
+    T ratio = mx > T(0) ? dx / echelle : T(0);
+    all_out = (double)(all * ratio);
+    plain_out = (double)(plain * ratio);
+    if (all_raw) *all_raw = all;
+    if (plain_raw) *plain_raw = plain;
+  }
+
+  // ---------------------------------------------------------------- multigrid
+  // delta3d_multigrid.rs:109-136 (EXTENSION: max_levels cap)
+  void build_hierarchy() {
+    levels.clear();
+    size_t x = nx, y = ny, z = nz;
+    T h = dx;
+    while (coarsens3(x, y, z) && (opt.max_levels < 0 || (int)levels.size() < opt.max_levels)) {
+      size_t cx = x / 2, cy = y / 2, cz = z / 2;
+      h *= T(2);
+      Level3<T> L;
+      L.nx = cx; L.ny = cy; L.nz = cz;
+      L.inv = T(1) / (h * h);
+      L.open_u.assign((cx + 1) * cy * cz, T(0));
+      L.open_v.assign(cx * (cy + 1) * cz, T(0));
+      L.open_w.assign(cx * cy * (cz + 1), T(0));
+      size_t n = cx * cy * cz;
+      L.kind.assign(n, T(0)); L.diag = L.kind; L.x = L.kind; L.r = L.kind; L.t = L.kind;
+      levels.push_back(std::move(L));
+      x = cx; y = cy; z = cz;
+    }
+    mz.assign(cells(), T(0));
+    mt.assign(cells(), T(0));
+  }
+
+  // delta3d_multigrid.rs:216-255
+  static void row_sums(const Level3<T>& L, const std::vector<T>* pp, std::vector<T>& out) {
+    const size_t nx_ = L.nx, ny_ = L.ny, nz_ = L.nz;
+    for (size_t k = 0; k < nz_; ++k)
+      for (size_t j = 0; j < ny_; ++j)
+        for (size_t i = 0; i < nx_; ++i) {
+          size_t cc = L.c(i, j, k);
+          if (L.kind[cc] <= T(0)) { out[cc] = T(0); continue; }
+          T pc = pp ? (*pp)[cc] : T(0);
+          T acc = T(0);
+          auto face = [&](T a, int64_t n, bool top) {
+            if (a == T(0)) return;
+            if (n >= 0 && L.kind[n] > T(0)) acc += pp ? a * (pc - (*pp)[n]) : a;
+            else if (n >= 0 && L.kind[n] < T(0)) acc += T(2) * a * (pp ? pc : T(1));
+            else if (n >= 0) {}
+            else if (top) acc += T(2) * a * (pp ? pc : T(1));
+          };
+          face(L.open_u[(k * ny_ + j) * (nx_ + 1) + i], i > 0 ? (int64_t)cc - 1 : -1, false);
+          face(L.open_u[(k * ny_ + j) * (nx_ + 1) + i + 1], i + 1 < nx_ ? (int64_t)cc + 1 : -1, false);
+          face(L.open_v[(k * (ny_ + 1) + j) * nx_ + i], j > 0 ? (int64_t)(cc - nx_) : -1, false);
+          face(L.open_v[(k * (ny_ + 1) + j + 1) * nx_ + i], j + 1 < ny_ ? (int64_t)(cc + nx_) : -1, false);
+          face(L.open_w[(k * ny_ + j) * nx_ + i], k > 0 ? (int64_t)(cc - nx_ * ny_) : -1, false);
+          face(L.open_w[((k + 1) * ny_ + j) * nx_ + i], k + 1 < nz_ ? (int64_t)(cc + nx_ * ny_) : -1, true);
+          out[cc] = acc * L.inv;
+        }
+  }
+
+  // delta3d_multigrid.rs:141-206
+  template <class OpenF, class KindF>
+  static void coarsen3(size_t fnx, size_t fny, OpenF open, KindF kind, Level3<T>& L) {
+    const size_t cx = L.nx, cy = L.ny, cz = L.nz;
+    auto fc = [&](size_t i, size_t j, size_t k) { return (k * fny + j) * fnx + i; };
+    auto fu_ = [&](size_t i, size_t j, size_t k) { return (k * fny + j) * (fnx + 1) + i; };
+    auto fv_ = [&](size_t i, size_t j, size_t k) { return (k * (fny + 1) + j) * fnx + i; };
+    auto fw_ = [&](size_t i, size_t j, size_t k) { return (k * fny + j) * fnx + i; };
+    for (size_t k = 0; k < cz; ++k)
+      for (size_t j = 0; j < cy; ++j)
+        for (size_t i = 0; i < cx; ++i) {
+          bool active = false, air = false;
+          for (auto& ch : CHILDREN) {
+            T vk = kind(fc(2 * i + ch[0], 2 * j + ch[1], 2 * k + ch[2]));
+            if (vk > T(0)) active = true; else if (vk < T(0)) air = true;
+          }
+          L.kind[L.c(i, j, k)] = active ? T(1) : air ? T(-1) : T(0);
+        }
+    for (size_t k = 0; k < cz; ++k)
+      for (size_t j = 0; j < cy; ++j)
+        for (size_t i = 0; i <= cx; ++i) {
+          T s = T(0);
+          for (auto& q : QUARTER) s += open(0, fu_(2 * i, 2 * j + q[0], 2 * k + q[1]));
+          L.open_u[(k * cy + j) * (cx + 1) + i] = T(0.25) * s;
+        }
+    for (size_t k = 0; k < cz; ++k)
+      for (size_t j = 0; j <= cy; ++j)
+        for (size_t i = 0; i < cx; ++i) {
+          T s = T(0);
+          for (auto& q : QUARTER) s += open(1, fv_(2 * i + q[0], 2 * j, 2 * k + q[1]));
+          L.open_v[(k * (cy + 1) + j) * cx + i] = T(0.25) * s;
+        }
+    for (size_t k = 0; k <= cz; ++k)
+      for (size_t j = 0; j < cy; ++j)
+        for (size_t i = 0; i < cx; ++i) {
+          T s = T(0);
+          for (auto& q : QUARTER) s += open(2, fw_(2 * i + q[0], 2 * j + q[1], 2 * k));
+          L.open_w[(k * cy + j) * cx + i] = T(0.25) * s;
+        }
+    row_sums(L, nullptr, L.diag);
+  }
+
+  // delta3d_multigrid.rs:357-367
+  T fine_kind3(size_t cc) const {
+    size_t i = cc % nx, j = (cc / nx) % ny, k = cc / (nx * ny);
+    if (wet3(i, j, k)) return T(1);
+    if (solid3(cc)) return T(0);
+    return T(-1);
+  }
+
+  // delta3d_multigrid.rs:370-387
+  void prepare_multigrid3() {
+    if (levels.empty()) return;
+    coarsen3(nx, ny, [&](int axis, size_t f) { return open3(axis, f); }, [&](size_t cc) { return fine_kind3(cc); }, levels[0]);
+    for (size_t l = 1; l < levels.size(); ++l) {
+      const Level3<T>& F = levels[l - 1];
+      coarsen3(F.nx, F.ny,
+               [&](int axis, size_t f) { return axis == 0 ? F.open_u[f] : axis == 1 ? F.open_v[f] : F.open_w[f]; },
+               [&](size_t cc) { return F.kind[cc]; }, levels[l]);
+    }
+  }
+
+  // delta3d_multigrid.rs:258-272
+  static void restrict3(size_t fnx, size_t fny, const std::vector<T>& fine, Level3<T>& L) {
+    for (size_t k = 0; k < L.nz; ++k)
+      for (size_t j = 0; j < L.ny; ++j)
+        for (size_t i = 0; i < L.nx; ++i) {
+          T s = T(0);
+          for (auto& ch : CHILDREN) s += fine[((2 * k + ch[2]) * fny + 2 * j + ch[1]) * fnx + 2 * i + ch[0]];
+          L.r[L.c(i, j, k)] = T(0.125) * s;
+        }
+  }
+  // delta3d_multigrid.rs:275-287
+  static void prolong_add3(size_t fnx, size_t fny, std::vector<T>& fine, const Level3<T>& L) {
+    for (size_t k = 0; k < L.nz; ++k)
+      for (size_t j = 0; j < L.ny; ++j)
+        for (size_t i = 0; i < L.nx; ++i) {
+          T val = L.x[L.c(i, j, k)];
+          for (auto& ch : CHILDREN) fine[((2 * k + ch[2]) * fny + 2 * j + ch[1]) * fnx + 2 * i + ch[0]] += val;
+        }
+  }
+
+  static T damping() { return T(6) / T(7); }  // delta3d_multigrid.rs:28
+
+  // delta3d_multigrid.rs:290-299
+  static void smooth3(Level3<T>& L) {
+    row_sums(L, &L.x, L.t);
+    for (size_t cc = 0; cc < L.cells(); ++cc)
+      if (L.diag[cc] > T(0)) L.x[cc] += damping() * (L.r[cc] - L.t[cc]) / L.diag[cc];
+  }
+  // EXTENSION: one colour of a Gauss-Seidel sweep on a coarse level. Cells of
+  // one colour do not couple, so the update uses the other colour's values.
+  static void gs_colour(Level3<T>& L, int colour) {
+    row_sums(L, &L.x, L.t);
+    for (size_t k = 0; k < L.nz; ++k)
+      for (size_t j = 0; j < L.ny; ++j)
+        for (size_t i = 0; i < L.nx; ++i) {
+          if ((int)((i + j + k) & 1) != colour) continue;
+          size_t cc = L.c(i, j, k);
+          if (L.diag[cc] > T(0)) L.x[cc] += (L.r[cc] - L.t[cc]) / L.diag[cc];
+        }
+  }
+  // A sweep: Jacobi (reference) or one red-black pass; `forward` = red then
+  // black (pre-smoothing), else black then red (post-smoothing, the adjoint).
+  void level_sweep(Level3<T>& L, bool forward) const {
+    if (opt.smoother == 0) { smooth3(L); return; }
+    gs_colour(L, forward ? 0 : 1);
+    gs_colour(L, forward ? 1 : 0);
+  }
+
+  // delta3d_multigrid.rs:303-332
+  void coarse_cycle3() {
+    size_t last = levels.size() - 1;
+    for (size_t l = 0; l <= last; ++l) {
+      Level3<T>& L = levels[l];
+      std::fill(L.x.begin(), L.x.end(), T(0));
+      if (l == last) {
+        if (opt.smoother == 0) {
+          for (int s = 0; s < opt.coarse_sweeps; ++s) smooth3(L);
+        } else {
+          // palindromic red-black sequence so the coarsest correction is symmetric
+          int half = opt.coarse_sweeps / 2;
+          for (int s = 0; s < half; ++s) level_sweep(L, true);
+          for (int s = 0; s < half; ++s) level_sweep(L, false);
+        }
+      } else {
+        for (int s = 0; s < opt.pre_sweeps; ++s) level_sweep(L, true);
+      }
+      if (l < last) {
+        row_sums(L, &L.x, L.t);
+        for (size_t cc = 0; cc < L.cells(); ++cc) L.t[cc] = L.r[cc] - L.t[cc];
+        restrict3(L.nx, L.ny, L.t, levels[l + 1]);
+      }
+    }
+    for (size_t l = last; l-- > 0;) {
+      Level3<T>& L = levels[l];
+      prolong_add3(L.nx, L.ny, L.x, levels[l + 1]);
+      for (int s = 0; s < opt.post_sweeps; ++s) level_sweep(L, false);
+    }
+  }
+
+  // Fine-level sweep on mz with the exact operator and prec
+  // (delta3d_multigrid.rs:398-405, :420-427; RBGS is the EXTENSION).
+  void fine_sweep(bool forward) {
+    if (opt.smoother == 0) {
+      apply_mobile3(mz, mt);
+      for (size_t cc = 0; cc < cells(); ++cc)
+        if (prec[cc] > T(0)) mz[cc] += damping() * prec[cc] * (res[cc] - mt[cc]);
+      return;
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+      int colour = forward ? pass : 1 - pass;
+      apply_mobile3(mz, mt);
+      for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+          for (size_t i = 0; i < nx; ++i) {
+            if ((int)((i + j + k) & 1) != colour) continue;
+            size_t cc = c(i, j, k);
+            if (prec[cc] > T(0)) mz[cc] += prec[cc] * (res[cc] - mt[cc]);
+          }
+    }
+  }
+
+  // delta3d_multigrid.rs:393-429
+  void v_cycle3() {
+    std::fill(mz.begin(), mz.end(), T(0));
+    for (int s = 0; s < opt.pre_sweeps; ++s) fine_sweep(true);
+    if (!levels.empty()) {
+      apply_mobile3(mz, mt);
+      for (size_t cc = 0; cc < cells(); ++cc) mt[cc] = res[cc] - mt[cc];
+      restrict3(nx, ny, mt, levels[0]);
+      coarse_cycle3();
+      prolong_add3(nx, ny, mz, levels[0]);
+      for (size_t cc = 0; cc < cells(); ++cc) if (prec[cc] == T(0)) mz[cc] = T(0);
+    }
+    for (int s = 0; s < opt.post_sweeps; ++s) fine_sweep(false);
+  }
+
+  // delta3d_multigrid.rs:433-445
+  Status prime_multigrid3(T beta, T& rz) {
+    v_cycle3();
+    rz = dot(res, mz);
+    for (size_t q = 0; q < dir.size(); ++q) dir[q] = beta == T(0) ? mz[q] : mz[q] + beta * dir[q];
+    if (!std::isfinite(rz)) return ERR_NOT_FINITE;
+    return OK;
+  }
+  // delta3d_multigrid.rs:448-457
+  Status precondition_multigrid3(T& rz) {
+    v_cycle3();
+    rz = dot(res, mz);
+    if (!std::isfinite(rz)) return ERR_NOT_FINITE;
+    return OK;
+  }
+  // delta3d_multigrid.rs:460-465
+  void direction_multigrid3(T beta) {
+    for (size_t q = 0; q < dir.size(); ++q) dir[q] = mz[q] + beta * dir[q];
+  }
+
+  Status prime(T& rz) {
+    if (mg_on) return prime_multigrid3(T(0), rz);
+    return prime_mobile3(T(0), rz);
+  }
+
+  // EXTENSION: a problem with no Dirichlet face anywhere (no ghost face on
+  // the fine level) is singular with the constants as null space.
+  bool detect_pure_neumann() const {
+    for (size_t k = 0; k < nz; ++k)
+      for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+          if (!wet3(i, j, k)) continue;
+          Row row = mobile_row(i, j, k);
+          for (int f = 0; f < 6; ++f) if (row.n[f] < 0 && row.a[f] > T(0)) return false;
+        }
+    return true;
+  }
+  void remove_wet_mean(std::vector<T>& x) const {
+    double s = 0; size_t n = 0;
+    for (size_t cc = 0; cc < cells(); ++cc) if (prec[cc] > T(0)) { s += (double)x[cc]; ++n; }
+    if (!n) return;
+    T m = (T)(s / (double)n);
+    for (size_t cc = 0; cc < cells(); ++cc) if (prec[cc] > T(0)) x[cc] -= m;
+  }
+
+  // delta3d_mobile.rs:465-609. On entry us/vs/ws hold the predicted faces and
+  // p the warm start.
+  Status project_mobile3(T scale, T k1, uint32_t max_iters, Report& rep) {
+    divergence(us, vs, ws, rhs);
+    rhs_mobile3(scale);
+    if (mg_on) prepare_multigrid3();
+    pure_neumann = opt.gauge && detect_pure_neumann();
+    if (pure_neumann) remove_wet_mean(rhs);
+    T b2;
+    Status st = norm2(rhs, b2);
+    if (st) return st;
+    for (size_t k = 0; k < nz; ++k)
+      for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i)
+          if (!wet3(i, j, k)) p[c(i, j, k)] = T(0);
+    apply_mobile3(p, tmp);
+    for (size_t q = 0; q < res.size(); ++q) res[q] = rhs[q] - tmp[q];
+    T rr;
+    if ((st = norm2(res, rr))) return st;
+    T rz = T(0);
+    bool primed = false;
+    uint32_t it = 0;
+    const T tol = (T)1e-12;
+    bool floor_stop = false;
+    T physical_target = std::numeric_limits<T>::infinity();
+    bool have_checkpoint = false;
+    uint64_t checkpoint = 0;
+    uint32_t power = 1, since = 0;
+    T actual_rr;
+    bool settled = false;
+    double set_all = 0, set_plain = 0;
+    for (;;) {
+      uint32_t before = it;
+      while (b2 > T(0) && (rr > tol * b2 || rr > physical_target) && it < max_iters) {
+        if (!primed) {
+          if ((st = prime(rz))) return st;
+          primed = true;
+        }
+        apply_mobile3(dir, tmp);
+        T dq = dot(dir, tmp);
+        if (!(dq > T(0))) break;
+        T alpha = rz / dq;
+        for (size_t q = 0; q < p.size(); ++q) {
+          p[q] += alpha * dir[q];
+          res[q] -= alpha * tmp[q];
+        }
+        T rn;
+        if ((st = norm2(res, rn))) return st;
+        if (mg_on) {
+          T zn;
+          if ((st = precondition_multigrid3(zn))) return st;
+          direction_multigrid3(zn / rz);
+          rz = zn;
+        } else {
+          T zn;
+          if ((st = dot_prec3(zn))) return st;
+          T beta = zn / rz;
+          T unused;
+          if ((st = prime_mobile3(beta, unused))) return st;
+          rz = zn;
+        }
+        rr = rn;
+        ++it;
+      }
+      apply_mobile3(p, tmp);
+      for (size_t q = 0; q < res.size(); ++q) res[q] = rhs[q] - tmp[q];
+      T actual;
+      if ((st = norm2(res, actual))) return st;
+      bool exhausted = it >= max_iters || it == before;
+      bool have_measured = false;
+      double m_all = 0, m_plain = 0;
+      if (actual <= tol * b2) {
+        if (exhausted) { actual_rr = actual; break; }
+        if (pure_neumann) remove_wet_mean(p);
+        correct_mobile3(k1);
+        double r_all, r_plain;
+        divergence_mobile3(r_all, r_plain);
+        if (r_plain <= 1e-5) { actual_rr = actual; settled = true; set_all = r_all; set_plain = r_plain; break; }
+        have_measured = true; m_all = r_all; m_plain = r_plain;
+        T ratio = (T)(1e-5 / r_plain);
+        physical_target = actual * ratio * ratio;
+      } else if (exhausted) {
+        actual_rr = actual;
+        break;
+      }
+      if (backward_error_mobile3() <= mobile_roundoff()) {
+        floor_stop = true;
+        actual_rr = actual;
+        if (have_measured) { settled = true; set_all = m_all; set_plain = m_plain; }
+        break;
+      }
+      uint64_t state = fingerprint(p);
+      if (have_checkpoint && checkpoint == state) {
+        floor_stop = true;
+        actual_rr = actual;
+        if (have_measured) { settled = true; set_all = m_all; set_plain = m_plain; }
+        break;
+      }
+      ++since;
+      if (since == power) {
+        have_checkpoint = true;
+        checkpoint = state;
+        power = power > 0x7fffffffu ? 0xffffffffu : power * 2;
+        since = 0;
+      }
+      rr = actual;
+      if ((st = prime(rz))) return st;
+    }
+    T residual = b2 > T(0) ? std::sqrt(actual_rr / b2) : T(0);
+    double d_all, d_plain;
+    if (settled) { d_all = set_all; d_plain = set_plain; }
+    else {
+      if (pure_neumann) remove_wet_mean(p);
+      correct_mobile3(k1);
+      divergence_mobile3(d_all, d_plain);
+    }
+    bool accepted = (actual_rr <= tol * b2 || floor_stop) && d_plain <= 1e-5;
+    rep.refinements = 0;
+    rep.iterations = it;
+    rep.degraded = (b2 > T(0) && !accepted) ? 1 : 0;
+    rep.residual = (double)residual;
+    rep.divergence = d_all;
+    rep.floor = floor_stop ? 1 : 0;
+    rep.backward_error = (double)backward_error_mobile3();
+    rep.divergence_plain = d_plain;
+    return OK;
+  }
+};
+
+}  // namespace orc

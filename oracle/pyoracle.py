@@ -1,0 +1,284 @@
+"""CPU ORACLE — test infrastructure only.
+
+ctypes bindings of ``oracle/liboracle.so`` (the C++ restatement of the
+reference projection in ``oracle.hpp``). Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg import
+this module, as the checker or as the timed CPU baseline; the product path
+never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class Geometry(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("dx", C.c_float),
+        ("rho", C.c_float), ("g_eff", C.c_float), ("rest", C.c_float),
+        ("homogeneous_ghost", C.c_int32), ("closed_lid", C.c_int32),
+        ("eta", C.c_void_p), ("eta_roundoff", C.c_void_p), ("solid", C.c_void_p),
+        ("open_u", C.c_void_p), ("open_v", C.c_void_p), ("open_w", C.c_void_p),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("multigrid", C.c_int32), ("smoother", C.c_int32), ("max_levels", C.c_int32),
+        ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32), ("coarse_sweeps", C.c_int32),
+        ("gauge", C.c_int32),
+    ]
+
+
+class Report(C.Structure):
+    _fields_ = [
+        ("refinements", C.c_uint32), ("iterations", C.c_uint32), ("degraded", C.c_int32),
+        ("floor", C.c_int32), ("residual", C.c_double), ("divergence", C.c_double),
+        ("backward_error", C.c_double), ("divergence_plain", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp, f32, u32, i32, sz = C.c_void_p, C.c_float, C.c_uint32, C.c_int, C.c_size_t
+        L.orc_project.argtypes = [i32, vp, vp, vp, vp, vp, vp, f32, f32, u32, vp, vp, vp, vp, vp, vp]
+        L.orc_apply.argtypes = [vp, vp, vp]
+        L.orc_apply64.argtypes = [vp, vp, vp]
+        L.orc_assemble.argtypes = [vp, vp, f32, vp, vp]
+        L.orc_divergence.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_correct.argtypes = [vp, vp, vp, vp, vp, f32, vp, vp, vp, vp, vp]
+        L.orc_vcycle.argtypes = [i32, vp, vp, vp, vp]
+        L.orc_classify.argtypes = [vp, vp, vp]
+        L.orc_coarse_level.argtypes = [vp, vp, i32, vp, vp]
+        L.orc_level_count3.argtypes = [sz, sz, sz]
+        L.orc_level_count3.restype = sz
+        L.orc_hierarchy_floats3.argtypes = [sz, sz, sz]
+        L.orc_hierarchy_floats3.restype = sz
+        L.orc_fingerprint.argtypes = [vp, sz]
+        L.orc_fingerprint.restype = C.c_uint64
+        L.orc_roundoff_3d.restype = C.c_float
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+@dataclass
+class Problem:
+    """Geometry arrays of one projection (reference layout, x fastest, z up)."""
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    eta: np.ndarray
+    rho: float = 1025.0
+    g_eff: float = 9.81
+    rest: float = 0.0
+    homogeneous_ghost: bool = True
+    closed_lid: bool = False
+    solid: np.ndarray | None = None
+    eta_roundoff: np.ndarray | None = None
+    open_u: np.ndarray | None = None
+    open_v: np.ndarray | None = None
+    open_w: np.ndarray | None = None
+
+    @property
+    def cells(self):
+        return self.nx * self.ny * self.nz
+
+    @property
+    def face_counts(self):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        return (nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)
+
+    def geometry(self):
+        self._keep = [np.ascontiguousarray(self.eta, np.float32)]
+        def keep(a, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dt)
+            self._keep.append(a)
+            return a.ctypes.data
+        g = Geometry()
+        g.nx, g.ny, g.nz, g.dx = self.nx, self.ny, self.nz, self.dx
+        g.rho, g.g_eff, g.rest = self.rho, self.g_eff, self.rest
+        g.homogeneous_ghost = int(self.homogeneous_ghost)
+        g.closed_lid = int(self.closed_lid)
+        g.eta = self._keep[0].ctypes.data
+        g.eta_roundoff = keep(self.eta_roundoff, np.float32)
+        g.solid = keep(self.solid, np.uint8)
+        g.open_u = keep(self.open_u, np.float32)
+        g.open_v = keep(self.open_v, np.float32)
+        g.open_w = keep(self.open_w, np.float32)
+        return g
+
+
+def options(multigrid=1, smoother=0, max_levels=-1, pre=2, post=2, coarse=8, gauge=1):
+    return Options(multigrid, smoother, max_levels, pre, post, coarse, gauge)
+
+
+def project(prob: Problem, us, vs, ws, scale, k1, max_iters=20000, opts=None, p_warm=None, precision=32):
+    """Run the projection; returns (report dict, p, u, v, w[, p64])."""
+    g = prob.geometry()
+    o = opts if opts is not None else options()
+    nu, nv, nw = prob.face_counts
+    us = np.ascontiguousarray(us, np.float32)
+    vs = np.ascontiguousarray(vs, np.float32)
+    ws = np.ascontiguousarray(ws, np.float32)
+    pw = None if p_warm is None else np.ascontiguousarray(p_warm, np.float32)
+    p = np.zeros(prob.cells, np.float32)
+    u = np.zeros(nu, np.float32)
+    v = np.zeros(nv, np.float32)
+    w = np.zeros(nw, np.float32)
+    p64 = np.zeros(prob.cells, np.float64) if precision == 64 else None
+    rep = Report()
+    st = lib().orc_project(precision, C.byref(g), C.byref(o), _ptr(us), _ptr(vs), _ptr(ws), _ptr(pw),
+                           scale, k1, max_iters, _ptr(p), _ptr(u), _ptr(v), _ptr(w), _ptr(p64), C.byref(rep))
+    if st != 0:
+        raise RuntimeError(f"oracle project failed: status {st}")
+    out = (rep.as_dict(), p, u, v, w)
+    return out + (p64,) if precision == 64 else out
+
+
+def apply(prob: Problem, x):
+    g = prob.geometry()
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros(prob.cells, np.float32)
+    assert lib().orc_apply(C.byref(g), _ptr(x), _ptr(out)) == 0
+    return out
+
+
+def apply64(prob: Problem, x):
+    g = prob.geometry()
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros(prob.cells, np.float64)
+    assert lib().orc_apply64(C.byref(g), _ptr(x), _ptr(out)) == 0
+    return out
+
+
+def assemble(prob: Problem, div, scale):
+    g = prob.geometry()
+    div = np.ascontiguousarray(div, np.float32)
+    rhs = np.zeros(prob.cells, np.float32)
+    prec = np.zeros(prob.cells, np.float32)
+    assert lib().orc_assemble(C.byref(g), _ptr(div), scale, _ptr(rhs), _ptr(prec)) == 0
+    return rhs, prec
+
+
+def divergence(prob: Problem, u, v, w):
+    g = prob.geometry()
+    u, v, w = (np.ascontiguousarray(a, np.float32) for a in (u, v, w))
+    out = np.zeros(prob.cells, np.float32)
+    assert lib().orc_divergence(C.byref(g), _ptr(u), _ptr(v), _ptr(w), _ptr(out)) == 0
+    return out
+
+
+def correct(prob: Problem, us, vs, ws, p, k1):
+    g = prob.geometry()
+    us, vs, ws, p = (np.ascontiguousarray(a, np.float32) for a in (us, vs, ws, p))
+    nu, nv, nw = prob.face_counts
+    u, v, w = np.zeros(nu, np.float32), np.zeros(nv, np.float32), np.zeros(nw, np.float32)
+    a, b = C.c_double(), C.c_double()
+    assert lib().orc_correct(C.byref(g), _ptr(us), _ptr(vs), _ptr(ws), _ptr(p), k1, _ptr(u), _ptr(v), _ptr(w),
+                             C.byref(a), C.byref(b)) == 0
+    return u, v, w, a.value, b.value
+
+
+def vcycle(prob: Problem, r, opts=None, precision=32):
+    g = prob.geometry()
+    o = opts if opts is not None else options()
+    r = np.ascontiguousarray(r, np.float32)
+    z = np.zeros(prob.cells, np.float64)
+    assert lib().orc_vcycle(precision, C.byref(g), C.byref(o), _ptr(r), _ptr(z)) == 0
+    return z
+
+
+def classify(prob: Problem):
+    g = prob.geometry()
+    kind = np.zeros(prob.cells, np.int8)
+    ghost = np.zeros(prob.cells, np.uint8)
+    assert lib().orc_classify(C.byref(g), _ptr(kind), _ptr(ghost)) == 0
+    return kind, ghost
+
+
+def coarse_level(prob: Problem, level, opts=None):
+    g = prob.geometry()
+    o = opts if opts is not None else options()
+    nx, ny, nz = prob.nx, prob.ny, prob.nz
+    for _ in range(level + 1):
+        nx, ny, nz = nx // 2, ny // 2, nz // 2
+    kind = np.zeros(nx * ny * nz, np.float32)
+    diag = np.zeros(nx * ny * nz, np.float32)
+    st = lib().orc_coarse_level(C.byref(g), C.byref(o), level, _ptr(kind), _ptr(diag))
+    if st != 0:
+        raise RuntimeError(f"no coarse level {level}")
+    return kind, diag
+
+
+def level_count3(nx, ny, nz):
+    return lib().orc_level_count3(nx, ny, nz)
+
+
+def hierarchy_floats3(nx, ny, nz):
+    return lib().orc_hierarchy_floats3(nx, ny, nz)
+
+
+def fingerprint(values):
+    a = np.ascontiguousarray(values, np.float32)
+    return lib().orc_fingerprint(_ptr(a), a.size)
+
+
+# ---------------------------------------------------------------- S385 bench case
+_libm = C.CDLL("libm.so.6")
+_libm.sinf.argtypes = [C.c_float]
+_libm.sinf.restype = C.c_float
+
+
+def _sinf(x):
+    # Rust's f32::sin is the platform libm's sinf.
+    return _libm.sinf(x)
+
+
+def s385_problem(nx: int) -> tuple[Problem, float, float]:
+    """The reference's S385 bench case (examples/delta3d_multigrille.rs:28-36,
+    58-79): an 8 x 4 x 6 m tank, flat bottom, surface 4 m plus a 1 cm sine
+    along x, rho = 1025, g = 9.81, dt = 2 ms. Returns (problem, scale, k1) of
+    the first mobile step from rest (delta3d_mobile.rs:836-837)."""
+    ny, nz = nx // 2, 3 * nx // 4
+    f32 = np.float32
+    dx = f32(8.0) / f32(nx)
+    tau = f32(2.0 * np.pi)
+    i = np.arange(nx, dtype=f32)
+    # LZ + A * (TAU * (i + 0.5) * dx / LX).sin(), every operation in f32
+    arg = ((tau * (i + f32(0.5))) * dx) / f32(8.0)
+    s = np.array([_sinf(float(a)) for a in arg], dtype=f32)
+    row = (f32(4.0) + f32(0.01) * s).astype(f32)
+    eta = np.tile(row, ny).astype(f32)
+    prob = Problem(nx, ny, nz, float(dx), eta, rho=1025.0, g_eff=9.81, rest=4.0, homogeneous_ghost=False)
+    dt = 2000 * 1e-6
+    scale = float(np.float32(-1025.0 / dt))
+    k1 = float(np.float32(dt / 1025.0))
+    return prob, scale, k1
