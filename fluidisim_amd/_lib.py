@@ -1,0 +1,122 @@
+"""ctypes bindings of libfsp.so, the C ABI declared in include/fsp.h.
+
+The library is the product: loading fails loudly when it has not been built
+(there is no fallback implementation).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfsp.so")
+
+FSP_OK, FSP_DOMAIN, FSP_NOT_FINITE, FSP_SHAPE, FSP_CONVERGENCE, FSP_DEVICE, FSP_INVALID = range(7)
+PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
+SMOOTH_JACOBI, SMOOTH_RBGS = 0, 1
+STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
+
+# Every entry point include/fsp.h declares.
+EXPORTS = [
+    "fsp_abi_version", "fsp_status_string", "fsp_default_options", "fsp_create", "fsp_destroy", "fsp_levels",
+    "fsp_set_geometry", "fsp_project", "fsp_project_device", "fsp_apply_operator_device", "fsp_assemble_device",
+    "fsp_precondition_device", "fsp_divergence_device", "fsp_profile_begin", "fsp_profile_end", "fsp_generate",
+]
+
+
+class Domain(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("dx", C.c_float)]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("preconditioner", C.c_int32), ("smoother", C.c_int32), ("max_levels", C.c_int32),
+        ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32), ("coarse_sweeps", C.c_int32),
+        ("gauge", C.c_int32), ("vcycle_storage", C.c_int32), ("replace_every", C.c_int32),
+        ("refuse_degraded", C.c_int32), ("device", C.c_int32),
+    ]
+
+
+class Surface(C.Structure):
+    _fields_ = [("rho", C.c_float), ("g_eff", C.c_float), ("rest", C.c_float),
+                ("homogeneous_ghost", C.c_int32), ("closed_lid", C.c_int32)]
+
+
+class Report(C.Structure):
+    _fields_ = [
+        ("refinements", C.c_uint32), ("iterations", C.c_uint32), ("degraded", C.c_int32), ("floor", C.c_int32),
+        ("residual", C.c_double), ("divergence", C.c_double), ("backward_error", C.c_double),
+        ("divergence_plain", C.c_double), ("ms_device", C.c_double), ("ms_transfer", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class FspError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        super().__init__(f"{what}: {status_string(status)} ({status})")
+
+
+_lib = None
+
+
+def lib():
+    """Load libfsp.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    L = C.CDLL(LIB_PATH)
+    vp, sz, f32, f64, i32, u32 = C.c_void_p, C.c_size_t, C.c_float, C.c_double, C.c_int32, C.c_uint32
+    L.fsp_abi_version.restype = i32
+    L.fsp_status_string.argtypes = [i32]
+    L.fsp_status_string.restype = C.c_char_p
+    L.fsp_default_options.argtypes = [vp]
+    L.fsp_create.argtypes = [vp, vp, vp]
+    L.fsp_create.restype = i32
+    L.fsp_destroy.argtypes = [vp]
+    L.fsp_levels.argtypes = [vp]
+    L.fsp_levels.restype = i32
+    L.fsp_set_geometry.argtypes = [vp, vp, vp, sz, vp, vp, sz, vp, sz, vp, sz, vp, sz]
+    L.fsp_set_geometry.restype = i32
+    L.fsp_project.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp, sz, i32, vp, sz, vp, sz, vp, sz, f32, f32, f64, u32, vp]
+    L.fsp_project.restype = i32
+    L.fsp_project_device.argtypes = [vp, vp, vp, vp, vp, i32, vp, vp, vp, f32, f32, f64, u32, vp]
+    L.fsp_project_device.restype = i32
+    L.fsp_apply_operator_device.argtypes = [vp, vp, vp]
+    L.fsp_apply_operator_device.restype = i32
+    L.fsp_assemble_device.argtypes = [vp, vp, f32, vp, vp]
+    L.fsp_assemble_device.restype = i32
+    L.fsp_precondition_device.argtypes = [vp, vp, vp]
+    L.fsp_precondition_device.restype = i32
+    L.fsp_divergence_device.argtypes = [vp, vp, vp, vp, vp]
+    L.fsp_divergence_device.restype = i32
+    L.fsp_profile_begin.argtypes = [vp, C.c_char_p, u32]
+    L.fsp_profile_begin.restype = i32
+    L.fsp_profile_end.argtypes = [vp, vp, vp, vp, u32]
+    L.fsp_profile_end.restype = i32
+    L.fsp_generate.argtypes = [i32, i32, i32, i32, C.c_uint64, vp, vp, vp, vp, vp]
+    L.fsp_generate.restype = i32
+    _lib = L
+    return L
+
+
+def status_string(status: int) -> str:
+    return lib().fsp_status_string(status).decode()
+
+
+def default_options(**kw) -> Options:
+    o = Options()
+    lib().fsp_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def check(status: int, what: str):
+    if status != FSP_OK:
+        raise FspError(status, what)

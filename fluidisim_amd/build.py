@@ -1,0 +1,46 @@
+"""Build recipe of libfsp.so (HIP kernels + C ABI for gfx950) in-tree.
+
+hipcc compiles the device code for gfx950 with packed-FP32 instructions
+disabled; the input generators are plain host C++ built with g++.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libfsp.so")
+ARCH = os.environ.get("FSP_OFFLOAD_ARCH", "gfx950")
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose: bool = True) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    fsp_o = os.path.join(BUILD, "fsp.o")
+    gen_o = os.path.join(BUILD, "generate.o")
+    _run([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, "fsp.hip"), "-o", fsp_o])
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", "-c",
+          os.path.join(CSRC, "generate.cpp"), "-o", gen_o])
+    tmp = LIB + ".tmp"
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-o", tmp, fsp_o, gen_o, "-pthread"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,846 @@
+// C ABI and host driver of the MI355X pressure projection.
+//
+// The driver restates the control flow of the reference's
+// Volume3::project_mobile3 (code/water-core/src/delta3d_mobile.rs:465-609):
+// assembly, warm start, conjugate-gradient passes, true residual recomputed
+// after each pass, acceptance on the plain-row divergence with target
+// tightening, floor stops on the backward error or a repeated state of p.
+// The iterations themselves run on the device: every scalar of the CG
+// (alpha, beta, the loop test) is computed by the last block of the kernel
+// that produced its reduction, so the host only polls a small control block
+// between batches of iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/fsp.h"
+#include "kernels.h"
+
+using namespace fsp;
+
+struct fsp_solver {
+  fsp_domain dom{};
+  fsp_options opt{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  size_t n = 0, nu = 0, nv = 0, nw = 0, ncols = 0;
+  bool geom = false;
+  int cut = 0;
+  fsp_surface surf{};
+  uint8_t *solid = nullptr, *ou = nullptr, *ov = nullptr, *ow = nullptr, *mask = nullptr;
+  float *eta = nullptr, *eta_rt = nullptr;
+  float *b = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *zt = nullptr, *d0 = nullptr,
+        *d1 = nullptr;
+  float *us = nullptr, *vs = nullptr, *ws = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
+  double* p64 = nullptr;  // the pressure, accumulated in f64
+  std::vector<Level> levels;
+  Ctl* ctl = nullptr;
+  double* partials = nullptr;
+  unsigned long long *hashes = nullptr, *hash_out = nullptr;
+  Ctl* hctl = nullptr;
+  unsigned long long* hhash = nullptr;
+  std::vector<void*> allocs;
+  // live timing of one kernel (fsp_profile_begin / fsp_profile_end)
+  bool prof_on = false;
+  std::string prof_name;
+  std::vector<hipEvent_t> prof_events;
+  size_t prof_used = 0;
+
+  Fine fine() const {
+    Fine G;
+    G.nx = dom.nx; G.ny = dom.ny; G.nz = dom.nz;
+    G.cut = cut;
+    G.n = n;
+    G.dx = dom.dx;
+    G.inv = 1.0f / (dom.dx * dom.dx);
+    G.rho = surf.rho; G.g = surf.g_eff; G.rest = surf.rest;
+    G.homogeneous = surf.homogeneous_ghost;
+    G.solid = solid; G.ou = ou; G.ov = ov; G.ow = ow;
+    G.eta = eta; G.eta_rt = eta_rt;
+    G.mask = mask;
+    return G;
+  }
+};
+
+namespace {
+
+#define FSP_HIP(call)                                  \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) {                            \
+      std::fprintf(stderr, "fsp: %s failed: %s\n", #call, hipGetErrorString(e_)); \
+      return FSP_DEVICE;                               \
+    }                                                  \
+  } while (0)
+
+unsigned grid_for(size_t n) {
+  size_t b = (n + BLOCK - 1) / BLOCK;
+  if (b > (size_t)MAX_BLOCKS) b = MAX_BLOCKS;
+  if (b == 0) b = 1;
+  return (unsigned)b;
+}
+
+// Launch on the solver's stream; when the kernel is the one being timed,
+// bracket the launch with a pair of events.
+#define FSP_LAUNCH(S, kern, grid, stream, ...)                                                     \
+  do {                                                                                         \
+    const bool prof_ = (S)->prof_on && (S)->prof_name == #kern &&                              \
+                       (S)->prof_used + 2 <= (S)->prof_events.size();                           \
+    if (prof_) (void)hipEventRecord((S)->prof_events[(S)->prof_used], stream);                 \
+    hipLaunchKernelGGL(kern, grid, dim3(BLOCK), 0, stream, __VA_ARGS__);                       \
+    if (prof_) {                                                                               \
+      (void)hipEventRecord((S)->prof_events[(S)->prof_used + 1], stream);                     \
+      (S)->prof_used += 2;                                                                     \
+    }                                                                                          \
+  } while (0)
+
+template <class T>
+fsp_status dalloc(fsp_solver* S, T** out, size_t count) {
+  void* ptr = nullptr;
+  size_t bytes = count * sizeof(T);
+  if (bytes == 0) bytes = 256;
+  FSP_HIP(hipMalloc(&ptr, bytes));
+  S->allocs.push_back(ptr);
+  *out = (T*)ptr;
+  return FSP_OK;
+}
+
+bool coarsens3(size_t nx, size_t ny, size_t nz) {
+  return nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0 && nx >= 4 && ny >= 4 && nz >= 4;
+}
+
+__global__ void k_reset(Ctl* ctl, int gauge) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned char* bytes = (unsigned char*)ctl;
+  for (size_t q = 0; q < sizeof(Ctl); ++q) bytes[q] = 0;
+  ctl->first_dir = 1;
+  ctl->physical_target = __longlong_as_double(0x7ff0000000000000ll);
+  ctl->gauge = gauge;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_fill_gated(float* x, size_t n, float value, Ctl* ctl) {
+  if (!ctl->active) return;
+  FSP_FOR(c, n) x[c] = value;
+}
+__global__ void __launch_bounds__(BLOCK) k_copy_gated(const float* a, float* b, size_t n, Ctl* ctl) {
+  if (!ctl->active) return;
+  FSP_FOR(c, n) b[c] = a[c];
+}
+
+struct Launch {
+  fsp_solver* S;
+  hipStream_t st;
+  Fine G;
+  explicit Launch(fsp_solver* s) : S(s), st(s->stream), G(s->fine()) {}
+
+  void mask() { FSP_LAUNCH(S, k_mask, dim3(grid_for(G.n)), st, G, S->mask, S->ctl, S->partials); }
+
+  // Coarse geometry of every level (prepare_multigrid3, delta3d_multigrid.rs:370-387).
+  void hierarchy() {
+    for (size_t l = 0; l < S->levels.size(); ++l) {
+      Level& L = S->levels[l];
+      const size_t nf = (size_t)(L.nx + 1) * L.ny * L.nz + (size_t)L.nx * (L.ny + 1) * L.nz +
+                        (size_t)L.nx * L.ny * (L.nz + 1);
+      if (l == 0) {
+        FSP_LAUNCH(S, k_ckind_fine, dim3(grid_for(L.n)), st, G, L);
+        FSP_LAUNCH(S, k_cfaces_fine, dim3(grid_for(nf)), st, G, L);
+      } else {
+        FSP_LAUNCH(S, k_ckind, dim3(grid_for(L.n)), st, S->levels[l - 1], L);
+        FSP_LAUNCH(S, k_cfaces, dim3(grid_for(nf)), st, S->levels[l - 1], L);
+      }
+      FSP_LAUNCH(S, k_cdiag, dim3(grid_for(L.n)), st, L);
+    }
+  }
+
+  void fill_gated(float* x, size_t n, float v) {
+    FSP_LAUNCH(S, k_fill_gated, dim3(grid_for(n)), st, x, n, v, S->ctl);
+  }
+
+  // Sweeps on coarse level l; `x` is the current iterate (nullptr = zero),
+  // returns the buffer holding the result.
+  float* coarse_sweeps(int l, float* x, int count, bool forward, bool coarsest) {
+    Level& L = S->levels[l];
+    const int smoother = S->opt.smoother;
+    if (count == 0) {
+      if (!x) { fill_gated(L.x, L.n, 0.f); return L.x; }
+      return x;
+    }
+    if (smoother == FSP_SMOOTH_JACOBI) {
+      float* cur = x;
+      for (int s = 0; s < count; ++s) {
+        float* out = (cur == L.x) ? L.t : L.x;
+        FSP_LAUNCH(S, k_cjacobi, dim3(grid_for(L.n)), st, L, cur, out, S->ctl);
+        cur = out;
+      }
+      return cur;
+    }
+    // red-black Gauss-Seidel, in place on L.x
+    int first = x ? 0 : 1;
+    if (x && x != L.x) {
+      FSP_LAUNCH(S, k_copy_gated, dim3(grid_for(L.n)), st, x, L.x, L.n, S->ctl);
+    }
+    auto colour = [&](int c) {
+      FSP_LAUNCH(S, k_cgs, dim3(grid_for(L.n)), st, L, L.x, c, first, S->ctl);
+      first = 0;
+    };
+    if (coarsest) {
+      const int half = count / 2;
+      for (int s = 0; s < half; ++s) { colour(0); colour(1); }
+      if (count & 1) colour(0);
+      for (int s = 0; s < half; ++s) { colour(1); colour(0); }
+    } else {
+      for (int s = 0; s < count; ++s) {
+        if (forward) { colour(0); colour(1); } else { colour(1); colour(0); }
+      }
+    }
+    return L.x;
+  }
+
+  // coarse_cycle3 (delta3d_multigrid.rs:303-332); returns the level-0 iterate.
+  float* coarse_cycle() {
+    const int last = (int)S->levels.size() - 1;
+    std::vector<float*> xcur(S->levels.size(), nullptr);
+    for (int l = 0; l <= last; ++l) {
+      const int sweeps = (l == last) ? S->opt.coarse_sweeps : S->opt.pre_sweeps;
+      xcur[l] = coarse_sweeps(l, nullptr, sweeps, true, l == last);
+      if (l < last) {
+        Level& L = S->levels[l];
+        Level& C = S->levels[l + 1];
+        FSP_LAUNCH(S, k_crestrict, dim3(grid_for(C.n)), st, L, C, xcur[l], S->ctl);
+      }
+    }
+    for (int l = last - 1; l >= 0; --l) {
+      Level& L = S->levels[l];
+      Level& C = S->levels[l + 1];
+      FSP_LAUNCH(S, k_cprolong, dim3(grid_for(L.n)), st, L, C, xcur[l + 1], xcur[l], S->ctl);
+      xcur[l] = coarse_sweeps(l, xcur[l], S->opt.post_sweeps, false, false);
+    }
+    return xcur[0];
+  }
+
+  // Fine sweeps; `cur` is the current iterate (nullptr = zero); returns the result buffer.
+  float* fine_sweeps(float* cur, int count, bool forward) {
+    if (count == 0) {
+      if (!cur) { fill_gated(S->z, G.n, 0.f); return S->z; }
+      return cur;
+    }
+    if (S->opt.smoother == FSP_SMOOTH_JACOBI) {
+      for (int s = 0; s < count; ++s) {
+        float* out = (cur == S->z) ? S->zt : S->z;
+        FSP_LAUNCH(S, k_fjacobi, dim3(grid_for(G.n)), st, G, cur, out, S->r, S->ctl);
+        cur = out;
+      }
+      return cur;
+    }
+    int first = cur ? 0 : 1;
+    if (cur && cur != S->z) {
+      FSP_LAUNCH(S, k_copy_gated, dim3(grid_for(G.n)), st, cur, S->z, G.n, S->ctl);
+    }
+    for (int s = 0; s < count; ++s) {
+      const int c0 = forward ? 0 : 1;
+      FSP_LAUNCH(S, k_fgs, dim3(grid_for(G.n)), st, G, S->z, S->r, c0, first, S->ctl);
+      first = 0;
+      FSP_LAUNCH(S, k_fgs, dim3(grid_for(G.n)), st, G, S->z, S->r, 1 - c0, 0, S->ctl);
+    }
+    return S->z;
+  }
+
+  // z = M^-1 r (v_cycle3, delta3d_multigrid.rs:393-429) into S->z.
+  void vcycle() {
+    // With Jacobi sweeps the iterate alternates between z and zt; start so
+    // that an even total ends in z.
+    float* cur = nullptr;
+    cur = fine_sweeps(nullptr, S->opt.pre_sweeps, true);
+    if (!S->levels.empty()) {
+      Level& L0 = S->levels[0];
+      FSP_LAUNCH(S, k_restrict_fine, dim3(grid_for(L0.n)), st, G, L0, cur, S->r, S->ctl);
+      float* x0 = coarse_cycle();
+      FSP_LAUNCH(S, k_prolong_fine, dim3(grid_for(G.n)), st, G, L0, x0, cur, S->ctl);
+    }
+    cur = fine_sweeps(cur, S->opt.post_sweeps, false);
+    if (cur != S->z) {
+      FSP_LAUNCH(S, k_copy_gated, dim3(grid_for(G.n)), st, cur, S->z, G.n, S->ctl);
+    }
+  }
+
+  void precondition(int op) {
+    if (S->opt.preconditioner == FSP_PRECOND_JACOBI) {
+      FSP_LAUNCH(S, k_jacobi_prec, dim3(grid_for(G.n)), st, G, S->r, S->z, op, S->ctl,
+                         S->partials);
+      return;
+    }
+    vcycle();
+    FSP_LAUNCH(S, k_dot, dim3(grid_for(G.n)), st, G.n, S->r, S->z, op, 1, S->ctl, S->partials);
+  }
+
+  void iteration() {
+    FSP_LAUNCH(S, k_spmv_dir, dim3(grid_for(G.n)), st, G, S->z, S->d0, S->d1, S->q, S->ctl,
+                       S->partials);
+    FSP_LAUNCH(S, k_update, dim3(grid_for(G.n)), st, G.n, S->p64, S->r, S->d0, S->d1, S->q,
+                       S->ctl, S->partials);
+    precondition(OP_RZ);
+  }
+
+  void set_pass(int active, double rr, double tol_b2, double target, uint32_t it, uint32_t max_iters) {
+    // A pass ends after replace_every iterations so the recurrence residual
+    // is replaced by the true one (f64) at least that often.
+    const uint32_t every = S->opt.replace_every > 0 ? (uint32_t)S->opt.replace_every : 0u;
+    const uint32_t limit = every ? (it + every < it ? 0xffffffffu : it + every) : 0xffffffffu;
+    hipLaunchKernelGGL(k_set_pass, dim3(1), dim3(64), 0, st, S->ctl, active, rr, tol_b2, target, it, max_iters, limit);
+  }
+};
+
+fsp_status sync_ctl(fsp_solver* S) {
+  FSP_HIP(hipMemcpyAsync(S->hctl, S->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, S->stream));
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  return FSP_OK;
+}
+
+void cleanup(fsp_solver* S) {
+  for (hipEvent_t e : S->prof_events) (void)hipEventDestroy(e);
+  S->prof_events.clear();
+  for (void* p : S->allocs) (void)hipFree(p);
+  S->allocs.clear();
+  if (S->hctl) (void)hipHostFree(S->hctl);
+  if (S->hhash) (void)hipHostFree(S->hhash);
+  if (S->ev0) (void)hipEventDestroy(S->ev0);
+  if (S->ev1) (void)hipEventDestroy(S->ev1);
+  if (S->stream) (void)hipStreamDestroy(S->stream);
+}
+
+// Correction and acceptance metric (delta3d_mobile.rs:560-561); returns
+// (divergence over all wet rows, over plain rows), normalised as the reference.
+fsp_status correct_and_measure(fsp_solver* S, const float* us, const float* vs, const float* ws, double* p, float k1,
+                               float* u, float* v, float* w, double& all, double& plain) {
+  Launch K(S);
+  if (S->opt.gauge) {
+    FSP_LAUNCH(S, k_sum_active<double>, dim3(grid_for(K.G.n)), K.st, K.G, (const double*)p,
+                       OP_MEAN_P, S->ctl, S->partials);
+    FSP_LAUNCH(S, k_sub_mean<double>, dim3(grid_for(K.G.n)), K.st, K.G, p, 1, -1, S->ctl,
+                       S->partials);
+  }
+  FSP_LAUNCH(S, k_correct, dim3(grid_for(S->nu + S->nv + S->nw)), K.st, K.G, us, vs, ws,
+                     (const double*)p, k1, u, v, w, S->ctl, S->partials);
+  FSP_LAUNCH(S, k_divmetric, dim3(grid_for(K.G.n)), K.st, K.G, u, v, w, S->ctl, S->partials);
+  fsp_status st = sync_ctl(S);
+  if (st) return st;
+  // delta3d_mobile.rs:422-432 (f32 arithmetic)
+  const float dmax_all = (float)S->hctl->red[OP_DIVMAX][0];
+  const float dmax_plain = (float)S->hctl->red[OP_DIVMAX][1];
+  const float mx = (float)S->hctl->red[OP_UMAX][0];
+  const float echelle = std::max(mx, 1e-4f);
+  const float ratio = mx > 0.f ? S->dom.dx / echelle : 0.f;
+  all = (double)(dmax_all * ratio);
+  plain = (double)(dmax_plain * ratio);
+  return FSP_OK;
+}
+
+fsp_status backward_error(fsp_solver* S, const double* p, double& be) {
+  Launch K(S);
+  FSP_LAUNCH(S, k_backward, dim3(grid_for(K.G.n)), K.st, K.G, p, S->b, S->r, S->ctl,
+                     S->partials);
+  fsp_status st = sync_ctl(S);
+  if (st) return st;
+  be = S->hctl->red[OP_BACKWARD][0];
+  return FSP_OK;
+}
+
+fsp_status fingerprint(fsp_solver* S, const double* p, uint64_t& h) {
+  const size_t chunks = (S->n + 1023) / 1024;
+  FSP_LAUNCH(S, k_hash_chunks, dim3(grid_for(chunks)), S->stream, p, S->n, S->hashes);
+  hipLaunchKernelGGL(k_hash_combine, dim3(1), dim3(64), 0, S->stream, S->hashes, chunks, S->hash_out);
+  FSP_HIP(hipMemcpyAsync(S->hhash, S->hash_out, sizeof(unsigned long long), hipMemcpyDeviceToHost, S->stream));
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  h = *S->hhash;
+  return FSP_OK;
+}
+
+// Geometry rebuilt from the surface at the start of every projection
+// (delta3d_mobile.rs:479-484).
+void build_geometry(fsp_solver* S, bool hierarchy) {
+  Launch K(S);
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, K.st, S->ctl, S->opt.gauge);
+  K.mask();
+  if (hierarchy && S->opt.preconditioner == FSP_PRECOND_MULTIGRID) K.hierarchy();
+}
+
+// The projection on device buffers (project_mobile3, delta3d_mobile.rs:465-609).
+fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws, float* p, int warm, float* u,
+               float* v, float* w, float scale, float k1, double rel_tol, uint32_t max_iters, fsp_report* rep) {
+  if (!S->geom) return FSP_DOMAIN;
+  Launch K(S);
+  const Fine& G = K.G;
+  FSP_HIP(hipEventRecord(S->ev0, S->stream));
+  build_geometry(S, true);
+  FSP_LAUNCH(S, k_assemble, dim3(grid_for(G.n)), K.st, G, us, vs, ws, (const float*)nullptr,
+                     scale, S->b, (float*)nullptr, S->ctl, S->partials);
+  if (S->opt.gauge) {
+    FSP_LAUNCH(S, k_sum_active<float>, dim3(grid_for(G.n)), K.st, G, (const float*)S->b,
+                       OP_MEAN_B, S->ctl, S->partials);
+    FSP_LAUNCH(S, k_sub_mean<float>, dim3(grid_for(G.n)), K.st, G, S->b, 0, OP_B2, S->ctl,
+                       S->partials);
+  }
+  double* p64 = S->p64;
+  FSP_LAUNCH(S, k_load_p, dim3(grid_for(G.n)), K.st, G, (const float*)p, p64, warm);
+  FSP_LAUNCH(S, k_residual, dim3(grid_for(G.n)), K.st, G, (const double*)p64, S->b, S->r, OP_RR,
+                     S->ctl, S->partials);
+  fsp_status st = sync_ctl(S);
+  if (st) return st;
+  if (S->hctl->nonfinite) return FSP_NOT_FINITE;
+
+  const double b2 = S->hctl->b2;
+  const double tol_b2 = rel_tol * rel_tol * b2;
+  double rr = S->hctl->actual;
+  double physical_target = std::numeric_limits<double>::infinity();
+  uint32_t it = 0;
+  bool primed = false, floor_stop = false;
+  bool have_ck = false;
+  uint64_t ck = 0;
+  uint32_t power = 1, since = 0;
+  double actual_rr = 0.0;
+  bool settled = false;
+  double set_all = 0.0, set_plain = 0.0;
+  // The floor certificate gamma_{m+4} (delta3d.rs:65-72, delta_projection.rs:194-203)
+  // with the unit roundoff of the pressure's precision (f64 here).
+  const double unit = 1.0 / 9007199254740992.0;
+  const double gm = S->dom.ny == 1 ? 8.0 : 10.0;
+  const double roundoff = gm * unit / (1.0 - gm * unit);
+  auto loop_test = [&](double rrv, uint32_t itv) {
+    return b2 > 0.0 && (rrv > tol_b2 || rrv > physical_target) && itv < max_iters;
+  };
+  for (;;) {
+    const uint32_t before = it;
+    if (loop_test(rr, it)) {
+      K.set_pass(1, rr, tol_b2, physical_target, it, max_iters);
+      if (!primed) {
+        K.precondition(OP_RZ_PRIME);
+        primed = true;
+      }
+      // Batches of iterations between polls, sized from the observed rate.
+      int batch = 1;
+      double rr_prev = rr;
+      uint32_t it_prev = it;
+      for (;;) {
+        for (int s = 0; s < batch; ++s) K.iteration();
+        if ((st = sync_ctl(S))) return st;
+        if (S->hctl->nonfinite) return FSP_NOT_FINITE;
+        if (!S->hctl->active) break;
+        const double rr_now = S->hctl->rr;
+        const uint32_t it_now = S->hctl->it;
+        if (it_now <= it_prev) {
+          std::fprintf(stderr, "fsp: the iteration count did not advance (%u)\n", it_now);
+          return FSP_DEVICE;
+        }
+        int next = 2;
+        if (it_now > it_prev && rr_now > 0.0 && rr_prev > 0.0 && rr_now < rr_prev) {
+          const double rate = std::pow(rr_now / rr_prev, 1.0 / (double)(it_now - it_prev));
+          const double target = std::min(tol_b2, physical_target);
+          if (rate < 1.0 && target > 0.0) {
+            const double est = std::ceil(std::log(target / rr_now) / std::log(rate));
+            next = (int)std::max(1.0, std::min(16.0, est));
+          }
+        }
+        next = (int)std::min<uint64_t>((uint64_t)next, (uint64_t)(max_iters - it_now));
+        batch = std::max(1, next);
+        rr_prev = rr_now;
+        it_prev = it_now;
+      }
+      it = S->hctl->it;
+      rr = S->hctl->rr;
+    }
+    // True residual, recomputed: it decides, never the recurrence (:546-553).
+    FSP_LAUNCH(S, k_residual, dim3(grid_for(G.n)), K.st, G, (const double*)p64, S->b, S->r,
+                       OP_RR, S->ctl, S->partials);
+    if ((st = sync_ctl(S))) return st;
+    if (S->hctl->nonfinite) return FSP_NOT_FINITE;
+    const double actual = S->hctl->actual;
+    const bool exhausted = it >= max_iters || it == before;
+    bool have_measured = false;
+    double m_all = 0.0, m_plain = 0.0;
+    if (actual <= tol_b2) {
+      if (exhausted) { actual_rr = actual; break; }
+      double a, pl;
+      if ((st = correct_and_measure(S, us, vs, ws, p64, k1, u, v, w, a, pl))) return st;
+      if (pl <= 1e-5) { actual_rr = actual; settled = true; set_all = a; set_plain = pl; break; }
+      have_measured = true; m_all = a; m_plain = pl;
+      const float ratio = (float)(1e-5 / pl);
+      physical_target = (double)((float)actual * ratio * ratio);
+    } else if (exhausted) {
+      actual_rr = actual;
+      break;
+    }
+    double be;
+    if ((st = backward_error(S, p64, be))) return st;
+    if (be <= roundoff) {
+      floor_stop = true;
+      actual_rr = actual;
+      if (have_measured) { settled = true; set_all = m_all; set_plain = m_plain; }
+      break;
+    }
+    uint64_t state;
+    if ((st = fingerprint(S, p64, state))) return st;
+    if (have_ck && ck == state) {
+      floor_stop = true;
+      actual_rr = actual;
+      if (have_measured) { settled = true; set_all = m_all; set_plain = m_plain; }
+      break;
+    }
+    ++since;
+    if (since == power) {
+      have_ck = true;
+      ck = state;
+      power = power > 0x7fffffffu ? 0xffffffffu : power * 2;
+      since = 0;
+    }
+    rr = actual;
+    // Re-prime on the true residual (:587) when another pass will run.
+    if (loop_test(rr, it)) {
+      K.set_pass(1, rr, tol_b2, physical_target, it, max_iters);
+      K.precondition(OP_RZ_PRIME);
+    }
+  }
+  double d_all, d_plain;
+  if (settled) { d_all = set_all; d_plain = set_plain; }
+  else if ((st = correct_and_measure(S, us, vs, ws, p64, k1, u, v, w, d_all, d_plain))) return st;
+  double be;
+  if ((st = backward_error(S, p64, be))) return st;
+  FSP_LAUNCH(S, k_store_p, dim3(grid_for(G.n)), K.st, G.n, (const double*)p64, p);
+  const bool accepted = (actual_rr <= tol_b2 || floor_stop) && d_plain <= 1e-5;
+  FSP_HIP(hipEventRecord(S->ev1, S->stream));
+  FSP_HIP(hipEventSynchronize(S->ev1));
+  float ms = 0.f;
+  FSP_HIP(hipEventElapsedTime(&ms, S->ev0, S->ev1));
+  fsp_report R{};
+  R.refinements = 0;
+  R.iterations = it;
+  R.degraded = (b2 > 0.0 && !accepted) ? 1 : 0;
+  R.residual = b2 > 0.0 ? (double)std::sqrt((float)(actual_rr / b2)) : 0.0;
+  R.divergence = d_all;
+  R.floor = floor_stop ? 1 : 0;
+  R.backward_error = be;
+  R.divergence_plain = d_plain;
+  R.ms_device = ms;
+  R.ms_transfer = 0.0;
+  if (rep) *rep = R;
+  if (R.degraded && S->opt.refuse_degraded) return FSP_CONVERGENCE;
+  return FSP_OK;
+}
+
+bool all_finite(const float* x, size_t n) {
+  for (size_t q = 0; q < n; ++q) if (!std::isfinite(x[q])) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t fsp_abi_version(void) { return FSP_ABI_VERSION; }
+
+const char* fsp_status_string(fsp_status s) {
+  switch (s) {
+    case FSP_OK: return "ok";
+    case FSP_DOMAIN: return "domain";
+    case FSP_NOT_FINITE: return "not finite";
+    case FSP_SHAPE: return "shape";
+    case FSP_CONVERGENCE: return "convergence";
+    case FSP_DEVICE: return "device";
+    case FSP_INVALID: return "invalid";
+  }
+  return "unknown";
+}
+
+void fsp_default_options(fsp_options* o) {
+  if (!o) return;
+  o->preconditioner = FSP_PRECOND_MULTIGRID;
+  o->smoother = FSP_SMOOTH_JACOBI;
+  o->max_levels = -1;
+  o->pre_sweeps = 2;
+  o->post_sweeps = 2;
+  o->coarse_sweeps = 8;
+  o->gauge = 1;
+  o->vcycle_storage = FSP_STORE_F32;
+  o->replace_every = 0;
+  o->refuse_degraded = 0;
+  o->device = 0;
+}
+
+fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** out) {
+  if (!d || !out) return FSP_INVALID;
+  *out = nullptr;
+  if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0 || !std::isfinite(d->dx) || !(d->dx > 0.f)) return FSP_DOMAIN;
+  const uint64_t cells = (uint64_t)d->nx * (uint64_t)d->ny * (uint64_t)d->nz;
+  if (cells >= (1ull << 31)) return FSP_DOMAIN;
+  fsp_options opt;
+  fsp_default_options(&opt);
+  if (o) opt = *o;
+  if (opt.preconditioner < 0 || opt.preconditioner > 1 || opt.smoother < 0 || opt.smoother > 1) return FSP_INVALID;
+  if (opt.pre_sweeps < 0 || opt.post_sweeps < 0 || opt.coarse_sweeps < 0 || opt.pre_sweeps > 64 ||
+      opt.post_sweeps > 64 || opt.coarse_sweeps > 1024)
+    return FSP_INVALID;
+  if (opt.vcycle_storage != FSP_STORE_F32) return FSP_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || opt.device < 0 || opt.device >= ndev) return FSP_DEVICE;
+  fsp_solver* S = new (std::nothrow) fsp_solver();
+  if (!S) return FSP_DEVICE;
+  S->dom = *d;
+  S->opt = opt;
+  fsp_status st = FSP_OK;
+  auto fail = [&](fsp_status s) { cleanup(S); delete S; return s; };
+  if (hipSetDevice(opt.device) != hipSuccess) return fail(FSP_DEVICE);
+  if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess) return fail(FSP_DEVICE);
+  if (hipEventCreate(&S->ev0) != hipSuccess || hipEventCreate(&S->ev1) != hipSuccess) return fail(FSP_DEVICE);
+  const size_t nx = d->nx, ny = d->ny, nz = d->nz;
+  S->n = nx * ny * nz;
+  S->nu = (nx + 1) * ny * nz;
+  S->nv = nx * (ny + 1) * nz;
+  S->nw = nx * ny * (nz + 1);
+  S->ncols = nx * ny;
+  if ((st = dalloc(S, &S->solid, S->n)) || (st = dalloc(S, &S->ou, S->nu)) || (st = dalloc(S, &S->ov, S->nv)) ||
+      (st = dalloc(S, &S->ow, S->nw)) || (st = dalloc(S, &S->mask, S->n)) || (st = dalloc(S, &S->eta, S->ncols)) ||
+      (st = dalloc(S, &S->eta_rt, S->ncols)))
+    return fail(st);
+  float** cellbufs[] = {&S->b, &S->r, &S->p, &S->q, &S->z, &S->zt, &S->d0, &S->d1};
+  for (float** bp : cellbufs) if ((st = dalloc(S, bp, S->n))) return fail(st);
+  if ((st = dalloc(S, &S->p64, S->n))) return fail(st);
+  if ((st = dalloc(S, &S->us, S->nu)) || (st = dalloc(S, &S->vs, S->nv)) || (st = dalloc(S, &S->ws, S->nw)) ||
+      (st = dalloc(S, &S->u, S->nu)) || (st = dalloc(S, &S->v, S->nv)) || (st = dalloc(S, &S->w, S->nw)))
+    return fail(st);
+  if ((st = dalloc(S, &S->ctl, 1)) || (st = dalloc(S, &S->partials, (size_t)NUM_SLOTS * MAX_BLOCKS * 2)) ||
+      (st = dalloc(S, &S->hashes, (S->n + 1023) / 1024)) || (st = dalloc(S, &S->hash_out, 1)))
+    return fail(st);
+  if (hipMemset(S->ctl, 0, sizeof(Ctl)) != hipSuccess) return fail(FSP_DEVICE);
+  if (hipHostMalloc((void**)&S->hctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess) return fail(FSP_DEVICE);
+  if (hipHostMalloc((void**)&S->hhash, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+    return fail(FSP_DEVICE);
+  if (opt.preconditioner == FSP_PRECOND_MULTIGRID) {
+    size_t x = nx, y = ny, z = nz;
+    float h = d->dx;
+    while (coarsens3(x, y, z) && (opt.max_levels < 0 || (int)S->levels.size() < opt.max_levels)) {
+      x /= 2; y /= 2; z /= 2;
+      h *= 2.f;
+      Level L{};
+      L.nx = (int)x; L.ny = (int)y; L.nz = (int)z;
+      L.n = x * y * z;
+      L.inv = 1.0f / (h * h);
+      if ((st = dalloc(S, &L.kind, L.n)) || (st = dalloc(S, &L.ou, (x + 1) * y * z)) ||
+          (st = dalloc(S, &L.ov, x * (y + 1) * z)) || (st = dalloc(S, &L.ow, x * y * (z + 1))) ||
+          (st = dalloc(S, &L.diag, L.n)) || (st = dalloc(S, &L.x, L.n)) || (st = dalloc(S, &L.r, L.n)) ||
+          (st = dalloc(S, &L.t, L.n)))
+        return fail(st);
+      S->levels.push_back(L);
+    }
+  }
+  *out = S;
+  return FSP_OK;
+}
+
+void fsp_destroy(fsp_solver* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->opt.device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  cleanup(s);
+  delete s;
+}
+
+int32_t fsp_levels(const fsp_solver* s) { return s ? (int32_t)s->levels.size() : -1; }
+
+fsp_status fsp_set_geometry(fsp_solver* S, const fsp_surface* surf, const float* eta, size_t n_eta,
+                            const float* eta_roundoff, const uint8_t* solid, size_t n_solid, const float* open_u,
+                            size_t n_open_u, const float* open_v, size_t n_open_v, const float* open_w,
+                            size_t n_open_w) {
+  if (!S || !surf) return FSP_INVALID;
+  if (!eta || n_eta != S->ncols) return FSP_SHAPE;
+  if ((solid && n_solid != S->n) || (open_u && n_open_u != S->nu) || (open_v && n_open_v != S->nv) ||
+      (open_w && n_open_w != S->nw))
+    return FSP_SHAPE;
+  if (!std::isfinite(surf->rho) || !(surf->rho > 0.f) || !std::isfinite(surf->g_eff) || !std::isfinite(surf->rest))
+    return FSP_NOT_FINITE;
+  if (!all_finite(eta, S->ncols) || (eta_roundoff && !all_finite(eta_roundoff, S->ncols))) return FSP_NOT_FINITE;
+  const float* opens[3] = {open_u, open_v, open_w};
+  const size_t lens[3] = {S->nu, S->nv, S->nw};
+  for (int a = 0; a < 3; ++a)
+    if (opens[a])
+      for (size_t q = 0; q < lens[a]; ++q)
+        if (!(opens[a][q] == 0.f || opens[a][q] == 1.f)) return FSP_DOMAIN;  // fractional apertures: not supported
+  const size_t nx = S->dom.nx, ny = S->dom.ny, nz = S->dom.nz;
+  const int cut = (solid || open_u || open_v || open_w || surf->closed_lid) ? 1 : 0;
+  std::vector<uint8_t> hs(S->n, 0), hu(S->nu, 1), hv(S->nv, 1), hw(S->nw, 1);
+  if (solid) for (size_t q = 0; q < S->n; ++q) hs[q] = solid[q] ? 1 : 0;
+  if (cut) {
+    // Derived apertures: a face touching a solid cell or the box walls is
+    // closed, the lid open unless closed_lid; explicit apertures win.
+    auto Sd = [&](size_t i, size_t j, size_t k) { return hs[(k * ny + j) * nx + i] != 0; };
+    for (size_t k = 0; k < nz; ++k)
+      for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i <= nx; ++i) {
+          const size_t f = (k * ny + j) * (nx + 1) + i;
+          uint8_t a = (i == 0 || i == nx || Sd(i - 1, j, k) || Sd(i, j, k)) ? 0 : 1;
+          hu[f] = open_u ? (uint8_t)(open_u[f] == 1.f) : a;
+        }
+    for (size_t k = 0; k < nz; ++k)
+      for (size_t j = 0; j <= ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+          const size_t f = (k * (ny + 1) + j) * nx + i;
+          uint8_t a = (j == 0 || j == ny || Sd(i, j - 1, k) || Sd(i, j, k)) ? 0 : 1;
+          hv[f] = open_v ? (uint8_t)(open_v[f] == 1.f) : a;
+        }
+    for (size_t k = 0; k <= nz; ++k)
+      for (size_t j = 0; j < ny; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+          const size_t f = (k * ny + j) * nx + i;
+          uint8_t a;
+          if (k == 0) a = 0;
+          else if (k == nz) a = (Sd(i, j, nz - 1) || surf->closed_lid) ? 0 : 1;
+          else a = (Sd(i, j, k - 1) || Sd(i, j, k)) ? 0 : 1;
+          uint8_t val = open_w ? (uint8_t)(open_w[f] == 1.f) : a;
+          if (k == nz && surf->closed_lid) val = 0;
+          hw[f] = val;
+        }
+  }
+  FSP_HIP(hipSetDevice(S->opt.device));
+  FSP_HIP(hipMemcpyAsync(S->solid, hs.data(), S->n, hipMemcpyHostToDevice, S->stream));
+  FSP_HIP(hipMemcpyAsync(S->ou, hu.data(), S->nu, hipMemcpyHostToDevice, S->stream));
+  FSP_HIP(hipMemcpyAsync(S->ov, hv.data(), S->nv, hipMemcpyHostToDevice, S->stream));
+  FSP_HIP(hipMemcpyAsync(S->ow, hw.data(), S->nw, hipMemcpyHostToDevice, S->stream));
+  FSP_HIP(hipMemcpyAsync(S->eta, eta, S->ncols * sizeof(float), hipMemcpyHostToDevice, S->stream));
+  if (eta_roundoff) {
+    FSP_HIP(hipMemcpyAsync(S->eta_rt, eta_roundoff, S->ncols * sizeof(float), hipMemcpyHostToDevice, S->stream));
+  } else {
+    FSP_HIP(hipMemsetAsync(S->eta_rt, 0, S->ncols * sizeof(float), S->stream));
+  }
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  S->surf = *surf;
+  S->cut = cut;
+  S->geom = true;
+  return FSP_OK;
+}
+
+fsp_status fsp_project_device(fsp_solver* S, const float* d_us, const float* d_vs, const float* d_ws, float* d_p,
+                              int32_t p_warm, float* d_u, float* d_v, float* d_w, float scale, float k1,
+                              double rel_tol, uint32_t max_iters, fsp_report* report) {
+  if (!S || !d_us || !d_vs || !d_ws || !d_p || !d_u || !d_v || !d_w) return FSP_INVALID;
+  if (!std::isfinite(scale) || !std::isfinite(k1) || !(rel_tol > 0.0)) return FSP_NOT_FINITE;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  return run(S, d_us, d_vs, d_ws, d_p, p_warm, d_u, d_v, d_w, scale, k1, rel_tol, max_iters, report);
+}
+
+fsp_status fsp_project(fsp_solver* S, const float* us, size_t n_us, const float* vs, size_t n_vs, const float* ws,
+                       size_t n_ws, float* p, size_t n_p, int32_t p_warm, float* u, size_t n_u, float* v, size_t n_v,
+                       float* w, size_t n_w, float scale, float k1, double rel_tol, uint32_t max_iters,
+                       fsp_report* report) {
+  if (!S || !us || !vs || !ws || !p || !u || !v || !w) return FSP_INVALID;
+  if (n_us != S->nu || n_vs != S->nv || n_ws != S->nw || n_p != S->n || n_u != S->nu || n_v != S->nv ||
+      n_w != S->nw)
+    return FSP_SHAPE;
+  if (!std::isfinite(scale) || !std::isfinite(k1) || !(rel_tol > 0.0)) return FSP_NOT_FINITE;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  auto t0 = std::chrono::steady_clock::now();
+  FSP_HIP(hipMemcpyAsync(S->us, us, S->nu * 4, hipMemcpyHostToDevice, S->stream));
+  FSP_HIP(hipMemcpyAsync(S->vs, vs, S->nv * 4, hipMemcpyHostToDevice, S->stream));
+  FSP_HIP(hipMemcpyAsync(S->ws, ws, S->nw * 4, hipMemcpyHostToDevice, S->stream));
+  if (p_warm) FSP_HIP(hipMemcpyAsync(S->p, p, S->n * 4, hipMemcpyHostToDevice, S->stream));
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  auto t1 = std::chrono::steady_clock::now();
+  fsp_report R{};
+  fsp_status st = run(S, S->us, S->vs, S->ws, S->p, p_warm, S->u, S->v, S->w, scale, k1, rel_tol, max_iters, &R);
+  if (st != FSP_OK) return st;
+  auto t2 = std::chrono::steady_clock::now();
+  FSP_HIP(hipMemcpyAsync(p, S->p, S->n * 4, hipMemcpyDeviceToHost, S->stream));
+  FSP_HIP(hipMemcpyAsync(u, S->u, S->nu * 4, hipMemcpyDeviceToHost, S->stream));
+  FSP_HIP(hipMemcpyAsync(v, S->v, S->nv * 4, hipMemcpyDeviceToHost, S->stream));
+  FSP_HIP(hipMemcpyAsync(w, S->w, S->nw * 4, hipMemcpyDeviceToHost, S->stream));
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  auto t3 = std::chrono::steady_clock::now();
+  R.ms_transfer = std::chrono::duration<double, std::milli>((t1 - t0) + (t3 - t2)).count();
+  if (report) *report = R;
+  return FSP_OK;
+}
+
+fsp_status fsp_apply_operator_device(fsp_solver* S, const float* d_p, float* d_out) {
+  if (!S || !d_p || !d_out) return FSP_INVALID;
+  if (!S->geom) return FSP_DOMAIN;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  build_geometry(S, false);
+  Launch K(S);
+  FSP_LAUNCH(S, k_apply, dim3(grid_for(K.G.n)), K.st, K.G, d_p, d_out);
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  return FSP_OK;
+}
+
+fsp_status fsp_assemble_device(fsp_solver* S, const float* d_div, float scale, float* d_rhs, float* d_prec) {
+  if (!S || !d_div || !d_rhs || !d_prec) return FSP_INVALID;
+  if (!S->geom) return FSP_DOMAIN;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  build_geometry(S, false);
+  Launch K(S);
+  FSP_LAUNCH(S, k_assemble, dim3(grid_for(K.G.n)), K.st, K.G, (const float*)nullptr,
+                     (const float*)nullptr, (const float*)nullptr, d_div, scale, d_rhs, d_prec, S->ctl, S->partials);
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  return FSP_OK;
+}
+
+fsp_status fsp_precondition_device(fsp_solver* S, const float* d_r, float* d_z) {
+  if (!S || !d_r || !d_z) return FSP_INVALID;
+  if (!S->geom) return FSP_DOMAIN;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  build_geometry(S, true);
+  Launch K(S);
+  FSP_HIP(hipMemcpyAsync(S->r, d_r, S->n * 4, hipMemcpyDeviceToDevice, S->stream));
+  K.set_pass(1, 0.0, 0.0, 0.0, 0, 1);
+  K.precondition(OP_GENERIC);
+  FSP_HIP(hipMemcpyAsync(d_z, S->z, S->n * 4, hipMemcpyDeviceToDevice, S->stream));
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  return FSP_OK;
+}
+
+fsp_status fsp_divergence_device(fsp_solver* S, const float* d_u, const float* d_v, const float* d_w, float* d_out) {
+  if (!S || !d_u || !d_v || !d_w || !d_out) return FSP_INVALID;
+  if (!S->geom) return FSP_DOMAIN;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  Launch K(S);
+  FSP_LAUNCH(S, k_divergence, dim3(grid_for(K.G.n)), K.st, K.G, d_u, d_v, d_w, d_out);
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  return FSP_OK;
+}
+
+fsp_status fsp_profile_begin(fsp_solver* S, const char* kernel, uint32_t max_launches) {
+  if (!S || !kernel || max_launches == 0) return FSP_INVALID;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  for (hipEvent_t e : S->prof_events) (void)hipEventDestroy(e);
+  S->prof_events.assign(2 * (size_t)max_launches, nullptr);
+  for (auto& e : S->prof_events) FSP_HIP(hipEventCreate(&e));
+  S->prof_name = kernel;
+  S->prof_used = 0;
+  S->prof_on = true;
+  return FSP_OK;
+}
+
+fsp_status fsp_profile_end(fsp_solver* S, uint32_t* launches, double* total_ms, float* durations_ms,
+                           uint32_t max_durations) {
+  if (!S) return FSP_INVALID;
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  double total = 0.0;
+  for (size_t q = 0; q + 1 < S->prof_used; q += 2) {
+    float ms = 0.f;
+    FSP_HIP(hipEventElapsedTime(&ms, S->prof_events[q], S->prof_events[q + 1]));
+    total += ms;
+    if (durations_ms && q / 2 < max_durations) durations_ms[q / 2] = ms;
+  }
+  if (launches) *launches = (uint32_t)(S->prof_used / 2);
+  if (total_ms) *total_ms = total;
+  S->prof_on = false;
+  S->prof_used = 0;
+  return FSP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1004 @@
+// Device code of the projection: fine operator, assembly, CG steps, V-cycle,
+// correction and metrics. Included once, by fsp.hip.
+//
+// Fine-level arithmetic follows the reference's face order x-, x+, y-, y+,
+// z-, z+ with FP contraction off, so the operator action, the right-hand
+// side, the Jacobi diagonal, the correction and the divergence are the
+// reference's bit for bit (code/water-core/src/delta3d_mobile.rs:126-343,
+// delta3d.rs:747-823). Reductions accumulate in f64.
+#pragma once
+#include "fsp_internal.h"
+
+namespace fsp {
+
+// ---------------------------------------------------------------- indexing
+__device__ __forceinline__ size_t cidx(const Fine& G, int i, int j, int k) {
+  return ((size_t)k * G.ny + j) * G.nx + i;
+}
+__device__ __forceinline__ size_t fuidx(const Fine& G, int i, int j, int k) {
+  return ((size_t)k * G.ny + j) * (G.nx + 1) + i;
+}
+__device__ __forceinline__ size_t fvidx(const Fine& G, int i, int j, int k) {
+  return ((size_t)k * (G.ny + 1) + j) * G.nx + i;
+}
+__device__ __forceinline__ size_t fwidx(const Fine& G, int i, int j, int k) {
+  return ((size_t)k * G.ny + j) * G.nx + i;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
+// walks a contiguous range of chunks and stencil neighbours share its L2.
+__device__ __forceinline__ unsigned logical_block() {
+  const unsigned G = gridDim.x, b = blockIdx.x;
+  if ((G & 7u) == 0u) return (b & 7u) * (G >> 3) + (b >> 3);
+  return b;
+}
+#define FSP_FOR(idx, count)                                                         \
+  for (size_t idx = (size_t)logical_block() * BLOCK + threadIdx.x; idx < (count); \
+       idx += (size_t)gridDim.x * BLOCK)
+
+__device__ __forceinline__ void decode(const Fine& G, size_t c, int& i, int& j, int& k) {
+  const uint32_t cc = (uint32_t)c;
+  const uint32_t nx = (uint32_t)G.nx, ny = (uint32_t)G.ny;
+  const uint32_t t = cc / nx;
+  i = (int)(cc - t * nx);
+  k = (int)(t / ny);
+  j = (int)(t - (uint32_t)k * ny);
+}
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ double load_agent(const double* p) {
+  unsigned long long bits =
+      __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double((long long)bits);
+}
+
+// Grid-wide reduction of two values. Every block writes its partial; the last
+// block to arrive (counted with a device-scope atomic) folds the partials in
+// block order, so the result does not depend on scheduling. Returns true in
+// thread 0 of the last block, with the totals in a and b.
+template <bool MAX>
+__device__ bool grid_reduce2(double& a, double& b, int slot, Ctl* ctl, double* partials) {
+  __shared__ double sa[BLOCK / 64], sb[BLOCK / 64];
+  __shared__ int last;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  a = MAX ? wave_max(a) : wave_sum(a);
+  b = MAX ? wave_max(b) : wave_sum(b);
+  if (lane == 0) { sa[wid] = a; sb[wid] = b; }
+  __syncthreads();
+  double* part = partials + (size_t)slot * MAX_BLOCKS * 2;
+  if (threadIdx.x == 0) {
+    double ta = sa[0], tb = sb[0];
+    for (int w = 1; w < BLOCK / 64; ++w) {
+      ta = MAX ? fmax(ta, sa[w]) : ta + sa[w];
+      tb = MAX ? fmax(tb, sb[w]) : tb + sb[w];
+    }
+    part[2 * blockIdx.x] = ta;
+    part[2 * blockIdx.x + 1] = tb;
+    __threadfence();
+    const unsigned prev = atomicAdd(&ctl->counters[slot], 1u);
+    last = (prev == gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  double pa = 0.0, pb = 0.0;
+  for (unsigned q = threadIdx.x; q < gridDim.x; q += BLOCK) {
+    const double xa = load_agent(&part[2 * q]), xb = load_agent(&part[2 * q + 1]);
+    pa = MAX ? fmax(pa, xa) : pa + xa;
+    pb = MAX ? fmax(pb, xb) : pb + xb;
+  }
+  pa = MAX ? wave_max(pa) : wave_sum(pa);
+  pb = MAX ? wave_max(pb) : wave_sum(pb);
+  __syncthreads();
+  if (lane == 0) { sa[wid] = pa; sb[wid] = pb; }
+  __syncthreads();
+  if (threadIdx.x != 0) return false;
+  a = sa[0];
+  b = sb[0];
+  for (int w = 1; w < BLOCK / 64; ++w) {
+    a = MAX ? fmax(a, sa[w]) : a + sa[w];
+    b = MAX ? fmax(b, sb[w]) : b + sb[w];
+  }
+  ctl->counters[slot] = 0u;
+  ctl->red[slot][0] = a;
+  ctl->red[slot][1] = b;
+  return true;
+}
+
+// Scalar updates of the conjugate gradient, done on the device by the last
+// block of the kernel that produced the reduction (delta3d_mobile.rs:513-545).
+__device__ void finish(int op, Ctl* ctl, double a, double b) {
+  switch (op) {
+    case OP_B2:
+      ctl->b2 = a;
+      if (!isfinite(a)) { ctl->nonfinite = 1; ctl->active = 0; }
+      break;
+    case OP_RR:
+      ctl->actual = a;
+      if (!isfinite(a)) { ctl->nonfinite = 1; ctl->active = 0; }
+      break;
+    case OP_DQ:
+      ctl->dq = a;
+      ctl->first_dir = 0;
+      if (!(a > 0.0)) {
+        ctl->active = 0;
+        ctl->stop_dq = 1;
+      } else {
+        ctl->alpha = ctl->rz / a;
+        ctl->dsel ^= 1;
+      }
+      break;
+    case OP_RN:
+      ctl->rn = a;
+      if (!isfinite(a)) { ctl->nonfinite = 1; ctl->active = 0; }
+      break;
+    case OP_RZ:
+      if (!isfinite(a)) { ctl->nonfinite = 1; ctl->active = 0; break; }
+      ctl->beta = a / ctl->rz;
+      ctl->rz = a;
+      ctl->rr = ctl->rn;
+      ctl->it += 1;
+      ctl->active = ((ctl->rr > ctl->tol_b2 || ctl->rr > ctl->physical_target) && ctl->it < ctl->max_iters &&
+                     ctl->it < ctl->pass_limit) ? 1 : 0;
+      break;
+    case OP_RZ_PRIME:
+      if (!isfinite(a)) { ctl->nonfinite = 1; ctl->active = 0; break; }
+      ctl->rz = a;
+      ctl->first_dir = 1;
+      break;
+    case OP_MEAN_B:
+      ctl->mean_b = (ctl->pure_neumann && b > 0.0) ? a / b : 0.0;
+      break;
+    case OP_MEAN_P:
+      ctl->mean_p = (ctl->pure_neumann && b > 0.0) ? a / b : 0.0;
+      break;
+    case OP_COUNT:
+      ctl->wet_cells = (uint32_t)a;
+      ctl->ghost_faces = (uint32_t)b;
+      ctl->pure_neumann = (ctl->gauge && b == 0.0 && a > 0.0) ? 1 : 0;
+      break;
+    default:
+      break;
+  }
+}
+
+template <bool MAX>
+__device__ __forceinline__ void reduce_finish(double a, double b, int op, Ctl* ctl, double* partials) {
+  if (grid_reduce2<MAX>(a, b, op, ctl, partials)) finish(op, ctl, a, b);
+}
+
+// ---------------------------------------------------------------- fine geometry
+__device__ __forceinline__ bool solid_at(const Fine& G, int i, int j, int k) { return G.solid[cidx(G, i, j, k)] != 0; }
+__device__ __forceinline__ float zc_of(const Fine& G, int k) {
+#pragma clang fp contract(off)
+  return ((float)k + 0.5f) * G.dx;
+}
+// delta3d_mobile.rs:97-103
+__device__ __forceinline__ bool wet_at(const Fine& G, int i, int j, int k) {
+  if (solid_at(G, i, j, k)) return false;
+  return zc_of(G, k) < G.eta[(size_t)j * G.nx + i];
+}
+// delta3d_mobile.rs:126-134
+__device__ __forceinline__ void ghost_up(const Fine& G, int i, int j, int k, float& it, float& val) {
+#pragma clang fp contract(off)
+  const size_t col = (size_t)j * G.nx + i;
+  const float h = G.eta[col];
+  const float theta = fmaxf((h - zc_of(G, k)) / G.dx, 1e-3f);
+  it = 1.0f / theta;
+  if (G.homogeneous) { val = 0.f; return; }
+  val = G.rho * G.g * ((h - G.rest) - G.eta_rt[col]);
+}
+// delta3d_mobile.rs:136-147
+__device__ __forceinline__ void ghost_side(const Fine& G, int i, int j, int k, int x, int y, float& it, float& val) {
+#pragma clang fp contract(off)
+  const float zc = zc_of(G, k);
+  const float h = G.eta[(size_t)j * G.nx + i];
+  const float theta = fmaxf((h - zc) / (h - G.eta[(size_t)y * G.nx + x]), 1e-3f);
+  it = 1.0f / theta;
+  if (G.homogeneous) { val = 0.f; return; }
+  val = G.rho * G.g * (zc - G.rest);
+}
+
+// The reference's mobile_row (delta3d_mobile.rs:152-188) for binary
+// apertures: type 1 = wet neighbour, 2 = ghost face, 0 = nothing.
+struct RowG {
+  uint8_t type[6];
+  float it[6];
+  float val[6];
+};
+
+__device__ void full_row(const Fine& G, int i, int j, int k, RowG& R) {
+#pragma unroll
+  for (int f = 0; f < 6; ++f) { R.type[f] = 0; R.it[f] = 0.f; R.val[f] = 0.f; }
+  const bool has[4] = {i > 0, i + 1 < G.nx, j > 0, j + 1 < G.ny};
+  const int xs[4] = {i - 1, i + 1, i, i};
+  const int ys[4] = {j, j, j - 1, j + 1};
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    if (!has[f]) continue;
+    uint8_t a;
+    if (f == 0) a = G.ou[fuidx(G, i, j, k)];
+    else if (f == 1) a = G.ou[fuidx(G, i + 1, j, k)];
+    else if (f == 2) a = G.ov[fvidx(G, i, j, k)];
+    else a = G.ov[fvidx(G, i, j + 1, k)];
+    const int x = xs[f], y = ys[f];
+    if (a == 0 || solid_at(G, x, y, k)) continue;
+    if (wet_at(G, x, y, k)) {
+      R.type[f] = 1;
+    } else {
+      R.type[f] = 2;
+      ghost_side(G, i, j, k, x, y, R.it[f], R.val[f]);
+    }
+  }
+  if (k > 0 && G.ow[fwidx(G, i, j, k)] && !solid_at(G, i, j, k - 1)) R.type[4] = 1;
+  if (G.ow[fwidx(G, i, j, k + 1)]) {
+    if (k + 1 < G.nz && wet_at(G, i, j, k + 1)) {
+      R.type[5] = 1;
+    } else {
+      R.type[5] = 2;
+      ghost_up(G, i, j, k, R.it[5], R.val[5]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- fine operator
+template <class T>
+__device__ __forceinline__ float ldv(const T* p, size_t i) { return (float)p[i]; }
+
+// (A x)_c (apply_mobile3, delta3d_mobile.rs:190-213) and the Jacobi
+// preconditioner prec_c = 1/(diag * inv) (rhs_mobile3, :215-241).
+template <class T>
+__device__ __forceinline__ float apply_prec(const Fine& G, int i, int j, int k, size_t c, uint8_t m, const T* x,
+                                            float& prec) {
+#pragma clang fp contract(off)
+  prec = 0.f;
+  if (!(m & M_WET)) return 0.f;
+  const size_t plane = (size_t)G.nx * G.ny;
+  const float pc = ldv(x, c);
+  float acc = 0.f, diag = 0.f;
+  if (!(m & M_GHOST)) {
+    if (m & M_XM) { acc += pc - ldv(x, c - 1); diag += 1.0f; }
+    if (m & M_XP) { acc += pc - ldv(x, c + 1); diag += 1.0f; }
+    if (m & M_YM) { acc += pc - ldv(x, c - G.nx); diag += 1.0f; }
+    if (m & M_YP) { acc += pc - ldv(x, c + G.nx); diag += 1.0f; }
+    if (m & M_ZM) { acc += pc - ldv(x, c - plane); diag += 1.0f; }
+    if (m & M_ZP) { acc += pc - ldv(x, c + plane); diag += 1.0f; }
+  } else {
+    RowG R;
+    full_row(G, i, j, k, R);
+    const size_t nb[6] = {c - 1, c + 1, c - G.nx, c + G.nx, c - plane, c + plane};
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      if (R.type[f] == 1) { acc += pc - ldv(x, nb[f]); diag += 1.0f; }
+      else if (R.type[f] == 2) { acc += pc * R.it[f]; diag += R.it[f]; }
+    }
+  }
+  prec = diag > 0.f ? 1.0f / (diag * G.inv) : 0.f;
+  return acc * G.inv;
+}
+
+template <class T>
+__device__ __forceinline__ float apply_row(const Fine& G, int i, int j, int k, size_t c, uint8_t m, const T* x) {
+  float prec;
+  return apply_prec(G, i, j, k, c, m, x, prec);
+}
+
+// (A x)_c in accumulator precision A from storage T: the f64 form computes
+// the true residual of the f64 pressure (b - A p), the f32 form is apply_row.
+template <class A, class T>
+__device__ __forceinline__ A apply_row_t(const Fine& G, int i, int j, int k, size_t c, uint8_t m, const T* x) {
+#pragma clang fp contract(off)
+  if (!(m & M_WET)) return (A)0;
+  const size_t plane = (size_t)G.nx * G.ny;
+  const A pc = (A)x[c];
+  A acc = (A)0;
+  if (!(m & M_GHOST)) {
+    if (m & M_XM) acc += pc - (A)x[c - 1];
+    if (m & M_XP) acc += pc - (A)x[c + 1];
+    if (m & M_YM) acc += pc - (A)x[c - G.nx];
+    if (m & M_YP) acc += pc - (A)x[c + G.nx];
+    if (m & M_ZM) acc += pc - (A)x[c - plane];
+    if (m & M_ZP) acc += pc - (A)x[c + plane];
+  } else {
+    RowG R;
+    full_row(G, i, j, k, R);
+    const size_t nb[6] = {c - 1, c + 1, c - G.nx, c + G.nx, c - plane, c + plane};
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      if (R.type[f] == 1) acc += pc - (A)x[nb[f]];
+      else if (R.type[f] == 2) acc += pc * (A)R.it[f];
+    }
+  }
+  return acc * (A)G.inv;
+}
+
+// The Jacobi preconditioner alone (rhs_mobile3's prec, delta3d_mobile.rs:227-237).
+__device__ __forceinline__ float row_prec(const Fine& G, int i, int j, int k, uint8_t m) {
+#pragma clang fp contract(off)
+  if (!(m & M_WET)) return 0.f;
+  float diag = 0.f;
+  if (!(m & M_GHOST)) {
+    for (int f = 0; f < 6; ++f) if (m & (1u << f)) diag += 1.0f;
+  } else {
+    RowG R;
+    full_row(G, i, j, k, R);
+    for (int f = 0; f < 6; ++f) {
+      if (R.type[f] == 1) diag += 1.0f;
+      else if (R.type[f] == 2) diag += R.it[f];
+    }
+  }
+  return diag > 0.f ? 1.0f / (diag * G.inv) : 0.f;
+}
+
+// prec == 0 exactly on dry cells and on wet cells without any face.
+__device__ __forceinline__ bool prec_positive(uint8_t m) { return (m & M_WET) && (m & (0x3F | M_GHOST)); }
+
+// delta3d.rs:747-763 (plain) and :806-823 (cut with fixed solids).
+__device__ __forceinline__ float divergence_at(const Fine& G, const float* u, const float* v, const float* w, int i,
+                                               int j, int k, size_t c) {
+#pragma clang fp contract(off)
+  if (G.cut && G.solid[c]) return 0.f;
+  const size_t a0 = fuidx(G, i, j, k), a1 = fuidx(G, i + 1, j, k);
+  const size_t b0 = fvidx(G, i, j, k), b1 = fvidx(G, i, j + 1, k);
+  const size_t c0 = fwidx(G, i, j, k), c1 = fwidx(G, i, j, k + 1);
+  float fl = u[a0], fr = u[a1], ff = v[b0], fk = v[b1], fb = w[c0], ft = w[c1];
+  if (G.cut) {
+    fl = (float)G.ou[a0] * fl; fr = (float)G.ou[a1] * fr;
+    ff = (float)G.ov[b0] * ff; fk = (float)G.ov[b1] * fk;
+    fb = (float)G.ow[c0] * fb; ft = (float)G.ow[c1] * ft;
+  }
+  return ((fr - fl + ft - fb) + (fk - ff)) / G.dx;
+}
+
+// ---------------------------------------------------------------- geometry kernels
+__global__ void __launch_bounds__(BLOCK) k_mask(Fine G, uint8_t* mask, Ctl* ctl, double* partials) {
+  double wet = 0.0, ghosts = 0.0;
+  FSP_FOR(c, G.n) {
+    int i, j, k;
+    decode(G, c, i, j, k);
+    uint8_t m = 0;
+    if (wet_at(G, i, j, k)) {
+      m = M_WET;
+      RowG R;
+      full_row(G, i, j, k, R);
+      for (int f = 0; f < 6; ++f) {
+        if (R.type[f] == 1) m |= (uint8_t)(1u << f);
+        else if (R.type[f] == 2) { m |= M_GHOST; ghosts += 1.0; }
+      }
+      wet += 1.0;
+    }
+    mask[c] = m;
+  }
+  reduce_finish<false>(wet, ghosts, OP_COUNT, ctl, partials);
+}
+
+// b = scale*div + ghost terms; prec (optional). div from faces or given.
+__global__ void __launch_bounds__(BLOCK) k_assemble(Fine G, const float* us, const float* vs, const float* ws,
+                                                    const float* div_in, float scale, float* b, float* prec_out,
+                                                    Ctl* ctl, double* partials) {
+#pragma clang fp contract(off)
+  double s2 = 0.0, unused = 0.0;
+  FSP_FOR(c, G.n) {
+    int i, j, k;
+    decode(G, c, i, j, k);
+    const uint8_t m = G.mask[c];
+    float bb = 0.f, pr = 0.f;
+    if (m & M_WET) {
+      const float div = div_in ? div_in[c] : divergence_at(G, us, vs, ws, i, j, k, c);
+      bb = scale * div;
+      float diag = 0.f;
+      if (!(m & M_GHOST)) {
+        for (int f = 0; f < 6; ++f) if (m & (1u << f)) diag += 1.0f;
+      } else {
+        RowG R;
+        full_row(G, i, j, k, R);
+        for (int f = 0; f < 6; ++f) {
+          if (R.type[f] == 1) diag += 1.0f;
+          else if (R.type[f] == 2) {
+            diag += R.it[f];
+            bb += R.val[f] * R.it[f] * G.inv;
+          }
+        }
+      }
+      pr = diag > 0.f ? 1.0f / (diag * G.inv) : 0.f;
+    }
+    b[c] = bb;
+    if (prec_out) prec_out[c] = pr;
+    s2 += (double)bb * (double)bb;
+  }
+  reduce_finish<false>(s2, unused, OP_B2, ctl, partials);
+}
+
+// Sum of x and count over rows with prec > 0 (gauge of a pure-Neumann problem).
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_sum_active(Fine G, const T* x, int op, Ctl* ctl, double* partials) {
+  double s = 0.0, n = 0.0;
+  FSP_FOR(c, G.n) {
+    if (prec_positive(G.mask[c])) { s += (double)x[c]; n += 1.0; }
+  }
+  reduce_finish<false>(s, n, op, ctl, partials);
+}
+
+// x -= mean on active rows when the problem is pure Neumann; for b also
+// recompute |b|^2.
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_sub_mean(Fine G, T* x, int which, int op, Ctl* ctl, double* partials) {
+  const int pure = ctl->pure_neumann;
+  const T mean = (T)(which == 0 ? ctl->mean_b : ctl->mean_p);
+  double s2 = 0.0, unused = 0.0;
+  FSP_FOR(c, G.n) {
+    T val = x[c];
+    if (pure && prec_positive(G.mask[c])) { val = val - mean; x[c] = val; }
+    s2 += (double)val * (double)val;
+  }
+  if (op >= 0) reduce_finish<false>(s2, unused, op, ctl, partials);
+}
+
+// Warm start (delta3d_mobile.rs:486-496): p from the caller, 0 on dry cells;
+// cold start: p = 0.
+__global__ void __launch_bounds__(BLOCK) k_load_p(Fine G, const float* pin, double* p, int warm) {
+  FSP_FOR(c, G.n) p[c] = (warm && (G.mask[c] & M_WET)) ? (double)pin[c] : 0.0;
+}
+__global__ void __launch_bounds__(BLOCK) k_store_p(size_t n, const double* p, float* pout) {
+  FSP_FOR(c, n) pout[c] = (float)p[c];
+}
+
+__global__ void __launch_bounds__(BLOCK) k_fill(float* x, size_t n, float value) {
+  FSP_FOR(c, n) x[c] = value;
+}
+
+// r = b - A p, |r|^2 (delta3d_mobile.rs:497-503, :547-553), with the
+// pressure held and the residual formed in f64; r is stored in f32.
+__global__ void __launch_bounds__(BLOCK) k_residual(Fine G, const double* p, const float* b, float* r, int op, Ctl* ctl,
+                                                    double* partials) {
+#pragma clang fp contract(off)
+  double s = 0.0, unused = 0.0;
+  FSP_FOR(c, G.n) {
+    int i, j, k;
+    decode(G, c, i, j, k);
+    const double rc = (double)b[c] - apply_row_t<double>(G, i, j, k, c, G.mask[c], p);
+    r[c] = (float)rc;
+    s += rc * rc;
+  }
+  if (op >= 0) reduce_finish<false>(s, unused, op, ctl, partials);
+}
+
+// out = A p (the trial hook, delta3d_mobile.rs:936-949)
+__global__ void __launch_bounds__(BLOCK) k_apply(Fine G, const float* p, float* out) {
+  FSP_FOR(c, G.n) {
+    int i, j, k;
+    decode(G, c, i, j, k);
+    out[c] = apply_row(G, i, j, k, c, G.mask[c], p);
+  }
+}
+
+// ---------------------------------------------------------------- CG steps
+// d = z (first) or z + beta d; q = A d; <d, q>.
+__global__ void __launch_bounds__(BLOCK) k_spmv_dir(Fine G, const float* z, float* d0, float* d1, float* q, Ctl* ctl,
+                                                    double* partials) {
+  if (!ctl->active) return;
+  const int sel = ctl->dsel, first = ctl->first_dir;
+  const float beta = (float)ctl->beta;
+  const float* dold = sel ? d1 : d0;
+  float* dnew = sel ? d0 : d1;
+  const size_t plane = (size_t)G.nx * G.ny;
+  double s = 0.0, unused = 0.0;
+  FSP_FOR(c, G.n) {
+    int i, j, k;
+    decode(G, c, i, j, k);
+    const uint8_t m = G.mask[c];
+    auto dv = [&](size_t n) { return first ? z[n] : z[n] + beta * dold[n]; };
+    const float dc = dv(c);
+    dnew[c] = dc;
+    float qc = 0.f;
+    if (m & M_WET) {
+#pragma clang fp contract(off)
+      float acc = 0.f;
+      if (!(m & M_GHOST)) {
+        if (m & M_XM) acc += dc - dv(c - 1);
+        if (m & M_XP) acc += dc - dv(c + 1);
+        if (m & M_YM) acc += dc - dv(c - G.nx);
+        if (m & M_YP) acc += dc - dv(c + G.nx);
+        if (m & M_ZM) acc += dc - dv(c - plane);
+        if (m & M_ZP) acc += dc - dv(c + plane);
+      } else {
+        RowG R;
+        full_row(G, i, j, k, R);
+        const size_t nb[6] = {c - 1, c + 1, c - G.nx, c + G.nx, c - plane, c + plane};
+        for (int f = 0; f < 6; ++f) {
+          if (R.type[f] == 1) acc += dc - dv(nb[f]);
+          else if (R.type[f] == 2) acc += dc * R.it[f];
+        }
+      }
+      qc = acc * G.inv;
+    }
+    q[c] = qc;
+    s += (double)dc * (double)qc;
+  }
+  reduce_finish<false>(s, unused, OP_DQ, ctl, partials);
+}
+
+// p += alpha d; r -= alpha q; |r|^2 (delta3d_mobile.rs:527-532)
+__global__ void __launch_bounds__(BLOCK) k_update(size_t n, double* p, float* r, const float* d0, const float* d1,
+                                                  const float* q, Ctl* ctl, double* partials) {
+  if (!ctl->active) return;
+  const double alpha64 = ctl->alpha;
+  const float alpha = (float)alpha64;
+  const float* d = ctl->dsel ? d1 : d0;
+  double s = 0.0, unused = 0.0;
+  FSP_FOR(c, n) {
+    p[c] += alpha64 * (double)d[c];
+    const float rc = r[c] - alpha * q[c];
+    r[c] = rc;
+    s += (double)rc * (double)rc;
+  }
+  reduce_finish<false>(s, unused, OP_RN, ctl, partials);
+}
+
+// <a, b> into a slot, gated on the loop flag when `gated`.
+__global__ void __launch_bounds__(BLOCK) k_dot(size_t n, const float* a, const float* b, int op, int gated, Ctl* ctl,
+                                               double* partials) {
+  if (gated && !ctl->active) return;
+  double s = 0.0, unused = 0.0;
+  FSP_FOR(c, n) s += (double)a[c] * (double)b[c];
+  reduce_finish<false>(s, unused, op, ctl, partials);
+}
+
+// Jacobi preconditioner z = prec * r with <r, z> (delta3d_mobile.rs:435-461).
+__global__ void __launch_bounds__(BLOCK) k_jacobi_prec(Fine G, const float* r, float* z, int op, Ctl* ctl,
+                                                       double* partials) {
+  if (!ctl->active) return;
+  double s = 0.0, unused = 0.0;
+  FSP_FOR(c, G.n) {
+    int i, j, k;
+    decode(G, c, i, j, k);
+    const float prec = row_prec(G, i, j, k, G.mask[c]);
+    const float zc = prec * r[c];
+    z[c] = zc;
+    s += (double)r[c] * (double)zc;
+  }
+  reduce_finish<false>(s, unused, op, ctl, partials);
+}
+
+// ---------------------------------------------------------------- fine smoothing
+// Damped Jacobi (delta3d_multigrid.rs:398-405): zout = zin + w prec (r - A zin);
+// zin == nullptr means zin = 0.
+__global__ void __launch_bounds__(BLOCK) k_fjacobi(Fine G, const float* zin, float* zout, const float* r, Ctl* ctl) {
+#pragma clang fp contract(off)
+  if (!ctl->active) return;
+  const float omega = 6.0f / 7.0f;
+  FSP_FOR(c, G.n) {
+    int i, j, k;
+    decode(G, c, i, j, k);
+    const uint8_t m = G.mask[c];
+    float prec, az = 0.f, zc = 0.f;
+    if (zin) {
+      az = apply_prec(G, i, j, k, c, m, zin, prec);
+      zc = zin[c];
+    } else {
+      prec = row_prec(G, i, j, k, m);
+    }
+    if (prec > 0.f) zc += omega * prec * (r[c] - az);
+    zout[c] = zc;
+  }
+}
+
+// One colour of a red-black Gauss-Seidel sweep, in place. `first`: z starts
+// at zero (colour 0 writes prec*r, the other colour 0).
+__global__ void __launch_bounds__(BLOCK) k_fgs(Fine G, float* z, const float* r, int colour, int first, Ctl* ctl) {
+#pragma clang fp contract(off)
+  if (!ctl->active) return;
+  FSP_FOR(c, G.n) {
+    int i, j, k;
+    decode(G, c, i, j, k);
+    const int col = (i + j + k) & 1;
+    if (first) {
+      if (col != colour) { z[c] = 0.f; continue; }
+      const float prec = row_prec(G, i, j, k, G.mask[c]);
+      z[c] = prec > 0.f ? prec * r[c] : 0.f;
+      continue;
+    }
+    if (col != colour) continue;
+    float prec;
+    const float az = apply_prec(G, i, j, k, c, G.mask[c], z, prec);
+    if (prec > 0.f) z[c] = z[c] + prec * (r[c] - az);
+  }
+}
+
+// ---------------------------------------------------------------- coarse levels
+__device__ __forceinline__ size_t lidx(const Level& L, int i, int j, int k) { return ((size_t)k * L.ny + j) * L.nx + i; }
+__device__ __forceinline__ void ldecode(const Level& L, size_t c, int& i, int& j, int& k) {
+  const uint32_t cc = (uint32_t)c, nx = (uint32_t)L.nx, ny = (uint32_t)L.ny;
+  const uint32_t t = cc / nx;
+  i = (int)(cc - t * nx);
+  k = (int)(t / ny);
+  j = (int)(t - (uint32_t)k * ny);
+}
+
+// row_sums (delta3d_multigrid.rs:216-255); x == nullptr gives the diagonal.
+__device__ __forceinline__ float coarse_row(const Level& L, int i, int j, int k, size_t c, const float* x) {
+#pragma clang fp contract(off)
+  if (L.kind[c] <= 0) return 0.f;
+  const float pc = x ? x[c] : 0.f;
+  float acc = 0.f;
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  const size_t plane = (size_t)nx * ny;
+  auto face = [&](float a, bool has, size_t n, bool top) {
+    if (a == 0.f) return;
+    if (has) {
+      const int8_t kn = L.kind[n];
+      if (kn > 0) acc += x ? a * (pc - x[n]) : a;
+      else if (kn < 0) acc += 2.0f * a * (x ? pc : 1.0f);
+    } else if (top) {
+      acc += 2.0f * a * (x ? pc : 1.0f);
+    }
+  };
+  face(L.ou[((size_t)k * ny + j) * (nx + 1) + i], i > 0, c - 1, false);
+  face(L.ou[((size_t)k * ny + j) * (nx + 1) + i + 1], i + 1 < nx, c + 1, false);
+  face(L.ov[((size_t)k * (ny + 1) + j) * nx + i], j > 0, c - nx, false);
+  face(L.ov[((size_t)k * (ny + 1) + j + 1) * nx + i], j + 1 < ny, c + nx, false);
+  face(L.ow[((size_t)k * ny + j) * nx + i], k > 0, c - plane, false);
+  face(L.ow[((size_t)(k + 1) * ny + j) * nx + i], k + 1 < nz, c + plane, true);
+  return acc * L.inv;
+}
+
+// kind of the first coarse level from the fine cells (delta3d_multigrid.rs:153-169, :357-367)
+__global__ void __launch_bounds__(BLOCK) k_ckind_fine(Fine G, Level L) {
+  FSP_FOR(c, L.n) {
+    int i, j, k;
+    ldecode(L, c, i, j, k);
+    bool active = false, air = false;
+    for (int ch = 0; ch < 8; ++ch) {
+      const int di = ch & 1, dj = (ch >> 1) & 1, dk = (ch >> 2) & 1;
+      const size_t f = cidx(G, 2 * i + di, 2 * j + dj, 2 * k + dk);
+      const uint8_t m = G.mask[f];
+      if (m & M_WET) active = true;
+      else if (!G.solid[f]) air = true;
+    }
+    L.kind[c] = active ? 1 : (air ? -1 : 0);
+  }
+}
+__global__ void __launch_bounds__(BLOCK) k_ckind(Level F, Level L) {
+  FSP_FOR(c, L.n) {
+    int i, j, k;
+    ldecode(L, c, i, j, k);
+    bool active = false, air = false;
+    for (int ch = 0; ch < 8; ++ch) {
+      const int di = ch & 1, dj = (ch >> 1) & 1, dk = (ch >> 2) & 1;
+      const int8_t v = F.kind[lidx(F, 2 * i + di, 2 * j + dj, 2 * k + dk)];
+      if (v > 0) active = true;
+      else if (v < 0) air = true;
+    }
+    L.kind[c] = active ? 1 : (air ? -1 : 0);
+  }
+}
+
+// Coarse apertures: a quarter of the four fine faces (delta3d_multigrid.rs:170-202).
+// QUARTER order (0,0), (1,0), (0,1), (1,1).
+template <class OpenU, class OpenV, class OpenW>
+__device__ __forceinline__ void coarse_faces(const Level& L, size_t f, OpenU ou, OpenV ov, OpenW ow) {
+#pragma clang fp contract(off)
+  const size_t cx = L.nx, cy = L.ny, cz = L.nz;
+  const size_t ncu = (cx + 1) * cy * cz, ncv = cx * (cy + 1) * cz;
+  if (f < ncu) {
+    const size_t i = f % (cx + 1), t = f / (cx + 1), j = t % cy, k = t / cy;
+    float s = 0.f;
+    s += ou(2 * i, 2 * j + 0, 2 * k + 0);
+    s += ou(2 * i, 2 * j + 1, 2 * k + 0);
+    s += ou(2 * i, 2 * j + 0, 2 * k + 1);
+    s += ou(2 * i, 2 * j + 1, 2 * k + 1);
+    L.ou[f] = 0.25f * s;
+  } else if (f < ncu + ncv) {
+    const size_t g = f - ncu;
+    const size_t i = g % cx, t = g / cx, j = t % (cy + 1), k = t / (cy + 1);
+    float s = 0.f;
+    s += ov(2 * i + 0, 2 * j, 2 * k + 0);
+    s += ov(2 * i + 1, 2 * j, 2 * k + 0);
+    s += ov(2 * i + 0, 2 * j, 2 * k + 1);
+    s += ov(2 * i + 1, 2 * j, 2 * k + 1);
+    L.ov[g] = 0.25f * s;
+  } else {
+    const size_t g = f - ncu - ncv;
+    const size_t i = g % cx, t = g / cx, j = t % cy, k = t / cy;
+    float s = 0.f;
+    s += ow(2 * i + 0, 2 * j + 0, 2 * k);
+    s += ow(2 * i + 1, 2 * j + 0, 2 * k);
+    s += ow(2 * i + 0, 2 * j + 1, 2 * k);
+    s += ow(2 * i + 1, 2 * j + 1, 2 * k);
+    L.ow[g] = 0.25f * s;
+  }
+}
+__global__ void __launch_bounds__(BLOCK) k_cfaces_fine(Fine G, Level L) {
+  const size_t cx = L.nx, cy = L.ny, cz = L.nz;
+  const size_t total = (cx + 1) * cy * cz + cx * (cy + 1) * cz + cx * cy * (cz + 1);
+  FSP_FOR(f, total) {
+    coarse_faces(
+        L, f, [&](size_t i, size_t j, size_t k) { return (float)G.ou[fuidx(G, (int)i, (int)j, (int)k)]; },
+        [&](size_t i, size_t j, size_t k) { return (float)G.ov[fvidx(G, (int)i, (int)j, (int)k)]; },
+        [&](size_t i, size_t j, size_t k) { return (float)G.ow[fwidx(G, (int)i, (int)j, (int)k)]; });
+  }
+}
+__global__ void __launch_bounds__(BLOCK) k_cfaces(Level F, Level L) {
+  const size_t cx = L.nx, cy = L.ny, cz = L.nz;
+  const size_t total = (cx + 1) * cy * cz + cx * (cy + 1) * cz + cx * cy * (cz + 1);
+  const size_t fnx = F.nx, fny = F.ny;
+  FSP_FOR(f, total) {
+    coarse_faces(
+        L, f, [&](size_t i, size_t j, size_t k) { return F.ou[(k * fny + j) * (fnx + 1) + i]; },
+        [&](size_t i, size_t j, size_t k) { return F.ov[(k * (fny + 1) + j) * fnx + i]; },
+        [&](size_t i, size_t j, size_t k) { return F.ow[(k * fny + j) * fnx + i]; });
+  }
+}
+__global__ void __launch_bounds__(BLOCK) k_cdiag(Level L) {
+  FSP_FOR(c, L.n) {
+    int i, j, k;
+    ldecode(L, c, i, j, k);
+    L.diag[c] = coarse_row(L, i, j, k, c, nullptr);
+  }
+}
+
+// Damped Jacobi on a coarse level (delta3d_multigrid.rs:290-299): xout = xin +
+// w (r - L xin)/diag; xin == nullptr means xin = 0.
+__global__ void __launch_bounds__(BLOCK) k_cjacobi(Level L, const float* xin, float* xout, Ctl* ctl) {
+#pragma clang fp contract(off)
+  if (!ctl->active) return;
+  const float omega = 6.0f / 7.0f;
+  FSP_FOR(c, L.n) {
+    int i, j, k;
+    ldecode(L, c, i, j, k);
+    const float t = xin ? coarse_row(L, i, j, k, c, xin) : 0.f;
+    float xc = xin ? xin[c] : 0.f;
+    const float dg = L.diag[c];
+    if (dg > 0.f) xc += omega * (L.r[c] - t) / dg;
+    xout[c] = xc;
+  }
+}
+// One colour of red-black Gauss-Seidel on a coarse level, in place.
+__global__ void __launch_bounds__(BLOCK) k_cgs(Level L, float* x, int colour, int first, Ctl* ctl) {
+#pragma clang fp contract(off)
+  if (!ctl->active) return;
+  FSP_FOR(c, L.n) {
+    int i, j, k;
+    ldecode(L, c, i, j, k);
+    const int col = (i + j + k) & 1;
+    const float dg = L.diag[c];
+    if (first) {
+      if (col != colour) { x[c] = 0.f; continue; }
+      x[c] = dg > 0.f ? L.r[c] / dg : 0.f;
+      continue;
+    }
+    if (col != colour) continue;
+    if (dg > 0.f) {
+      const float t = coarse_row(L, i, j, k, c, x);
+      x[c] = x[c] + (L.r[c] - t) / dg;
+    }
+  }
+}
+
+// Residual of the fine level restricted to the first coarse level
+// (delta3d_multigrid.rs:406-411, restrict3 :258-272).
+__global__ void __launch_bounds__(BLOCK) k_restrict_fine(Fine G, Level L, const float* z, const float* r, Ctl* ctl) {
+#pragma clang fp contract(off)
+  if (!ctl->active) return;
+  FSP_FOR(c, L.n) {
+    int i, j, k;
+    ldecode(L, c, i, j, k);
+    float s = 0.f;
+    for (int ch = 0; ch < 8; ++ch) {
+      const int fi = 2 * i + (ch & 1), fj = 2 * j + ((ch >> 1) & 1), fk = 2 * k + ((ch >> 2) & 1);
+      const size_t f = cidx(G, fi, fj, fk);
+      s += r[f] - apply_row(G, fi, fj, fk, f, G.mask[f], z);
+    }
+    L.r[c] = 0.125f * s;
+  }
+}
+// Coarse residual restricted one level down (delta3d_multigrid.rs:312-321).
+__global__ void __launch_bounds__(BLOCK) k_crestrict(Level F, Level L, const float* xf, Ctl* ctl) {
+#pragma clang fp contract(off)
+  if (!ctl->active) return;
+  FSP_FOR(c, L.n) {
+    int i, j, k;
+    ldecode(L, c, i, j, k);
+    float s = 0.f;
+    for (int ch = 0; ch < 8; ++ch) {
+      const int fi = 2 * i + (ch & 1), fj = 2 * j + ((ch >> 1) & 1), fk = 2 * k + ((ch >> 2) & 1);
+      const size_t f = lidx(F, fi, fj, fk);
+      s += F.r[f] - coarse_row(F, fi, fj, fk, f, xf);
+    }
+    L.r[c] = 0.125f * s;
+  }
+}
+// Prolongation by injection (delta3d_multigrid.rs:275-287); on the fine
+// level dry rows are forced back to 0 (:414-418).
+__global__ void __launch_bounds__(BLOCK) k_prolong_fine(Fine G, Level L, const float* xc, float* z, Ctl* ctl) {
+  if (!ctl->active) return;
+  FSP_FOR(c, G.n) {
+    int i, j, k;
+    decode(G, c, i, j, k);
+    const uint8_t m = G.mask[c];
+    if (!prec_positive(m)) { z[c] = 0.f; continue; }
+    z[c] += xc[lidx(L, i >> 1, j >> 1, k >> 1)];
+  }
+}
+__global__ void __launch_bounds__(BLOCK) k_cprolong(Level F, Level L, const float* xc, float* xf, Ctl* ctl) {
+  if (!ctl->active) return;
+  FSP_FOR(c, F.n) {
+    int i, j, k;
+    ldecode(F, c, i, j, k);
+    xf[c] += xc[lidx(L, i >> 1, j >> 1, k >> 1)];
+  }
+}
+
+// ---------------------------------------------------------------- correction and metrics
+// Gradient subtract (delta3d_mobile.rs:284-343), one thread per face, from
+// the f64 pressure (the gradient is formed in f64, the face rounded to f32),
+// and max |u| over all faces (:422-427).
+__device__ __forceinline__ float corrected_side(const Fine& G, float us, const double* p, float k1, int x, int y, int i,
+                                                int j, int k) {
+#pragma clang fp contract(off)
+  const size_t l = cidx(G, x, y, k), r = cidx(G, i, j, k);
+  if (G.solid[l] || G.solid[r]) return us;
+  const bool wl = wet_at(G, x, y, k), wr = wet_at(G, i, j, k);
+  const double kk = (double)k1, dx = (double)G.dx;
+  double ch;
+  if (wl && wr) ch = kk * (p[r] - p[l]) / dx;
+  else if (wl) { float a, b; ghost_side(G, x, y, k, i, j, a, b); ch = kk * ((double)b - p[l]) * (double)a / dx; }
+  else if (wr) { float a, b; ghost_side(G, i, j, k, x, y, a, b); ch = kk * (p[r] - (double)b) * (double)a / dx; }
+  else return us;
+  return (float)((double)us - ch);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_correct(Fine G, const float* us, const float* vs, const float* ws,
+                                                   const double* p, float k1, float* u, float* v, float* w, Ctl* ctl,
+                                                   double* partials) {
+#pragma clang fp contract(off)
+  const size_t nx = G.nx, ny = G.ny, nz = G.nz;
+  const size_t nu = (nx + 1) * ny * nz, nv = nx * (ny + 1) * nz, nw = nx * ny * (nz + 1);
+  double mx = 0.0, unused = 0.0;
+  FSP_FOR(f, nu + nv + nw) {
+    float val;
+    if (f < nu) {
+      const int i = (int)(f % (nx + 1));
+      const size_t t = f / (nx + 1);
+      const int j = (int)(t % ny), k = (int)(t / ny);
+      val = us[f];
+      if (i > 0 && i < (int)nx && G.ou[f]) val = corrected_side(G, val, p, k1, i - 1, j, i, j, k);
+      u[f] = val;
+    } else if (f < nu + nv) {
+      const size_t g = f - nu;
+      const int i = (int)(g % nx);
+      const size_t t = g / nx;
+      const int j = (int)(t % (ny + 1)), k = (int)(t / (ny + 1));
+      val = vs[g];
+      if (j > 0 && j < (int)ny && G.ov[g]) val = corrected_side(G, val, p, k1, i, j - 1, i, j, k);
+      v[g] = val;
+    } else {
+      const size_t g = f - nu - nv;
+      const int i = (int)(g % nx);
+      const size_t t = g / nx;
+      const int j = (int)(t % ny), k = (int)(t / ny);
+      val = ws[g];
+      if (k >= 1 && G.ow[g] && wet_at(G, i, j, k - 1)) {
+        const size_t below = cidx(G, i, j, k - 1);
+        const double kk = (double)k1, dx = (double)G.dx;
+        if (k < (int)nz && wet_at(G, i, j, k)) {
+          val = (float)((double)val - kk * (p[cidx(G, i, j, k)] - p[below]) / dx);
+        } else {
+          float a, b;
+          ghost_up(G, i, j, k - 1, a, b);
+          val = (float)((double)val - kk * ((double)b - p[below]) * (double)a / dx);
+        }
+      }
+      w[g] = val;
+    }
+    mx = fmax(mx, (double)fabsf(val));
+  }
+  reduce_finish<true>(mx, unused, OP_UMAX, ctl, partials);
+}
+
+// max |div u| over wet rows and over plain rows (delta3d_mobile.rs:398-421).
+__global__ void __launch_bounds__(BLOCK) k_divmetric(Fine G, const float* u, const float* v, const float* w, Ctl* ctl,
+                                                     double* partials) {
+  double all = 0.0, plain = 0.0;
+  FSP_FOR(c, G.n) {
+    const uint8_t m = G.mask[c];
+    if (!(m & M_WET)) continue;
+    int i, j, k;
+    decode(G, c, i, j, k);
+    const double d = (double)fabsf(divergence_at(G, u, v, w, i, j, k, c));
+    all = fmax(all, d);
+    if (!(m & M_GHOST)) plain = fmax(plain, d);
+  }
+  reduce_finish<true>(all, plain, OP_DIVMAX, ctl, partials);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_divergence(Fine G, const float* u, const float* v, const float* w,
+                                                      float* out) {
+  FSP_FOR(c, G.n) {
+    int i, j, k;
+    decode(G, c, i, j, k);
+    out[c] = divergence_at(G, u, v, w, i, j, k, c);
+  }
+}
+
+// Oettli-Prager backward error of the true residual (delta3d_mobile.rs:243-271).
+__global__ void __launch_bounds__(BLOCK) k_backward(Fine G, const double* p, const float* b, const float* r, Ctl* ctl,
+                                                    double* partials) {
+#pragma clang fp contract(off)
+  const size_t plane = (size_t)G.nx * G.ny;
+  double worst = 0.0, unused = 0.0;
+  FSP_FOR(c, G.n) {
+    const uint8_t m = G.mask[c];
+    if (!(m & M_WET)) continue;
+    int i, j, k;
+    decode(G, c, i, j, k);
+    const double pc = fabs(p[c]);
+    double acc = 0.0;
+    RowG R;
+    full_row(G, i, j, k, R);
+    const size_t nb[6] = {c - 1, c + 1, c - G.nx, c + G.nx, c - plane, c + plane};
+    for (int f = 0; f < 6; ++f) {
+      if (R.type[f] == 1) acc += pc + fabs(p[nb[f]]);
+      else if (R.type[f] == 2) acc += pc * (double)R.it[f];
+    }
+    const double scale = fabs((double)b[c]) + acc * (double)G.inv;
+    if (scale > 0.0) worst = fmax(worst, fabs((double)r[c]) / scale);
+  }
+  reduce_finish<true>(worst, unused, OP_BACKWARD, ctl, partials);
+}
+
+// Fingerprint of p: FNV-1a 64 over the bytes of chunks of 1024 values, then over the
+// chunk hashes in order. It detects a bitwise repeat of p like the
+// reference's sequential FNV-1a (delta3d.rs:1360-1369).
+__global__ void __launch_bounds__(BLOCK) k_hash_chunks(const double* p, size_t n, unsigned long long* hashes) {
+  const size_t chunks = (n + 1023) / 1024;
+  FSP_FOR(q, chunks) {
+    unsigned long long h = 0xcbf29ce484222325ull;
+    const size_t e = (q + 1) * 1024 < n ? (q + 1) * 1024 : n;
+    for (size_t s = q * 1024; s < e; ++s) {
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(p[s]);
+      for (int b = 0; b < 8; ++b) {
+        h ^= (unsigned long long)((bits >> (8 * b)) & 0xffu);
+        h *= 0x100000001b3ull;
+      }
+    }
+    hashes[q] = h;
+  }
+}
+__global__ void k_hash_combine(const unsigned long long* hashes, size_t chunks, unsigned long long* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long h = 0xcbf29ce484222325ull;
+  for (size_t q = 0; q < chunks; ++q) {
+    unsigned long long x = hashes[q];
+    for (int b = 0; b < 8; ++b) {
+      h ^= (x >> (8 * b)) & 0xffull;
+      h *= 0x100000001b3ull;
+    }
+  }
+  *out = h;
+}
+
+// Host-side parameters of a CG pass, written on the stream.
+__global__ void k_set_pass(Ctl* ctl, int active, double rr, double tol_b2, double physical_target, uint32_t it,
+                           uint32_t max_iters, uint32_t pass_limit) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  ctl->active = active;
+  ctl->rr = rr;
+  ctl->tol_b2 = tol_b2;
+  ctl->physical_target = physical_target;
+  ctl->it = it;
+  ctl->max_iters = max_iters;
+  ctl->pass_limit = pass_limit;
+  ctl->stop_dq = 0;
+}
+
+}  // namespace fsp

@@ -1,0 +1,76 @@
+"""The projection workloads (synthetic, seeded), in reference orientation.
+
+Sizes are given as (X, Y, Z) with Y vertical and mapped to the reference's
+Domain3 {nx: X, ny: Z, nz: Y} (z up). Every workload has dx = 1 and
+rho = dt = 1, so scale = -rho/dt = -1 and k1 = dt/rho = 1 (b = -div u*).
+
+  closed_box    all fluid, closed box (Neumann on all six faces), faces
+                i.i.d. U[-1,1]; 3-level V-cycle, 2 RBGS pre/post sweeps.
+  tank          free surface h(x,z) = Y/2 + 4 sines, air (p = 0) above,
+                8 solid spheres, 16-mode smooth velocity of RMS 1;
+                multigrid over every coarse level the grid allows.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .volume import Domain3, generate
+
+
+@dataclass
+class Workload:
+    name: str
+    domain: Domain3
+    eta: np.ndarray
+    solid: np.ndarray | None
+    us: np.ndarray
+    vs: np.ndarray
+    ws: np.ndarray
+    homogeneous_ghost: bool
+    closed_lid: bool
+    scale: float = -1.0
+    k1: float = 1.0
+    rho: float = 1.0
+    g_eff: float = 9.81
+    rest: float = 0.0
+    options: dict = field(default_factory=dict)
+    seed: int = 0
+
+    def describe(self):
+        d = self.domain
+        return {"workload": self.name, "nx": d.nx, "ny": d.ny, "nz": d.nz, "cells": d.cells, "seed": self.seed}
+
+
+def closed_box(n: int = 64, seed: int = 1, levels: int = 2) -> Workload:
+    eta, solid, us, vs, ws = generate(0, n, n, n, seed)
+    return Workload(f"closed_box_{n}", Domain3(n, n, n, 1.0), eta, None, us, vs, ws,
+                    homogeneous_ghost=True, closed_lid=True, rest=float(n),
+                    options=dict(preconditioner=_lib.PRECOND_MULTIGRID, smoother=_lib.SMOOTH_RBGS,
+                                 max_levels=levels, pre_sweeps=2, post_sweeps=2, coarse_sweeps=8),
+                    seed=seed)
+
+
+def tank(X: int = 128, Y: int = 128, Z: int = 128, seed: int = 1) -> Workload:
+    nx, ny, nz = X, Z, Y
+    eta, solid, us, vs, ws = generate(1, nx, ny, nz, seed)
+    return Workload(f"tank_{X}x{Y}x{Z}", Domain3(nx, ny, nz, 1.0), eta, solid, us, vs, ws,
+                    homogeneous_ghost=True, closed_lid=True, rest=float(nz) / 2,
+                    options=dict(preconditioner=_lib.PRECOND_MULTIGRID, smoother=_lib.SMOOTH_RBGS,
+                                 max_levels=-1, pre_sweeps=2, post_sweeps=2, coarse_sweeps=8),
+                    seed=seed)
+
+
+def build_volume(wl: Workload, **option_overrides):
+    """A configured Volume3 holding the workload's geometry and predicted faces."""
+    from .volume import Volume3
+    opts = dict(wl.options)
+    opts.update(option_overrides)
+    vol = Volume3.configure(wl.domain, rho=wl.rho, g_eff=wl.g_eff, **opts)
+    vol.set_surface_mode(wl.homogeneous_ghost, wl.closed_lid)
+    vol.set_solid(wl.solid)
+    vol.set_free_surface(wl.eta, wl.rest)
+    vol.set_velocity(wl.us, wl.vs, wl.ws)
+    return vol

@@ -1,0 +1,166 @@
+/*
+ * fsp.h — C ABI of the MI355X pressure projection for the δ-3D volumetric
+ * solver (Fluidisim water-core). Plain pointers and sizes only.
+ *
+ * Layout (the reference's, code/water-core/src/delta3d.rs:26-28, 187-206):
+ *   cells  c  = (k*ny + j)*nx + i          x fastest, k (z) up
+ *   u faces   = (k*ny + j)*(nx+1) + i      (nx+1)*ny*nz
+ *   v faces   = (k*(ny+1) + j)*nx + i      nx*(ny+1)*nz
+ *   w faces   = (k*ny + j)*nx + i          nx*ny*(nz+1)   (k = nz is the lid)
+ *   columns   = j*nx + i                   nx*ny
+ *
+ * Threading: one caller per solver handle; separate handles are independent.
+ * All device memory is reserved by fsp_create (the reference's invariant
+ * I-06, delta3d.rs:208-241): fsp_project* allocates nothing.
+ */
+#ifndef FSP_H
+#define FSP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSP_ABI_VERSION 1
+
+/* Replaces the reference's Error (code/water-core/src/delta_projection.rs:133-150);
+ * FSP_DEVICE and FSP_INVALID have no counterpart (device failure, bad handle/options). */
+typedef enum {
+  FSP_OK = 0,
+  FSP_DOMAIN = 1,       /* Error::Domain: empty domain, unsupported geometry */
+  FSP_NOT_FINITE = 2,   /* Error::NotFinite: non-finite input or intermediate */
+  FSP_SHAPE = 3,        /* Error::Shape: wrong array length */
+  FSP_CONVERGENCE = 4,  /* Error::Convergence: degraded result refused (fsp_options.refuse_degraded) */
+  FSP_DEVICE = 5,
+  FSP_INVALID = 6
+} fsp_status;
+
+/* Replaces Domain3 (delta3d.rs:44-51). */
+typedef struct {
+  int32_t nx, ny, nz;
+  float dx;
+} fsp_domain;
+
+enum { FSP_PRECOND_JACOBI = 0, FSP_PRECOND_MULTIGRID = 1 };
+enum { FSP_SMOOTH_JACOBI = 0, FSP_SMOOTH_RBGS = 1 };
+enum { FSP_STORE_F32 = 0, FSP_STORE_F16 = 1, FSP_STORE_BF16 = 2 };
+
+/* Solver settings. fsp_default_options gives the reference's multigrid
+ * preconditioner (delta3d_multigrid.rs:28-34: damped Jacobi ω=6/7, 2 pre,
+ * 2 post, 8 coarsest sweeps, every coarse level the reference rule allows). */
+typedef struct {
+  int32_t preconditioner;  /* FSP_PRECOND_* */
+  int32_t smoother;        /* FSP_SMOOTH_* */
+  int32_t max_levels;      /* coarse levels cap; -1 = reference rule (delta3d_multigrid.rs:73-87) */
+  int32_t pre_sweeps, post_sweeps, coarse_sweeps;
+  int32_t gauge;           /* 1: remove the mean of b and p when no Dirichlet face exists */
+  int32_t vcycle_storage;  /* FSP_STORE_*: storage type of the V-cycle vectors */
+  int32_t replace_every;   /* true-residual replacement period in iterations (0 = never) */
+  int32_t refuse_degraded; /* 1: a degraded projection returns FSP_CONVERGENCE, outputs untouched */
+  int32_t device;          /* HIP device ordinal */
+} fsp_options;
+
+/* Surface state (Volume3 fields rho, g_eff, rest, homogeneous_ghost;
+ * delta3d.rs:102-110, delta3d_mobile.rs:126-147). */
+typedef struct {
+  float rho, g_eff, rest;
+  int32_t homogeneous_ghost; /* 1: ghost values 0 (p = 0 at the surface) */
+  int32_t closed_lid;        /* 1: every top face closed (aperture 0): Neumann lid */
+} fsp_surface;
+
+/* Replaces Report (delta_projection.rs:164-188). */
+typedef struct {
+  uint32_t refinements;
+  uint32_t iterations;
+  int32_t degraded;
+  int32_t floor;
+  double residual;
+  double divergence;
+  double backward_error;
+  double divergence_plain;
+  /* timing of the call, milliseconds (no reference counterpart) */
+  double ms_device;          /* device time of the projection (events around it) */
+  double ms_transfer;        /* host<->device copies of fsp_project (0 for the device variant) */
+} fsp_report;
+
+typedef struct fsp_solver fsp_solver;
+
+int32_t fsp_abi_version(void);
+const char* fsp_status_string(fsp_status s);
+void fsp_default_options(fsp_options* out);
+
+/* Replaces Volume3::configure + enable_multigrid (delta3d.rs:210-294,
+ * delta3d_multigrid.rs:339-349): reserves every device buffer. */
+fsp_status fsp_create(const fsp_domain* domain, const fsp_options* options, fsp_solver** out);
+void fsp_destroy(fsp_solver* s);
+/* Coarse levels the solver holds (Volume3::multigrid_levels, delta3d_multigrid.rs:352-354). */
+int32_t fsp_levels(const fsp_solver* s);
+
+/* Replaces set_free_surface and the cut geometry (delta3d_mobile.rs:9-25,
+ * delta3d.rs:300-319). Host pointers. eta: nx*ny column heights (required);
+ * eta_roundoff: nx*ny or NULL; solid: cells, nonzero = solid, or NULL;
+ * open_u/v/w: face apertures or NULL (derived: a face touching a solid cell or
+ * the box walls is closed, the lid open). Only apertures 0 and 1 are
+ * supported (else FSP_DOMAIN). */
+fsp_status fsp_set_geometry(fsp_solver* s, const fsp_surface* surface,
+                            const float* eta, size_t n_eta,
+                            const float* eta_roundoff,
+                            const uint8_t* solid, size_t n_solid,
+                            const float* open_u, size_t n_open_u,
+                            const float* open_v, size_t n_open_v,
+                            const float* open_w, size_t n_open_w);
+
+/* Replaces Volume3::project_mobile3 (delta3d_mobile.rs:465-609). Host
+ * pointers. us/vs/ws: predicted faces; p: in = warm start (p_warm != 0) or
+ * ignored, out = pressure; u/v/w out: corrected faces. scale = -rho/dt,
+ * k1 = dt/rho (delta3d_mobile.rs:836-837). rel_tol: relative true residual
+ * (1e-6 in the reference, :507). On any non-OK status no output is written. */
+fsp_status fsp_project(fsp_solver* s,
+                       const float* us, size_t n_us, const float* vs, size_t n_vs, const float* ws, size_t n_ws,
+                       float* p, size_t n_p, int32_t p_warm,
+                       float* u, size_t n_u, float* v, size_t n_v, float* w, size_t n_w,
+                       float scale, float k1, double rel_tol, uint32_t max_iters, fsp_report* report);
+
+/* The same on device pointers (resident use; no host copies). */
+fsp_status fsp_project_device(fsp_solver* s, const float* d_us, const float* d_vs, const float* d_ws,
+                              float* d_p, int32_t p_warm, float* d_u, float* d_v, float* d_w,
+                              float scale, float k1, double rel_tol, uint32_t max_iters, fsp_report* report);
+
+/* Replaces apply_pressure_operator_for_trials (delta3d_mobile.rs:936-949):
+ * out = A p with the current geometry. Device pointers. */
+fsp_status fsp_apply_operator_device(fsp_solver* s, const float* d_p, float* d_out);
+
+/* Replaces assemble_pressure_problem_for_trials (delta3d_mobile.rs:960-979):
+ * rhs and Jacobi prec from a divergence field. Device pointers. */
+fsp_status fsp_assemble_device(fsp_solver* s, const float* d_div, float scale, float* d_rhs, float* d_prec);
+
+/* One preconditioner application z = M^-1 r on the current geometry
+ * (v_cycle3, delta3d_multigrid.rs:393-429). Device pointers. */
+fsp_status fsp_precondition_device(fsp_solver* s, const float* d_r, float* d_z);
+
+/* Divergence of faces (delta3d.rs:747-763, :806-823). Device pointers. */
+fsp_status fsp_divergence_device(fsp_solver* s, const float* d_u, const float* d_v, const float* d_w, float* d_out);
+
+/* Live timing of one kernel (no reference counterpart; a measurement hook).
+ * fsp_profile_begin reserves max_launches event pairs and brackets every
+ * launch of the kernel named `kernel` (its symbol name, e.g. "k_fgs") with
+ * them; fsp_profile_end synchronises and returns the launch count, the
+ * summed device time and (if durations_ms is not NULL) up to max_durations
+ * per-launch times in launch order. */
+fsp_status fsp_profile_begin(fsp_solver* s, const char* kernel, uint32_t max_launches);
+fsp_status fsp_profile_end(fsp_solver* s, uint32_t* launches, double* total_ms, float* durations_ms,
+                           uint32_t max_durations);
+
+/* Seeded synthetic inputs (host memory). cfg: 0 = closed box, i.i.d.
+ * U[-1,1] faces; 1 = free-surface tank with 8 spheres and a 16-mode smooth
+ * velocity. Sizes in reference orientation (nz is vertical). Arrays may be
+ * NULL when not wanted. scale_ref scales heights/radii (edge/128 for patches). */
+fsp_status fsp_generate(int32_t cfg, int32_t nx, int32_t ny, int32_t nz, uint64_t seed,
+                        float* eta, uint8_t* solid, float* us, float* vs, float* ws);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FSP_H */

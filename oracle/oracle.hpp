@@ -77,9 +77,12 @@ struct Geometry {
   const float* open_w;
 };
 
-// delta3d.rs:69-72 and delta_projection.rs:200-203
-template <class T> inline T roundoff_3d() { const T u = T(1) / T(16777216); return T(10) * u / (T(1) - T(10) * u); }
-template <class T> inline T roundoff_2d() { const T u = T(1) / T(16777216); return T(8) * u / (T(1) - T(8) * u); }
+// delta3d.rs:69-72 and delta_projection.rs:200-203: gamma_{m+4} = (m+4)u/(1-(m+4)u)
+// with u the unit roundoff of T (2^-24 in f32 as in the reference; 2^-53 in
+// the f64 oracle, whose floor is that of f64).
+template <class T> inline T unit_roundoff() { return std::numeric_limits<T>::epsilon() / T(2); }
+template <class T> inline T roundoff_3d() { const T u = unit_roundoff<T>(); return T(10) * u / (T(1) - T(10) * u); }
+template <class T> inline T roundoff_2d() { const T u = unit_roundoff<T>(); return T(8) * u / (T(1) - T(8) * u); }
 
 // delta3d.rs:1360-1369 — FNV-1a over the little-endian bytes of each f32.
 inline uint64_t fingerprint_f32(const float* v, size_t n) {
@@ -672,6 +675,7 @@ template <class T> class Volume {
           // palindromic red-black sequence so the coarsest correction is symmetric
           int half = opt.coarse_sweeps / 2;
           for (int s = 0; s < half; ++s) level_sweep(L, true);
+          if (opt.coarse_sweeps & 1) gs_colour(L, 0);
           for (int s = 0; s < half; ++s) level_sweep(L, false);
         }
       } else {
