@@ -27,13 +27,13 @@ HBM_PEAK_GBS = 8000.0
 # must touch, read once and written once (u8 row code = 1 B, f32 = 4 B,
 # f64 = 8 B; a red-black colour pass reads and writes half the cells).
 BYTES_PER_CELL = {
-    "k_fgs": 1 + 4 + 2 + 2,             # mask, z (all, for neighbours), r (colour), z write (colour)
-    "k_spmv_dir": 1 + 4 + 4 + 4 + 4,    # mask, z, d_old -> d_new, q
-    "k_update": 8 + 8 + 4 + 4 + 4 + 4,  # p64 rw, d, q, r rw
-    "k_residual": 1 + 8 + 4 + 4,        # mask, p64, b -> r
-    "k_restrict_fine": 1 + 4 + 4 + 0.5,  # mask, z, r -> coarse r
-    "k_prolong_fine": 1 + 4 + 4 + 0.5,  # mask, z rw, coarse x
-    "k_dot": 4 + 4,
+    "k_q_gs<0>": 1 + 4 + 4 + 4,          # red-black pass: mask, z (all), r, z write (whole quads)
+    "k_q_gs<1>": 1 + 4 + 4,              # first pass from zero: mask, r -> z
+    "k_q_gs<2>": 1 + 4 + 0.5 + 4 + 4,    # prolongation + pass: mask, z, coarse x, r -> z'
+    "k_q_spmv_dir": 1 + 4 + 4 + 4 + 4,   # mask, z, d_old -> d_new, q
+    "k_q_update": 8 + 8 + 4 + 4 + 4 + 4, # p64 rw, d, q, r rw
+    "k_q_restrict": 1 + 4 + 4 + 0.5,     # mask, z, r -> coarse r
+    "k_residual": 1 + 8 + 4 + 4,         # mask, p64, b -> r
 }
 
 
@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", default="256x128x256", help="X x Y x Z, Y vertical")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--kernel", default="k_fgs", help="kernel timed live for the roofline")
+    ap.add_argument("--kernel", default="k_q_gs<0>", help="kernel timed live for the roofline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", default="1,3", help="oracle sample: two iteration budgets")
     return ap.parse_args()
@@ -65,7 +65,8 @@ def cpu_baseline(wl, gpu_iterations, iters_pair):
                       homogeneous_ghost=wl.homogeneous_ghost, closed_lid=wl.closed_lid, solid=wl.solid)
     o = wl.options
     opts = po.options(multigrid=1, smoother=o["smoother"], max_levels=o["max_levels"], pre=o["pre_sweeps"],
-                      post=o["post_sweeps"], coarse=o["coarse_sweeps"])
+                      post=o["post_sweeps"], coarse=o["coarse_sweeps"],
+                      coarse_op=o.get("coarse_operator", 0))
     times = []
     for k in iters_pair:
         t0 = time.perf_counter()
@@ -153,7 +154,7 @@ def main():
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": wl.name, "domain": [wl.domain.nx, wl.domain.ny, wl.domain.nz],
-                   "cells": wl.domain.cells, "solver": "plain MGPCG (RBGS V-cycle, 2+2 sweeps, all levels)",
+                   "cells": wl.domain.cells, "solver": "plain MGPCG (RBGS V-cycle 2+2, Galerkin coarse levels, 8 coarsest sweeps)",
                    "precision": "f32 vectors, f64 pressure/residual/reductions", "rel_tol": 1e-6,
                    "iterations": last.iterations, "residual": last.residual,
                    "divergence_plain": last.divergence_plain, "degraded": last.degraded,

@@ -15,6 +15,7 @@ FSP_OK, FSP_DOMAIN, FSP_NOT_FINITE, FSP_SHAPE, FSP_CONVERGENCE, FSP_DEVICE, FSP_
 PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
 SMOOTH_JACOBI, SMOOTH_RBGS = 0, 1
 STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
+COARSE_REDISCRETISED, COARSE_GALERKIN = 0, 1
 
 # Every entry point include/fsp.h declares.
 EXPORTS = [
@@ -33,7 +34,7 @@ class Options(C.Structure):
         ("preconditioner", C.c_int32), ("smoother", C.c_int32), ("max_levels", C.c_int32),
         ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32), ("coarse_sweeps", C.c_int32),
         ("gauge", C.c_int32), ("vcycle_storage", C.c_int32), ("replace_every", C.c_int32),
-        ("refuse_degraded", C.c_int32), ("device", C.c_int32),
+        ("refuse_degraded", C.c_int32), ("device", C.c_int32), ("coarse_operator", C.c_int32),
     ]
 
 

@@ -23,6 +23,7 @@
 
 #include "../../include/fsp.h"
 #include "kernels.h"
+#include "kernels_fast.h"
 
 using namespace fsp;
 
@@ -37,11 +38,20 @@ struct fsp_solver {
   fsp_surface surf{};
   uint8_t *solid = nullptr, *ou = nullptr, *ov = nullptr, *ow = nullptr, *mask = nullptr;
   float *eta = nullptr, *eta_rt = nullptr;
+  uint16_t* gk0 = nullptr;  // per column: lowest ghost row
+  float* gtab = nullptr;    // per column: 1/theta of G_SLOTS ghost rows
   float *b = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *zt = nullptr, *d0 = nullptr,
         *d1 = nullptr;
   float *us = nullptr, *vs = nullptr, *ws = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
   double* p64 = nullptr;  // the pressure, accumulated in f64
   std::vector<Level> levels;
+  // fast paths: quad kernels on the fine level (nx % 4 == 0), one-workgroup
+  // tail for the coarse levels from tail_first on
+  bool quad = false;
+  QuadGrid qfine{}, qrestr{};
+  int tail_first = 0;
+  size_t tail_lds = 0;
+  float* zres = nullptr;  // buffer holding M^-1 r after precondition()
   Ctl* ctl = nullptr;
   double* partials = nullptr;
   unsigned long long *hashes = nullptr, *hash_out = nullptr;
@@ -66,6 +76,11 @@ struct fsp_solver {
     G.solid = solid; G.ou = ou; G.ov = ov; G.ow = ow;
     G.eta = eta; G.eta_rt = eta_rt;
     G.mask = mask;
+    G.gk0 = gk0;
+    G.gtab = gtab;
+    // prec of a plain row with n faces, 1/(n * inv), as rhs_mobile3 computes it
+    G.prec_tab[0] = 0.f;
+    for (int q = 1; q < 7; ++q) G.prec_tab[q] = 1.0f / ((float)q * G.inv);
     return G;
   }
 };
@@ -101,6 +116,14 @@ unsigned grid_for(size_t n) {
       (S)->prof_used += 2;                                                                     \
     }                                                                                          \
   } while (0)
+
+// Fold the per-block partials of a reducing kernel (k_finish) right after it.
+void launch_finish(hipStream_t st, int op, unsigned nblocks, bool is_max, int gated, Ctl* ctl, double* partials) {
+  if (is_max) hipLaunchKernelGGL(k_finish<true>, dim3(1), dim3(BLOCK), 0, st, op, nblocks, gated, ctl, partials);
+  else hipLaunchKernelGGL(k_finish<false>, dim3(1), dim3(BLOCK), 0, st, op, nblocks, gated, ctl, partials);
+}
+#define FSP_FINISH(S, stream, op, grid, is_max, gated) \
+  launch_finish(stream, op, (grid).x, is_max, gated, (S)->ctl, (S)->partials)
 
 template <class T>
 fsp_status dalloc(fsp_solver* S, T** out, size_t count) {
@@ -141,7 +164,14 @@ struct Launch {
   Fine G;
   explicit Launch(fsp_solver* s) : S(s), st(s->stream), G(s->fine()) {}
 
-  void mask() { FSP_LAUNCH(S, k_mask, dim3(grid_for(G.n)), st, G, S->mask, S->ctl, S->partials); }
+  void mask() {
+    const dim3 g(grid_for(G.n));
+    FSP_LAUNCH(S, k_mask, g, st, G, S->mask, S->ctl, S->partials);
+    FSP_FINISH(S, st, OP_COUNT, g, false, 0);
+    // ghost-row table from the fresh row codes (k_mask's finish does not touch it)
+    Fine Gm = G;
+    FSP_LAUNCH(S, k_ghost_table, dim3(grid_for(S->ncols)), st, Gm, S->gk0, S->gtab);
+  }
 
   // Coarse geometry of every level (prepare_multigrid3, delta3d_multigrid.rs:370-387).
   void hierarchy() {
@@ -151,10 +181,16 @@ struct Launch {
                         (size_t)L.nx * L.ny * (L.nz + 1);
       if (l == 0) {
         FSP_LAUNCH(S, k_ckind_fine, dim3(grid_for(L.n)), st, G, L);
-        FSP_LAUNCH(S, k_cfaces_fine, dim3(grid_for(nf)), st, G, L);
+        if (L.galerkin) {
+          FSP_LAUNCH(S, k_cfaces_fine_gal, dim3(grid_for(nf)), st, G, L);
+          FSP_LAUNCH(S, k_cdd_fine, dim3(grid_for(L.n)), st, G, L);
+        } else {
+          FSP_LAUNCH(S, k_cfaces_fine, dim3(grid_for(nf)), st, G, L);
+        }
       } else {
         FSP_LAUNCH(S, k_ckind, dim3(grid_for(L.n)), st, S->levels[l - 1], L);
         FSP_LAUNCH(S, k_cfaces, dim3(grid_for(nf)), st, S->levels[l - 1], L);
+        if (L.galerkin) FSP_LAUNCH(S, k_cdd, dim3(grid_for(L.n)), st, S->levels[l - 1], L);
       }
       FSP_LAUNCH(S, k_cdiag, dim3(grid_for(L.n)), st, L);
     }
@@ -204,11 +240,25 @@ struct Launch {
     return L.x;
   }
 
+  // The tail levels in one workgroup: level tail_first's r is restricted;
+  // leaves its x in levels[tail_first].x.
+  void tail() {
+    TailSet T{};
+    T.count = (int)S->levels.size() - S->tail_first;
+    for (int l = 0; l < T.count; ++l) T.lv[l] = S->levels[S->tail_first + l];
+    T.smoother = S->opt.smoother;
+    T.pre = S->opt.pre_sweeps;
+    T.post = S->opt.post_sweeps;
+    T.coarse = S->opt.coarse_sweeps;
+    hipLaunchKernelGGL(k_tail, dim3(1), dim3(TAIL_THREADS), S->tail_lds, st, T, S->ctl);
+  }
+
   // coarse_cycle3 (delta3d_multigrid.rs:303-332); returns the level-0 iterate.
   float* coarse_cycle() {
     const int last = (int)S->levels.size() - 1;
+    const int tf = S->tail_first;
     std::vector<float*> xcur(S->levels.size(), nullptr);
-    for (int l = 0; l <= last; ++l) {
+    for (int l = 0; l < tf; ++l) {
       const int sweeps = (l == last) ? S->opt.coarse_sweeps : S->opt.pre_sweeps;
       xcur[l] = coarse_sweeps(l, nullptr, sweeps, true, l == last);
       if (l < last) {
@@ -217,7 +267,11 @@ struct Launch {
         FSP_LAUNCH(S, k_crestrict, dim3(grid_for(C.n)), st, L, C, xcur[l], S->ctl);
       }
     }
-    for (int l = last - 1; l >= 0; --l) {
+    if (tf <= last) {
+      tail();
+      xcur[tf] = S->levels[tf].x;
+    }
+    for (int l = std::min(tf, last) - 1; l >= 0; --l) {
       Level& L = S->levels[l];
       Level& C = S->levels[l + 1];
       FSP_LAUNCH(S, k_cprolong, dim3(grid_for(L.n)), st, L, C, xcur[l + 1], xcur[l], S->ctl);
@@ -271,21 +325,84 @@ struct Launch {
     }
   }
 
+  bool quad_vcycle() const {
+    return S->quad && S->opt.smoother == FSP_SMOOTH_RBGS && S->opt.pre_sweeps >= 1 && S->opt.post_sweeps >= 1;
+  }
+
+  // The V-cycle on quads with red-black smoothing: first pass from zero,
+  // residual fused with the restriction, prolongation fused with the first
+  // post-smoothing pass, <r, z> fused with the last one. Returns the buffer
+  // holding z.
+  float* vcycle_quad(int dot_op) {
+    const QuadGrid& Q = S->qfine;
+    const dim3 g((unsigned)(Q.nbx * Q.nby * Q.planes));
+    Level L0 = S->levels.empty() ? Level{} : S->levels[0];
+    const float* none = nullptr;
+    FSP_LAUNCH(S, k_q_gs<1>, g, st, G, Q, (const float*)S->z, S->z, S->r, 0, L0, none, -1, S->ctl, S->partials);
+    FSP_LAUNCH(S, k_q_gs<0>, g, st, G, Q, (const float*)S->z, S->z, S->r, 1, L0, none, -1, S->ctl, S->partials);
+    for (int s = 1; s < S->opt.pre_sweeps; ++s) {
+      FSP_LAUNCH(S, k_q_gs<0>, g, st, G, Q, (const float*)S->z, S->z, S->r, 0, L0, none, -1, S->ctl, S->partials);
+      FSP_LAUNCH(S, k_q_gs<0>, g, st, G, Q, (const float*)S->z, S->z, S->r, 1, L0, none, -1, S->ctl, S->partials);
+    }
+    float* out = S->z;
+    int passes = 2 * S->opt.post_sweeps;
+    int done = 0;
+    if (!S->levels.empty()) {
+      const QuadGrid& R = S->qrestr;
+      const dim3 gr((unsigned)(R.nbx * R.nby * R.planes));
+      FSP_LAUNCH(S, k_q_restrict, gr, st, G, R, (const float*)S->z, (const float*)S->r, L0, S->ctl);
+      const float* x0 = coarse_cycle();
+      FSP_LAUNCH(S, k_q_gs<2>, g, st, G, Q, (const float*)S->z, S->zt, S->r, 1, L0, x0,
+                 passes == 1 ? dot_op : -1, S->ctl, S->partials);
+      if (passes == 1 && dot_op >= 0) FSP_FINISH(S, st, dot_op, g, false, 1);
+      out = S->zt;
+      done = 1;
+    }
+    for (int pz = done; pz < passes; ++pz) {
+      const int colour = (pz % 2 == 0) ? 1 : 0;
+      FSP_LAUNCH(S, k_q_gs<0>, g, st, G, Q, (const float*)out, out, S->r, colour, L0, none,
+                 pz == passes - 1 ? dot_op : -1, S->ctl, S->partials);
+      if (pz == passes - 1 && dot_op >= 0) FSP_FINISH(S, st, dot_op, g, false, 1);
+    }
+    return out;
+  }
+
   void precondition(int op) {
     if (S->opt.preconditioner == FSP_PRECOND_JACOBI) {
       FSP_LAUNCH(S, k_jacobi_prec, dim3(grid_for(G.n)), st, G, S->r, S->z, op, S->ctl,
                          S->partials);
+      FSP_FINISH(S, st, op, dim3(grid_for(G.n)), false, 1);
+      S->zres = S->z;
+      return;
+    }
+    if (quad_vcycle()) {
+      S->zres = vcycle_quad(op);
       return;
     }
     vcycle();
     FSP_LAUNCH(S, k_dot, dim3(grid_for(G.n)), st, G.n, S->r, S->z, op, 1, S->ctl, S->partials);
+    FSP_FINISH(S, st, op, dim3(grid_for(G.n)), false, 1);
+    S->zres = S->z;
   }
 
   void iteration() {
-    FSP_LAUNCH(S, k_spmv_dir, dim3(grid_for(G.n)), st, G, S->z, S->d0, S->d1, S->q, S->ctl,
-                       S->partials);
-    FSP_LAUNCH(S, k_update, dim3(grid_for(G.n)), st, G.n, S->p64, S->r, S->d0, S->d1, S->q,
-                       S->ctl, S->partials);
+    if (S->quad) {
+      const QuadGrid& Q = S->qfine;
+      const dim3 g((unsigned)(Q.nbx * Q.nby * Q.planes));
+      FSP_LAUNCH(S, k_q_spmv_dir, g, st, G, Q, (const float*)S->zres, S->d0, S->d1, S->q, S->ctl, S->partials);
+      FSP_FINISH(S, st, OP_DQ, g, false, 1);
+      const size_t nquads = G.n / 4;
+      const dim3 gu((unsigned)((nquads + QBLOCK - 1) / QBLOCK));
+      FSP_LAUNCH(S, k_q_update, gu, st, nquads, S->p64, S->r, (const float*)S->d0, (const float*)S->d1,
+                 (const float*)S->q, S->ctl, S->partials);
+      FSP_FINISH(S, st, OP_RN, gu, false, 1);
+    } else {
+      FSP_LAUNCH(S, k_spmv_dir, dim3(grid_for(G.n)), st, G, S->zres, S->d0, S->d1, S->q, S->ctl, S->partials);
+      FSP_FINISH(S, st, OP_DQ, dim3(grid_for(G.n)), false, 1);
+      FSP_LAUNCH(S, k_update, dim3(grid_for(G.n)), st, G.n, S->p64, S->r, S->d0, S->d1, S->q, S->ctl,
+                 S->partials);
+      FSP_FINISH(S, st, OP_RN, dim3(grid_for(G.n)), false, 1);
+    }
     precondition(OP_RZ);
   }
 
@@ -324,12 +441,15 @@ fsp_status correct_and_measure(fsp_solver* S, const float* us, const float* vs, 
   if (S->opt.gauge) {
     FSP_LAUNCH(S, k_sum_active<double>, dim3(grid_for(K.G.n)), K.st, K.G, (const double*)p,
                        OP_MEAN_P, S->ctl, S->partials);
+    FSP_FINISH(S, K.st, OP_MEAN_P, dim3(grid_for(K.G.n)), false, 0);
     FSP_LAUNCH(S, k_sub_mean<double>, dim3(grid_for(K.G.n)), K.st, K.G, p, 1, -1, S->ctl,
                        S->partials);
   }
   FSP_LAUNCH(S, k_correct, dim3(grid_for(S->nu + S->nv + S->nw)), K.st, K.G, us, vs, ws,
                      (const double*)p, k1, u, v, w, S->ctl, S->partials);
+  FSP_FINISH(S, K.st, OP_UMAX, dim3(grid_for(S->nu + S->nv + S->nw)), true, 0);
   FSP_LAUNCH(S, k_divmetric, dim3(grid_for(K.G.n)), K.st, K.G, u, v, w, S->ctl, S->partials);
+  FSP_FINISH(S, K.st, OP_DIVMAX, dim3(grid_for(K.G.n)), true, 0);
   fsp_status st = sync_ctl(S);
   if (st) return st;
   // delta3d_mobile.rs:422-432 (f32 arithmetic)
@@ -347,6 +467,7 @@ fsp_status backward_error(fsp_solver* S, const double* p, double& be) {
   Launch K(S);
   FSP_LAUNCH(S, k_backward, dim3(grid_for(K.G.n)), K.st, K.G, p, S->b, S->r, S->ctl,
                      S->partials);
+  FSP_FINISH(S, K.st, OP_BACKWARD, dim3(grid_for(K.G.n)), true, 0);
   fsp_status st = sync_ctl(S);
   if (st) return st;
   be = S->hctl->red[OP_BACKWARD][0];
@@ -382,16 +503,20 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
   build_geometry(S, true);
   FSP_LAUNCH(S, k_assemble, dim3(grid_for(G.n)), K.st, G, us, vs, ws, (const float*)nullptr,
                      scale, S->b, (float*)nullptr, S->ctl, S->partials);
+  FSP_FINISH(S, K.st, OP_B2, dim3(grid_for(G.n)), false, 0);
   if (S->opt.gauge) {
     FSP_LAUNCH(S, k_sum_active<float>, dim3(grid_for(G.n)), K.st, G, (const float*)S->b,
                        OP_MEAN_B, S->ctl, S->partials);
+    FSP_FINISH(S, K.st, OP_MEAN_B, dim3(grid_for(G.n)), false, 0);
     FSP_LAUNCH(S, k_sub_mean<float>, dim3(grid_for(G.n)), K.st, G, S->b, 0, OP_B2, S->ctl,
                        S->partials);
+    FSP_FINISH(S, K.st, OP_B2, dim3(grid_for(G.n)), false, 0);
   }
   double* p64 = S->p64;
   FSP_LAUNCH(S, k_load_p, dim3(grid_for(G.n)), K.st, G, (const float*)p, p64, warm);
   FSP_LAUNCH(S, k_residual, dim3(grid_for(G.n)), K.st, G, (const double*)p64, S->b, S->r, OP_RR,
                      S->ctl, S->partials);
+  FSP_FINISH(S, K.st, OP_RR, dim3(grid_for(G.n)), false, 0);
   fsp_status st = sync_ctl(S);
   if (st) return st;
   if (S->hctl->nonfinite) return FSP_NOT_FINITE;
@@ -459,6 +584,7 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
     // True residual, recomputed: it decides, never the recurrence (:546-553).
     FSP_LAUNCH(S, k_residual, dim3(grid_for(G.n)), K.st, G, (const double*)p64, S->b, S->r,
                        OP_RR, S->ctl, S->partials);
+    FSP_FINISH(S, K.st, OP_RR, dim3(grid_for(G.n)), false, 0);
     if ((st = sync_ctl(S))) return st;
     if (S->hctl->nonfinite) return FSP_NOT_FINITE;
     const double actual = S->hctl->actual;
@@ -571,6 +697,7 @@ void fsp_default_options(fsp_options* o) {
   o->replace_every = 0;
   o->refuse_degraded = 0;
   o->device = 0;
+  o->coarse_operator = FSP_COARSE_REDISCRETISED;
 }
 
 fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** out) {
@@ -587,6 +714,7 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
       opt.post_sweeps > 64 || opt.coarse_sweeps > 1024)
     return FSP_INVALID;
   if (opt.vcycle_storage != FSP_STORE_F32) return FSP_INVALID;
+  if (opt.coarse_operator < 0 || opt.coarse_operator > 1) return FSP_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || opt.device < 0 || opt.device >= ndev) return FSP_DEVICE;
   fsp_solver* S = new (std::nothrow) fsp_solver();
@@ -606,7 +734,8 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
   S->ncols = nx * ny;
   if ((st = dalloc(S, &S->solid, S->n)) || (st = dalloc(S, &S->ou, S->nu)) || (st = dalloc(S, &S->ov, S->nv)) ||
       (st = dalloc(S, &S->ow, S->nw)) || (st = dalloc(S, &S->mask, S->n)) || (st = dalloc(S, &S->eta, S->ncols)) ||
-      (st = dalloc(S, &S->eta_rt, S->ncols)))
+      (st = dalloc(S, &S->eta_rt, S->ncols)) || (st = dalloc(S, &S->gk0, S->ncols)) ||
+      (st = dalloc(S, &S->gtab, S->ncols * (size_t)G_SLOTS * 8)))
     return fail(st);
   float** cellbufs[] = {&S->b, &S->r, &S->p, &S->q, &S->z, &S->zt, &S->d0, &S->d1};
   for (float** bp : cellbufs) if ((st = dalloc(S, bp, S->n))) return fail(st);
@@ -614,7 +743,7 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
   if ((st = dalloc(S, &S->us, S->nu)) || (st = dalloc(S, &S->vs, S->nv)) || (st = dalloc(S, &S->ws, S->nw)) ||
       (st = dalloc(S, &S->u, S->nu)) || (st = dalloc(S, &S->v, S->nv)) || (st = dalloc(S, &S->w, S->nw)))
     return fail(st);
-  if ((st = dalloc(S, &S->ctl, 1)) || (st = dalloc(S, &S->partials, (size_t)NUM_SLOTS * MAX_BLOCKS * 2)) ||
+  if ((st = dalloc(S, &S->ctl, 1)) || (st = dalloc(S, &S->partials, (size_t)NUM_SLOTS * MAX_PARTIALS * 2)) ||
       (st = dalloc(S, &S->hashes, (S->n + 1023) / 1024)) || (st = dalloc(S, &S->hash_out, 1)))
     return fail(st);
   if (hipMemset(S->ctl, 0, sizeof(Ctl)) != hipSuccess) return fail(FSP_DEVICE);
@@ -634,11 +763,38 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
       if ((st = dalloc(S, &L.kind, L.n)) || (st = dalloc(S, &L.ou, (x + 1) * y * z)) ||
           (st = dalloc(S, &L.ov, x * (y + 1) * z)) || (st = dalloc(S, &L.ow, x * y * (z + 1))) ||
           (st = dalloc(S, &L.diag, L.n)) || (st = dalloc(S, &L.x, L.n)) || (st = dalloc(S, &L.r, L.n)) ||
-          (st = dalloc(S, &L.t, L.n)))
+          (st = dalloc(S, &L.t, L.n)) || (st = dalloc(S, &L.dd, L.n)))
         return fail(st);
+      L.galerkin = opt.coarse_operator == FSP_COARSE_GALERKIN ? 1 : 0;
       S->levels.push_back(L);
     }
   }
+  // fast paths
+  auto make_q = [](int nq, int rows, int planes) {
+    QuadGrid Q{};
+    int qx = 1;
+    while (qx < nq && qx < 64) qx <<= 1;
+    Q.nq = nq; Q.QX = qx; Q.RY = QBLOCK / qx;
+    Q.nbx = (nq + qx - 1) / qx;
+    Q.nby = (rows + Q.RY - 1) / Q.RY;
+    Q.rows = rows; Q.planes = planes;
+    return Q;
+  };
+  if (nx % 4 == 0) {
+    S->qfine = make_q((int)nx / 4, (int)ny, (int)nz);
+    S->qrestr = make_q((int)nx / 4, (int)ny / 2, (int)nz / 2);
+    const size_t blocks = (size_t)S->qfine.nbx * S->qfine.nby * S->qfine.planes;
+    S->quad = blocks <= (size_t)MAX_PARTIALS && (S->n / 4 + QBLOCK - 1) / QBLOCK <= (size_t)MAX_PARTIALS;
+  }
+  S->tail_first = (int)S->levels.size();
+  for (int t = (int)S->levels.size() - 1; t >= 0; --t) {
+    if (S->levels[t].n > 4096 || (int)S->levels.size() - t > TAIL_MAX_LEVELS) break;
+    const size_t bytes = tail_lds_bytes(&S->levels[t], (int)S->levels.size() - t, opt.smoother);
+    if (bytes > 64 * 1024) break;
+    S->tail_first = t;
+    S->tail_lds = bytes;
+  }
+  S->zres = S->z;
   *out = S;
   return FSP_OK;
 }
@@ -784,6 +940,7 @@ fsp_status fsp_assemble_device(fsp_solver* S, const float* d_div, float scale, f
   Launch K(S);
   FSP_LAUNCH(S, k_assemble, dim3(grid_for(K.G.n)), K.st, K.G, (const float*)nullptr,
                      (const float*)nullptr, (const float*)nullptr, d_div, scale, d_rhs, d_prec, S->ctl, S->partials);
+  FSP_FINISH(S, K.st, OP_B2, dim3(grid_for(K.G.n)), false, 0);
   FSP_HIP(hipStreamSynchronize(S->stream));
   return FSP_OK;
 }
@@ -797,7 +954,7 @@ fsp_status fsp_precondition_device(fsp_solver* S, const float* d_r, float* d_z) 
   FSP_HIP(hipMemcpyAsync(S->r, d_r, S->n * 4, hipMemcpyDeviceToDevice, S->stream));
   K.set_pass(1, 0.0, 0.0, 0.0, 0, 1);
   K.precondition(OP_GENERIC);
-  FSP_HIP(hipMemcpyAsync(d_z, S->z, S->n * 4, hipMemcpyDeviceToDevice, S->stream));
+  FSP_HIP(hipMemcpyAsync(d_z, S->zres, S->n * 4, hipMemcpyDeviceToDevice, S->stream));
   FSP_HIP(hipStreamSynchronize(S->stream));
   return FSP_OK;
 }

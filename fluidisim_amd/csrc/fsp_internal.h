@@ -18,6 +18,8 @@ enum : uint8_t {
 constexpr int BLOCK = 256;        // threads per workgroup (4 wavefronts of 64)
 constexpr int MAX_BLOCKS = 2048;  // grid cap of grid-stride kernels (8 per CU)
 constexpr int NUM_SLOTS = 16;     // reduction slots
+constexpr int MAX_PARTIALS = 65536;  // blocks a reducing kernel may have
+constexpr int G_SLOTS = 8;        // ghost rows tabulated per column
 
 // Reduction slots / finish operations.
 enum Op : int {
@@ -69,6 +71,9 @@ struct Fine {
   const float* eta;      // columns
   const float* eta_rt;   // columns
   const uint8_t* mask;   // cells (row codes)
+  const uint16_t* gk0;   // columns: lowest ghost row (0xFFFF: none)
+  const float* gtab;     // columns x G_SLOTS x 8: 1/theta of the ghost faces
+  float prec_tab[7];     // 1/(n * inv) for a plain row with n faces
 };
 
 // One coarse level (delta3d_multigrid.rs:37-52).
@@ -84,6 +89,8 @@ struct Level {
   float* x;
   float* r;
   float* t;
+  float* dd;             // Galerkin levels: Dirichlet part of the diagonal (units of inv)
+  int galerkin;          // 1: P^T A P / 16 coarse operator, 0: re-discretised
 };
 
 }  // namespace fsp

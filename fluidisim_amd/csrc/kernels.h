@@ -55,63 +55,29 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-__device__ __forceinline__ double load_agent(const double* p) {
-  unsigned long long bits =
-      __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double((long long)bits);
-}
-
-// Grid-wide reduction of two values. Every block writes its partial; the last
-// block to arrive (counted with a device-scope atomic) folds the partials in
-// block order, so the result does not depend on scheduling. Returns true in
-// thread 0 of the last block, with the totals in a and b.
+// Block-level reduction of two values into this block's slot of `partials`.
+// The grid total is folded by k_finish, launched right after the producing
+// kernel: no atomics and no device-wide fences on the hot path, and the
+// totals do not depend on scheduling.
 template <bool MAX>
-__device__ bool grid_reduce2(double& a, double& b, int slot, Ctl* ctl, double* partials) {
+__device__ __forceinline__ void block_partial(double a, double b, int slot, double* partials) {
   __shared__ double sa[BLOCK / 64], sb[BLOCK / 64];
-  __shared__ int last;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
   a = MAX ? wave_max(a) : wave_sum(a);
   b = MAX ? wave_max(b) : wave_sum(b);
   if (lane == 0) { sa[wid] = a; sb[wid] = b; }
   __syncthreads();
-  double* part = partials + (size_t)slot * MAX_BLOCKS * 2;
   if (threadIdx.x == 0) {
     double ta = sa[0], tb = sb[0];
-    for (int w = 1; w < BLOCK / 64; ++w) {
+    for (int w = 1; w < nw; ++w) {
       ta = MAX ? fmax(ta, sa[w]) : ta + sa[w];
       tb = MAX ? fmax(tb, sb[w]) : tb + sb[w];
     }
+    double* part = partials + (size_t)slot * MAX_PARTIALS * 2;
     part[2 * blockIdx.x] = ta;
     part[2 * blockIdx.x + 1] = tb;
-    __threadfence();
-    const unsigned prev = atomicAdd(&ctl->counters[slot], 1u);
-    last = (prev == gridDim.x - 1) ? 1 : 0;
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  double pa = 0.0, pb = 0.0;
-  for (unsigned q = threadIdx.x; q < gridDim.x; q += BLOCK) {
-    const double xa = load_agent(&part[2 * q]), xb = load_agent(&part[2 * q + 1]);
-    pa = MAX ? fmax(pa, xa) : pa + xa;
-    pb = MAX ? fmax(pb, xb) : pb + xb;
-  }
-  pa = MAX ? wave_max(pa) : wave_sum(pa);
-  pb = MAX ? wave_max(pb) : wave_sum(pb);
-  __syncthreads();
-  if (lane == 0) { sa[wid] = pa; sb[wid] = pb; }
-  __syncthreads();
-  if (threadIdx.x != 0) return false;
-  a = sa[0];
-  b = sb[0];
-  for (int w = 1; w < BLOCK / 64; ++w) {
-    a = MAX ? fmax(a, sa[w]) : a + sa[w];
-    b = MAX ? fmax(b, sb[w]) : b + sb[w];
-  }
-  ctl->counters[slot] = 0u;
-  ctl->red[slot][0] = a;
-  ctl->red[slot][1] = b;
-  return true;
 }
 
 // Scalar updates of the conjugate gradient, done on the device by the last
@@ -173,7 +139,47 @@ __device__ void finish(int op, Ctl* ctl, double a, double b) {
 
 template <bool MAX>
 __device__ __forceinline__ void reduce_finish(double a, double b, int op, Ctl* ctl, double* partials) {
-  if (grid_reduce2<MAX>(a, b, op, ctl, partials)) finish(op, ctl, a, b);
+  (void)ctl;
+  block_partial<MAX>(a, b, op, partials);
+}
+
+// Folds the partials of `nblocks` blocks in block order and applies the
+// finish operation. `gated`: skipped like its producer once the loop stopped.
+template <bool MAX>
+__global__ void __launch_bounds__(BLOCK) k_finish(int op, unsigned nblocks, int gated, Ctl* ctl, double* partials) {
+  if (gated && !ctl->active) return;
+  __shared__ double sa[BLOCK / 64], sb[BLOCK / 64];
+  const double2* part = reinterpret_cast<const double2*>(partials + (size_t)op * MAX_PARTIALS * 2);
+  // eight independent loads in flight per step, folded in a fixed order
+  double a = 0.0, b = 0.0;
+  for (unsigned base = threadIdx.x; base < nblocks; base += 8 * BLOCK) {
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const unsigned q = base + (unsigned)u * BLOCK;
+      v[u] = q < nblocks ? part[q] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      a = MAX ? fmax(a, v[u].x) : a + v[u].x;
+      b = MAX ? fmax(b, v[u].y) : b + v[u].y;
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  a = MAX ? wave_max(a) : wave_sum(a);
+  b = MAX ? wave_max(b) : wave_sum(b);
+  if (lane == 0) { sa[wid] = a; sb[wid] = b; }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  a = sa[0];
+  b = sb[0];
+  for (int w = 1; w < BLOCK / 64; ++w) {
+    a = MAX ? fmax(a, sa[w]) : a + sa[w];
+    b = MAX ? fmax(b, sb[w]) : b + sb[w];
+  }
+  ctl->red[op][0] = a;
+  ctl->red[op][1] = b;
+  finish(op, ctl, a, b);
 }
 
 // ---------------------------------------------------------------- fine geometry
@@ -254,49 +260,41 @@ __device__ void full_row(const Fine& G, int i, int j, int k, RowG& R) {
 template <class T>
 __device__ __forceinline__ float ldv(const T* p, size_t i) { return (float)p[i]; }
 
-// (A x)_c (apply_mobile3, delta3d_mobile.rs:190-213) and the Jacobi
-// preconditioner prec_c = 1/(diag * inv) (rhs_mobile3, :215-241).
-template <class T>
-__device__ __forceinline__ float apply_prec(const Fine& G, int i, int j, int k, size_t c, uint8_t m, const T* x,
-                                            float& prec) {
-#pragma clang fp contract(off)
-  prec = 0.f;
-  if (!(m & M_WET)) return 0.f;
-  const size_t plane = (size_t)G.nx * G.ny;
-  const float pc = ldv(x, c);
-  float acc = 0.f, diag = 0.f;
-  if (!(m & M_GHOST)) {
-    if (m & M_XM) { acc += pc - ldv(x, c - 1); diag += 1.0f; }
-    if (m & M_XP) { acc += pc - ldv(x, c + 1); diag += 1.0f; }
-    if (m & M_YM) { acc += pc - ldv(x, c - G.nx); diag += 1.0f; }
-    if (m & M_YP) { acc += pc - ldv(x, c + G.nx); diag += 1.0f; }
-    if (m & M_ZM) { acc += pc - ldv(x, c - plane); diag += 1.0f; }
-    if (m & M_ZP) { acc += pc - ldv(x, c + plane); diag += 1.0f; }
-  } else {
-    RowG R;
-    full_row(G, i, j, k, R);
-    const size_t nb[6] = {c - 1, c + 1, c - G.nx, c + G.nx, c - plane, c + plane};
+// 1/theta of the six faces of a ghost row, from the per-column table built
+// once per projection (k_ghost_table); false if the row is outside the
+// column's table, in which case the caller derives it from the geometry.
+__device__ __forceinline__ bool ghost_its(const Fine& G, int i, int j, int k, float it[6]) {
+  const size_t col = (size_t)j * G.nx + i;
+  const int slot = k - (int)G.gk0[col];
+  if (slot < 0 || slot >= G_SLOTS) return false;
+  const float* e = G.gtab + (col * G_SLOTS + slot) * 8;
+  const float4 a = *reinterpret_cast<const float4*>(e);
+  const float2 b = *reinterpret_cast<const float2*>(e + 4);
+  it[0] = a.x; it[1] = a.y; it[2] = a.z; it[3] = a.w; it[4] = b.x; it[5] = b.y;
+  return true;
+}
+
+// Per-face coefficients of a row with ghost faces: type 1 neighbour, 2 ghost.
+__device__ __forceinline__ void ghost_row(const Fine& G, int i, int j, int k, uint8_t m, uint8_t type[6], float it[6]) {
+  if (ghost_its(G, i, j, k, it)) {
 #pragma unroll
-    for (int f = 0; f < 6; ++f) {
-      if (R.type[f] == 1) { acc += pc - ldv(x, nb[f]); diag += 1.0f; }
-      else if (R.type[f] == 2) { acc += pc * R.it[f]; diag += R.it[f]; }
-    }
+    for (int f = 0; f < 6; ++f) type[f] = (m & (1u << f)) ? 1 : (it[f] > 0.f ? 2 : 0);
+    return;
   }
-  prec = diag > 0.f ? 1.0f / (diag * G.inv) : 0.f;
-  return acc * G.inv;
+  RowG R;
+  full_row(G, i, j, k, R);
+#pragma unroll
+  for (int f = 0; f < 6; ++f) { type[f] = R.type[f]; it[f] = R.it[f]; }
 }
 
-template <class T>
-__device__ __forceinline__ float apply_row(const Fine& G, int i, int j, int k, size_t c, uint8_t m, const T* x) {
-  float prec;
-  return apply_prec(G, i, j, k, c, m, x, prec);
-}
-
-// (A x)_c in accumulator precision A from storage T: the f64 form computes
-// the true residual of the f64 pressure (b - A p), the f32 form is apply_row.
+// (A x)_c in accumulator precision A from storage T, in the reference's face
+// order (apply_mobile3, delta3d_mobile.rs:190-213), and prec as rhs_mobile3
+// computes it (:227-237).
 template <class A, class T>
-__device__ __forceinline__ A apply_row_t(const Fine& G, int i, int j, int k, size_t c, uint8_t m, const T* x) {
+__device__ __forceinline__ A apply_row_t(const Fine& G, int i, int j, int k, size_t c, uint8_t m, const T* x,
+                                         float* prec_out = nullptr) {
 #pragma clang fp contract(off)
+  if (prec_out) *prec_out = 0.f;
   if (!(m & M_WET)) return (A)0;
   const size_t plane = (size_t)G.nx * G.ny;
   const A pc = (A)x[c];
@@ -308,33 +306,46 @@ __device__ __forceinline__ A apply_row_t(const Fine& G, int i, int j, int k, siz
     if (m & M_YP) acc += pc - (A)x[c + G.nx];
     if (m & M_ZM) acc += pc - (A)x[c - plane];
     if (m & M_ZP) acc += pc - (A)x[c + plane];
+    if (prec_out) *prec_out = G.prec_tab[__popc(m & 0x3Fu)];
   } else {
-    RowG R;
-    full_row(G, i, j, k, R);
+    uint8_t type[6];
+    float it[6];
+    ghost_row(G, i, j, k, m, type, it);
     const size_t nb[6] = {c - 1, c + 1, c - G.nx, c + G.nx, c - plane, c + plane};
+    float diag = 0.f;
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-      if (R.type[f] == 1) acc += pc - (A)x[nb[f]];
-      else if (R.type[f] == 2) acc += pc * (A)R.it[f];
+      if (type[f] == 1) { acc += pc - (A)x[nb[f]]; diag += 1.0f; }
+      else if (type[f] == 2) { acc += pc * (A)it[f]; diag += it[f]; }
     }
+    if (prec_out) *prec_out = diag > 0.f ? 1.0f / (diag * G.inv) : 0.f;
   }
   return acc * (A)G.inv;
+}
+
+template <class T>
+__device__ __forceinline__ float apply_prec(const Fine& G, int i, int j, int k, size_t c, uint8_t m, const T* x,
+                                            float& prec) {
+  return apply_row_t<float>(G, i, j, k, c, m, x, &prec);
+}
+
+template <class T>
+__device__ __forceinline__ float apply_row(const Fine& G, int i, int j, int k, size_t c, uint8_t m, const T* x) {
+  return apply_row_t<float>(G, i, j, k, c, m, x);
 }
 
 // The Jacobi preconditioner alone (rhs_mobile3's prec, delta3d_mobile.rs:227-237).
 __device__ __forceinline__ float row_prec(const Fine& G, int i, int j, int k, uint8_t m) {
 #pragma clang fp contract(off)
   if (!(m & M_WET)) return 0.f;
+  if (!(m & M_GHOST)) return G.prec_tab[__popc(m & 0x3Fu)];
+  uint8_t type[6];
+  float it[6];
+  ghost_row(G, i, j, k, m, type, it);
   float diag = 0.f;
-  if (!(m & M_GHOST)) {
-    for (int f = 0; f < 6; ++f) if (m & (1u << f)) diag += 1.0f;
-  } else {
-    RowG R;
-    full_row(G, i, j, k, R);
-    for (int f = 0; f < 6; ++f) {
-      if (R.type[f] == 1) diag += 1.0f;
-      else if (R.type[f] == 2) diag += R.it[f];
-    }
+  for (int f = 0; f < 6; ++f) {
+    if (type[f] == 1) diag += 1.0f;
+    else if (type[f] == 2) diag += it[f];
   }
   return diag > 0.f ? 1.0f / (diag * G.inv) : 0.f;
 }
@@ -379,6 +390,36 @@ __global__ void __launch_bounds__(BLOCK) k_mask(Fine G, uint8_t* mask, Ctl* ctl,
     mask[c] = m;
   }
   reduce_finish<false>(wet, ghosts, OP_COUNT, ctl, partials);
+}
+
+// Per-column table of the ghost rows' 1/theta (six faces, 0 where the face
+// is not a ghost), for the G_SLOTS rows from the column's lowest ghost row:
+// the operator reads it instead of re-deriving the surface geometry. Rows
+// past the table fall back to full_row.
+__global__ void __launch_bounds__(BLOCK) k_ghost_table(Fine G, uint16_t* gk0, float* gtab) {
+  const size_t cols = (size_t)G.nx * G.ny;
+  const size_t plane = cols;
+  FSP_FOR(col, cols) {
+    const int i = (int)(col % (size_t)G.nx), j = (int)(col / (size_t)G.nx);
+    int k0 = -1;
+    for (int k = 0; k < G.nz; ++k)
+      if (G.mask[(size_t)k * plane + col] & M_GHOST) { k0 = k; break; }
+    gk0[col] = (uint16_t)(k0 < 0 ? 0xFFFF : k0);
+    if (k0 < 0) continue;
+    for (int s = 0; s < G_SLOTS && k0 + s < G.nz; ++s) {
+      const int k = k0 + s;
+      float* e = gtab + (col * G_SLOTS + s) * 8;
+      float it[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      if (G.mask[(size_t)k * plane + col] & M_GHOST) {
+        RowG R;
+        full_row(G, i, j, k, R);
+        for (int f = 0; f < 6; ++f) it[f] = R.type[f] == 2 ? R.it[f] : 0.f;
+      }
+      for (int f = 0; f < 6; ++f) e[f] = it[f];
+      e[6] = 0.f;
+      e[7] = 0.f;
+    }
+  }
 }
 
 // b = scale*div + ghost terms; prec (optional). div from faces or given.
@@ -632,13 +673,14 @@ __device__ __forceinline__ float coarse_row(const Level& L, int i, int j, int k,
   float acc = 0.f;
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   const size_t plane = (size_t)nx * ny;
+  const bool gal = L.galerkin != 0;
   auto face = [&](float a, bool has, size_t n, bool top) {
     if (a == 0.f) return;
     if (has) {
       const int8_t kn = L.kind[n];
       if (kn > 0) acc += x ? a * (pc - x[n]) : a;
-      else if (kn < 0) acc += 2.0f * a * (x ? pc : 1.0f);
-    } else if (top) {
+      else if (!gal && kn < 0) acc += 2.0f * a * (x ? pc : 1.0f);
+    } else if (!gal && top) {
       acc += 2.0f * a * (x ? pc : 1.0f);
     }
   };
@@ -648,6 +690,7 @@ __device__ __forceinline__ float coarse_row(const Level& L, int i, int j, int k,
   face(L.ov[((size_t)k * (ny + 1) + j + 1) * nx + i], j + 1 < ny, c + nx, false);
   face(L.ow[((size_t)k * ny + j) * nx + i], k > 0, c - plane, false);
   face(L.ow[((size_t)(k + 1) * ny + j) * nx + i], k + 1 < nz, c + plane, true);
+  if (gal) acc += x ? L.dd[c] * pc : L.dd[c];
   return acc * L.inv;
 }
 
@@ -736,6 +779,62 @@ __global__ void __launch_bounds__(BLOCK) k_cfaces(Level F, Level L) {
         L, f, [&](size_t i, size_t j, size_t k) { return F.ou[(k * fny + j) * (fnx + 1) + i]; },
         [&](size_t i, size_t j, size_t k) { return F.ov[(k * (fny + 1) + j) * fnx + i]; },
         [&](size_t i, size_t j, size_t k) { return F.ow[(k * fny + j) * fnx + i]; });
+  }
+}
+// Galerkin coarse levels (P^T A P / 16, P piecewise constant on wet cells):
+// a coarse face carries a quarter of the couplings of its four fine faces
+// (a fine face couples when both its cells are wet and it is open), and the
+// Dirichlet part of the diagonal is a quarter of the children's ghost terms.
+__global__ void __launch_bounds__(BLOCK) k_cfaces_fine_gal(Fine G, Level L) {
+  const size_t cx = L.nx, cy = L.ny, cz = L.nz;
+  const size_t total = (cx + 1) * cy * cz + cx * (cy + 1) * cz + cx * cy * (cz + 1);
+  FSP_FOR(f, total) {
+    coarse_faces(
+        L, f,
+        [&](size_t i, size_t j, size_t k) {
+          return (i > 0 && i < (size_t)G.nx && (G.mask[cidx(G, (int)i, (int)j, (int)k)] & M_XM)) ? 1.0f : 0.0f;
+        },
+        [&](size_t i, size_t j, size_t k) {
+          return (j > 0 && j < (size_t)G.ny && (G.mask[cidx(G, (int)i, (int)j, (int)k)] & M_YM)) ? 1.0f : 0.0f;
+        },
+        [&](size_t i, size_t j, size_t k) {
+          return (k > 0 && k < (size_t)G.nz && (G.mask[cidx(G, (int)i, (int)j, (int)k)] & M_ZM)) ? 1.0f : 0.0f;
+        });
+  }
+}
+__global__ void __launch_bounds__(BLOCK) k_cdd_fine(Fine G, Level L) {
+#pragma clang fp contract(off)
+  FSP_FOR(c, L.n) {
+    int i, j, k;
+    ldecode(L, c, i, j, k);
+    float s = 0.f;
+    for (int ch = 0; ch < 8; ++ch) {
+      const int fi = 2 * i + (ch & 1), fj = 2 * j + ((ch >> 1) & 1), fk = 2 * k + ((ch >> 2) & 1);
+      const uint8_t m = G.mask[cidx(G, fi, fj, fk)];
+      float g = 0.f;
+      if ((m & M_WET) && (m & M_GHOST)) {
+        uint8_t type[6];
+        float it[6];
+        ghost_row(G, fi, fj, fk, m, type, it);
+        for (int f = 0; f < 6; ++f)
+          if (type[f] == 2) g += it[f];
+      }
+      s += g;
+    }
+    L.dd[c] = 0.25f * s;
+  }
+}
+__global__ void __launch_bounds__(BLOCK) k_cdd(Level F, Level L) {
+#pragma clang fp contract(off)
+  FSP_FOR(c, L.n) {
+    int i, j, k;
+    ldecode(L, c, i, j, k);
+    float s = 0.f;
+    for (int ch = 0; ch < 8; ++ch) {
+      const int fi = 2 * i + (ch & 1), fj = 2 * j + ((ch >> 1) & 1), fk = 2 * k + ((ch >> 2) & 1);
+      s += F.dd[lidx(F, fi, fj, fk)];
+    }
+    L.dd[c] = 0.25f * s;
   }
 }
 __global__ void __launch_bounds__(BLOCK) k_cdiag(Level L) {

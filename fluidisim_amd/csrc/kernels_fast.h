@@ -1,0 +1,549 @@
+// Fast path of the fine level and of the small coarse levels.
+//
+// Fine level ("quad" kernels, nx % 4 == 0): one thread owns four consecutive
+// cells along x, so every lane of a wavefront is busy in both colours of a
+// red-black pass and every access is a 16-byte vector access; x neighbours
+// across quads come from the adjacent lane by wavefront shuffle, y and z
+// neighbours are vector loads served mostly by L2. The 1-D grid is mapped to
+// (x-segment, y-block, z) tiles so that each XCD walks a contiguous range of
+// planes and its stencil neighbours stay in its own L2. Per-cell arithmetic
+// is the fine operator's (face order x-, x+, y-, y+, z-, z+, no contraction),
+// so results equal the generic kernels bit for bit.
+//
+// Small coarse levels: one workgroup runs every level below a size threshold
+// with the vectors and coefficients in LDS, replacing dozens of tiny launches
+// per V-cycle by one.
+#pragma once
+#include "kernels.h"
+
+namespace fsp {
+
+constexpr int QBLOCK = 256;
+
+struct QuadGrid {
+  int nq;   // quads per row (nx / 4)
+  int QX;   // threads along x per block (power of two, <= 64)
+  int RY;   // rows per block (QBLOCK / QX)
+  int nbx;  // blocks along x
+  int nby;  // blocks along y (over `rows`)
+  int rows; // rows covered (ny, or ny/2 for the restriction)
+  int planes;
+};
+
+// Tile coordinates of this thread; returns false for lanes past the edge
+// (they still take part in shuffles).
+__device__ __forceinline__ bool quad_coords(const QuadGrid& Q, int& qx, int& y, int& z) {
+  const unsigned G = gridDim.x, b = blockIdx.x;
+  const unsigned L = ((G & 7u) == 0u) ? (b & 7u) * (G >> 3) + (b >> 3) : b;
+  const unsigned bx = L % (unsigned)Q.nbx;
+  const unsigned t = L / (unsigned)Q.nbx;
+  const unsigned by = t % (unsigned)Q.nby;
+  z = (int)(t / (unsigned)Q.nby);
+  qx = (int)bx * Q.QX + (int)(threadIdx.x & (unsigned)(Q.QX - 1));
+  y = (int)by * Q.RY + (int)(threadIdx.x / (unsigned)Q.QX);
+  return qx < Q.nq && y < Q.rows && z < Q.planes;
+}
+
+__device__ __forceinline__ float q_get(const float4& v, int e) { return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w; }
+__device__ __forceinline__ void q_set(float4& v, int e, float x) {
+  if (e == 0) v.x = x; else if (e == 1) v.y = x; else if (e == 2) v.z = x; else v.w = x;
+}
+__device__ __forceinline__ float4 ld4(const float* p, size_t c) { return *reinterpret_cast<const float4*>(p + c); }
+__device__ __forceinline__ void st4(float* p, size_t c, const float4& v) { *reinterpret_cast<float4*>(p + c) = v; }
+__device__ __forceinline__ float4 f4zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+
+// Row of the fine operator from neighbour values already in registers, in the
+// reference's face order; prec as rhs_mobile3 computes it.
+__device__ __forceinline__ float row_local(const Fine& G, int i, int j, int k, uint8_t m, float pc, const float nb[6],
+                                          float& prec) {
+#pragma clang fp contract(off)
+  prec = 0.f;
+  if (!(m & M_WET)) return 0.f;
+  float acc = 0.f;
+  if (!(m & M_GHOST)) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+      if (m & (1u << f)) acc += pc - nb[f];
+    prec = G.prec_tab[__popc(m & 0x3Fu)];
+  } else {
+    uint8_t type[6];
+    float it[6];
+    ghost_row(G, i, j, k, m, type, it);
+    float diag = 0.f;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      if (type[f] == 1) { acc += pc - nb[f]; diag += 1.0f; }
+      else if (type[f] == 2) { acc += pc * it[f]; diag += it[f]; }
+    }
+    prec = diag > 0.f ? 1.0f / (diag * G.inv) : 0.f;
+  }
+  return acc * G.inv;
+}
+
+// The four cells' x neighbours: within the quad, from the adjacent lanes, or
+// (at the edge of a lane segment) from memory through `edge`.
+// Every lane takes part in the shuffles; only valid lanes read memory.
+template <class Edge>
+__device__ __forceinline__ void x_neighbours(const QuadGrid& Q, bool valid, int qx, const float4& v, float& left,
+                                             float& right, Edge edge) {
+  const float l_sh = __shfl_up(v.w, 1, 64);
+  const float r_sh = __shfl_down(v.x, 1, 64);
+  const int lane_in_seg = (int)(threadIdx.x & (unsigned)(Q.QX - 1));
+  left = 0.f;
+  right = 0.f;
+  if (!valid) return;
+  if (qx > 0) left = lane_in_seg > 0 ? l_sh : edge(-1);
+  if (qx + 1 < Q.nq) right = (lane_in_seg < Q.QX - 1) ? r_sh : edge(4);
+}
+
+// ---------------------------------------------------------------- red-black pass
+// One colour of a red-black Gauss-Seidel sweep on quads, in place.
+// MODE 1: the first pass from z = 0 (colour cells get prec*r, the others 0).
+// MODE 2: z is first prolonged from the first coarse level (z += x_c of the
+//         parent, 0 on rows with prec == 0), then the colour is updated;
+//         neighbours are prolonged on the fly, so this mode reads z and
+//         writes zout (a different buffer).
+// dot_op >= 0: also reduce <r, z> over every cell into that slot.
+template <int MODE>
+__global__ void __launch_bounds__(QBLOCK) k_q_gs(Fine G, QuadGrid Q, const float* z, float* zout, const float* r,
+                                                 int colour, Level L0, const float* xc, int dot_op, Ctl* ctl,
+                                                 double* partials) {
+#pragma clang fp contract(off)
+  if (!ctl->active) return;
+  int qx, y, k;
+  const bool valid = quad_coords(Q, qx, y, k);
+  const size_t plane = (size_t)G.nx * G.ny;
+  const size_t c0 = valid ? ((size_t)k * G.ny + y) * G.nx + 4 * (size_t)qx : 0;
+  double s = 0.0, unused = 0.0;
+  if (MODE == 1) {
+    if (valid) {
+      const uint32_t mq = *reinterpret_cast<const uint32_t*>(G.mask + c0);
+      const float4 rq = ld4(r, c0);
+      float4 out = f4zero();
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (((e + y + k) & 1) != colour) continue;
+        const uint8_t m = (uint8_t)(mq >> (8 * e));
+        const float prec = row_prec(G, 4 * qx + e, y, k, m);
+        q_set(out, e, prec > 0.f ? prec * q_get(rq, e) : 0.f);
+      }
+      st4(zout, c0, out);
+      if (dot_op >= 0) s = (double)rq.x * out.x + (double)rq.y * out.y + (double)rq.z * out.z + (double)rq.w * out.w;
+    }
+    if (dot_op >= 0) reduce_finish<false>(s, unused, dot_op, ctl, partials);
+    return;
+  }
+  // parent values of the prolongation
+  auto parent = [&](int yy, int kk, int xq) -> float2 {
+    return *reinterpret_cast<const float2*>(xc + lidx(L0, 2 * xq, yy >> 1, kk >> 1));
+  };
+  float4 zc = f4zero(), rq = f4zero();
+  uint32_t mq = 0;
+  float4 ym = f4zero(), yp = f4zero(), zm = f4zero(), zp = f4zero();
+  if (valid) {
+    mq = *reinterpret_cast<const uint32_t*>(G.mask + c0);
+    zc = ld4(z, c0);
+    rq = ld4(r, c0);
+    if (y > 0) ym = ld4(z, c0 - G.nx);
+    if (y + 1 < G.ny) yp = ld4(z, c0 + G.nx);
+    if (k > 0) zm = ld4(z, c0 - plane);
+    if (k + 1 < G.nz) zp = ld4(z, c0 + plane);
+    if (MODE == 2) {
+      float2 pc = parent(y, k, qx);
+      zc.x += pc.x; zc.y += pc.x; zc.z += pc.y; zc.w += pc.y;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (!prec_positive((uint8_t)(mq >> (8 * e)))) q_set(zc, e, 0.f);
+      if (y > 0) { float2 a = parent(y - 1, k, qx); ym.x += a.x; ym.y += a.x; ym.z += a.y; ym.w += a.y; }
+      if (y + 1 < G.ny) { float2 a = parent(y + 1, k, qx); yp.x += a.x; yp.y += a.x; yp.z += a.y; yp.w += a.y; }
+      if (k > 0) { float2 a = parent(y, k - 1, qx); zm.x += a.x; zm.y += a.x; zm.z += a.y; zm.w += a.y; }
+      if (k + 1 < G.nz) { float2 a = parent(y, k + 1, qx); zp.x += a.x; zp.y += a.x; zp.z += a.y; zp.w += a.y; }
+    }
+  }
+  float left, right;
+  x_neighbours(Q, valid, qx, zc, left, right, [&](int off) {
+    float v = z[c0 + off];
+    if (MODE == 2) {
+      const int xx = 4 * qx + off;
+      v += xc[lidx(L0, xx >> 1, y >> 1, k >> 1)];
+    }
+    return v;
+  });
+  if (valid) {
+    float4 out = zc;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (((e + y + k) & 1) != colour) continue;
+      const uint8_t m = (uint8_t)(mq >> (8 * e));
+      const float pc = q_get(zc, e);
+      const float xl = e == 0 ? left : q_get(zc, e - 1);
+      const float xr = e == 3 ? right : q_get(zc, e + 1);
+      const float nb[6] = {xl, xr, q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
+      float prec;
+      const float az = row_local(G, 4 * qx + e, y, k, m, pc, nb, prec);
+      if (prec > 0.f) q_set(out, e, pc + prec * (q_get(rq, e) - az));
+    }
+    st4(zout, c0, out);
+    if (dot_op >= 0) s = (double)rq.x * out.x + (double)rq.y * out.y + (double)rq.z * out.z + (double)rq.w * out.w;
+  }
+  if (dot_op >= 0) reduce_finish<false>(s, unused, dot_op, ctl, partials);
+}
+
+// ---------------------------------------------------------------- residual + restriction
+// One thread per (fine quad column qx, coarse row J, coarse plane K): the
+// residual t = r - A z of its 16 fine cells, summed per coarse cell in the
+// reference's child order and scaled by 1/8 (restrict3, delta3d_multigrid.rs:258-272).
+__global__ void __launch_bounds__(QBLOCK) k_q_restrict(Fine G, QuadGrid Q, const float* z, const float* r, Level L0,
+                                                       Ctl* ctl) {
+#pragma clang fp contract(off)
+  if (!ctl->active) return;
+  int qx, J, K;
+  const bool valid = quad_coords(Q, qx, J, K);
+  const size_t plane = (size_t)G.nx * G.ny;
+  float t[2][2][4];  // [dz][dy][e]
+#pragma unroll
+  for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy) {
+      const int y = 2 * J + dy, k = 2 * K + dz;
+      const size_t c0 = valid ? ((size_t)k * G.ny + y) * G.nx + 4 * (size_t)qx : 0;
+      float4 zc = f4zero(), rq = f4zero(), ym = f4zero(), yp = f4zero(), zm = f4zero(), zp = f4zero();
+      uint32_t mq = 0;
+      if (valid) {
+        mq = *reinterpret_cast<const uint32_t*>(G.mask + c0);
+        zc = ld4(z, c0);
+        rq = ld4(r, c0);
+        if (y > 0) ym = ld4(z, c0 - G.nx);
+        if (y + 1 < G.ny) yp = ld4(z, c0 + G.nx);
+        if (k > 0) zm = ld4(z, c0 - plane);
+        if (k + 1 < G.nz) zp = ld4(z, c0 + plane);
+      }
+      float left, right;
+      x_neighbours(Q, valid, qx, zc, left, right, [&](int off) { return z[c0 + off]; });
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const uint8_t m = (uint8_t)(mq >> (8 * e));
+        const float pc = q_get(zc, e);
+        const float xl = e == 0 ? left : q_get(zc, e - 1);
+        const float xr = e == 3 ? right : q_get(zc, e + 1);
+        const float nb[6] = {xl, xr, q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
+        float prec;
+        const float az = valid ? row_local(G, 4 * qx + e, y, k, m, pc, nb, prec) : 0.f;
+        t[dz][dy][e] = q_get(rq, e) - az;
+      }
+    }
+  if (!valid) return;
+  float2 out;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    float s = 0.f;
+    s += t[0][0][2 * h]; s += t[0][0][2 * h + 1];
+    s += t[0][1][2 * h]; s += t[0][1][2 * h + 1];
+    s += t[1][0][2 * h]; s += t[1][0][2 * h + 1];
+    s += t[1][1][2 * h]; s += t[1][1][2 * h + 1];
+    if (h == 0) out.x = 0.125f * s; else out.y = 0.125f * s;
+  }
+  *reinterpret_cast<float2*>(L0.r + lidx(L0, 2 * qx, J, K)) = out;
+}
+
+// ---------------------------------------------------------------- CG steps on quads
+// d = z (first) or z + beta d_old, q = A d, <d, q> (delta3d_mobile.rs:520-523, :460-465).
+__global__ void __launch_bounds__(QBLOCK) k_q_spmv_dir(Fine G, QuadGrid Q, const float* z, float* d0, float* d1,
+                                                       float* q, Ctl* ctl, double* partials) {
+#pragma clang fp contract(off)
+  if (!ctl->active) return;
+  const int sel = ctl->dsel, first = ctl->first_dir;
+  const float beta = (float)ctl->beta;
+  const float* dold = sel ? d1 : d0;
+  float* dnew = sel ? d0 : d1;
+  int qx, y, k;
+  const bool valid = quad_coords(Q, qx, y, k);
+  const size_t plane = (size_t)G.nx * G.ny;
+  const size_t c0 = valid ? ((size_t)k * G.ny + y) * G.nx + 4 * (size_t)qx : 0;
+  auto dir4 = [&](size_t c) {
+    float4 a = ld4(z, c);
+    if (!first) {
+      const float4 b = ld4(dold, c);
+      a.x = a.x + beta * b.x; a.y = a.y + beta * b.y; a.z = a.z + beta * b.z; a.w = a.w + beta * b.w;
+    }
+    return a;
+  };
+  float4 dc = f4zero(), ym = f4zero(), yp = f4zero(), zm = f4zero(), zp = f4zero();
+  uint32_t mq = 0;
+  if (valid) {
+    mq = *reinterpret_cast<const uint32_t*>(G.mask + c0);
+    dc = dir4(c0);
+    if (y > 0) ym = dir4(c0 - G.nx);
+    if (y + 1 < G.ny) yp = dir4(c0 + G.nx);
+    if (k > 0) zm = dir4(c0 - plane);
+    if (k + 1 < G.nz) zp = dir4(c0 + plane);
+  }
+  float left, right;
+  x_neighbours(Q, valid, qx, dc, left, right, [&](int off) {
+    const size_t c = c0 + off;
+    return first ? z[c] : z[c] + beta * dold[c];
+  });
+  double s = 0.0, unused = 0.0;
+  if (valid) {
+    float4 qo;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const uint8_t m = (uint8_t)(mq >> (8 * e));
+      const float pc = q_get(dc, e);
+      const float xl = e == 0 ? left : q_get(dc, e - 1);
+      const float xr = e == 3 ? right : q_get(dc, e + 1);
+      const float nb[6] = {xl, xr, q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
+      float prec;
+      q_set(qo, e, row_local(G, 4 * qx + e, y, k, m, pc, nb, prec));
+    }
+    st4(dnew, c0, dc);
+    st4(q, c0, qo);
+    s = (double)dc.x * qo.x + (double)dc.y * qo.y + (double)dc.z * qo.z + (double)dc.w * qo.w;
+  }
+  reduce_finish<false>(s, unused, OP_DQ, ctl, partials);
+}
+
+// p += alpha d (f64), r -= alpha q, |r|^2; four cells per thread.
+__global__ void __launch_bounds__(QBLOCK) k_q_update(size_t nquads, double* p, float* r, const float* d0,
+                                                     const float* d1, const float* q, Ctl* ctl, double* partials) {
+  if (!ctl->active) return;
+  const double alpha64 = ctl->alpha;
+  const float alpha = (float)alpha64;
+  const float* d = ctl->dsel ? d1 : d0;
+  double s = 0.0, unused = 0.0;
+  const size_t t = (size_t)blockIdx.x * QBLOCK + threadIdx.x;
+  if (t < nquads) {
+    const size_t c = 4 * t;
+    const float4 dv = ld4(d, c), qv = ld4(q, c);
+    float4 rv = ld4(r, c);
+    double2* pp = reinterpret_cast<double2*>(p + c);
+    double2 p0 = pp[0], p1 = pp[1];
+    p0.x += alpha64 * (double)dv.x; p0.y += alpha64 * (double)dv.y;
+    p1.x += alpha64 * (double)dv.z; p1.y += alpha64 * (double)dv.w;
+    pp[0] = p0;
+    pp[1] = p1;
+    rv.x = rv.x - alpha * qv.x; rv.y = rv.y - alpha * qv.y; rv.z = rv.z - alpha * qv.z; rv.w = rv.w - alpha * qv.w;
+    st4(r, c, rv);
+    s = (double)rv.x * rv.x + (double)rv.y * rv.y + (double)rv.z * rv.z + (double)rv.w * rv.w;
+  }
+  reduce_finish<false>(s, unused, OP_RN, ctl, partials);
+}
+
+// ---------------------------------------------------------------- small coarse levels in one workgroup
+constexpr int TAIL_THREADS = 1024;
+constexpr int TAIL_MAX_LEVELS = 8;
+
+struct TailSet {
+  Level lv[TAIL_MAX_LEVELS];  // global arrays of the tail levels (first .. last)
+  int count;
+  int smoother;               // FSP_SMOOTH_*
+  int pre, post, coarse;
+};
+
+// A tail level inside LDS: sizes and float offsets of its arrays.
+struct TailLvl {
+  int nx, ny, nz, n;
+  float inv;
+  int o_kind, o_ou, o_ov, o_ow, o_diag, o_x, o_r, o_t, o_dd;
+  int galerkin;
+};
+
+__device__ __forceinline__ void tail_decode(const TailLvl& L, int c, int& i, int& j, int& k) {
+  const int t = c / L.nx;
+  i = c - t * L.nx;
+  k = t / L.ny;
+  j = t - k * L.ny;
+}
+
+// row_sums (delta3d_multigrid.rs:216-255) on LDS arrays; x at offset xo.
+__device__ __forceinline__ float tail_row(const float* lds, const TailLvl& L, int i, int j, int k, int c, int xo) {
+#pragma clang fp contract(off)
+  const int8_t* kind = reinterpret_cast<const int8_t*>(lds + L.o_kind);
+  if (kind[c] <= 0) return 0.f;
+  const float* x = lds + xo;
+  const float* ou = lds + L.o_ou;
+  const float* ov = lds + L.o_ov;
+  const float* ow = lds + L.o_ow;
+  const float pc = x[c];
+  float acc = 0.f;
+  const int nx = L.nx, ny = L.ny, nz = L.nz, plane = nx * ny;
+  const bool gal = L.galerkin != 0;
+  auto face = [&](float a, bool has, int n, bool top) {
+    if (a == 0.f) return;
+    if (has) {
+      const int8_t kn = kind[n];
+      if (kn > 0) acc += a * (pc - x[n]);
+      else if (!gal && kn < 0) acc += 2.0f * a * pc;
+    } else if (!gal && top) {
+      acc += 2.0f * a * pc;
+    }
+  };
+  face(ou[(k * ny + j) * (nx + 1) + i], i > 0, c - 1, false);
+  face(ou[(k * ny + j) * (nx + 1) + i + 1], i + 1 < nx, c + 1, false);
+  face(ov[(k * (ny + 1) + j) * nx + i], j > 0, c - nx, false);
+  face(ov[(k * (ny + 1) + j + 1) * nx + i], j + 1 < ny, c + nx, false);
+  face(ow[(k * ny + j) * nx + i], k > 0, c - plane, false);
+  face(ow[((k + 1) * ny + j) * nx + i], k + 1 < nz, c + plane, true);
+  if (gal) acc += lds[L.o_dd + c] * pc;
+  return acc * L.inv;
+}
+
+// One smoothing pass on tail level l. RBGS: colour `col` in place (first: x
+// starts at zero). Jacobi: t = x + w(r - Lx)/diag, then x = t.
+__device__ __forceinline__ void tail_pass(float* lds, const TailLvl& L, int smoother, int col, bool first) {
+#pragma clang fp contract(off)
+  float* x = lds + L.o_x;
+  const float* r = lds + L.o_r;
+  const float* diag = lds + L.o_diag;
+  if (smoother == 0) {
+    float* t = lds + L.o_t;
+    const float omega = 6.0f / 7.0f;
+    for (int c = threadIdx.x; c < L.n; c += TAIL_THREADS) {
+      int i, j, k;
+      tail_decode(L, c, i, j, k);
+      const float row = first ? 0.f : tail_row(lds, L, i, j, k, c, L.o_x);
+      float xc = first ? 0.f : x[c];
+      const float dg = diag[c];
+      if (dg > 0.f) xc += omega * (r[c] - row) / dg;
+      t[c] = xc;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < L.n; c += TAIL_THREADS) x[c] = t[c];
+    __syncthreads();
+    return;
+  }
+  for (int c = threadIdx.x; c < L.n; c += TAIL_THREADS) {
+    int i, j, k;
+    tail_decode(L, c, i, j, k);
+    const float dg = diag[c];
+    if (((i + j + k) & 1) != col) {
+      if (first) x[c] = 0.f;
+      continue;
+    }
+    if (first) { x[c] = dg > 0.f ? r[c] / dg : 0.f; continue; }
+    if (dg > 0.f) x[c] = x[c] + (r[c] - tail_row(lds, L, i, j, k, c, L.o_x)) / dg;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void tail_sweeps(float* lds, const TailLvl& L, int smoother, int count, bool forward,
+                                            bool coarsest, bool from_zero) {
+  if (count == 0) {
+    if (from_zero) {
+      for (int c = threadIdx.x; c < L.n; c += TAIL_THREADS) lds[L.o_x + c] = 0.f;
+      __syncthreads();
+    }
+    return;
+  }
+  bool fz = from_zero;
+  if (smoother == 0) {
+    for (int s = 0; s < count; ++s) { tail_pass(lds, L, 0, 0, fz); fz = false; }
+    return;
+  }
+  if (coarsest) {
+    const int half = count / 2;
+    for (int s = 0; s < half; ++s) { tail_pass(lds, L, 1, 0, fz); fz = false; tail_pass(lds, L, 1, 1, false); }
+    if (count & 1) { tail_pass(lds, L, 1, 0, fz); fz = false; }
+    for (int s = 0; s < half; ++s) { tail_pass(lds, L, 1, 1, fz); fz = false; tail_pass(lds, L, 1, 0, false); }
+    return;
+  }
+  for (int s = 0; s < count; ++s) {
+    const int c0 = forward ? 0 : 1;
+    tail_pass(lds, L, 1, c0, fz);
+    fz = false;
+    tail_pass(lds, L, 1, 1 - c0, false);
+  }
+}
+
+// Descends and ascends every tail level in LDS (coarse_cycle3,
+// delta3d_multigrid.rs:303-332), starting from the first tail level whose r
+// is already restricted; writes that level's x back to global.
+__global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailSet T, Ctl* ctl) {
+#pragma clang fp contract(off)
+  if (!ctl->active) return;
+  extern __shared__ float lds[];
+  __shared__ TailLvl tl[TAIL_MAX_LEVELS];
+  if (threadIdx.x == 0) {
+    int off = 0;
+    for (int l = 0; l < T.count; ++l) {
+      const Level& g = T.lv[l];
+      TailLvl& L = tl[l];
+      L.nx = g.nx; L.ny = g.ny; L.nz = g.nz; L.n = (int)g.n; L.inv = g.inv;
+      L.o_kind = off; off += (L.n + 3) / 4;
+      L.o_ou = off; off += (g.nx + 1) * g.ny * g.nz;
+      L.o_ov = off; off += g.nx * (g.ny + 1) * g.nz;
+      L.o_ow = off; off += g.nx * g.ny * (g.nz + 1);
+      L.o_diag = off; off += L.n;
+      L.o_x = off; off += L.n;
+      L.o_r = off; off += L.n;
+      if (T.smoother == 0) { L.o_t = off; off += L.n; } else { L.o_t = L.o_x; }
+      L.galerkin = g.galerkin;
+      if (g.galerkin) { L.o_dd = off; off += L.n; } else { L.o_dd = L.o_diag; }
+    }
+  }
+  __syncthreads();
+  for (int l = 0; l < T.count; ++l) {
+    const Level& g = T.lv[l];
+    const TailLvl L = tl[l];
+    int8_t* kind = reinterpret_cast<int8_t*>(lds + L.o_kind);
+    const int nu = (g.nx + 1) * g.ny * g.nz, nv = g.nx * (g.ny + 1) * g.nz, nw = g.nx * g.ny * (g.nz + 1);
+    for (int q = threadIdx.x; q < L.n; q += TAIL_THREADS) {
+      kind[q] = g.kind[q];
+      lds[L.o_diag + q] = g.diag[q];
+      if (L.galerkin) lds[L.o_dd + q] = g.dd[q];
+    }
+    for (int q = threadIdx.x; q < nu; q += TAIL_THREADS) lds[L.o_ou + q] = g.ou[q];
+    for (int q = threadIdx.x; q < nv; q += TAIL_THREADS) lds[L.o_ov + q] = g.ov[q];
+    for (int q = threadIdx.x; q < nw; q += TAIL_THREADS) lds[L.o_ow + q] = g.ow[q];
+    if (l == 0)
+      for (int q = threadIdx.x; q < L.n; q += TAIL_THREADS) lds[L.o_r + q] = g.r[q];
+  }
+  __syncthreads();
+  const int last = T.count - 1;
+  for (int l = 0; l <= last; ++l) {
+    const TailLvl L = tl[l];
+    tail_sweeps(lds, L, T.smoother, l == last ? T.coarse : T.pre, true, l == last, true);
+    if (l < last) {
+      const TailLvl C = tl[l + 1];
+      for (int c = threadIdx.x; c < C.n; c += TAIL_THREADS) {
+        int i, j, k;
+        tail_decode(C, c, i, j, k);
+        float s = 0.f;
+        for (int ch = 0; ch < 8; ++ch) {
+          const int fi = 2 * i + (ch & 1), fj = 2 * j + ((ch >> 1) & 1), fk = 2 * k + ((ch >> 2) & 1);
+          const int f = (fk * L.ny + fj) * L.nx + fi;
+          s += lds[L.o_r + f] - tail_row(lds, L, fi, fj, fk, f, L.o_x);
+        }
+        lds[C.o_r + c] = 0.125f * s;
+      }
+      __syncthreads();
+    }
+  }
+  for (int l = last - 1; l >= 0; --l) {
+    const TailLvl L = tl[l];
+    const TailLvl C = tl[l + 1];
+    for (int c = threadIdx.x; c < L.n; c += TAIL_THREADS) {
+      int i, j, k;
+      tail_decode(L, c, i, j, k);
+      lds[L.o_x + c] += lds[C.o_x + ((k >> 1) * C.ny + (j >> 1)) * C.nx + (i >> 1)];
+    }
+    __syncthreads();
+    tail_sweeps(lds, L, T.smoother, T.post, false, false, false);
+  }
+  const TailLvl L0 = tl[0];
+  for (int q = threadIdx.x; q < L0.n; q += TAIL_THREADS) T.lv[0].x[q] = lds[L0.o_x + q];
+}
+
+// LDS bytes k_tail needs for the given levels.
+inline size_t tail_lds_bytes(const Level* lv, int count, int smoother) {
+  size_t floats = 0;
+  for (int l = 0; l < count; ++l) {
+    const Level& g = lv[l];
+    const size_t nu = (size_t)(g.nx + 1) * g.ny * g.nz, nv = (size_t)g.nx * (g.ny + 1) * g.nz,
+                 nw = (size_t)g.nx * g.ny * (g.nz + 1);
+    floats += (g.n + 3) / 4 + nu + nv + nw + (smoother == 0 ? 4 : 3) * g.n + (g.galerkin ? g.n : 0);
+  }
+  return floats * sizeof(float);
+}
+
+}  // namespace fsp

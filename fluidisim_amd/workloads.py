@@ -9,6 +9,7 @@ rho = dt = 1, so scale = -rho/dt = -1 and k1 = dt/rho = 1 (b = -div u*).
   tank          free surface h(x,z) = Y/2 + 4 sines, air (p = 0) above,
                 8 solid spheres, 16-mode smooth velocity of RMS 1;
                 multigrid over every coarse level the grid allows.
+Both use Galerkin coarse operators (P^T A P / 16).
 """
 from __future__ import annotations
 
@@ -49,7 +50,8 @@ def closed_box(n: int = 64, seed: int = 1, levels: int = 2) -> Workload:
     return Workload(f"closed_box_{n}", Domain3(n, n, n, 1.0), eta, None, us, vs, ws,
                     homogeneous_ghost=True, closed_lid=True, rest=float(n),
                     options=dict(preconditioner=_lib.PRECOND_MULTIGRID, smoother=_lib.SMOOTH_RBGS,
-                                 max_levels=levels, pre_sweeps=2, post_sweeps=2, coarse_sweeps=8),
+                                 max_levels=levels, pre_sweeps=2, post_sweeps=2, coarse_sweeps=8,
+                                 coarse_operator=_lib.COARSE_GALERKIN),
                     seed=seed)
 
 
@@ -59,7 +61,8 @@ def tank(X: int = 128, Y: int = 128, Z: int = 128, seed: int = 1) -> Workload:
     return Workload(f"tank_{X}x{Y}x{Z}", Domain3(nx, ny, nz, 1.0), eta, solid, us, vs, ws,
                     homogeneous_ghost=True, closed_lid=True, rest=float(nz) / 2,
                     options=dict(preconditioner=_lib.PRECOND_MULTIGRID, smoother=_lib.SMOOTH_RBGS,
-                                 max_levels=-1, pre_sweeps=2, post_sweeps=2, coarse_sweeps=8),
+                                 max_levels=-1, pre_sweeps=2, post_sweeps=2, coarse_sweeps=8,
+                                 coarse_operator=_lib.COARSE_GALERKIN),
                     seed=seed)
 
 

@@ -46,6 +46,11 @@ typedef struct {
 enum { FSP_PRECOND_JACOBI = 0, FSP_PRECOND_MULTIGRID = 1 };
 enum { FSP_SMOOTH_JACOBI = 0, FSP_SMOOTH_RBGS = 1 };
 enum { FSP_STORE_F32 = 0, FSP_STORE_F16 = 1, FSP_STORE_BF16 = 2 };
+/* Coarse operators: the reference's re-discretisation (averaged apertures,
+ * Dirichlet at a coarse half cell; delta3d_multigrid.rs:141-255), or the
+ * Galerkin product P^T A P / 16 with piecewise-constant P (equal to the
+ * re-discretisation in the interior, exact at the surface and at solids). */
+enum { FSP_COARSE_REDISCRETISED = 0, FSP_COARSE_GALERKIN = 1 };
 
 /* Solver settings. fsp_default_options gives the reference's multigrid
  * preconditioner (delta3d_multigrid.rs:28-34: damped Jacobi ω=6/7, 2 pre,
@@ -60,6 +65,7 @@ typedef struct {
   int32_t replace_every;   /* true-residual replacement period in iterations (0 = never) */
   int32_t refuse_degraded; /* 1: a degraded projection returns FSP_CONVERGENCE, outputs untouched */
   int32_t device;          /* HIP device ordinal */
+  int32_t coarse_operator; /* FSP_COARSE_* */
 } fsp_options;
 
 /* Surface state (Volume3 fields rho, g_eff, rest, homogeneous_ghost;

@@ -190,6 +190,16 @@ int orc_coarse_level(const Geometry* g, const Options* o, int level, float* kind
 size_t orc_level_count3(size_t nx, size_t ny, size_t nz) { return level_count3(nx, ny, nz); }
 size_t orc_hierarchy_floats3(size_t nx, size_t ny, size_t nz) { return hierarchy_floats3(nx, ny, nz); }
 uint64_t orc_fingerprint(const float* v, size_t n) { return fingerprint_f32(v, n); }
+
+// FNV-1a 64 over raw bytes, with the constants fingerprint_f32 uses.
+uint64_t orc_fnv_bytes(const uint8_t* b, size_t n) {
+  uint64_t h = 0xcbf29ce484222325ull;
+  for (size_t i = 0; i < n; ++i) {
+    h ^= (uint64_t)b[i];
+    h *= 0x100000001b3ull;
+  }
+  return h;
+}
 float orc_roundoff_3d() { return roundoff_3d<float>(); }
 
 }  // extern "C"

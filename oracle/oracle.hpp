@@ -58,9 +58,11 @@ struct Options {
   int32_t post_sweeps;    // delta3d_multigrid.rs:33 (2)
   int32_t coarse_sweeps;  // delta3d_multigrid.rs:34 (8)
   int32_t gauge;          // EXTENSION: 1 = remove the mean of b and of p on a pure-Neumann problem
+  int32_t coarse_op;      // 0: re-discretised coarse levels (delta3d_multigrid.rs:141-255);
+                          // 1: EXTENSION Galerkin P^T A P / 16 with piecewise-constant P
 };
 
-inline Options default_options() { return Options{1, 0, -1, 2, 2, 8, 1}; }
+inline Options default_options() { return Options{1, 0, -1, 2, 2, 8, 1, 0}; }
 
 // Geometry: the subset of Volume3's state that the projection reads.
 struct Geometry {
@@ -133,6 +135,8 @@ template <class T> struct Level3 {
   size_t nx, ny, nz;
   T inv;
   std::vector<T> open_u, open_v, open_w, kind, diag, x, r, t;
+  std::vector<T> dd;   // Galerkin levels: Dirichlet part of the diagonal (units of inv)
+  bool galerkin = false;
   size_t cells() const { return nx * ny * nz; }
   size_t c(size_t i, size_t j, size_t k) const { return (k * ny + j) * nx + i; }
 };
@@ -517,7 +521,8 @@ template <class T> class Volume {
       L.open_v.assign(cx * (cy + 1) * cz, T(0));
       L.open_w.assign(cx * cy * (cz + 1), T(0));
       size_t n = cx * cy * cz;
-      L.kind.assign(n, T(0)); L.diag = L.kind; L.x = L.kind; L.r = L.kind; L.t = L.kind;
+      L.kind.assign(n, T(0)); L.diag = L.kind; L.x = L.kind; L.r = L.kind; L.t = L.kind; L.dd = L.kind;
+      L.galerkin = opt.coarse_op == 1;
       levels.push_back(std::move(L));
       x = cx; y = cy; z = cz;
     }
@@ -535,9 +540,11 @@ template <class T> class Volume {
           if (L.kind[cc] <= T(0)) { out[cc] = T(0); continue; }
           T pc = pp ? (*pp)[cc] : T(0);
           T acc = T(0);
+          const bool gal = L.galerkin;
           auto face = [&](T a, int64_t n, bool top) {
             if (a == T(0)) return;
             if (n >= 0 && L.kind[n] > T(0)) acc += pp ? a * (pc - (*pp)[n]) : a;
+            else if (gal) {}
             else if (n >= 0 && L.kind[n] < T(0)) acc += T(2) * a * (pp ? pc : T(1));
             else if (n >= 0) {}
             else if (top) acc += T(2) * a * (pp ? pc : T(1));
@@ -548,6 +555,7 @@ template <class T> class Volume {
           face(L.open_v[(k * (ny_ + 1) + j + 1) * nx_ + i], j + 1 < ny_ ? (int64_t)(cc + nx_) : -1, false);
           face(L.open_w[(k * ny_ + j) * nx_ + i], k > 0 ? (int64_t)(cc - nx_ * ny_) : -1, false);
           face(L.open_w[((k + 1) * ny_ + j) * nx_ + i], k + 1 < nz_ ? (int64_t)(cc + nx_ * ny_) : -1, true);
+          if (gal) acc += pp ? L.dd[cc] * pc : L.dd[cc];
           out[cc] = acc * L.inv;
         }
   }
@@ -603,8 +611,83 @@ template <class T> class Volume {
   }
 
   // delta3d_multigrid.rs:370-387
+  // EXTENSION (coarse_op = 1): Galerkin coarse levels L_{l+1} = P^T L_l P / 16
+  // with P the piecewise-constant prolongation onto wet cells. A coarse face
+  // carries a quarter of the couplings of the four fine faces it covers
+  // (only faces between two active cells couple); the Dirichlet (ghost) part
+  // of the diagonal of the eight children is summed and divided by four. In
+  // the interior this is the reference's re-discretised operator exactly.
+  T ghost_diag(size_t i, size_t j, size_t k) const {
+    if (!wet3(i, j, k)) return T(0);
+    Row row = mobile_row(i, j, k);
+    T g = T(0);
+    for (int f = 0; f < 6; ++f)
+      if (row.n[f] < 0 && row.a[f] > T(0)) g += row.a[f] * row.it[f];
+    return g;
+  }
+  T fine_coupling(int axis, size_t i, size_t j, size_t k) const {
+    // aperture of the face below (i, j, k) along `axis` if it couples two wet cells
+    if (axis == 0 && (i == 0 || i >= nx)) return T(0);
+    if (axis == 1 && (j == 0 || j >= ny)) return T(0);
+    if (axis == 2 && (k == 0 || k >= nz)) return T(0);
+    if (!wet3(i, j, k)) return T(0);
+    Row row = mobile_row(i, j, k);
+    const int f = axis == 0 ? 0 : axis == 1 ? 2 : 4;
+    return row.n[f] >= 0 ? row.a[f] : T(0);
+  }
+  void prepare_galerkin() {
+    Level3<T>& L0 = levels[0];
+    coarsen3(nx, ny, [&](int, size_t) { return T(1); }, [&](size_t cc) { return fine_kind3(cc); }, L0);
+    const size_t cx = L0.nx, cy = L0.ny, cz = L0.nz;
+    for (size_t k = 0; k < cz; ++k)
+      for (size_t j = 0; j < cy; ++j)
+        for (size_t i = 0; i <= cx; ++i) {
+          T s = T(0);
+          for (auto& q : QUARTER) s += fine_coupling(0, 2 * i, 2 * j + q[0], 2 * k + q[1]);
+          L0.open_u[(k * cy + j) * (cx + 1) + i] = T(0.25) * s;
+        }
+    for (size_t k = 0; k < cz; ++k)
+      for (size_t j = 0; j <= cy; ++j)
+        for (size_t i = 0; i < cx; ++i) {
+          T s = T(0);
+          for (auto& q : QUARTER) s += fine_coupling(1, 2 * i + q[0], 2 * j, 2 * k + q[1]);
+          L0.open_v[(k * (cy + 1) + j) * cx + i] = T(0.25) * s;
+        }
+    for (size_t k = 0; k <= cz; ++k)
+      for (size_t j = 0; j < cy; ++j)
+        for (size_t i = 0; i < cx; ++i) {
+          T s = T(0);
+          for (auto& q : QUARTER) s += fine_coupling(2, 2 * i + q[0], 2 * j + q[1], 2 * k);
+          L0.open_w[(k * cy + j) * cx + i] = T(0.25) * s;
+        }
+    for (size_t k = 0; k < cz; ++k)
+      for (size_t j = 0; j < cy; ++j)
+        for (size_t i = 0; i < cx; ++i) {
+          T s = T(0);
+          for (auto& ch : CHILDREN) s += ghost_diag(2 * i + ch[0], 2 * j + ch[1], 2 * k + ch[2]);
+          L0.dd[L0.c(i, j, k)] = T(0.25) * s;
+        }
+    row_sums(L0, nullptr, L0.diag);
+    for (size_t l = 1; l < levels.size(); ++l) {
+      const Level3<T>& F = levels[l - 1];
+      Level3<T>& L = levels[l];
+      coarsen3(F.nx, F.ny,
+               [&](int axis, size_t f) { return axis == 0 ? F.open_u[f] : axis == 1 ? F.open_v[f] : F.open_w[f]; },
+               [&](size_t cc) { return F.kind[cc]; }, L);
+      for (size_t k = 0; k < L.nz; ++k)
+        for (size_t j = 0; j < L.ny; ++j)
+          for (size_t i = 0; i < L.nx; ++i) {
+            T s = T(0);
+            for (auto& ch : CHILDREN) s += F.dd[F.c(2 * i + ch[0], 2 * j + ch[1], 2 * k + ch[2])];
+            L.dd[L.c(i, j, k)] = T(0.25) * s;
+          }
+      row_sums(L, nullptr, L.diag);
+    }
+  }
+
   void prepare_multigrid3() {
     if (levels.empty()) return;
+    if (opt.coarse_op == 1) { prepare_galerkin(); return; }
     coarsen3(nx, ny, [&](int axis, size_t f) { return open3(axis, f); }, [&](size_t cc) { return fine_kind3(cc); }, levels[0]);
     for (size_t l = 1; l < levels.size(); ++l) {
       const Level3<T>& F = levels[l - 1];

@@ -33,7 +33,7 @@ class Options(C.Structure):
     _fields_ = [
         ("multigrid", C.c_int32), ("smoother", C.c_int32), ("max_levels", C.c_int32),
         ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32), ("coarse_sweeps", C.c_int32),
-        ("gauge", C.c_int32),
+        ("gauge", C.c_int32), ("coarse_op", C.c_int32),
     ]
 
 
@@ -78,6 +78,8 @@ def lib():
         L.orc_fingerprint.argtypes = [vp, sz]
         L.orc_fingerprint.restype = C.c_uint64
         L.orc_roundoff_3d.restype = C.c_float
+        L.orc_fnv_bytes.argtypes = [C.c_char_p, sz]
+        L.orc_fnv_bytes.restype = C.c_uint64
         _lib = L
     return _lib
 
@@ -136,8 +138,8 @@ class Problem:
         return g
 
 
-def options(multigrid=1, smoother=0, max_levels=-1, pre=2, post=2, coarse=8, gauge=1):
-    return Options(multigrid, smoother, max_levels, pre, post, coarse, gauge)
+def options(multigrid=1, smoother=0, max_levels=-1, pre=2, post=2, coarse=8, gauge=1, coarse_op=0):
+    return Options(multigrid, smoother, max_levels, pre, post, coarse, gauge, coarse_op)
 
 
 def project(prob: Problem, us, vs, ws, scale, k1, max_iters=20000, opts=None, p_warm=None, precision=32):
@@ -244,6 +246,10 @@ def level_count3(nx, ny, nz):
 
 def hierarchy_floats3(nx, ny, nz):
     return lib().orc_hierarchy_floats3(nx, ny, nz)
+
+
+def fnv_bytes(data: bytes):
+    return lib().orc_fnv_bytes(data, len(data))
 
 
 def fingerprint(values):
