@@ -12,6 +12,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -25,16 +26,19 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic HBM bytes per fine cell of one launch: every array the kernel
 # must touch, read once and written once (u8 row code = 1 B, f32 = 4 B,
-# f64 = 8 B; a red-black colour pass reads and writes half the cells).
-BYTES_PER_CELL = {
-    "k_q_gs<0>": 1 + 4 + 4 + 4,          # red-black pass: mask, z (all), r, z write (whole quads)
-    "k_q_gs<1>": 1 + 4 + 4,              # first pass from zero: mask, r -> z
-    "k_q_gs<2>": 1 + 4 + 0.5 + 4 + 4,    # prolongation + pass: mask, z, coarse x, r -> z'
-    "k_q_spmv_dir": 1 + 4 + 4 + 4 + 4,   # mask, z, d_old -> d_new, q
-    "k_q_update": 8 + 8 + 4 + 4 + 4 + 4, # p64 rw, d, q, r rw
-    "k_q_restrict": 1 + 4 + 4 + 0.5,     # mask, z, r -> coarse r
-    "k_residual": 1 + 8 + 4 + 4,         # mask, p64, b -> r
-}
+# f64 = 8 B, V-cycle value = zb B: 4 in f32 storage, 2 in f16/bf16; a
+# coarse value is 1/8 of a fine one).
+def bytes_per_cell(kernel, zb):
+    table = {
+        "k_q_gs<0>": 1 + zb + 4 + zb,              # red-black pass: mask, z, r -> z (whole quads)
+        "k_q_gs<1>": 1 + 4 + zb,                   # first pass from zero: mask, r -> z
+        "k_q_gs<2>": 1 + zb + zb / 8 + 4 + zb,     # prolongation + pass: mask, z, coarse x, r -> z'
+        "k_q_spmv_dir": 1 + zb + 4 + 4 + 4,        # mask, z, d_old -> d_new, q
+        "k_q_update": 8 + 8 + 4 + 4 + 4 + 4,       # p64 rw, d, q, r rw
+        "k_q_restrict": 1 + zb + 4 + zb / 8,       # mask, z, r -> coarse r
+        "k_residual": 1 + 8 + 4 + 4,               # mask, p64, b -> r
+    }
+    return table.get(kernel)
 
 
 def parse():
@@ -46,8 +50,47 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--kernel", default="k_q_gs<0>", help="kernel timed live for the roofline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-iters", default="1,3", help="oracle sample: two iteration budgets")
+    ap.add_argument("--cpu-iters", default="2,8", help="oracle sample: two iteration budgets")
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
+                    help="V-cycle vector storage (arithmetic stays f32)")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "fluidisim_amd", "learned_weights.npz"),
+                    help="trained network for the learned-FCG line (skipped when the file is absent)")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--copy-probe-only", action="store_true",
+                    help="run only the 256 MiB device copy (counter calibration) and exit")
     return ap.parse_args()
+
+
+def copy_probe(torch, mib=256, reps=10):
+    """Achievable HBM bandwidth: a plain device-to-device copy of `mib` MiB,
+    timed with events; counts read + write bytes."""
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return 2.0 * 4 * n / (ms * 1e-3) / 1e9
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    counter summary (FETCH_SIZE and WRITE_SIZE per dispatch, already scaled by
+    the copy-kernel calibration); None when no summary covers the pair."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"\|\s*`([^`]+)`\s*\|\s*`([^`]+)`\s*\|\s*([0-9.]+)\s*\|\s*([0-9.]+)\s*\|\s*([0-9]+)\s*\|", line)
+        if m and m.group(1) == kernel and m.group(2) == workload:
+            return int(m.group(5))
+    return None
 
 
 def log(msg):
@@ -85,6 +128,35 @@ def cpu_baseline(wl, gpu_iterations, iters_pair):
     }
 
 
+def learned_fcg(wl, args, torch, _lib, W):
+    """The same projection with the trained network inside flexible CG
+    (MGPCG fallback), timed like the plain one."""
+    from fluidisim_amd.learned import Network
+    net = Network()
+    net.load(args.weights)
+    vol = W.build_volume(wl, preconditioner=_lib.PRECOND_LEARNED, learned_budget=200)
+    net.attach(vol)
+
+    def step():
+        return vol.project_mobile3(wl.scale, wl.k1, 2000, rel_tol=1e-6, warm=False)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+    r = reps[-1]
+    log(f"[bench] learned FCG: {ms:.3f} ms/projection, {r.iterations} iterations, residual {r.residual:.2e}")
+    vol.close()
+    net.close()
+    with np.load(args.weights, allow_pickle=False) as f:
+        trained = {k: (f[k].tolist() if f[k].size < 16 else None) for k in f.files if k != "theta"}
+    return {"ms_per_projection": round(ms, 4), "iterations": r.iterations, "residual": r.residual,
+            "divergence_plain": r.divergence_plain, "degraded": r.degraded, "training": trained}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -92,6 +164,9 @@ def main():
               f"described, not built.", flush=True)
         sys.exit(2)
     import torch
+    if args.copy_probe_only:
+        print(json.dumps({"copy_probe_gbs": round(copy_probe(torch), 1)}), flush=True)
+        return
     from fluidisim_amd import _lib
     from fluidisim_amd import workloads as W
 
@@ -99,7 +174,8 @@ def main():
     t0 = time.perf_counter()
     wl = W.tank(X, Y, Z, seed=args.seed)
     log(f"[bench] workload {wl.name}: {wl.domain.cells} cells, generated in {time.perf_counter() - t0:.1f} s")
-    vol = W.build_volume(wl)
+    storage = {"f32": _lib.STORE_F32, "f16": _lib.STORE_F16, "bf16": _lib.STORE_BF16}[args.storage]
+    vol = W.build_volume(wl, vcycle_storage=storage)
     log(f"[bench] solver ready, {vol.multigrid_levels()} coarse levels")
 
     def step():
@@ -128,7 +204,7 @@ def main():
     # duration (HIP events on the library's stream over the timed steps).
     # Launches queued after convergence return at once; they are excluded
     # (duration under a fifth of the median of the longer half).
-    bpc = BYTES_PER_CELL.get(args.kernel)
+    bpc = bytes_per_cell(args.kernel, 4 if args.storage == "f32" else 2)
     roofline = None
     if bpc is not None and launches.value > 0:
         d = durations[: min(launches.value, max_launches)].astype(np.float64)
@@ -138,10 +214,15 @@ def main():
         bytes_launch = bpc * wl.domain.cells
         achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": args.kernel,
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.kernel, wl.name),
+                    "kernel": args.kernel, "copy_probe_gbs": round(copy_probe(torch), 1),
                     "launches": int(eff.size), "noop_launches": int(d.size - eff.size),
                     "avg_us": round(avg_ms * 1000.0, 2), "bytes_per_launch": int(bytes_launch)}
         log(f"[bench] {args.kernel}: {eff.size} launches, {avg_ms * 1000:.1f} us avg, {achieved:.0f} GB/s")
+
+    learned = None
+    if not args.no_learned and os.path.exists(args.weights):
+        learned = learned_fcg(wl, args, torch, _lib, W)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -155,12 +236,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "domain": [wl.domain.nx, wl.domain.ny, wl.domain.nz],
                    "cells": wl.domain.cells, "solver": "plain MGPCG (RBGS V-cycle 2+2, Galerkin coarse levels, 8 coarsest sweeps)",
-                   "precision": "f32 vectors, f64 pressure/residual/reductions", "rel_tol": 1e-6,
+                   "precision": "f32 CG vectors, f64 pressure/residual/reductions",
+                   "vcycle_storage": args.storage, "rel_tol": 1e-6,
                    "iterations": last.iterations, "residual": last.residual,
                    "divergence_plain": last.divergence_plain, "degraded": last.degraded,
                    "ms_device_last": round(last.ms_device, 4), "seed": args.seed},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "learned_fcg": learned,
     }
     print(json.dumps(line), flush=True)
     vol.close()

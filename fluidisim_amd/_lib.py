@@ -1,4 +1,5 @@
-"""ctypes bindings of libfsp.so, the C ABI declared in include/fsp.h.
+"""ctypes bindings of libfsp.so, the C ABI declared in include/fsp.h and
+include/fsp_learned.h.
 
 The library is the product: loading fails loudly when it has not been built
 (there is no fallback implementation).
@@ -12,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfsp.so")
 
 FSP_OK, FSP_DOMAIN, FSP_NOT_FINITE, FSP_SHAPE, FSP_CONVERGENCE, FSP_DEVICE, FSP_INVALID = range(7)
-PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
+PRECOND_JACOBI, PRECOND_MULTIGRID, PRECOND_LEARNED = 0, 1, 2
 SMOOTH_JACOBI, SMOOTH_RBGS = 0, 1
 STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
 COARSE_REDISCRETISED, COARSE_GALERKIN = 0, 1
@@ -22,6 +23,10 @@ EXPORTS = [
     "fsp_abi_version", "fsp_status_string", "fsp_default_options", "fsp_create", "fsp_destroy", "fsp_levels",
     "fsp_set_geometry", "fsp_project", "fsp_project_device", "fsp_apply_operator_device", "fsp_assemble_device",
     "fsp_precondition_device", "fsp_divergence_device", "fsp_profile_begin", "fsp_profile_end", "fsp_generate",
+    # include/fsp_learned.h
+    "fsp_net_param_count", "fsp_net_create", "fsp_net_destroy", "fsp_net_set_params", "fsp_net_get_params",
+    "fsp_net_reserve", "fsp_net_features_device", "fsp_net_forward_device", "fsp_net_loss_grad_device",
+    "fsp_net_train_step_device", "fsp_attach_net", "fsp_net_mfma_probe",
 ]
 
 
@@ -35,6 +40,7 @@ class Options(C.Structure):
         ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32), ("coarse_sweeps", C.c_int32),
         ("gauge", C.c_int32), ("vcycle_storage", C.c_int32), ("replace_every", C.c_int32),
         ("refuse_degraded", C.c_int32), ("device", C.c_int32), ("coarse_operator", C.c_int32),
+        ("graph", C.c_int32), ("learned_budget", C.c_int32),
     ]
 
 
@@ -102,6 +108,29 @@ def lib():
     L.fsp_profile_end.restype = i32
     L.fsp_generate.argtypes = [i32, i32, i32, i32, C.c_uint64, vp, vp, vp, vp, vp]
     L.fsp_generate.restype = i32
+    L.fsp_net_param_count.restype = u32
+    L.fsp_net_create.argtypes = [i32, vp]
+    L.fsp_net_create.restype = i32
+    L.fsp_net_destroy.argtypes = [vp]
+    L.fsp_net_destroy.restype = None
+    L.fsp_net_set_params.argtypes = [vp, vp, sz]
+    L.fsp_net_set_params.restype = i32
+    L.fsp_net_get_params.argtypes = [vp, vp, sz]
+    L.fsp_net_get_params.restype = i32
+    L.fsp_net_reserve.argtypes = [vp, i32, i32, i32]
+    L.fsp_net_reserve.restype = i32
+    L.fsp_net_features_device.argtypes = [vp, vp, vp, vp, f32, vp]
+    L.fsp_net_features_device.restype = i32
+    L.fsp_net_forward_device.argtypes = [vp, i32, i32, i32, vp, vp]
+    L.fsp_net_forward_device.restype = i32
+    L.fsp_net_loss_grad_device.argtypes = [vp, i32, i32, i32, vp, vp, vp, vp, vp]
+    L.fsp_net_loss_grad_device.restype = i32
+    L.fsp_net_train_step_device.argtypes = [vp, i32, i32, i32, vp, vp, vp, f32, vp, vp]
+    L.fsp_net_train_step_device.restype = i32
+    L.fsp_attach_net.argtypes = [vp, vp]
+    L.fsp_attach_net.restype = i32
+    L.fsp_net_mfma_probe.argtypes = [vp, vp, vp]
+    L.fsp_net_mfma_probe.restype = i32
     _lib = L
     return L
 

@@ -31,12 +31,23 @@ def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     fsp_o = os.path.join(BUILD, "fsp.o")
+    net_o = os.path.join(BUILD, "learned.o")
     gen_o = os.path.join(BUILD, "generate.o")
-    _run([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, "fsp.hip"), "-o", fsp_o])
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", "-c",
-          os.path.join(CSRC, "generate.cpp"), "-o", gen_o])
+    jobs = [
+        [hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, "fsp.hip"), "-o", fsp_o],
+        [hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, "learned.hip"), "-o", net_o],
+        ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", "-c",
+         os.path.join(CSRC, "generate.cpp"), "-o", gen_o],
+    ]
+    procs = []
+    for cmd in jobs:
+        print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+    failed = [cmd for cmd, p in zip(jobs, procs) if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
     tmp = LIB + ".tmp"
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-o", tmp, fsp_o, gen_o, "-pthread"])
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-o", tmp, fsp_o, net_o, gen_o, "-pthread"])
     os.replace(tmp, LIB)
     return LIB
 

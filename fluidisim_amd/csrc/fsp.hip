@@ -19,11 +19,15 @@
 #include <limits>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fsp.h"
+#include "../../include/fsp_learned.h"
 #include "kernels.h"
 #include "kernels_fast.h"
+#include "kernels_coarse.h"
+#include "learned_internal.h"
 
 using namespace fsp;
 
@@ -49,6 +53,7 @@ struct fsp_solver {
   // tail for the coarse levels from tail_first on
   bool quad = false;
   QuadGrid qfine{}, qrestr{};
+  std::vector<QuadGrid> cq, cqr;  // compact levels: quads of the level, and of its restriction
   int tail_first = 0;
   size_t tail_lds = 0;
   float* zres = nullptr;  // buffer holding M^-1 r after precondition()
@@ -59,6 +64,12 @@ struct fsp_solver {
   unsigned long long* hhash = nullptr;
   std::vector<void*> allocs;
   // live timing of one kernel (fsp_profile_begin / fsp_profile_end)
+  // one CG iteration captured as a graph, replayed per iteration
+  hipGraph_t iter_graph[2] = {};      // one CG iteration: [0] as configured, [1] learned phase
+  hipGraphExec_t iter_exec[2] = {};
+  fsp_net* net = nullptr;             // learned preconditioner (not owned)
+  int net_phase = 0;                  // 1 while the current pass uses the network
+  uint32_t net_until = 0;             // iteration at which the learned phase ends
   bool prof_on = false;
   std::string prof_name;
   std::vector<hipEvent_t> prof_events;
@@ -105,17 +116,41 @@ unsigned grid_for(size_t n) {
 
 // Launch on the solver's stream; when the kernel is the one being timed,
 // bracket the launch with a pair of events.
-#define FSP_LAUNCH(S, kern, grid, stream, ...)                                                     \
+// Launch with BLOCK threads; `name` is what fsp_profile_begin matches (the
+// kernel without its storage argument).
+#define FSP_LAUNCH_N(S, name, kern, grid, stream, ...)                                             \
   do {                                                                                         \
-    const bool prof_ = (S)->prof_on && (S)->prof_name == #kern &&                              \
+    const bool prof_ = (S)->prof_on && (S)->prof_name == name &&                               \
                        (S)->prof_used + 2 <= (S)->prof_events.size();                           \
     if (prof_) (void)hipEventRecord((S)->prof_events[(S)->prof_used], stream);                 \
-    hipLaunchKernelGGL(kern, grid, dim3(BLOCK), 0, stream, __VA_ARGS__);                       \
+    kern<<<grid, dim3(BLOCK), 0, stream>>>(__VA_ARGS__);                                       \
     if (prof_) {                                                                               \
       (void)hipEventRecord((S)->prof_events[(S)->prof_used + 1], stream);                     \
       (S)->prof_used += 2;                                                                     \
     }                                                                                          \
   } while (0)
+#define FSP_LAUNCH(S, kern, grid, stream, ...) FSP_LAUNCH_N(S, #kern, kern, grid, stream, __VA_ARGS__)
+
+// Kernels templated on the V-cycle storage ST of the enclosing function.
+#define K_QGS(M) k_q_gs<M, ST>
+#define K_QRESTRICT k_q_restrict<ST>
+#define K_QSPMV k_q_spmv_dir<ST>
+#define K_CGS k_cgs<ST>
+#define K_CRESTRICT k_crestrict<ST>
+#define K_CPROLONG k_cprolong<ST>
+#define K_GCGS(M) k_gc_gs<M, ST>
+#define K_GCRESTRICT k_gc_restrict<ST>
+template <int ST> using TS = typename Store<ST>::T;
+
+// Calls f with the storage type of `storage` (FSP_STORE_*) as a constant.
+template <class F>
+auto with_storage(int storage, F&& f) {
+  switch (storage) {
+    case FSP_STORE_F16: return f(std::integral_constant<int, ST_F16>{});
+    case FSP_STORE_BF16: return f(std::integral_constant<int, ST_BF16>{});
+    default: return f(std::integral_constant<int, ST_F32>{});
+  }
+}
 
 // Fold the per-block partials of a reducing kernel (k_finish) right after it.
 void launch_finish(hipStream_t st, int op, unsigned nblocks, bool is_max, int gated, Ctl* ctl, double* partials) {
@@ -158,6 +193,18 @@ __global__ void __launch_bounds__(BLOCK) k_copy_gated(const float* a, float* b, 
   FSP_FOR(c, n) b[c] = a[c];
 }
 
+// The tail of Galerkin levels with red-black smoothing runs in the compact
+// form (couplings and dd only).
+bool gc_tail_ok(const fsp_options& o) {
+  return o.coarse_operator == FSP_COARSE_GALERKIN && o.smoother == FSP_SMOOTH_RBGS;
+}
+
+// The fine V-cycle runs on quads (red-black smoothing, at least one pre- and
+// one post-sweep).
+bool quad_vcycle_ok(bool quad, const fsp_options& o) {
+  return quad && o.smoother == FSP_SMOOTH_RBGS && o.pre_sweeps >= 1 && o.post_sweeps >= 1;
+}
+
 struct Launch {
   fsp_solver* S;
   hipStream_t st;
@@ -193,6 +240,7 @@ struct Launch {
         if (L.galerkin) FSP_LAUNCH(S, k_cdd, dim3(grid_for(L.n)), st, S->levels[l - 1], L);
       }
       FSP_LAUNCH(S, k_cdiag, dim3(grid_for(L.n)), st, L);
+      if (L.compact) FSP_LAUNCH(S, k_gc_build, dim3(grid_for(L.n)), st, L);
     }
   }
 
@@ -201,30 +249,32 @@ struct Launch {
   }
 
   // Sweeps on coarse level l; `x` is the current iterate (nullptr = zero),
-  // returns the buffer holding the result.
-  float* coarse_sweeps(int l, float* x, int count, bool forward, bool coarsest) {
+  // returns the buffer holding the result. Jacobi sweeps exist in f32 only.
+  template <int ST>
+  TS<ST>* coarse_sweeps(int l, TS<ST>* x, int count, bool forward, bool coarsest) {
     Level& L = S->levels[l];
+    TS<ST>* lx = reinterpret_cast<TS<ST>*>(L.x);
     const int smoother = S->opt.smoother;
     if (count == 0) {
-      if (!x) { fill_gated(L.x, L.n, 0.f); return L.x; }
+      if (!x) { fill_gated(L.x, L.n, 0.f); return lx; }
       return x;
     }
-    if (smoother == FSP_SMOOTH_JACOBI) {
-      float* cur = x;
-      for (int s = 0; s < count; ++s) {
-        float* out = (cur == L.x) ? L.t : L.x;
-        FSP_LAUNCH(S, k_cjacobi, dim3(grid_for(L.n)), st, L, cur, out, S->ctl);
-        cur = out;
+    if constexpr (ST == ST_F32) {
+      if (smoother == FSP_SMOOTH_JACOBI) {
+        float* cur = x;
+        for (int s = 0; s < count; ++s) {
+          float* out = (cur == L.x) ? L.t : L.x;
+          FSP_LAUNCH(S, k_cjacobi, dim3(grid_for(L.n)), st, L, cur, out, S->ctl);
+          cur = out;
+        }
+        return cur;
       }
-      return cur;
+      if (x && x != L.x) FSP_LAUNCH(S, k_copy_gated, dim3(grid_for(L.n)), st, x, L.x, L.n, S->ctl);
     }
     // red-black Gauss-Seidel, in place on L.x
     int first = x ? 0 : 1;
-    if (x && x != L.x) {
-      FSP_LAUNCH(S, k_copy_gated, dim3(grid_for(L.n)), st, x, L.x, L.n, S->ctl);
-    }
     auto colour = [&](int c) {
-      FSP_LAUNCH(S, k_cgs, dim3(grid_for(L.n)), st, L, L.x, c, first, S->ctl);
+      FSP_LAUNCH_N(S, "k_cgs", K_CGS, dim3(grid_for(L.n)), st, L, lx, c, first, S->ctl);
       first = 0;
     };
     if (coarsest) {
@@ -237,11 +287,12 @@ struct Launch {
         if (forward) { colour(0); colour(1); } else { colour(1); colour(0); }
       }
     }
-    return L.x;
+    return lx;
   }
 
   // The tail levels in one workgroup: level tail_first's r is restricted;
   // leaves its x in levels[tail_first].x.
+  template <int ST>
   void tail() {
     TailSet T{};
     T.count = (int)S->levels.size() - S->tail_first;
@@ -250,32 +301,91 @@ struct Launch {
     T.pre = S->opt.pre_sweeps;
     T.post = S->opt.post_sweeps;
     T.coarse = S->opt.coarse_sweeps;
-    hipLaunchKernelGGL(k_tail, dim3(1), dim3(TAIL_THREADS), S->tail_lds, st, T, S->ctl);
+    if (gc_tail_ok(S->opt)) k_gc_tail<ST><<<dim3(1), dim3(TAIL_THREADS), S->tail_lds, st>>>(T, S->ctl);
+    else k_tail<ST><<<dim3(1), dim3(TAIL_THREADS), S->tail_lds, st>>>(T, S->ctl);
+  }
+
+  static dim3 qgrid(const QuadGrid& Q) { return dim3((unsigned)(Q.nbx * Q.nby * Q.planes)); }
+
+  // Pre-smoothing of a compact level from zero, in place on L.x: forward
+  // red-black sweeps, or the palindromic sequence on the coarsest level.
+  template <int ST>
+  TS<ST>* gc_sweeps(int l, int count, bool coarsest) {
+    Level& L = S->levels[l];
+    TS<ST>* x = reinterpret_cast<TS<ST>*>(L.x);
+    if (count == 0) { fill_gated(L.x, L.n, 0.f); return x; }
+    const dim3 g = qgrid(S->cq[l]);
+    const TS<ST>* none = nullptr;
+    bool first = true;
+    auto colour = [&](int c) {
+      if (first) FSP_LAUNCH_N(S, "k_gc_gs<1>", K_GCGS(1), g, st, L, S->cq[l], (const TS<ST>*)x, x, c, L, none, S->ctl);
+      else FSP_LAUNCH_N(S, "k_gc_gs<0>", K_GCGS(0), g, st, L, S->cq[l], (const TS<ST>*)x, x, c, L, none, S->ctl);
+      first = false;
+    };
+    if (coarsest) {
+      const int half = count / 2;
+      for (int s = 0; s < half; ++s) { colour(0); colour(1); }
+      if (count & 1) colour(0);
+      for (int s = 0; s < half; ++s) { colour(1); colour(0); }
+    } else {
+      for (int s = 0; s < count; ++s) { colour(0); colour(1); }
+    }
+    return x;
+  }
+
+  // Prolongation from the coarser level's iterate `xp` fused with the first
+  // post-smoothing pass (L.x -> L.t), then the remaining passes on L.t.
+  template <int ST>
+  TS<ST>* gc_ascend(int l, const TS<ST>* xp) {
+    Level& L = S->levels[l];
+    Level& C = S->levels[l + 1];
+    TS<ST>* x = reinterpret_cast<TS<ST>*>(L.x);
+    TS<ST>* t = reinterpret_cast<TS<ST>*>(L.t);
+    const dim3 g = qgrid(S->cq[l]);
+    const TS<ST>* none = nullptr;
+    const int passes = 2 * S->opt.post_sweeps;
+    FSP_LAUNCH_N(S, "k_gc_gs<2>", K_GCGS(2), g, st, L, S->cq[l], (const TS<ST>*)x, t, 1, C, xp, S->ctl);
+    for (int pz = 1; pz < passes; ++pz) {
+      const int colour = (pz % 2 == 0) ? 1 : 0;
+      FSP_LAUNCH_N(S, "k_gc_gs<0>", K_GCGS(0), g, st, L, S->cq[l], (const TS<ST>*)t, t, colour, L, none, S->ctl);
+    }
+    return t;
   }
 
   // coarse_cycle3 (delta3d_multigrid.rs:303-332); returns the level-0 iterate.
-  float* coarse_cycle() {
+  template <int ST>
+  TS<ST>* coarse_cycle() {
     const int last = (int)S->levels.size() - 1;
     const int tf = S->tail_first;
-    std::vector<float*> xcur(S->levels.size(), nullptr);
+    std::vector<TS<ST>*> xcur(S->levels.size(), nullptr);
     for (int l = 0; l < tf; ++l) {
+      Level& L = S->levels[l];
       const int sweeps = (l == last) ? S->opt.coarse_sweeps : S->opt.pre_sweeps;
-      xcur[l] = coarse_sweeps(l, nullptr, sweeps, true, l == last);
+      if (L.compact) xcur[l] = gc_sweeps<ST>(l, sweeps, l == last);
+      else xcur[l] = coarse_sweeps<ST>(l, nullptr, sweeps, true, l == last);
       if (l < last) {
-        Level& L = S->levels[l];
         Level& C = S->levels[l + 1];
-        FSP_LAUNCH(S, k_crestrict, dim3(grid_for(C.n)), st, L, C, xcur[l], S->ctl);
+        if (L.compact)
+          FSP_LAUNCH_N(S, "k_gc_restrict", K_GCRESTRICT, qgrid(S->cqr[l]), st, L, S->cqr[l], (const TS<ST>*)xcur[l], C,
+                       S->ctl);
+        else
+          FSP_LAUNCH_N(S, "k_crestrict", K_CRESTRICT, dim3(grid_for(C.n)), st, L, C, (const TS<ST>*)xcur[l], S->ctl);
       }
     }
     if (tf <= last) {
-      tail();
-      xcur[tf] = S->levels[tf].x;
+      tail<ST>();
+      xcur[tf] = reinterpret_cast<TS<ST>*>(S->levels[tf].x);
     }
     for (int l = std::min(tf, last) - 1; l >= 0; --l) {
       Level& L = S->levels[l];
       Level& C = S->levels[l + 1];
-      FSP_LAUNCH(S, k_cprolong, dim3(grid_for(L.n)), st, L, C, xcur[l + 1], xcur[l], S->ctl);
-      xcur[l] = coarse_sweeps(l, xcur[l], S->opt.post_sweeps, false, false);
+      if (L.compact) {
+        xcur[l] = gc_ascend<ST>(l, xcur[l + 1]);
+        continue;
+      }
+      FSP_LAUNCH_N(S, "k_cprolong", K_CPROLONG, dim3(grid_for(L.n)), st, L, C, (const TS<ST>*)xcur[l + 1], xcur[l],
+                   S->ctl);
+      xcur[l] = coarse_sweeps<ST>(l, xcur[l], S->opt.post_sweeps, false, false);
     }
     return xcur[0];
   }
@@ -307,7 +417,7 @@ struct Launch {
     return S->z;
   }
 
-  // z = M^-1 r (v_cycle3, delta3d_multigrid.rs:393-429) into S->z.
+  // z = M^-1 r (v_cycle3, delta3d_multigrid.rs:393-429) into S->z; f32 storage.
   void vcycle() {
     // With Jacobi sweeps the iterate alternates between z and zt; start so
     // that an even total ends in z.
@@ -316,7 +426,7 @@ struct Launch {
     if (!S->levels.empty()) {
       Level& L0 = S->levels[0];
       FSP_LAUNCH(S, k_restrict_fine, dim3(grid_for(L0.n)), st, G, L0, cur, S->r, S->ctl);
-      float* x0 = coarse_cycle();
+      float* x0 = coarse_cycle<ST_F32>();
       FSP_LAUNCH(S, k_prolong_fine, dim3(grid_for(G.n)), st, G, L0, x0, cur, S->ctl);
     }
     cur = fine_sweeps(cur, S->opt.post_sweeps, false);
@@ -325,49 +435,64 @@ struct Launch {
     }
   }
 
-  bool quad_vcycle() const {
-    return S->quad && S->opt.smoother == FSP_SMOOTH_RBGS && S->opt.pre_sweeps >= 1 && S->opt.post_sweeps >= 1;
-  }
+  bool quad_vcycle() const { return quad_vcycle_ok(S->quad, S->opt); }
 
   // The V-cycle on quads with red-black smoothing: first pass from zero,
   // residual fused with the restriction, prolongation fused with the first
   // post-smoothing pass, <r, z> fused with the last one. Returns the buffer
-  // holding z.
-  float* vcycle_quad(int dot_op) {
+  // holding z (storage ST).
+  template <int ST>
+  TS<ST>* vcycle_quad(int dot_op) {
     const QuadGrid& Q = S->qfine;
     const dim3 g((unsigned)(Q.nbx * Q.nby * Q.planes));
     Level L0 = S->levels.empty() ? Level{} : S->levels[0];
-    const float* none = nullptr;
-    FSP_LAUNCH(S, k_q_gs<1>, g, st, G, Q, (const float*)S->z, S->z, S->r, 0, L0, none, -1, S->ctl, S->partials);
-    FSP_LAUNCH(S, k_q_gs<0>, g, st, G, Q, (const float*)S->z, S->z, S->r, 1, L0, none, -1, S->ctl, S->partials);
+    TS<ST>* z = reinterpret_cast<TS<ST>*>(S->z);
+    TS<ST>* zt = reinterpret_cast<TS<ST>*>(S->zt);
+    const TS<ST>* none = nullptr;
+    FSP_LAUNCH_N(S, "k_q_gs<1>", K_QGS(1), g, st, G, Q, (const TS<ST>*)z, z, S->r, 0, L0, none, -1, S->ctl,
+                 S->partials);
+    FSP_LAUNCH_N(S, "k_q_gs<0>", K_QGS(0), g, st, G, Q, (const TS<ST>*)z, z, S->r, 1, L0, none, -1, S->ctl,
+                 S->partials);
     for (int s = 1; s < S->opt.pre_sweeps; ++s) {
-      FSP_LAUNCH(S, k_q_gs<0>, g, st, G, Q, (const float*)S->z, S->z, S->r, 0, L0, none, -1, S->ctl, S->partials);
-      FSP_LAUNCH(S, k_q_gs<0>, g, st, G, Q, (const float*)S->z, S->z, S->r, 1, L0, none, -1, S->ctl, S->partials);
+      FSP_LAUNCH_N(S, "k_q_gs<0>", K_QGS(0), g, st, G, Q, (const TS<ST>*)z, z, S->r, 0, L0, none, -1, S->ctl,
+                   S->partials);
+      FSP_LAUNCH_N(S, "k_q_gs<0>", K_QGS(0), g, st, G, Q, (const TS<ST>*)z, z, S->r, 1, L0, none, -1, S->ctl,
+                   S->partials);
     }
-    float* out = S->z;
+    TS<ST>* out = z;
     int passes = 2 * S->opt.post_sweeps;
     int done = 0;
     if (!S->levels.empty()) {
       const QuadGrid& R = S->qrestr;
       const dim3 gr((unsigned)(R.nbx * R.nby * R.planes));
-      FSP_LAUNCH(S, k_q_restrict, gr, st, G, R, (const float*)S->z, (const float*)S->r, L0, S->ctl);
-      const float* x0 = coarse_cycle();
-      FSP_LAUNCH(S, k_q_gs<2>, g, st, G, Q, (const float*)S->z, S->zt, S->r, 1, L0, x0,
-                 passes == 1 ? dot_op : -1, S->ctl, S->partials);
+      FSP_LAUNCH_N(S, "k_q_restrict", K_QRESTRICT, gr, st, G, R, (const TS<ST>*)z, (const float*)S->r, L0, S->ctl);
+      const TS<ST>* x0 = coarse_cycle<ST>();
+      FSP_LAUNCH_N(S, "k_q_gs<2>", K_QGS(2), g, st, G, Q, (const TS<ST>*)z, zt, S->r, 1, L0, x0,
+                   passes == 1 ? dot_op : -1, S->ctl, S->partials);
       if (passes == 1 && dot_op >= 0) FSP_FINISH(S, st, dot_op, g, false, 1);
-      out = S->zt;
+      out = zt;
       done = 1;
     }
     for (int pz = done; pz < passes; ++pz) {
       const int colour = (pz % 2 == 0) ? 1 : 0;
-      FSP_LAUNCH(S, k_q_gs<0>, g, st, G, Q, (const float*)out, out, S->r, colour, L0, none,
-                 pz == passes - 1 ? dot_op : -1, S->ctl, S->partials);
+      FSP_LAUNCH_N(S, "k_q_gs<0>", K_QGS(0), g, st, G, Q, (const TS<ST>*)out, out, S->r, colour, L0, none,
+                   pz == passes - 1 ? dot_op : -1, S->ctl, S->partials);
       if (pz == passes - 1 && dot_op >= 0) FSP_FINISH(S, st, dot_op, g, false, 1);
     }
     return out;
   }
 
   void precondition(int op) {
+    if (S->net_phase) {
+      // z = network(r) inside flexible CG; sigma from |r|^2 on the device
+      const double* rr = op == OP_RZ ? &S->ctl->rn : &S->ctl->rr;
+      (void)net_apply(S->net, st, S->r, S->mask, S->solid, rr, &S->ctl->wet_cells, &S->ctl->active, S->z);
+      FSP_LAUNCH(S, k_dot_rz_qz, dim3(grid_for(G.n)), st, G.n, (const float*)S->r, (const float*)S->z,
+                 (const float*)S->q, op, S->ctl, S->partials);
+      FSP_FINISH(S, st, op, dim3(grid_for(G.n)), false, 1);
+      S->zres = S->z;
+      return;
+    }
     if (S->opt.preconditioner == FSP_PRECOND_JACOBI) {
       FSP_LAUNCH(S, k_jacobi_prec, dim3(grid_for(G.n)), st, G, S->r, S->z, op, S->ctl,
                          S->partials);
@@ -376,7 +501,9 @@ struct Launch {
       return;
     }
     if (quad_vcycle()) {
-      S->zres = vcycle_quad(op);
+      S->zres = with_storage(S->opt.vcycle_storage, [&](auto c) {
+        return reinterpret_cast<float*>(vcycle_quad<decltype(c)::value>(op));
+      });
       return;
     }
     vcycle();
@@ -385,11 +512,27 @@ struct Launch {
     S->zres = S->z;
   }
 
+  // The preconditioned vector in f32 (tests and the device entry point).
+  void export_z(float* out) {
+    with_storage(S->opt.vcycle_storage, [&](auto c) {
+      constexpr int ST = decltype(c)::value;
+      FSP_LAUNCH(S, k_unpack<ST>, dim3(grid_for(G.n)), st, G.n, (const TS<ST>*)S->zres, out);
+      return 0;
+    });
+  }
+
   void iteration() {
     if (S->quad) {
       const QuadGrid& Q = S->qfine;
       const dim3 g((unsigned)(Q.nbx * Q.nby * Q.planes));
-      FSP_LAUNCH(S, k_q_spmv_dir, g, st, G, Q, (const float*)S->zres, S->d0, S->d1, S->q, S->ctl, S->partials);
+      const int storage = (S->opt.preconditioner != FSP_PRECOND_JACOBI && !S->net_phase) ? S->opt.vcycle_storage
+                                                                                           : FSP_STORE_F32;
+      with_storage(storage, [&](auto c) {
+        constexpr int ST = decltype(c)::value;
+        FSP_LAUNCH_N(S, "k_q_spmv_dir", K_QSPMV, g, st, G, Q, (const TS<ST>*)S->zres, S->d0, S->d1, S->q, S->ctl,
+                     S->partials);
+        return 0;
+      });
       FSP_FINISH(S, st, OP_DQ, g, false, 1);
       const size_t nquads = G.n / 4;
       const dim3 gu((unsigned)((nquads + QBLOCK - 1) / QBLOCK));
@@ -410,8 +553,10 @@ struct Launch {
     // A pass ends after replace_every iterations so the recurrence residual
     // is replaced by the true one (f64) at least that often.
     const uint32_t every = S->opt.replace_every > 0 ? (uint32_t)S->opt.replace_every : 0u;
-    const uint32_t limit = every ? (it + every < it ? 0xffffffffu : it + every) : 0xffffffffu;
-    hipLaunchKernelGGL(k_set_pass, dim3(1), dim3(64), 0, st, S->ctl, active, rr, tol_b2, target, it, max_iters, limit);
+    uint32_t limit = every ? (it + every < it ? 0xffffffffu : it + every) : 0xffffffffu;
+    if (S->net_phase) limit = std::min(limit, S->net_until);
+    hipLaunchKernelGGL(k_set_pass, dim3(1), dim3(64), 0, st, S->ctl, active, rr, tol_b2, target, it, max_iters, limit,
+                       S->net_phase);
   }
 };
 
@@ -422,6 +567,12 @@ fsp_status sync_ctl(fsp_solver* S) {
 }
 
 void cleanup(fsp_solver* S) {
+  for (int g = 0; g < 2; ++g) {
+    if (S->iter_exec[g]) (void)hipGraphExecDestroy(S->iter_exec[g]);
+    if (S->iter_graph[g]) (void)hipGraphDestroy(S->iter_graph[g]);
+    S->iter_exec[g] = nullptr;
+    S->iter_graph[g] = nullptr;
+  }
   for (hipEvent_t e : S->prof_events) (void)hipEventDestroy(e);
   S->prof_events.clear();
   for (void* p : S->allocs) (void)hipFree(p);
@@ -484,19 +635,52 @@ fsp_status fingerprint(fsp_solver* S, const double* p, uint64_t& h) {
   return FSP_OK;
 }
 
+// Captures one CG iteration (every kernel reads its scalars from the device
+// control block and every buffer is the solver's own, so the graph stays
+// valid for the solver's lifetime) and instantiates it once.
+fsp_status capture_iteration(fsp_solver* S) {
+  const int slot = S->net_phase;
+  if (S->iter_exec[slot]) return FSP_OK;
+  const bool prof = S->prof_on;
+  S->prof_on = false;
+  FSP_HIP(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
+  {
+    Launch K(S);
+    K.iteration();
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(S->stream, &g);
+  S->prof_on = prof;
+  if (e != hipSuccess || !g) {
+    std::fprintf(stderr, "fsp: graph capture failed: %s\n", hipGetErrorString(e));
+    return FSP_DEVICE;
+  }
+  S->iter_graph[slot] = g;
+  FSP_HIP(hipGraphInstantiate(&S->iter_exec[slot], g, nullptr, nullptr, 0));
+  return FSP_OK;
+}
+
 // Geometry rebuilt from the surface at the start of every projection
 // (delta3d_mobile.rs:479-484).
 void build_geometry(fsp_solver* S, bool hierarchy) {
   Launch K(S);
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, K.st, S->ctl, S->opt.gauge);
   K.mask();
-  if (hierarchy && S->opt.preconditioner == FSP_PRECOND_MULTIGRID) K.hierarchy();
+  if (hierarchy && S->opt.preconditioner != FSP_PRECOND_JACOBI) K.hierarchy();
 }
 
 // The projection on device buffers (project_mobile3, delta3d_mobile.rs:465-609).
 fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws, float* p, int warm, float* u,
                float* v, float* w, float scale, float k1, double rel_tol, uint32_t max_iters, fsp_report* rep) {
   if (!S->geom) return FSP_DOMAIN;
+  S->net_phase = 0;
+  if (S->opt.preconditioner == FSP_PRECOND_LEARNED) {
+    if (!S->net) return FSP_INVALID;
+    const fsp_status ns = net_reserve(S->net, S->dom.nx, S->dom.ny, S->dom.nz);
+    if (ns) return ns;
+    S->net_phase = 1;
+    S->net_until = S->opt.learned_budget > 0 ? (uint32_t)S->opt.learned_budget : 50u;
+  }
   Launch K(S);
   const Fine& G = K.G;
   FSP_HIP(hipEventRecord(S->ev0, S->stream));
@@ -543,6 +727,7 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
   };
   for (;;) {
     const uint32_t before = it;
+    bool net_failed = false;
     if (loop_test(rr, it)) {
       K.set_pass(1, rr, tol_b2, physical_target, it, max_iters);
       if (!primed) {
@@ -554,9 +739,17 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
       double rr_prev = rr;
       uint32_t it_prev = it;
       for (;;) {
-        for (int s = 0; s < batch; ++s) K.iteration();
+        if (S->opt.graph && !S->prof_on && (st = capture_iteration(S))) return st;
+        for (int s = 0; s < batch; ++s) {
+          if (S->iter_exec[S->net_phase] && !S->prof_on) FSP_HIP(hipGraphLaunch(S->iter_exec[S->net_phase], S->stream));
+          else K.iteration();
+        }
         if ((st = sync_ctl(S))) return st;
-        if (S->hctl->nonfinite) return FSP_NOT_FINITE;
+        if (S->hctl->nonfinite) {
+          if (!S->net_phase) return FSP_NOT_FINITE;
+          net_failed = true;  // the network's output: fall back to MGPCG from here
+          break;
+        }
         if (!S->hctl->active) break;
         const double rr_now = S->hctl->rr;
         const uint32_t it_now = S->hctl->it;
@@ -580,6 +773,7 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
       }
       it = S->hctl->it;
       rr = S->hctl->rr;
+      if (net_failed) K.set_pass(0, rr, tol_b2, physical_target, it, max_iters);  // clears the flag
     }
     // True residual, recomputed: it decides, never the recurrence (:546-553).
     FSP_LAUNCH(S, k_residual, dim3(grid_for(G.n)), K.st, G, (const double*)p64, S->b, S->r,
@@ -588,7 +782,15 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
     if ((st = sync_ctl(S))) return st;
     if (S->hctl->nonfinite) return FSP_NOT_FINITE;
     const double actual = S->hctl->actual;
-    const bool exhausted = it >= max_iters || it == before;
+    // The learned phase ends at its budget, on a breakdown (<d, A d> <= 0 or a
+    // non-finite network output) or when a pass made no progress; MGPCG
+    // then continues from the current pressure.
+    bool switched = false;
+    if (S->net_phase && (net_failed || S->hctl->stop_dq || it >= S->net_until || it == before)) {
+      S->net_phase = 0;
+      switched = true;
+    }
+    const bool exhausted = it >= max_iters || (it == before && !switched);
     bool have_measured = false;
     double m_all = 0.0, m_plain = 0.0;
     if (actual <= tol_b2) {
@@ -698,6 +900,8 @@ void fsp_default_options(fsp_options* o) {
   o->refuse_degraded = 0;
   o->device = 0;
   o->coarse_operator = FSP_COARSE_REDISCRETISED;
+  o->graph = 1;
+  o->learned_budget = 0;
 }
 
 fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** out) {
@@ -709,11 +913,12 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
   fsp_options opt;
   fsp_default_options(&opt);
   if (o) opt = *o;
-  if (opt.preconditioner < 0 || opt.preconditioner > 1 || opt.smoother < 0 || opt.smoother > 1) return FSP_INVALID;
+  if (opt.preconditioner < 0 || opt.preconditioner > 2 || opt.smoother < 0 || opt.smoother > 1) return FSP_INVALID;
+  if (opt.learned_budget < 0) return FSP_INVALID;
   if (opt.pre_sweeps < 0 || opt.post_sweeps < 0 || opt.coarse_sweeps < 0 || opt.pre_sweeps > 64 ||
       opt.post_sweeps > 64 || opt.coarse_sweeps > 1024)
     return FSP_INVALID;
-  if (opt.vcycle_storage != FSP_STORE_F32) return FSP_INVALID;
+  if (opt.vcycle_storage < FSP_STORE_F32 || opt.vcycle_storage > FSP_STORE_BF16) return FSP_INVALID;
   if (opt.coarse_operator < 0 || opt.coarse_operator > 1) return FSP_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || opt.device < 0 || opt.device >= ndev) return FSP_DEVICE;
@@ -750,7 +955,7 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
   if (hipHostMalloc((void**)&S->hctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess) return fail(FSP_DEVICE);
   if (hipHostMalloc((void**)&S->hhash, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
     return fail(FSP_DEVICE);
-  if (opt.preconditioner == FSP_PRECOND_MULTIGRID) {
+  if (opt.preconditioner != FSP_PRECOND_JACOBI) {  // the learned preconditioner falls back to the V-cycle
     size_t x = nx, y = ny, z = nz;
     float h = d->dx;
     while (coarsens3(x, y, z) && (opt.max_levels < 0 || (int)S->levels.size() < opt.max_levels)) {
@@ -789,11 +994,38 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
   S->tail_first = (int)S->levels.size();
   for (int t = (int)S->levels.size() - 1; t >= 0; --t) {
     if (S->levels[t].n > 4096 || (int)S->levels.size() - t > TAIL_MAX_LEVELS) break;
-    const size_t bytes = tail_lds_bytes(&S->levels[t], (int)S->levels.size() - t, opt.smoother);
+    const int count = (int)S->levels.size() - t;
+    const size_t bytes = gc_tail_ok(opt) ? gc_tail_lds_bytes(&S->levels[t], count)
+                                         : tail_lds_bytes(&S->levels[t], count, opt.smoother);
     if (bytes > 64 * 1024) break;
     S->tail_first = t;
     S->tail_lds = bytes;
   }
+  // Compact Galerkin levels: every level above the tail whose couplings are
+  // exact in binary16 (level <= 5) and whose rows split into quads.
+  S->cq.assign(S->levels.size(), QuadGrid{});
+  S->cqr.assign(S->levels.size(), QuadGrid{});
+  if (opt.coarse_operator == FSP_COARSE_GALERKIN && opt.preconditioner != FSP_PRECOND_JACOBI &&
+      quad_vcycle_ok(S->quad, opt)) {
+    for (int l = 0; l < S->tail_first && l < 5; ++l) {
+      Level& L = S->levels[l];
+      if (L.nx % 4 != 0) continue;
+      QuadGrid q = make_q(L.nx / 4, L.ny, L.nz);
+      const size_t blocks = (size_t)q.nbx * q.nby * q.planes;
+      if (blocks > 0xFFFFFFFFu) continue;
+      if ((st = dalloc(S, &L.cx, L.n)) || (st = dalloc(S, &L.cy, L.n)) || (st = dalloc(S, &L.cz, L.n)))
+        return fail(st);
+      L.compact = 1;
+      S->cq[l] = q;
+      if (l + 1 < (int)S->levels.size()) S->cqr[l] = make_q(L.nx / 4, L.ny / 2, L.nz / 2);
+    }
+  }
+  // 16-bit V-cycle storage exists on the red-black quad path only
+  if (opt.vcycle_storage != FSP_STORE_F32 &&
+      !(opt.preconditioner != FSP_PRECOND_JACOBI && quad_vcycle_ok(S->quad, opt)))
+    return fail(FSP_INVALID);
+  // the network's output is an f32 vector; it shares z with the V-cycle
+  if (opt.preconditioner == FSP_PRECOND_LEARNED && opt.vcycle_storage != FSP_STORE_F32) return fail(FSP_INVALID);
   S->zres = S->z;
   *out = S;
   return FSP_OK;
@@ -945,6 +1177,53 @@ fsp_status fsp_assemble_device(fsp_solver* S, const float* d_div, float scale, f
   return FSP_OK;
 }
 
+fsp_status fsp_attach_net(fsp_solver* S, fsp_net* net) {
+  if (!S) return FSP_INVALID;
+  if (net && !net_shape_ok(S->dom.nx, S->dom.ny, S->dom.nz)) return FSP_SHAPE;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  S->net = net;
+  // graphs of the learned phase hold the previous network's buffers
+  if (S->iter_exec[1]) (void)hipGraphExecDestroy(S->iter_exec[1]);
+  if (S->iter_graph[1]) (void)hipGraphDestroy(S->iter_graph[1]);
+  S->iter_exec[1] = nullptr;
+  S->iter_graph[1] = nullptr;
+  if (net) return net_reserve(net, S->dom.nx, S->dom.ny, S->dom.nz);
+  return FSP_OK;
+}
+
+fsp_status fsp_net_features_device(fsp_net* net, fsp_solver* S, const float* d_r, uint16_t* d_x, float sigma,
+                                   float* sigma_out) {
+  if (!net || !S || !d_r || !d_x) return FSP_INVALID;
+  if (!S->geom) return FSP_DOMAIN;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  fsp_status st = net_reserve(net, S->dom.nx, S->dom.ny, S->dom.nz);
+  if (st) return st;
+  build_geometry(S, false);
+  Launch K(S);
+  const double* rr = nullptr;
+  if (!(sigma > 0.f)) {
+    // |r|^2 over wet cells into the control block (OP_GENERIC slot)
+    FSP_LAUNCH(S, k_dot, dim3(grid_for(S->n)), S->stream, S->n, d_r, d_r, OP_GENERIC, 0, S->ctl, S->partials);
+    FSP_FINISH(S, S->stream, OP_GENERIC, dim3(grid_for(S->n)), false, 0);
+    rr = &S->ctl->red[OP_GENERIC][0];
+  }
+  if ((st = net_features(net, S->stream, d_r, S->mask, S->solid, rr, &S->ctl->wet_cells, sigma, d_x))) return st;
+  float s_used = sigma;
+  if (sigma_out) {
+    FSP_HIP(hipStreamSynchronize(S->stream));
+    if (!(sigma > 0.f)) {
+      if ((st = sync_ctl(S))) return st;
+      const double r2 = S->hctl->red[OP_GENERIC][0];
+      const double nw = S->hctl->wet_cells > 0 ? (double)S->hctl->wet_cells : 1.0;
+      s_used = r2 > 0.0 ? (float)std::sqrt(nw / r2) : 1.0f;
+    }
+    *sigma_out = s_used;
+  }
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  return FSP_OK;
+}
+
 fsp_status fsp_precondition_device(fsp_solver* S, const float* d_r, float* d_z) {
   if (!S || !d_r || !d_z) return FSP_INVALID;
   if (!S->geom) return FSP_DOMAIN;
@@ -954,7 +1233,7 @@ fsp_status fsp_precondition_device(fsp_solver* S, const float* d_r, float* d_z) 
   FSP_HIP(hipMemcpyAsync(S->r, d_r, S->n * 4, hipMemcpyDeviceToDevice, S->stream));
   K.set_pass(1, 0.0, 0.0, 0.0, 0, 1);
   K.precondition(OP_GENERIC);
-  FSP_HIP(hipMemcpyAsync(d_z, S->zres, S->n * 4, hipMemcpyDeviceToDevice, S->stream));
+  K.export_z(d_z);
   FSP_HIP(hipStreamSynchronize(S->stream));
   return FSP_OK;
 }

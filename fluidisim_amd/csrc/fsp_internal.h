@@ -52,6 +52,7 @@ struct Ctl {
   int32_t stop_dq;       // the loop stopped on <d, A d> <= 0
   int32_t pure_neumann;  // no Dirichlet face on the fine level (and gauge on)
   int32_t gauge;
+  int32_t flexible;      // 1: beta = -alpha <z', q> / <z, r> (flexible CG), else <z', r'> / <z, r>
   uint32_t wet_cells, ghost_faces;
   uint32_t counters[NUM_SLOTS];
 };
@@ -91,6 +92,110 @@ struct Level {
   float* t;
   float* dd;             // Galerkin levels: Dirichlet part of the diagonal (units of inv)
   int galerkin;          // 1: P^T A P / 16 coarse operator, 0: re-discretised
+  int compact;           // 1: smoothed by the compact quad kernels (Galerkin, binary16 couplings)
+  uint16_t* cx;          // compact levels: binary16 coupling of each cell's x- face
+  uint16_t* cy;          // ... y- face
+  uint16_t* cz;          // ... z- face
+};
+
+// ---------------------------------------------------------------- V-cycle storage
+// V-cycle vectors (the fine iterate z and every level's x and r) may be held
+// in 16-bit storage: arithmetic stays f32, and each value is rounded to the
+// storage type (round to nearest even) when it is stored. Store<ST>::rnd is
+// that rounding as an f32 value, so code that keeps a stored value in a
+// register uses exactly what a reload would give.
+enum : int { ST_F32 = 0, ST_F16 = 1, ST_BF16 = 2 };
+
+template <int ST> struct Store;
+
+template <> struct Store<ST_F32> {
+  using T = float;
+  static __device__ __forceinline__ float rnd(float v) { return v; }
+  static __device__ __forceinline__ float ld(const T* p, size_t c) { return p[c]; }
+  static __device__ __forceinline__ void st(T* p, size_t c, float v) { p[c] = v; }
+  static __device__ __forceinline__ float2 ld2(const T* p, size_t c) { return *reinterpret_cast<const float2*>(p + c); }
+  static __device__ __forceinline__ void st2(T* p, size_t c, float2 v) { *reinterpret_cast<float2*>(p + c) = v; }
+  static __device__ __forceinline__ float4 ld4(const T* p, size_t c) { return *reinterpret_cast<const float4*>(p + c); }
+  static __device__ __forceinline__ void st4(T* p, size_t c, float4 v) { *reinterpret_cast<float4*>(p + c) = v; }
+};
+
+// bfloat16: the upper half of the f32 bit pattern.
+template <> struct Store<ST_BF16> {
+  using T = uint16_t;
+  static __device__ __forceinline__ float rnd(float v) {
+    uint32_t u = __float_as_uint(v);
+    if ((u & 0x7F800000u) == 0x7F800000u) return v;  // inf / nan unchanged
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xFFFF0000u);
+  }
+  static __device__ __forceinline__ float up(uint32_t h) { return __uint_as_float(h << 16); }
+  static __device__ __forceinline__ uint32_t down(float v) { return __float_as_uint(rnd(v)) >> 16; }
+  static __device__ __forceinline__ float ld(const T* p, size_t c) { return up(p[c]); }
+  static __device__ __forceinline__ void st(T* p, size_t c, float v) { p[c] = (T)down(v); }
+  static __device__ __forceinline__ float2 ld2(const T* p, size_t c) {
+    const uint32_t w = *reinterpret_cast<const uint32_t*>(p + c);
+    return make_float2(up(w & 0xFFFFu), up(w >> 16));
+  }
+  static __device__ __forceinline__ void st2(T* p, size_t c, float2 v) {
+    *reinterpret_cast<uint32_t*>(p + c) = down(v.x) | (down(v.y) << 16);
+  }
+  static __device__ __forceinline__ float4 ld4(const T* p, size_t c) {
+    const uint2 w = *reinterpret_cast<const uint2*>(p + c);
+    return make_float4(up(w.x & 0xFFFFu), up(w.x >> 16), up(w.y & 0xFFFFu), up(w.y >> 16));
+  }
+  static __device__ __forceinline__ void st4(T* p, size_t c, float4 v) {
+    *reinterpret_cast<uint2*>(p + c) = make_uint2(down(v.x) | (down(v.y) << 16), down(v.z) | (down(v.w) << 16));
+  }
+};
+
+// IEEE binary16, converted in software (round to nearest even, subnormals
+// kept, overflow to infinity) so that no packed or truncating hardware
+// conversion can be substituted for it.
+__device__ __forceinline__ uint32_t f32_to_f16_bits(float f) {
+  const uint32_t x = __float_as_uint(f);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  const uint32_t ax = x & 0x7FFFFFFFu;
+  if (ax >= 0x7F800000u) return sign | (ax > 0x7F800000u ? 0x7E00u : 0x7C00u);
+  if (ax >= 0x477FF000u) return sign | 0x7C00u;  // >= 65520 rounds to infinity
+  if (ax < 0x38800000u) {                         // below 2^-14: step 2^-24
+    const uint32_t m = (uint32_t)rintf(__uint_as_float(ax) * 16777216.0f);
+    return sign | m;
+  }
+  const uint32_t r = ax + 0xFFFu + ((ax >> 13) & 1u);
+  return sign | ((r >> 13) - (112u << 10));
+}
+__device__ __forceinline__ float f16_bits_to_f32(uint32_t h) {
+  const uint32_t sign = (h & 0x8000u) << 16;
+  const uint32_t e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+  if (e == 0) {
+    const float v = (float)m * 5.9604644775390625e-08f;  // exact
+    return sign ? -v : v;
+  }
+  if (e == 31) return __uint_as_float(sign | 0x7F800000u | (m << 13));
+  return __uint_as_float(sign | ((e + 112u) << 23) | (m << 13));
+}
+
+template <> struct Store<ST_F16> {
+  using T = uint16_t;
+  static __device__ __forceinline__ float rnd(float v) { return f16_bits_to_f32(f32_to_f16_bits(v)); }
+  static __device__ __forceinline__ float up(uint32_t h) { return f16_bits_to_f32(h); }
+  static __device__ __forceinline__ uint32_t down(float v) { return f32_to_f16_bits(v); }
+  static __device__ __forceinline__ float ld(const T* p, size_t c) { return up(p[c]); }
+  static __device__ __forceinline__ void st(T* p, size_t c, float v) { p[c] = (T)down(v); }
+  static __device__ __forceinline__ float2 ld2(const T* p, size_t c) {
+    const uint32_t w = *reinterpret_cast<const uint32_t*>(p + c);
+    return make_float2(up(w & 0xFFFFu), up(w >> 16));
+  }
+  static __device__ __forceinline__ void st2(T* p, size_t c, float2 v) {
+    *reinterpret_cast<uint32_t*>(p + c) = down(v.x) | (down(v.y) << 16);
+  }
+  static __device__ __forceinline__ float4 ld4(const T* p, size_t c) {
+    const uint2 w = *reinterpret_cast<const uint2*>(p + c);
+    return make_float4(up(w.x & 0xFFFFu), up(w.x >> 16), up(w.y & 0xFFFFu), up(w.y >> 16));
+  }
+  static __device__ __forceinline__ void st4(T* p, size_t c, float4 v) {
+    *reinterpret_cast<uint2*>(p + c) = make_uint2(down(v.x) | (down(v.y) << 16), down(v.z) | (down(v.w) << 16));
+  }
 };
 
 }  // namespace fsp

@@ -108,8 +108,9 @@ __device__ void finish(int op, Ctl* ctl, double a, double b) {
       if (!isfinite(a)) { ctl->nonfinite = 1; ctl->active = 0; }
       break;
     case OP_RZ:
-      if (!isfinite(a)) { ctl->nonfinite = 1; ctl->active = 0; break; }
-      ctl->beta = a / ctl->rz;
+      if (!isfinite(a) || (ctl->flexible && !isfinite(b))) { ctl->nonfinite = 1; ctl->active = 0; break; }
+      // flexible CG (Polak-Ribiere): <z', r' - r> = -alpha <z', q> since r' = r - alpha q
+      ctl->beta = ctl->flexible ? -(ctl->alpha * b) / ctl->rz : a / ctl->rz;
       ctl->rz = a;
       ctl->rr = ctl->rn;
       ctl->it += 1;
@@ -594,6 +595,19 @@ __global__ void __launch_bounds__(BLOCK) k_dot(size_t n, const float* a, const f
   reduce_finish<false>(s, unused, op, ctl, partials);
 }
 
+// <r, z> and <q, z> in one pass (flexible CG).
+__global__ void __launch_bounds__(BLOCK) k_dot_rz_qz(size_t n, const float* r, const float* z, const float* q, int op,
+                                                     Ctl* ctl, double* partials) {
+  if (!ctl->active) return;
+  double a = 0.0, b = 0.0;
+  FSP_FOR(c, n) {
+    const double zc = (double)z[c];
+    a += (double)r[c] * zc;
+    b += (double)q[c] * zc;
+  }
+  reduce_finish<false>(a, b, op, ctl, partials);
+}
+
 // Jacobi preconditioner z = prec * r with <r, z> (delta3d_mobile.rs:435-461).
 __global__ void __launch_bounds__(BLOCK) k_jacobi_prec(Fine G, const float* r, float* z, int op, Ctl* ctl,
                                                        double* partials) {
@@ -666,10 +680,14 @@ __device__ __forceinline__ void ldecode(const Level& L, size_t c, int& i, int& j
 }
 
 // row_sums (delta3d_multigrid.rs:216-255); x == nullptr gives the diagonal.
-__device__ __forceinline__ float coarse_row(const Level& L, int i, int j, int k, size_t c, const float* x) {
+// x is held in storage ST.
+template <int ST = ST_F32>
+__device__ __forceinline__ float coarse_row(const Level& L, int i, int j, int k, size_t c,
+                                            const typename Store<ST>::T* x) {
 #pragma clang fp contract(off)
+  using SV = Store<ST>;
   if (L.kind[c] <= 0) return 0.f;
-  const float pc = x ? x[c] : 0.f;
+  const float pc = x ? SV::ld(x, c) : 0.f;
   float acc = 0.f;
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   const size_t plane = (size_t)nx * ny;
@@ -678,7 +696,7 @@ __device__ __forceinline__ float coarse_row(const Level& L, int i, int j, int k,
     if (a == 0.f) return;
     if (has) {
       const int8_t kn = L.kind[n];
-      if (kn > 0) acc += x ? a * (pc - x[n]) : a;
+      if (kn > 0) acc += x ? a * (pc - SV::ld(x, n)) : a;
       else if (!gal && kn < 0) acc += 2.0f * a * (x ? pc : 1.0f);
     } else if (!gal && top) {
       acc += 2.0f * a * (x ? pc : 1.0f);
@@ -841,7 +859,7 @@ __global__ void __launch_bounds__(BLOCK) k_cdiag(Level L) {
   FSP_FOR(c, L.n) {
     int i, j, k;
     ldecode(L, c, i, j, k);
-    L.diag[c] = coarse_row(L, i, j, k, c, nullptr);
+    L.diag[c] = coarse_row<ST_F32>(L, i, j, k, c, nullptr);
   }
 }
 
@@ -861,24 +879,28 @@ __global__ void __launch_bounds__(BLOCK) k_cjacobi(Level L, const float* xin, fl
     xout[c] = xc;
   }
 }
-// One colour of red-black Gauss-Seidel on a coarse level, in place.
-__global__ void __launch_bounds__(BLOCK) k_cgs(Level L, float* x, int colour, int first, Ctl* ctl) {
+// One colour of red-black Gauss-Seidel on a coarse level, in place; x and
+// L.r are held in storage ST.
+template <int ST>
+__global__ void __launch_bounds__(BLOCK) k_cgs(Level L, typename Store<ST>::T* x, int colour, int first, Ctl* ctl) {
 #pragma clang fp contract(off)
+  using SV = Store<ST>;
   if (!ctl->active) return;
+  const typename SV::T* lr = reinterpret_cast<const typename SV::T*>(L.r);
   FSP_FOR(c, L.n) {
     int i, j, k;
     ldecode(L, c, i, j, k);
     const int col = (i + j + k) & 1;
     const float dg = L.diag[c];
     if (first) {
-      if (col != colour) { x[c] = 0.f; continue; }
-      x[c] = dg > 0.f ? L.r[c] / dg : 0.f;
+      if (col != colour) { SV::st(x, c, 0.f); continue; }
+      SV::st(x, c, dg > 0.f ? SV::ld(lr, c) / dg : 0.f);
       continue;
     }
     if (col != colour) continue;
     if (dg > 0.f) {
-      const float t = coarse_row(L, i, j, k, c, x);
-      x[c] = x[c] + (L.r[c] - t) / dg;
+      const float t = coarse_row<ST>(L, i, j, k, c, x);
+      SV::st(x, c, SV::ld(x, c) + (SV::ld(lr, c) - t) / dg);
     }
   }
 }
@@ -901,9 +923,13 @@ __global__ void __launch_bounds__(BLOCK) k_restrict_fine(Fine G, Level L, const 
   }
 }
 // Coarse residual restricted one level down (delta3d_multigrid.rs:312-321).
-__global__ void __launch_bounds__(BLOCK) k_crestrict(Level F, Level L, const float* xf, Ctl* ctl) {
+template <int ST>
+__global__ void __launch_bounds__(BLOCK) k_crestrict(Level F, Level L, const typename Store<ST>::T* xf, Ctl* ctl) {
 #pragma clang fp contract(off)
+  using SV = Store<ST>;
   if (!ctl->active) return;
+  const typename SV::T* fr = reinterpret_cast<const typename SV::T*>(F.r);
+  typename SV::T* lr = reinterpret_cast<typename SV::T*>(L.r);
   FSP_FOR(c, L.n) {
     int i, j, k;
     ldecode(L, c, i, j, k);
@@ -911,9 +937,9 @@ __global__ void __launch_bounds__(BLOCK) k_crestrict(Level F, Level L, const flo
     for (int ch = 0; ch < 8; ++ch) {
       const int fi = 2 * i + (ch & 1), fj = 2 * j + ((ch >> 1) & 1), fk = 2 * k + ((ch >> 2) & 1);
       const size_t f = lidx(F, fi, fj, fk);
-      s += F.r[f] - coarse_row(F, fi, fj, fk, f, xf);
+      s += SV::ld(fr, f) - coarse_row<ST>(F, fi, fj, fk, f, xf);
     }
-    L.r[c] = 0.125f * s;
+    SV::st(lr, c, 0.125f * s);
   }
 }
 // Prolongation by injection (delta3d_multigrid.rs:275-287); on the fine
@@ -928,12 +954,16 @@ __global__ void __launch_bounds__(BLOCK) k_prolong_fine(Fine G, Level L, const f
     z[c] += xc[lidx(L, i >> 1, j >> 1, k >> 1)];
   }
 }
-__global__ void __launch_bounds__(BLOCK) k_cprolong(Level F, Level L, const float* xc, float* xf, Ctl* ctl) {
+template <int ST>
+__global__ void __launch_bounds__(BLOCK) k_cprolong(Level F, Level L, const typename Store<ST>::T* xc,
+                                                    typename Store<ST>::T* xf, Ctl* ctl) {
+#pragma clang fp contract(off)
+  using SV = Store<ST>;
   if (!ctl->active) return;
   FSP_FOR(c, F.n) {
     int i, j, k;
     ldecode(F, c, i, j, k);
-    xf[c] += xc[lidx(L, i >> 1, j >> 1, k >> 1)];
+    SV::st(xf, c, SV::ld(xf, c) + SV::ld(xc, lidx(L, i >> 1, j >> 1, k >> 1)));
   }
 }
 
@@ -1088,9 +1118,10 @@ __global__ void k_hash_combine(const unsigned long long* hashes, size_t chunks, 
 
 // Host-side parameters of a CG pass, written on the stream.
 __global__ void k_set_pass(Ctl* ctl, int active, double rr, double tol_b2, double physical_target, uint32_t it,
-                           uint32_t max_iters, uint32_t pass_limit) {
+                           uint32_t max_iters, uint32_t pass_limit, int flexible) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   ctl->active = active;
+  ctl->flexible = flexible;
   ctl->rr = rr;
   ctl->tol_b2 = tol_b2;
   ctl->physical_target = physical_target;
@@ -1098,6 +1129,7 @@ __global__ void k_set_pass(Ctl* ctl, int active, double rr, double tol_b2, doubl
   ctl->max_iters = max_iters;
   ctl->pass_limit = pass_limit;
   ctl->stop_dq = 0;
+  ctl->nonfinite = 0;
 }
 
 }  // namespace fsp

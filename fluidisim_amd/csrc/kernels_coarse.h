@@ -1,0 +1,402 @@
+// Compact Galerkin coarse levels on quads.
+//
+// A Galerkin coarse level (P^T A P / 16, P piecewise constant) couples a cell
+// to a neighbour only through a face whose coupling is non-zero, and such a
+// face always joins two active cells; an inactive cell has no coupling and no
+// Dirichlet part. The row of the operator is therefore a function of the
+// couplings and the Dirichlet part alone, with no cell kinds:
+//
+//   row(x)_c = inv * ( sum over faces f in order x-, x+, y-, y+, z-, z+ with
+//                      a_f != 0 of a_f (x_c - x_f)  +  dd_c x_c )
+//   diag_c   = inv * ( sum of those a_f  +  dd_c )
+//
+// which is row_sums (delta3d_multigrid.rs:216-255) on such a level, term for
+// term. The couplings are stored cell-aligned (a cell holds its x-, y- and z-
+// faces; its + faces are the next cell's - faces) in binary16: on level l
+// (1-based) they are multiples of 4^-l no larger than 1, exact in binary16 up
+// to l = 5. Every load of a pass is independent of the others (no kind test
+// before a neighbour load), so a pass costs one round of memory latency.
+#pragma once
+#include "kernels_fast.h"
+
+namespace fsp {
+
+typedef _Float16 gc_h4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4 gc_ld4(const uint16_t* a, size_t c) {
+  const gc_h4 w = *reinterpret_cast<const gc_h4*>(a + c);
+  return make_float4((float)w.x, (float)w.y, (float)w.z, (float)w.w);
+}
+__device__ __forceinline__ float gc_ld(const uint16_t* a, size_t c) {
+  return (float)*reinterpret_cast<const _Float16*>(a + c);
+}
+
+// Cell-aligned binary16 couplings of a Galerkin level from its face arrays.
+__global__ void __launch_bounds__(BLOCK) k_gc_build(Level L) {
+  FSP_FOR(c, L.n) {
+    int i, j, k;
+    ldecode(L, c, i, j, k);
+    const float ax = i > 0 ? L.ou[((size_t)k * L.ny + j) * (L.nx + 1) + i] : 0.f;
+    const float ay = j > 0 ? L.ov[((size_t)k * (L.ny + 1) + j) * L.nx + i] : 0.f;
+    const float az = k > 0 ? L.ow[((size_t)k * L.ny + j) * L.nx + i] : 0.f;
+    reinterpret_cast<_Float16*>(L.cx)[c] = (_Float16)ax;
+    reinterpret_cast<_Float16*>(L.cy)[c] = (_Float16)ay;
+    reinterpret_cast<_Float16*>(L.cz)[c] = (_Float16)az;
+  }
+}
+
+// The six couplings of the four cells of a quad, and their Dirichlet parts.
+struct GcQuad {
+  float4 xm, ym, yp, zm, zp, dd;
+  float xp_last;  // x+ face of the last cell (x- face of the next quad's first)
+};
+
+__device__ __forceinline__ GcQuad gc_quad(const Level& L, size_t c0, int x0, int y, int k) {
+  const size_t plane = (size_t)L.nx * L.ny;
+  GcQuad g;
+  g.xm = gc_ld4(L.cx, c0);
+  g.xp_last = (x0 + 4 < L.nx) ? gc_ld(L.cx, c0 + 4) : 0.f;
+  g.ym = gc_ld4(L.cy, c0);
+  g.yp = (y + 1 < L.ny) ? gc_ld4(L.cy, c0 + L.nx) : f4zero();
+  g.zm = gc_ld4(L.cz, c0);
+  g.zp = (k + 1 < L.nz) ? gc_ld4(L.cz, c0 + plane) : f4zero();
+  g.dd = ld4(L.dd, c0);
+  return g;
+}
+
+__device__ __forceinline__ void gc_faces(const GcQuad& g, int e, float a[6]) {
+  a[0] = q_get(g.xm, e);
+  a[1] = e < 3 ? q_get(g.xm, e + 1) : g.xp_last;
+  a[2] = q_get(g.ym, e);
+  a[3] = q_get(g.yp, e);
+  a[4] = q_get(g.zm, e);
+  a[5] = q_get(g.zp, e);
+}
+
+__device__ __forceinline__ float gc_diag(const float a[6], float dd, float inv) {
+#pragma clang fp contract(off)
+  float acc = 0.f;
+#pragma unroll
+  for (int f = 0; f < 6; ++f)
+    if (a[f] != 0.f) acc += a[f];
+  acc += dd;
+  return acc * inv;
+}
+
+__device__ __forceinline__ float gc_row(const float a[6], float dd, float inv, float pc, const float nb[6]) {
+#pragma clang fp contract(off)
+  float acc = 0.f;
+#pragma unroll
+  for (int f = 0; f < 6; ++f)
+    if (a[f] != 0.f) acc += a[f] * (pc - nb[f]);
+  acc += dd * pc;
+  return acc * inv;
+}
+
+// One colour of a red-black pass on a compact level, quads of four x-cells.
+// MODE 0: in place (x == xout).
+// MODE 1: first pass from x = 0: colour cells r/diag, the others 0.
+// MODE 2: x is first prolonged from the coarser level C (x += parent, as
+//         prolong_add3 stores it), then the colour is updated; out of place.
+// x, xout, the parents xp and L.r are held in storage ST.
+template <int MODE, int ST>
+__global__ void __launch_bounds__(QBLOCK) k_gc_gs(Level L, QuadGrid Q, const typename Store<ST>::T* x,
+                                                  typename Store<ST>::T* xout, int colour, Level C,
+                                                  const typename Store<ST>::T* xp, Ctl* ctl) {
+#pragma clang fp contract(off)
+  using SV = Store<ST>;
+  using T = typename SV::T;
+  if (!ctl->active) return;
+  int qx, y, k;
+  const bool valid = quad_coords(Q, qx, y, k);
+  const size_t plane = (size_t)L.nx * L.ny;
+  const size_t c0 = valid ? ((size_t)k * L.ny + y) * L.nx + 4 * (size_t)qx : 0;
+  const T* lr = reinterpret_cast<const T*>(L.r);
+  GcQuad g{};
+  float4 rq = f4zero();
+  if (valid) {
+    g = gc_quad(L, c0, 4 * qx, y, k);
+    rq = SV::ld4(lr, c0);
+  }
+  if (MODE == 1) {
+    if (!valid) return;
+    float4 out = f4zero();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (((4 * qx + e + y + k) & 1) != colour) continue;
+      float a[6];
+      gc_faces(g, e, a);
+      const float dg = gc_diag(a, q_get(g.dd, e), L.inv);
+      if (dg > 0.f) q_set(out, e, SV::rnd(q_get(rq, e) / dg));
+    }
+    SV::st4(xout, c0, out);
+    return;
+  }
+  auto parent = [&](int yy, int kk) -> float2 { return SV::ld2(xp, lidx(C, 2 * qx, yy >> 1, kk >> 1)); };
+  auto prolong = [&](float4& v, float2 a) {
+    v.x = SV::rnd(v.x + a.x); v.y = SV::rnd(v.y + a.x); v.z = SV::rnd(v.z + a.y); v.w = SV::rnd(v.w + a.y);
+  };
+  float4 xc = f4zero(), ym = f4zero(), yp = f4zero(), zm = f4zero(), zp = f4zero();
+  if (valid) {
+    xc = SV::ld4(x, c0);
+    if (y > 0) ym = SV::ld4(x, c0 - L.nx);
+    if (y + 1 < L.ny) yp = SV::ld4(x, c0 + L.nx);
+    if (k > 0) zm = SV::ld4(x, c0 - plane);
+    if (k + 1 < L.nz) zp = SV::ld4(x, c0 + plane);
+    if (MODE == 2) {
+      prolong(xc, parent(y, k));
+      if (y > 0) prolong(ym, parent(y - 1, k));
+      if (y + 1 < L.ny) prolong(yp, parent(y + 1, k));
+      if (k > 0) prolong(zm, parent(y, k - 1));
+      if (k + 1 < L.nz) prolong(zp, parent(y, k + 1));
+    }
+  }
+  float left, right;
+  x_neighbours(Q, valid, qx, xc, left, right, [&](int off) {
+    float v = SV::ld(x, c0 + off);
+    if (MODE == 2) v = SV::rnd(v + SV::ld(xp, lidx(C, (4 * qx + off) >> 1, y >> 1, k >> 1)));
+    return v;
+  });
+  if (!valid) return;
+  float4 out = xc;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (((4 * qx + e + y + k) & 1) != colour) continue;
+    float a[6];
+    gc_faces(g, e, a);
+    const float dd = q_get(g.dd, e);
+    const float dg = gc_diag(a, dd, L.inv);
+    if (!(dg > 0.f)) continue;
+    const float pc = q_get(xc, e);
+    const float nb[6] = {e == 0 ? left : q_get(xc, e - 1), e == 3 ? right : q_get(xc, e + 1),
+                         q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
+    const float row = gc_row(a, dd, L.inv, pc, nb);
+    q_set(out, e, SV::rnd(pc + (q_get(rq, e) - row) / dg));
+  }
+  SV::st4(xout, c0, out);
+}
+
+// Residual r - row(x) of a compact level restricted to the next level
+// (delta3d_multigrid.rs:312-321, restrict3 :258-272): one thread per (quad
+// of the fine level, coarse row J, coarse plane K), 16 fine cells -> 2
+// coarse cells, children summed in the reference's order.
+template <int ST>
+__global__ void __launch_bounds__(QBLOCK) k_gc_restrict(Level F, QuadGrid Q, const typename Store<ST>::T* xf, Level C,
+                                                        Ctl* ctl) {
+#pragma clang fp contract(off)
+  using SV = Store<ST>;
+  using T = typename SV::T;
+  if (!ctl->active) return;
+  int qx, J, K;
+  const bool valid = quad_coords(Q, qx, J, K);
+  const size_t plane = (size_t)F.nx * F.ny;
+  const T* fr = reinterpret_cast<const T*>(F.r);
+  float t[2][2][4];
+#pragma unroll
+  for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy) {
+      const int y = 2 * J + dy, k = 2 * K + dz;
+      const size_t c0 = valid ? ((size_t)k * F.ny + y) * F.nx + 4 * (size_t)qx : 0;
+      GcQuad g{};
+      float4 xc = f4zero(), rq = f4zero(), ym = f4zero(), yp = f4zero(), zm = f4zero(), zp = f4zero();
+      if (valid) {
+        g = gc_quad(F, c0, 4 * qx, y, k);
+        rq = SV::ld4(fr, c0);
+        xc = SV::ld4(xf, c0);
+        if (y > 0) ym = SV::ld4(xf, c0 - F.nx);
+        if (y + 1 < F.ny) yp = SV::ld4(xf, c0 + F.nx);
+        if (k > 0) zm = SV::ld4(xf, c0 - plane);
+        if (k + 1 < F.nz) zp = SV::ld4(xf, c0 + plane);
+      }
+      float left, right;
+      x_neighbours(Q, valid, qx, xc, left, right, [&](int off) { return SV::ld(xf, c0 + off); });
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        float a[6];
+        gc_faces(g, e, a);
+        const float pc = q_get(xc, e);
+        const float nb[6] = {e == 0 ? left : q_get(xc, e - 1), e == 3 ? right : q_get(xc, e + 1),
+                             q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
+        t[dz][dy][e] = q_get(rq, e) - gc_row(a, q_get(g.dd, e), F.inv, pc, nb);
+      }
+    }
+  if (!valid) return;
+  float2 out;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    float s = 0.f;
+    s += t[0][0][2 * h]; s += t[0][0][2 * h + 1];
+    s += t[0][1][2 * h]; s += t[0][1][2 * h + 1];
+    s += t[1][0][2 * h]; s += t[1][0][2 * h + 1];
+    s += t[1][1][2 * h]; s += t[1][1][2 * h + 1];
+    if (h == 0) out.x = 0.125f * s; else out.y = 0.125f * s;
+  }
+  SV::st2(reinterpret_cast<T*>(C.r), lidx(C, 2 * qx, J, K), out);
+}
+
+}  // namespace fsp
+
+namespace fsp {
+
+// ---------------------------------------------------------------- compact tail
+// Every Galerkin level below the compact ones in one workgroup, in LDS:
+// per cell x, r (values rounded to storage ST), the x-, y-, z- couplings in
+// f32 and dd (6 floats). Same arithmetic as k_gc_gs / k_gc_restrict.
+struct GcTailLvl {
+  int nx, ny, nz, n;
+  float inv;
+  int o_x, o_r, o_fx, o_fy, o_fz, o_dd;
+};
+
+__device__ __forceinline__ void gct_faces(const float* lds, const GcTailLvl& L, int i, int j, int k, int c,
+                                          float a[6]) {
+  const int plane = L.nx * L.ny;
+  a[0] = lds[L.o_fx + c];
+  a[1] = i + 1 < L.nx ? lds[L.o_fx + c + 1] : 0.f;
+  a[2] = lds[L.o_fy + c];
+  a[3] = j + 1 < L.ny ? lds[L.o_fy + c + L.nx] : 0.f;
+  a[4] = lds[L.o_fz + c];
+  a[5] = k + 1 < L.nz ? lds[L.o_fz + c + plane] : 0.f;
+}
+
+__device__ __forceinline__ float gct_row(const float* lds, const GcTailLvl& L, int i, int j, int k, int c,
+                                         const float a[6]) {
+  const float* x = lds + L.o_x;
+  const int plane = L.nx * L.ny;
+  const float nb[6] = {i > 0 ? x[c - 1] : 0.f, i + 1 < L.nx ? x[c + 1] : 0.f,
+                       j > 0 ? x[c - L.nx] : 0.f, j + 1 < L.ny ? x[c + L.nx] : 0.f,
+                       k > 0 ? x[c - plane] : 0.f, k + 1 < L.nz ? x[c + plane] : 0.f};
+  return gc_row(a, lds[L.o_dd + c], L.inv, x[c], nb);
+}
+
+// One red-black colour on a tail level (first: x starts at zero).
+template <int ST>
+__device__ __forceinline__ void gct_pass(float* lds, const GcTailLvl& L, int col, bool first) {
+#pragma clang fp contract(off)
+  using SV = Store<ST>;
+  float* x = lds + L.o_x;
+  const float* r = lds + L.o_r;
+  for (int c = threadIdx.x; c < L.n; c += TAIL_THREADS) {
+    const int t = c / L.nx, i = c - t * L.nx, k = t / L.ny, j = t - k * L.ny;
+    if (((i + j + k) & 1) != col) {
+      if (first) x[c] = 0.f;
+      continue;
+    }
+    float a[6];
+    gct_faces(lds, L, i, j, k, c, a);
+    const float dg = gc_diag(a, lds[L.o_dd + c], L.inv);
+    if (first) { x[c] = dg > 0.f ? SV::rnd(r[c] / dg) : 0.f; continue; }
+    if (dg > 0.f) x[c] = SV::rnd(x[c] + (r[c] - gct_row(lds, L, i, j, k, c, a)) / dg);
+  }
+  __syncthreads();
+}
+
+template <int ST>
+__device__ __forceinline__ void gct_sweeps(float* lds, const GcTailLvl& L, int count, bool forward, bool coarsest,
+                                           bool from_zero) {
+  if (count == 0) {
+    if (from_zero) {
+      for (int c = threadIdx.x; c < L.n; c += TAIL_THREADS) lds[L.o_x + c] = 0.f;
+      __syncthreads();
+    }
+    return;
+  }
+  bool fz = from_zero;
+  if (coarsest) {
+    const int half = count / 2;
+    for (int s = 0; s < half; ++s) { gct_pass<ST>(lds, L, 0, fz); fz = false; gct_pass<ST>(lds, L, 1, false); }
+    if (count & 1) { gct_pass<ST>(lds, L, 0, fz); fz = false; }
+    for (int s = 0; s < half; ++s) { gct_pass<ST>(lds, L, 1, fz); fz = false; gct_pass<ST>(lds, L, 0, false); }
+    return;
+  }
+  for (int s = 0; s < count; ++s) {
+    const int c0 = forward ? 0 : 1;
+    gct_pass<ST>(lds, L, c0, fz);
+    fz = false;
+    gct_pass<ST>(lds, L, 1 - c0, false);
+  }
+}
+
+// coarse_cycle3 (delta3d_multigrid.rs:303-332) over the tail levels, red-black
+// smoothing, Galerkin levels; the first level's r is restricted, its x is
+// written back.
+template <int ST>
+__global__ void __launch_bounds__(TAIL_THREADS) k_gc_tail(TailSet T, Ctl* ctl) {
+#pragma clang fp contract(off)
+  using SV = Store<ST>;
+  if (!ctl->active) return;
+  extern __shared__ float lds[];
+  __shared__ GcTailLvl tl[TAIL_MAX_LEVELS];
+  if (threadIdx.x == 0) {
+    int off = 0;
+    for (int l = 0; l < T.count; ++l) {
+      const Level& g = T.lv[l];
+      GcTailLvl& L = tl[l];
+      L.nx = g.nx; L.ny = g.ny; L.nz = g.nz; L.n = (int)g.n; L.inv = g.inv;
+      L.o_x = off; off += L.n;
+      L.o_r = off; off += L.n;
+      L.o_fx = off; off += L.n;
+      L.o_fy = off; off += L.n;
+      L.o_fz = off; off += L.n;
+      L.o_dd = off; off += L.n;
+    }
+  }
+  __syncthreads();
+  // every level's couplings in one round of independent loads
+  for (int l = 0; l < T.count; ++l) {
+    const Level& g = T.lv[l];
+    const GcTailLvl L = tl[l];
+    for (int c = threadIdx.x; c < L.n; c += TAIL_THREADS) {
+      const int t = c / L.nx, i = c - t * L.nx, k = t / L.ny, j = t - k * L.ny;
+      lds[L.o_fx + c] = i > 0 ? g.ou[((size_t)k * L.ny + j) * (L.nx + 1) + i] : 0.f;
+      lds[L.o_fy + c] = j > 0 ? g.ov[((size_t)k * (L.ny + 1) + j) * L.nx + i] : 0.f;
+      lds[L.o_fz + c] = k > 0 ? g.ow[((size_t)k * L.ny + j) * L.nx + i] : 0.f;
+      lds[L.o_dd + c] = g.dd[c];
+      if (l == 0) lds[L.o_r + c] = SV::ld(reinterpret_cast<const typename SV::T*>(g.r), c);
+    }
+  }
+  __syncthreads();
+  const int last = T.count - 1;
+  for (int l = 0; l <= last; ++l) {
+    const GcTailLvl L = tl[l];
+    gct_sweeps<ST>(lds, L, l == last ? T.coarse : T.pre, true, l == last, true);
+    if (l < last) {
+      const GcTailLvl C = tl[l + 1];
+      for (int c = threadIdx.x; c < C.n; c += TAIL_THREADS) {
+        const int t = c / C.nx, i = c - t * C.nx, k = t / C.ny, j = t - k * C.ny;
+        float s = 0.f;
+        for (int ch = 0; ch < 8; ++ch) {
+          const int fi = 2 * i + (ch & 1), fj = 2 * j + ((ch >> 1) & 1), fk = 2 * k + ((ch >> 2) & 1);
+          const int f = (fk * L.ny + fj) * L.nx + fi;
+          float a[6];
+          gct_faces(lds, L, fi, fj, fk, f, a);
+          s += lds[L.o_r + f] - gct_row(lds, L, fi, fj, fk, f, a);
+        }
+        lds[C.o_r + c] = SV::rnd(0.125f * s);
+      }
+      __syncthreads();
+    }
+  }
+  for (int l = last - 1; l >= 0; --l) {
+    const GcTailLvl L = tl[l];
+    const GcTailLvl C = tl[l + 1];
+    for (int c = threadIdx.x; c < L.n; c += TAIL_THREADS) {
+      const int t = c / L.nx, i = c - t * L.nx, k = t / L.ny, j = t - k * L.ny;
+      lds[L.o_x + c] = SV::rnd(lds[L.o_x + c] + lds[C.o_x + ((k >> 1) * C.ny + (j >> 1)) * C.nx + (i >> 1)]);
+    }
+    __syncthreads();
+    gct_sweeps<ST>(lds, L, T.post, false, false, false);
+  }
+  const GcTailLvl L0 = tl[0];
+  for (int q = threadIdx.x; q < L0.n; q += TAIL_THREADS)
+    SV::st(reinterpret_cast<typename SV::T*>(T.lv[0].x), q, lds[L0.o_x + q]);
+}
+
+inline size_t gc_tail_lds_bytes(const Level* lv, int count) {
+  size_t floats = 0;
+  for (int l = 0; l < count; ++l) floats += 6 * lv[l].n;
+  return floats * sizeof(float);
+}
+
+}  // namespace fsp

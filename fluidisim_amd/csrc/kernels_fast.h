@@ -104,11 +104,15 @@ __device__ __forceinline__ void x_neighbours(const QuadGrid& Q, bool valid, int 
 //         neighbours are prolonged on the fly, so this mode reads z and
 //         writes zout (a different buffer).
 // dot_op >= 0: also reduce <r, z> over every cell into that slot.
-template <int MODE>
-__global__ void __launch_bounds__(QBLOCK) k_q_gs(Fine G, QuadGrid Q, const float* z, float* zout, const float* r,
-                                                 int colour, Level L0, const float* xc, int dot_op, Ctl* ctl,
+// z, zout and the parents xc are held in storage ST (r is the f32 CG
+// residual); every value entering the arithmetic is a stored value.
+template <int MODE, int ST>
+__global__ void __launch_bounds__(QBLOCK) k_q_gs(Fine G, QuadGrid Q, const typename Store<ST>::T* z,
+                                                 typename Store<ST>::T* zout, const float* r, int colour, Level L0,
+                                                 const typename Store<ST>::T* xc, int dot_op, Ctl* ctl,
                                                  double* partials) {
 #pragma clang fp contract(off)
+  using SV = Store<ST>;
   if (!ctl->active) return;
   int qx, y, k;
   const bool valid = quad_coords(Q, qx, y, k);
@@ -125,47 +129,48 @@ __global__ void __launch_bounds__(QBLOCK) k_q_gs(Fine G, QuadGrid Q, const float
         if (((e + y + k) & 1) != colour) continue;
         const uint8_t m = (uint8_t)(mq >> (8 * e));
         const float prec = row_prec(G, 4 * qx + e, y, k, m);
-        q_set(out, e, prec > 0.f ? prec * q_get(rq, e) : 0.f);
+        q_set(out, e, prec > 0.f ? SV::rnd(prec * q_get(rq, e)) : 0.f);
       }
-      st4(zout, c0, out);
+      SV::st4(zout, c0, out);
       if (dot_op >= 0) s = (double)rq.x * out.x + (double)rq.y * out.y + (double)rq.z * out.z + (double)rq.w * out.w;
     }
     if (dot_op >= 0) reduce_finish<false>(s, unused, dot_op, ctl, partials);
     return;
   }
   // parent values of the prolongation
-  auto parent = [&](int yy, int kk, int xq) -> float2 {
-    return *reinterpret_cast<const float2*>(xc + lidx(L0, 2 * xq, yy >> 1, kk >> 1));
+  auto parent = [&](int yy, int kk, int xq) -> float2 { return SV::ld2(xc, lidx(L0, 2 * xq, yy >> 1, kk >> 1)); };
+  // prolongation z + parent, rounded to storage (prolong_add3 stores it)
+  auto prolong = [&](float4& v, float2 a) {
+    v.x = SV::rnd(v.x + a.x); v.y = SV::rnd(v.y + a.x); v.z = SV::rnd(v.z + a.y); v.w = SV::rnd(v.w + a.y);
   };
   float4 zc = f4zero(), rq = f4zero();
   uint32_t mq = 0;
   float4 ym = f4zero(), yp = f4zero(), zm = f4zero(), zp = f4zero();
   if (valid) {
     mq = *reinterpret_cast<const uint32_t*>(G.mask + c0);
-    zc = ld4(z, c0);
+    zc = SV::ld4(z, c0);
     rq = ld4(r, c0);
-    if (y > 0) ym = ld4(z, c0 - G.nx);
-    if (y + 1 < G.ny) yp = ld4(z, c0 + G.nx);
-    if (k > 0) zm = ld4(z, c0 - plane);
-    if (k + 1 < G.nz) zp = ld4(z, c0 + plane);
+    if (y > 0) ym = SV::ld4(z, c0 - G.nx);
+    if (y + 1 < G.ny) yp = SV::ld4(z, c0 + G.nx);
+    if (k > 0) zm = SV::ld4(z, c0 - plane);
+    if (k + 1 < G.nz) zp = SV::ld4(z, c0 + plane);
     if (MODE == 2) {
-      float2 pc = parent(y, k, qx);
-      zc.x += pc.x; zc.y += pc.x; zc.z += pc.y; zc.w += pc.y;
+      prolong(zc, parent(y, k, qx));
 #pragma unroll
       for (int e = 0; e < 4; ++e)
         if (!prec_positive((uint8_t)(mq >> (8 * e)))) q_set(zc, e, 0.f);
-      if (y > 0) { float2 a = parent(y - 1, k, qx); ym.x += a.x; ym.y += a.x; ym.z += a.y; ym.w += a.y; }
-      if (y + 1 < G.ny) { float2 a = parent(y + 1, k, qx); yp.x += a.x; yp.y += a.x; yp.z += a.y; yp.w += a.y; }
-      if (k > 0) { float2 a = parent(y, k - 1, qx); zm.x += a.x; zm.y += a.x; zm.z += a.y; zm.w += a.y; }
-      if (k + 1 < G.nz) { float2 a = parent(y, k + 1, qx); zp.x += a.x; zp.y += a.x; zp.z += a.y; zp.w += a.y; }
+      if (y > 0) prolong(ym, parent(y - 1, k, qx));
+      if (y + 1 < G.ny) prolong(yp, parent(y + 1, k, qx));
+      if (k > 0) prolong(zm, parent(y, k - 1, qx));
+      if (k + 1 < G.nz) prolong(zp, parent(y, k + 1, qx));
     }
   }
   float left, right;
   x_neighbours(Q, valid, qx, zc, left, right, [&](int off) {
-    float v = z[c0 + off];
+    float v = SV::ld(z, c0 + off);
     if (MODE == 2) {
       const int xx = 4 * qx + off;
-      v += xc[lidx(L0, xx >> 1, y >> 1, k >> 1)];
+      v = SV::rnd(v + SV::ld(xc, lidx(L0, xx >> 1, y >> 1, k >> 1)));
     }
     return v;
   });
@@ -181,9 +186,9 @@ __global__ void __launch_bounds__(QBLOCK) k_q_gs(Fine G, QuadGrid Q, const float
       const float nb[6] = {xl, xr, q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
       float prec;
       const float az = row_local(G, 4 * qx + e, y, k, m, pc, nb, prec);
-      if (prec > 0.f) q_set(out, e, pc + prec * (q_get(rq, e) - az));
+      if (prec > 0.f) q_set(out, e, SV::rnd(pc + prec * (q_get(rq, e) - az)));
     }
-    st4(zout, c0, out);
+    SV::st4(zout, c0, out);
     if (dot_op >= 0) s = (double)rq.x * out.x + (double)rq.y * out.y + (double)rq.z * out.z + (double)rq.w * out.w;
   }
   if (dot_op >= 0) reduce_finish<false>(s, unused, dot_op, ctl, partials);
@@ -193,9 +198,11 @@ __global__ void __launch_bounds__(QBLOCK) k_q_gs(Fine G, QuadGrid Q, const float
 // One thread per (fine quad column qx, coarse row J, coarse plane K): the
 // residual t = r - A z of its 16 fine cells, summed per coarse cell in the
 // reference's child order and scaled by 1/8 (restrict3, delta3d_multigrid.rs:258-272).
-__global__ void __launch_bounds__(QBLOCK) k_q_restrict(Fine G, QuadGrid Q, const float* z, const float* r, Level L0,
-                                                       Ctl* ctl) {
+template <int ST>
+__global__ void __launch_bounds__(QBLOCK) k_q_restrict(Fine G, QuadGrid Q, const typename Store<ST>::T* z,
+                                                       const float* r, Level L0, Ctl* ctl) {
 #pragma clang fp contract(off)
+  using SV = Store<ST>;
   if (!ctl->active) return;
   int qx, J, K;
   const bool valid = quad_coords(Q, qx, J, K);
@@ -211,15 +218,15 @@ __global__ void __launch_bounds__(QBLOCK) k_q_restrict(Fine G, QuadGrid Q, const
       uint32_t mq = 0;
       if (valid) {
         mq = *reinterpret_cast<const uint32_t*>(G.mask + c0);
-        zc = ld4(z, c0);
+        zc = SV::ld4(z, c0);
         rq = ld4(r, c0);
-        if (y > 0) ym = ld4(z, c0 - G.nx);
-        if (y + 1 < G.ny) yp = ld4(z, c0 + G.nx);
-        if (k > 0) zm = ld4(z, c0 - plane);
-        if (k + 1 < G.nz) zp = ld4(z, c0 + plane);
+        if (y > 0) ym = SV::ld4(z, c0 - G.nx);
+        if (y + 1 < G.ny) yp = SV::ld4(z, c0 + G.nx);
+        if (k > 0) zm = SV::ld4(z, c0 - plane);
+        if (k + 1 < G.nz) zp = SV::ld4(z, c0 + plane);
       }
       float left, right;
-      x_neighbours(Q, valid, qx, zc, left, right, [&](int off) { return z[c0 + off]; });
+      x_neighbours(Q, valid, qx, zc, left, right, [&](int off) { return SV::ld(z, c0 + off); });
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const uint8_t m = (uint8_t)(mq >> (8 * e));
@@ -243,14 +250,23 @@ __global__ void __launch_bounds__(QBLOCK) k_q_restrict(Fine G, QuadGrid Q, const
     s += t[1][1][2 * h]; s += t[1][1][2 * h + 1];
     if (h == 0) out.x = 0.125f * s; else out.y = 0.125f * s;
   }
-  *reinterpret_cast<float2*>(L0.r + lidx(L0, 2 * qx, J, K)) = out;
+  SV::st2(reinterpret_cast<typename SV::T*>(L0.r), lidx(L0, 2 * qx, J, K), out);
+}
+
+// A vector held in storage ST, widened to f32.
+template <int ST>
+__global__ void __launch_bounds__(BLOCK) k_unpack(size_t n, const typename Store<ST>::T* z, float* out) {
+  FSP_FOR(c, n) out[c] = Store<ST>::ld(z, c);
 }
 
 // ---------------------------------------------------------------- CG steps on quads
 // d = z (first) or z + beta d_old, q = A d, <d, q> (delta3d_mobile.rs:520-523, :460-465).
-__global__ void __launch_bounds__(QBLOCK) k_q_spmv_dir(Fine G, QuadGrid Q, const float* z, float* d0, float* d1,
-                                                       float* q, Ctl* ctl, double* partials) {
+// z is held in storage ST.
+template <int ST>
+__global__ void __launch_bounds__(QBLOCK) k_q_spmv_dir(Fine G, QuadGrid Q, const typename Store<ST>::T* z, float* d0,
+                                                       float* d1, float* q, Ctl* ctl, double* partials) {
 #pragma clang fp contract(off)
+  using SV = Store<ST>;
   if (!ctl->active) return;
   const int sel = ctl->dsel, first = ctl->first_dir;
   const float beta = (float)ctl->beta;
@@ -261,7 +277,7 @@ __global__ void __launch_bounds__(QBLOCK) k_q_spmv_dir(Fine G, QuadGrid Q, const
   const size_t plane = (size_t)G.nx * G.ny;
   const size_t c0 = valid ? ((size_t)k * G.ny + y) * G.nx + 4 * (size_t)qx : 0;
   auto dir4 = [&](size_t c) {
-    float4 a = ld4(z, c);
+    float4 a = SV::ld4(z, c);
     if (!first) {
       const float4 b = ld4(dold, c);
       a.x = a.x + beta * b.x; a.y = a.y + beta * b.y; a.z = a.z + beta * b.z; a.w = a.w + beta * b.w;
@@ -281,7 +297,7 @@ __global__ void __launch_bounds__(QBLOCK) k_q_spmv_dir(Fine G, QuadGrid Q, const
   float left, right;
   x_neighbours(Q, valid, qx, dc, left, right, [&](int off) {
     const size_t c = c0 + off;
-    return first ? z[c] : z[c] + beta * dold[c];
+    return first ? SV::ld(z, c) : SV::ld(z, c) + beta * dold[c];
   });
   double s = 0.0, unused = 0.0;
   if (valid) {
@@ -389,9 +405,12 @@ __device__ __forceinline__ float tail_row(const float* lds, const TailLvl& L, in
 }
 
 // One smoothing pass on tail level l. RBGS: colour `col` in place (first: x
-// starts at zero). Jacobi: t = x + w(r - Lx)/diag, then x = t.
+// starts at zero). Jacobi: t = x + w(r - Lx)/diag, then x = t. Stored values
+// are rounded to storage ST.
+template <int ST>
 __device__ __forceinline__ void tail_pass(float* lds, const TailLvl& L, int smoother, int col, bool first) {
 #pragma clang fp contract(off)
+  using SV = Store<ST>;
   float* x = lds + L.o_x;
   const float* r = lds + L.o_r;
   const float* diag = lds + L.o_diag;
@@ -404,7 +423,7 @@ __device__ __forceinline__ void tail_pass(float* lds, const TailLvl& L, int smoo
       const float row = first ? 0.f : tail_row(lds, L, i, j, k, c, L.o_x);
       float xc = first ? 0.f : x[c];
       const float dg = diag[c];
-      if (dg > 0.f) xc += omega * (r[c] - row) / dg;
+      if (dg > 0.f) xc = SV::rnd(xc + omega * (r[c] - row) / dg);
       t[c] = xc;
     }
     __syncthreads();
@@ -420,12 +439,13 @@ __device__ __forceinline__ void tail_pass(float* lds, const TailLvl& L, int smoo
       if (first) x[c] = 0.f;
       continue;
     }
-    if (first) { x[c] = dg > 0.f ? r[c] / dg : 0.f; continue; }
-    if (dg > 0.f) x[c] = x[c] + (r[c] - tail_row(lds, L, i, j, k, c, L.o_x)) / dg;
+    if (first) { x[c] = dg > 0.f ? SV::rnd(r[c] / dg) : 0.f; continue; }
+    if (dg > 0.f) x[c] = SV::rnd(x[c] + (r[c] - tail_row(lds, L, i, j, k, c, L.o_x)) / dg);
   }
   __syncthreads();
 }
 
+template <int ST>
 __device__ __forceinline__ void tail_sweeps(float* lds, const TailLvl& L, int smoother, int count, bool forward,
                                             bool coarsest, bool from_zero) {
   if (count == 0) {
@@ -437,29 +457,33 @@ __device__ __forceinline__ void tail_sweeps(float* lds, const TailLvl& L, int sm
   }
   bool fz = from_zero;
   if (smoother == 0) {
-    for (int s = 0; s < count; ++s) { tail_pass(lds, L, 0, 0, fz); fz = false; }
+    for (int s = 0; s < count; ++s) { tail_pass<ST>(lds, L, 0, 0, fz); fz = false; }
     return;
   }
   if (coarsest) {
     const int half = count / 2;
-    for (int s = 0; s < half; ++s) { tail_pass(lds, L, 1, 0, fz); fz = false; tail_pass(lds, L, 1, 1, false); }
-    if (count & 1) { tail_pass(lds, L, 1, 0, fz); fz = false; }
-    for (int s = 0; s < half; ++s) { tail_pass(lds, L, 1, 1, fz); fz = false; tail_pass(lds, L, 1, 0, false); }
+    for (int s = 0; s < half; ++s) { tail_pass<ST>(lds, L, 1, 0, fz); fz = false; tail_pass<ST>(lds, L, 1, 1, false); }
+    if (count & 1) { tail_pass<ST>(lds, L, 1, 0, fz); fz = false; }
+    for (int s = 0; s < half; ++s) { tail_pass<ST>(lds, L, 1, 1, fz); fz = false; tail_pass<ST>(lds, L, 1, 0, false); }
     return;
   }
   for (int s = 0; s < count; ++s) {
     const int c0 = forward ? 0 : 1;
-    tail_pass(lds, L, 1, c0, fz);
+    tail_pass<ST>(lds, L, 1, c0, fz);
     fz = false;
-    tail_pass(lds, L, 1, 1 - c0, false);
+    tail_pass<ST>(lds, L, 1, 1 - c0, false);
   }
 }
 
 // Descends and ascends every tail level in LDS (coarse_cycle3,
 // delta3d_multigrid.rs:303-332), starting from the first tail level whose r
 // is already restricted; writes that level's x back to global.
+// Level vectors x and r are held in storage ST, in global memory and (as
+// rounded f32 values) in LDS.
+template <int ST>
 __global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailSet T, Ctl* ctl) {
 #pragma clang fp contract(off)
+  using SV = Store<ST>;
   if (!ctl->active) return;
   extern __shared__ float lds[];
   __shared__ TailLvl tl[TAIL_MAX_LEVELS];
@@ -496,13 +520,14 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailSet T, Ctl* ctl) {
     for (int q = threadIdx.x; q < nv; q += TAIL_THREADS) lds[L.o_ov + q] = g.ov[q];
     for (int q = threadIdx.x; q < nw; q += TAIL_THREADS) lds[L.o_ow + q] = g.ow[q];
     if (l == 0)
-      for (int q = threadIdx.x; q < L.n; q += TAIL_THREADS) lds[L.o_r + q] = g.r[q];
+      for (int q = threadIdx.x; q < L.n; q += TAIL_THREADS)
+        lds[L.o_r + q] = SV::ld(reinterpret_cast<const typename SV::T*>(g.r), q);
   }
   __syncthreads();
   const int last = T.count - 1;
   for (int l = 0; l <= last; ++l) {
     const TailLvl L = tl[l];
-    tail_sweeps(lds, L, T.smoother, l == last ? T.coarse : T.pre, true, l == last, true);
+    tail_sweeps<ST>(lds, L, T.smoother, l == last ? T.coarse : T.pre, true, l == last, true);
     if (l < last) {
       const TailLvl C = tl[l + 1];
       for (int c = threadIdx.x; c < C.n; c += TAIL_THREADS) {
@@ -514,7 +539,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailSet T, Ctl* ctl) {
           const int f = (fk * L.ny + fj) * L.nx + fi;
           s += lds[L.o_r + f] - tail_row(lds, L, fi, fj, fk, f, L.o_x);
         }
-        lds[C.o_r + c] = 0.125f * s;
+        lds[C.o_r + c] = SV::rnd(0.125f * s);
       }
       __syncthreads();
     }
@@ -525,13 +550,14 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailSet T, Ctl* ctl) {
     for (int c = threadIdx.x; c < L.n; c += TAIL_THREADS) {
       int i, j, k;
       tail_decode(L, c, i, j, k);
-      lds[L.o_x + c] += lds[C.o_x + ((k >> 1) * C.ny + (j >> 1)) * C.nx + (i >> 1)];
+      lds[L.o_x + c] = SV::rnd(lds[L.o_x + c] + lds[C.o_x + ((k >> 1) * C.ny + (j >> 1)) * C.nx + (i >> 1)]);
     }
     __syncthreads();
-    tail_sweeps(lds, L, T.smoother, T.post, false, false, false);
+    tail_sweeps<ST>(lds, L, T.smoother, T.post, false, false, false);
   }
   const TailLvl L0 = tl[0];
-  for (int q = threadIdx.x; q < L0.n; q += TAIL_THREADS) T.lv[0].x[q] = lds[L0.o_x + q];
+  for (int q = threadIdx.x; q < L0.n; q += TAIL_THREADS)
+    SV::st(reinterpret_cast<typename SV::T*>(T.lv[0].x), q, lds[L0.o_x + q]);
 }
 
 // LDS bytes k_tail needs for the given levels.

@@ -1,0 +1,839 @@
+// Learned preconditioner: a 3-level 3D convolutional encoder-decoder
+// (8/16/32 channels, 3x3x3 kernels, ReLU, additive skip connections) with
+// its forward and backward passes as binary16 MFMA implicit GEMMs (f32
+// accumulation), an Adam optimiser on f32 master weights, and the glue the
+// projection uses to apply it inside flexible CG. No counterpart in the
+// reference solver.
+//
+// Activations are channels-last binary16, cells (k*ny+j)*nx+i. Level l of
+// the network is the grid coarsened 2^l times (average pooling down, nearest
+// upsampling folded into the next convolution's loads). Layer weights are
+// W[co][tap][ci], tap = (dz+1)*9 + (dy+1)*3 + (dx+1), zero padding at the
+// domain boundary.
+//
+// Convolution as a GEMM on v_mfma_f32_16x16x16_f16: a wavefront owns 16
+// consecutive x-cells of one row (M = 16) and all output channels (N, in
+// 16-channel tiles); K runs over (tap, input channel) in chunks of 16, each
+// lane loading 4 consecutive channels of one tap (8 bytes).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/fsp.h"
+#include "../../include/fsp_learned.h"
+#include "learned_internal.h"
+
+namespace fsp {
+namespace net {
+
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int WAVES = 4;  // wavefronts per workgroup of the convolution kernels
+constexpr int THREADS = 64 * WAVES;
+
+struct LayerDesc {
+  int level, cin, cout, relu, skip, up;  // skip: layer index added before the ReLU (-1 none); up: input from level+1
+};
+// e1a e1b | e2a e2b | e3a e3b | d2a d2b | d1a d1b | head
+constexpr int NL = 11;
+constexpr LayerDesc LAYERS[NL] = {
+    {0, 4, 8, 1, -1, 0},  {0, 8, 8, 1, -1, 0},   {1, 8, 16, 1, -1, 0}, {1, 16, 16, 1, -1, 0},
+    {2, 16, 32, 1, -1, 0}, {2, 32, 32, 1, -1, 0}, {1, 32, 16, 1, 3, 1}, {1, 16, 16, 1, -1, 0},
+    {0, 16, 8, 1, 1, 1},  {0, 8, 8, 1, -1, 0},   {0, 8, 1, 0, -1, 0},
+};
+
+__host__ __device__ constexpr int kpad(int cin) { return ((27 * cin + 15) / 16) * 16; }
+__host__ __device__ constexpr int npad(int cout) { return ((cout + 15) / 16) * 16; }
+// channels of the gradient flowing into a layer's backward-data convolution
+__host__ __device__ constexpr int gch(int cout) { return cout < 4 ? 4 : cout; }
+
+size_t param_count() {
+  size_t n = 0;
+  for (const LayerDesc& L : LAYERS) n += (size_t)L.cout * 27 * L.cin + L.cout;
+  return n;
+}
+
+// ---------------------------------------------------------------- convolution
+struct ConvArgs {
+  int nx, ny, nz;        // output grid
+  int cnx, cny, cnz;     // input grid when the input is upsampled (else = output grid)
+  const _Float16* in;    // [cells_in][CIN]
+  const _Float16* wt;    // [npad(COUT)][kpad(CIN)]
+  const float* bias;     // [COUT] (forward only)
+  const _Float16* skip;  // [cells][COUT] or null
+  _Float16* out;         // [cells][COUT]
+  float* yout;           // head: [cells]
+  const int32_t* gate;   // return at once while *gate == 0 (null: always run)
+};
+
+// EPI 0: bias (+ skip) + ReLU -> binary16; 1: bias -> f32 channel 0 (head);
+// 2: plain product -> binary16 (backward data).
+template <int CIN, int COUT, int UP, int EPI>
+__global__ void __launch_bounds__(THREADS) k_conv(ConvArgs a) {
+  constexpr int NT = npad(COUT) / 16;
+  constexpr int KP = kpad(CIN);
+  constexpr int CHUNKS = KP / 16;
+  if (a.gate && !*a.gate) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  const int segs = a.nx / 16;
+  if (wave >= segs * a.ny * a.nz) return;  // uniform per wavefront
+  const int seg = wave % segs, row = wave / segs;
+  const int y = row % a.ny, z = row / a.ny;
+  const int m = lane & 15, g = lane >> 4;
+  const int x = seg * 16 + m;  // this lane's A row (cell)
+  f4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int ch = 0; ch < CHUNKS; ++ch) {
+    const int kk = ch * 16 + 4 * g;
+    const int tap = kk / CIN, ci = kk - tap * CIN;
+    h4 av = h4{0, 0, 0, 0};
+    if (tap < 27) {
+      const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+      const int xx = x + dx, yy = y + dy, zz = z + dz;
+      if (xx >= 0 && xx < a.nx && yy >= 0 && yy < a.ny && zz >= 0 && zz < a.nz) {
+        size_t v;
+        if (UP) v = ((size_t)(zz >> 1) * a.cny + (yy >> 1)) * a.cnx + (xx >> 1);
+        else v = ((size_t)zz * a.ny + yy) * a.nx + xx;
+        av = *reinterpret_cast<const h4*>(a.in + v * CIN + ci);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const h4 bv = *reinterpret_cast<const h4*>(a.wt + (size_t)(t * 16 + m) * KP + kk);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, acc[t], 0, 0, 0);
+    }
+  }
+  // lane holds D[cell 4g+v][channel t*16+m]
+  const size_t rowbase = ((size_t)z * a.ny + y) * a.nx + seg * 16;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int co = t * 16 + m;
+    if (co >= COUT) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const size_t cell = rowbase + 4 * g + v;
+      float val = acc[t][v];
+      if (EPI == 2) {
+        a.out[cell * COUT + co] = (_Float16)val;
+        continue;
+      }
+      val += a.bias[co];
+      if (EPI == 1) {
+        a.yout[cell] = val;
+        continue;
+      }
+      if (a.skip) val += (float)a.skip[cell * COUT + co];
+      a.out[cell * COUT + co] = (_Float16)(val > 0.f ? val : 0.f);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- weight gradient
+// Partial dW[co][k] = sum over the cells of one partition of g[cell][co] *
+// in[cell + tap][ci] (k = tap*CIN + ci), one wavefront per (partition, k
+// chunk, co tile); D = A B with A[co][cell] = g, B[cell][k] = shifted input.
+struct WgradArgs {
+  int nx, ny, nz, cnx, cny, cnz;
+  const _Float16* g;    // [cells][gs]: gradient at the layer's pre-activation (channels 0..COUT-1)
+  int gs;
+  const _Float16* in;   // layer input
+  float* partial;       // [parts][npad(COUT)][kpad(CIN)]
+  int parts;
+};
+
+template <int CIN, int COUT, int UP>
+__global__ void __launch_bounds__(THREADS) k_wgrad(WgradArgs a) {
+  constexpr int MT = npad(COUT) / 16;
+  constexpr int KP = kpad(CIN);
+  constexpr int CHUNKS = KP / 16;
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  const int jobs = CHUNKS * MT;
+  if (wave >= jobs * a.parts) return;
+  const int job = wave % jobs, part = wave / jobs;
+  const int ch = job % CHUNKS, mt = job / CHUNKS;
+  const int m = lane & 15, g4 = lane >> 4;
+  const int kk = ch * 16 + m;  // this lane's B column
+  const int tap = kk / CIN, ci = kk - tap * CIN;
+  const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+  const int co = mt * 16 + m;  // this lane's A row
+  const int segs = a.nx / 16;
+  const long total = (long)segs * a.ny * a.nz;
+  const long per = (total + a.parts - 1) / a.parts;
+  const long s0 = per * part, s1 = s0 + per < total ? s0 + per : total;
+  f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+  for (long s = s0; s < s1; ++s) {
+    const int seg = (int)(s % segs);
+    const long row = s / segs;
+    const int y = (int)(row % a.ny), z = (int)(row / a.ny);
+    const size_t rowbase = ((size_t)z * a.ny + y) * a.nx + seg * 16;
+    h4 av, bv;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int xc = seg * 16 + 4 * g4 + v;
+      av[v] = co < COUT ? a.g[(rowbase + 4 * g4 + v) * a.gs + co] : (_Float16)0;
+      _Float16 b = 0;
+      const int xx = xc + dx, yy = y + dy, zz = z + dz;
+      if (tap < 27 && xx >= 0 && xx < a.nx && yy >= 0 && yy < a.ny && zz >= 0 && zz < a.nz) {
+        size_t q;
+        if (UP) q = ((size_t)(zz >> 1) * a.cny + (yy >> 1)) * a.cnx + (xx >> 1);
+        else q = ((size_t)zz * a.ny + yy) * a.nx + xx;
+        b = a.in[q * CIN + ci];
+      }
+      bv[v] = b;
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, acc, 0, 0, 0);
+  }
+  // lane holds D[co 4*g4+v][k m] of this (co tile, k chunk)
+  float* P = a.partial + (size_t)part * npad(COUT) * KP;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) P[(size_t)(mt * 16 + 4 * g4 + v) * KP + ch * 16 + m] = acc[v];
+}
+
+// Sum of the partitions in order; dW in the flat parameter layout, scaled
+// by 1/scale; flags a non-finite value.
+__global__ void k_wgrad_reduce(const float* partial, int parts, int cin, int cout, float inv_scale, float* grad,
+                               int* bad) {
+  const int KP = kpad(cin), NP = npad(cout);
+  const int n = cout * 27 * cin;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int co = q / (27 * cin), k = q - co * 27 * cin;
+    float s = 0.f;
+    for (int p = 0; p < parts; ++p) s += partial[((size_t)p * NP + co) * KP + k];
+    s *= inv_scale;
+    if (!isfinite(s)) *bad = 1;
+    grad[q] = s;
+  }
+}
+
+// Bias gradient: sum over cells of g[cell][co], one workgroup per channel.
+__global__ void __launch_bounds__(256) k_bgrad(const _Float16* g, size_t cells, int gs, float inv_scale, float* grad,
+                                               int* bad) {
+  __shared__ float red[256];
+  const int co = blockIdx.x;
+  float s = 0.f;
+  for (size_t c = threadIdx.x; c < cells; c += 256) s += (float)g[c * gs + co];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float v = red[0] * inv_scale;
+    if (!isfinite(v)) *bad = 1;
+    grad[co] = v;
+  }
+}
+
+// ---------------------------------------------------------------- elementwise
+__global__ void k_pool(const _Float16* in, _Float16* out, int cnx, int cny, int cnz, int C, const int32_t* gate) {
+  if (gate && !*gate) return;
+  const size_t total = (size_t)cnx * cny * cnz * C;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(q % C);
+    const size_t cell = q / C;
+    const int i = (int)(cell % cnx), j = (int)((cell / cnx) % cny), k = (int)(cell / ((size_t)cnx * cny));
+    const int fnx = 2 * cnx, fny = 2 * cny;
+    float s = 0.f;
+    for (int dz = 0; dz < 2; ++dz)
+      for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx)
+          s += (float)in[(((size_t)(2 * k + dz) * fny + 2 * j + dy) * fnx + 2 * i + dx) * C + c];
+    out[q] = (_Float16)(s * 0.125f);
+  }
+}
+// g_fine = g_coarse[parent] / 8 (+ add[fine])
+__global__ void k_pool_back(const _Float16* gc, const _Float16* add, _Float16* gf, int fnx, int fny, int fnz, int C) {
+  const size_t total = (size_t)fnx * fny * fnz * C;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(q % C);
+    const size_t cell = q / C;
+    const int i = (int)(cell % fnx), j = (int)((cell / fnx) % fny), k = (int)(cell / ((size_t)fnx * fny));
+    const size_t p = ((size_t)(k >> 1) * (fny >> 1) + (j >> 1)) * (fnx >> 1) + (i >> 1);
+    float v = (float)gc[p * C + c] * 0.125f;
+    if (add) v += (float)add[q];
+    gf[q] = (_Float16)v;
+  }
+}
+// g_coarse = sum of the eight children's g_fine
+__global__ void k_up_back(const _Float16* gf, _Float16* gc, int cnx, int cny, int cnz, int C) {
+  const size_t total = (size_t)cnx * cny * cnz * C;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(q % C);
+    const size_t cell = q / C;
+    const int i = (int)(cell % cnx), j = (int)((cell / cnx) % cny), k = (int)(cell / ((size_t)cnx * cny));
+    const int fnx = 2 * cnx, fny = 2 * cny;
+    float s = 0.f;
+    for (int dz = 0; dz < 2; ++dz)
+      for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx)
+          s += (float)gf[(((size_t)(2 * k + dz) * fny + 2 * j + dy) * fnx + 2 * i + dx) * C + c];
+    gc[q] = (_Float16)s;
+  }
+}
+// g = g * (h > 0)
+__global__ void k_relu_back(const _Float16* h, _Float16* g, size_t n) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    if (!((float)h[q] > 0.f)) g[q] = (_Float16)0.f;
+}
+// g += a
+__global__ void k_add16(_Float16* g, const _Float16* a, size_t n) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    g[q] = (_Float16)((float)g[q] + (float)a[q]);
+}
+
+// Loss: mean of (y - t)^2 over wet cells; per-block f64 partials; the
+// output gradient scale * 2 (y - t) wet / n_wet as a 4-channel binary16
+// tensor (channels 1-3 zero) for the head's backward-data convolution.
+__global__ void __launch_bounds__(256) k_loss(const float* y, const float* t, const uint8_t* wet, size_t cells,
+                                              double inv_n, float gscale, _Float16* gy, double* partial) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < cells; c += (size_t)gridDim.x * 256) {
+    const float d = wet[c] ? y[c] - t[c] : 0.f;
+    s += (double)d * d;
+    const float gv = (float)(2.0 * inv_n) * d * gscale;
+    gy[4 * c] = (_Float16)gv;
+    gy[4 * c + 1] = (_Float16)0.f;
+    gy[4 * c + 2] = (_Float16)0.f;
+    gy[4 * c + 3] = (_Float16)0.f;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] * inv_n;
+}
+
+// Forward weights (W[co][tap][ci] -> [npad(cout)][kpad(cin)], k = tap*cin+ci)
+// and backward-data weights (W'[ci][tap'][c] = W[c][26-tap'][ci], c over
+// gch(cout) channels) in binary16, from the f32 master parameters.
+__global__ void k_pack(const float* theta, int cin, int cout, _Float16* wt, _Float16* wb) {
+  const int KP = kpad(cin), NP = npad(cout);
+  const int gc = gch(cout), KPb = kpad(gc), NPb = npad(cin);
+  const int nf = NP * KP, nb = NPb * KPb;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nf + nb; q += gridDim.x * blockDim.x) {
+    if (q < nf) {
+      const int co = q / KP, k = q - co * KP;
+      float v = 0.f;
+      if (co < cout && k < 27 * cin) v = theta[(size_t)co * 27 * cin + k];
+      wt[q] = (_Float16)v;
+    } else if (wb) {
+      const int r = q - nf;
+      const int ci = r / KPb, k = r - ci * KPb;
+      float v = 0.f;
+      if (ci < cin && k < 27 * gc) {
+        const int tp = k / gc, c = k - tp * gc;
+        if (c < cout) v = theta[((size_t)c * 27 + (26 - tp)) * cin + ci];
+      }
+      wb[r] = (_Float16)v;
+    }
+  }
+}
+
+// Adam (Kingma & Ba 2015) on the f32 master parameters; skipped entirely
+// when a gradient was non-finite.
+__global__ void k_adam(float* theta, float* m1, float* m2, const float* grad, size_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2, const int* bad) {
+  if (*bad) return;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    const float gq = grad[q];
+    const float a = b1 * m1[q] + (1.f - b1) * gq;
+    const float b = b2 * m2[q] + (1.f - b2) * gq * gq;
+    m1[q] = a;
+    m2[q] = b;
+    theta[q] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+  }
+}
+
+// Network input of a residual: (r * sigma, wet, ghost row, solid) in binary16,
+// sigma = sqrt(n_wet / |r|^2) taken from the device (f64) or given.
+__global__ void k_features(const float* r, const uint8_t* mask, const uint8_t* solid, size_t cells,
+                           const double* rr, const uint32_t* nwet, float sigma_given, _Float16* x, float* sigma_out,
+                           const int32_t* gate) {
+  if (gate && !*gate) return;
+  float sigma = sigma_given;
+  if (rr) {
+    const double s = *rr > 0.0 ? sqrt((double)(*nwet > 0 ? *nwet : 1u) / *rr) : 1.0;
+    sigma = (float)s;
+  }
+  if (sigma_out && blockIdx.x == 0 && threadIdx.x == 0) *sigma_out = sigma;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += (size_t)gridDim.x * blockDim.x) {
+    const uint8_t m = mask[c];
+    const bool wet = (m & 64u) != 0;
+    h4 v;
+    v[0] = (_Float16)(wet ? r[c] * sigma : 0.f);
+    v[1] = (_Float16)(wet ? 1.f : 0.f);
+    v[2] = (_Float16)((wet && (m & 128u)) ? 1.f : 0.f);
+    v[3] = (_Float16)((solid && solid[c]) ? 1.f : 0.f);
+    *reinterpret_cast<h4*>(x + 4 * c) = v;
+  }
+}
+
+// z = y / sigma on wet cells, 0 elsewhere.
+__global__ void k_unscale(const float* y, const uint8_t* mask, const float* sigma, size_t cells, float* z,
+                          const int32_t* gate) {
+  if (gate && !*gate) return;
+  const float s = *sigma;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += (size_t)gridDim.x * blockDim.x)
+    z[c] = (mask[c] & 64u) ? y[c] / s : 0.f;
+}
+
+// One 16x16x16 product, for checking the MFMA operand layout.
+__global__ void k_mfma_probe(const _Float16* A, const _Float16* B, float* D) {
+  const int l = threadIdx.x, m = l & 15, g = l >> 4;
+  h4 a, b;
+  for (int v = 0; v < 4; ++v) {
+    a[v] = A[m * 16 + 4 * g + v];
+    b[v] = B[(4 * g + v) * 16 + m];
+  }
+  f4 d = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+  for (int v = 0; v < 4; ++v) D[(4 * g + v) * 16 + m] = d[v];
+}
+
+}  // namespace net
+}  // namespace fsp
+
+// ================================================================ host side
+namespace fsp {
+namespace net {
+
+#define NET_HIP(x)                          \
+  do {                                      \
+    if ((x) != hipSuccess) return FSP_DEVICE; \
+  } while (0)
+
+inline unsigned blocks_for(size_t n, unsigned threads = 256, unsigned cap = 4096) {
+  size_t b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  return (unsigned)(b < cap ? b : cap);
+}
+
+}  // namespace net
+}  // namespace fsp
+
+using namespace fsp::net;
+
+struct fsp_net {
+  int device = 0;
+  hipStream_t stream = nullptr;  // the network's own stream
+  hipStream_t cur = nullptr;     // stream of the call in progress
+  const int32_t* gate = nullptr; // gate of the forward pass in progress
+  std::vector<void*> allocs;
+  size_t nparam_allocs = 0;  // leading allocations that belong to the parameters (kept across shapes)
+  // parameters
+  size_t P = 0;
+  float *theta = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr;
+  _Float16* wt[NL] = {};
+  _Float16* wb[NL] = {};
+  size_t poff[NL] = {};  // offset of layer l's W in theta (b follows)
+  int* bad = nullptr;
+  int* hbad = nullptr;
+  double* hloss = nullptr;
+  uint64_t step = 0;
+  float gscale = 0.f;
+  // activations for one shape
+  int nx = 0, ny = 0, nz = 0;
+  size_t V[3] = {};
+  _Float16* x0 = nullptr;
+  _Float16* act[NL] = {};
+  float* y = nullptr;
+  _Float16 *p1 = nullptr, *p2 = nullptr;
+  // backward
+  _Float16* gpre[NL] = {};  // gradient at each layer's pre-activation
+  _Float16 *gy = nullptr, *gf16 = nullptr, *gf32 = nullptr, *gp2 = nullptr, *gp1 = nullptr;
+  float* partial = nullptr;
+  size_t partial_floats = 0;
+  double* lpart = nullptr;
+  float* sigma = nullptr;
+  uint8_t* wet = nullptr;
+};
+
+namespace {
+
+template <class T>
+fsp_status nalloc(fsp_net* N, T** out, size_t count) {
+  void* p = nullptr;
+  size_t bytes = count * sizeof(T);
+  if (bytes == 0) bytes = 256;
+  if (hipMalloc(&p, bytes) != hipSuccess) return FSP_DEVICE;
+  if (hipMemsetAsync(p, 0, bytes, N->stream) != hipSuccess) return FSP_DEVICE;
+  N->allocs.push_back(p);
+  *out = static_cast<T*>(p);
+  return FSP_OK;
+}
+
+void free_shape(fsp_net* N, size_t keep) {
+  while (N->allocs.size() > keep) {
+    (void)hipFree(N->allocs.back());
+    N->allocs.pop_back();
+  }
+}
+
+template <int CIN, int COUT, int UP, int EPI>
+void conv(hipStream_t st, int nx, int ny, int nz, const _Float16* in, const _Float16* wt, const float* bias,
+          const _Float16* skip, _Float16* out, float* yout, const int32_t* gate = nullptr) {
+  ConvArgs a{};
+  a.gate = gate;
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  a.cnx = UP ? nx / 2 : nx; a.cny = UP ? ny / 2 : ny; a.cnz = UP ? nz / 2 : nz;
+  a.in = in; a.wt = wt; a.bias = bias; a.skip = skip; a.out = out; a.yout = yout;
+  const size_t waves = (size_t)(nx / 16) * ny * nz;
+  const unsigned blocks = (unsigned)((waves + WAVES - 1) / WAVES);
+  k_conv<CIN, COUT, UP, EPI><<<blocks, THREADS, 0, st>>>(a);
+}
+
+// dW of layer l (g: gradient at its pre-activation with channel stride gs)
+template <int CIN, int COUT, int UP>
+void wgrad(fsp_net* N, int l, int nx, int ny, int nz, const _Float16* g, const _Float16* in, int gs = COUT) {
+  constexpr int MT = npad(COUT) / 16, CH = kpad(CIN) / 16;
+  const size_t segs = (size_t)(nx / 16) * ny * nz;
+  int parts = (int)((8192 + CH * MT - 1) / (CH * MT));
+  if ((size_t)parts > segs) parts = (int)segs;
+  if ((size_t)parts * npad(COUT) * kpad(CIN) > N->partial_floats) parts = (int)(N->partial_floats / (npad(COUT) * kpad(CIN)));
+  if (parts < 1) parts = 1;
+  WgradArgs a{};
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  a.cnx = UP ? nx / 2 : nx; a.cny = UP ? ny / 2 : ny; a.cnz = UP ? nz / 2 : nz;
+  a.g = g; a.gs = gs; a.in = in; a.partial = N->partial; a.parts = parts;
+  const size_t waves = (size_t)CH * MT * parts;
+  k_wgrad<CIN, COUT, UP><<<(unsigned)((waves + WAVES - 1) / WAVES), THREADS, 0, N->cur>>>(a);
+  const float inv = 1.0f / N->gscale;
+  k_wgrad_reduce<<<blocks_for((size_t)COUT * 27 * CIN), 256, 0, N->cur>>>(N->partial, parts, CIN, COUT, inv,
+                                                                             N->grad + N->poff[l], N->bad);
+  k_bgrad<<<COUT, 256, 0, N->cur>>>(g, (size_t)nx * ny * nz, gs, inv, N->grad + N->poff[l] + (size_t)COUT * 27 * CIN,
+                                       N->bad);
+}
+
+int lvl_n(const fsp_net* N, int level, int axis) {
+  const int d = axis == 0 ? N->nx : axis == 1 ? N->ny : N->nz;
+  return d >> level;
+}
+
+void forward(fsp_net* N) {
+  hipStream_t st = N->cur;
+  const int32_t* gt = N->gate;
+  const int X0 = N->nx, Y0 = N->ny, Z0 = N->nz;
+  const int X1 = X0 / 2, Y1 = Y0 / 2, Z1 = Z0 / 2, X2 = X0 / 4, Y2 = Y0 / 4, Z2 = Z0 / 4;
+  auto B = [&](int l) { return N->theta + N->poff[l] + (size_t)LAYERS[l].cout * 27 * LAYERS[l].cin; };
+  conv<4, 8, 0, 0>(st, X0, Y0, Z0, N->x0, N->wt[0], B(0), nullptr, N->act[0], nullptr, gt);
+  conv<8, 8, 0, 0>(st, X0, Y0, Z0, N->act[0], N->wt[1], B(1), nullptr, N->act[1], nullptr, gt);
+  k_pool<<<blocks_for(N->V[1] * 8), 256, 0, st>>>(N->act[1], N->p1, X1, Y1, Z1, 8, gt);
+  conv<8, 16, 0, 0>(st, X1, Y1, Z1, N->p1, N->wt[2], B(2), nullptr, N->act[2], nullptr, gt);
+  conv<16, 16, 0, 0>(st, X1, Y1, Z1, N->act[2], N->wt[3], B(3), nullptr, N->act[3], nullptr, gt);
+  k_pool<<<blocks_for(N->V[2] * 16), 256, 0, st>>>(N->act[3], N->p2, X2, Y2, Z2, 16, gt);
+  conv<16, 32, 0, 0>(st, X2, Y2, Z2, N->p2, N->wt[4], B(4), nullptr, N->act[4], nullptr, gt);
+  conv<32, 32, 0, 0>(st, X2, Y2, Z2, N->act[4], N->wt[5], B(5), nullptr, N->act[5], nullptr, gt);
+  conv<32, 16, 1, 0>(st, X1, Y1, Z1, N->act[5], N->wt[6], B(6), N->act[3], N->act[6], nullptr, gt);
+  conv<16, 16, 0, 0>(st, X1, Y1, Z1, N->act[6], N->wt[7], B(7), nullptr, N->act[7], nullptr, gt);
+  conv<16, 8, 1, 0>(st, X0, Y0, Z0, N->act[7], N->wt[8], B(8), N->act[1], N->act[8], nullptr, gt);
+  conv<8, 8, 0, 0>(st, X0, Y0, Z0, N->act[8], N->wt[9], B(9), nullptr, N->act[9], nullptr, gt);
+  conv<8, 1, 0, 1>(st, X0, Y0, Z0, N->act[9], N->wt[10], B(10), nullptr, nullptr, N->y, gt);
+}
+
+void relu_back(fsp_net* N, const _Float16* h, _Float16* g, size_t n) {
+  k_relu_back<<<blocks_for(n), 256, 0, N->cur>>>(h, g, n);
+}
+
+// Gradients of the loss (its output gradient already in N->gy) into N->grad.
+void backward(fsp_net* N) {
+  hipStream_t st = N->cur;
+  const int X0 = N->nx, Y0 = N->ny, Z0 = N->nz;
+  const int X1 = X0 / 2, Y1 = Y0 / 2, Z1 = Z0 / 2, X2 = X0 / 4, Y2 = Y0 / 4, Z2 = Z0 / 4;
+  const size_t V0 = N->V[0], V1 = N->V[1], V2 = N->V[2];
+  _Float16** g = N->gpre;
+  // head (gradient gy: 4 channels, channel 0 used)
+  wgrad<8, 1, 0>(N, 10, X0, Y0, Z0, N->gy, N->act[9], 4);
+  conv<4, 8, 0, 2>(st, X0, Y0, Z0, N->gy, N->wb[10], nullptr, nullptr, g[9], nullptr);
+  relu_back(N, N->act[9], g[9], V0 * 8);
+  wgrad<8, 8, 0>(N, 9, X0, Y0, Z0, g[9], N->act[8]);
+  conv<8, 8, 0, 2>(st, X0, Y0, Z0, g[9], N->wb[9], nullptr, nullptr, g[8], nullptr);
+  relu_back(N, N->act[8], g[8], V0 * 8);
+  wgrad<16, 8, 1>(N, 8, X0, Y0, Z0, g[8], N->act[7]);
+  conv<8, 16, 0, 2>(st, X0, Y0, Z0, g[8], N->wb[8], nullptr, nullptr, N->gf16, nullptr);
+  k_up_back<<<blocks_for(V1 * 16), 256, 0, st>>>(N->gf16, g[7], X1, Y1, Z1, 16);
+  relu_back(N, N->act[7], g[7], V1 * 16);
+  wgrad<16, 16, 0>(N, 7, X1, Y1, Z1, g[7], N->act[6]);
+  conv<16, 16, 0, 2>(st, X1, Y1, Z1, g[7], N->wb[7], nullptr, nullptr, g[6], nullptr);
+  relu_back(N, N->act[6], g[6], V1 * 16);
+  wgrad<32, 16, 1>(N, 6, X1, Y1, Z1, g[6], N->act[5]);
+  conv<16, 32, 0, 2>(st, X1, Y1, Z1, g[6], N->wb[6], nullptr, nullptr, N->gf32, nullptr);
+  k_up_back<<<blocks_for(V2 * 32), 256, 0, st>>>(N->gf32, g[5], X2, Y2, Z2, 32);
+  relu_back(N, N->act[5], g[5], V2 * 32);
+  wgrad<32, 32, 0>(N, 5, X2, Y2, Z2, g[5], N->act[4]);
+  conv<32, 32, 0, 2>(st, X2, Y2, Z2, g[5], N->wb[5], nullptr, nullptr, g[4], nullptr);
+  relu_back(N, N->act[4], g[4], V2 * 32);
+  wgrad<16, 32, 0>(N, 4, X2, Y2, Z2, g[4], N->p2);
+  conv<32, 16, 0, 2>(st, X2, Y2, Z2, g[4], N->wb[4], nullptr, nullptr, N->gp2, nullptr);
+  k_pool_back<<<blocks_for(V1 * 16), 256, 0, st>>>(N->gp2, g[6], g[3], X1, Y1, Z1, 16);
+  relu_back(N, N->act[3], g[3], V1 * 16);
+  wgrad<16, 16, 0>(N, 3, X1, Y1, Z1, g[3], N->act[2]);
+  conv<16, 16, 0, 2>(st, X1, Y1, Z1, g[3], N->wb[3], nullptr, nullptr, g[2], nullptr);
+  relu_back(N, N->act[2], g[2], V1 * 16);
+  wgrad<8, 16, 0>(N, 2, X1, Y1, Z1, g[2], N->p1);
+  conv<16, 8, 0, 2>(st, X1, Y1, Z1, g[2], N->wb[2], nullptr, nullptr, N->gp1, nullptr);
+  k_pool_back<<<blocks_for(V0 * 8), 256, 0, st>>>(N->gp1, g[8], g[1], X0, Y0, Z0, 8);
+  relu_back(N, N->act[1], g[1], V0 * 8);
+  wgrad<8, 8, 0>(N, 1, X0, Y0, Z0, g[1], N->act[0]);
+  conv<8, 8, 0, 2>(st, X0, Y0, Z0, g[1], N->wb[1], nullptr, nullptr, g[0], nullptr);
+  relu_back(N, N->act[0], g[0], V0 * 8);
+  wgrad<4, 8, 0>(N, 0, X0, Y0, Z0, g[0], N->x0);
+}
+
+}  // namespace
+
+namespace fsp {
+
+bool net_shape_ok(int nx, int ny, int nz) { return nx > 0 && ny > 0 && nz > 0 && nx % 64 == 0 && ny % 4 == 0 && nz % 4 == 0; }
+
+fsp_status net_reserve(fsp_net* N, int nx, int ny, int nz) {
+  if (!net_shape_ok(nx, ny, nz)) return FSP_SHAPE;
+  if (N->nx == nx && N->ny == ny && N->nz == nz) return FSP_OK;
+  NET_HIP(hipSetDevice(N->device));
+  NET_HIP(hipStreamSynchronize(N->stream));
+  free_shape(N, N->nparam_allocs);
+  N->nx = N->ny = N->nz = 0;
+  const size_t V0 = (size_t)nx * ny * nz, V1 = V0 / 8, V2 = V0 / 64;
+  N->V[0] = V0; N->V[1] = V1; N->V[2] = V2;
+  fsp_status st;
+  if ((st = nalloc(N, &N->x0, V0 * 4)) || (st = nalloc(N, &N->y, V0)) || (st = nalloc(N, &N->p1, V1 * 8)) ||
+      (st = nalloc(N, &N->p2, V2 * 16)) || (st = nalloc(N, &N->gy, V0 * 4)) || (st = nalloc(N, &N->gf16, V0 * 16)) ||
+      (st = nalloc(N, &N->gf32, V1 * 32)) || (st = nalloc(N, &N->gp2, V2 * 16)) || (st = nalloc(N, &N->gp1, V1 * 8)) ||
+      (st = nalloc(N, &N->sigma, 1)) || (st = nalloc(N, &N->wet, V0)) || (st = nalloc(N, &N->lpart, 4096)))
+    return st;
+  for (int l = 0; l < NL; ++l) {
+    const size_t Vl = N->V[LAYERS[l].level];
+    if (l < NL - 1 && (st = nalloc(N, &N->act[l], Vl * LAYERS[l].cout))) return st;
+    if ((st = nalloc(N, &N->gpre[l], Vl * gch(LAYERS[l].cout)))) return st;
+  }
+  // weight-gradient partials: up to 8192 wavefronts' worth, at least one partition per layer
+  size_t most = 0;
+  for (const LayerDesc& L : LAYERS) most = std::max(most, (size_t)npad(L.cout) * kpad(L.cin));
+  N->partial_floats = std::max(most * 64, (size_t)8192 * 256);
+  if ((st = nalloc(N, &N->partial, N->partial_floats))) return st;
+  N->nx = nx; N->ny = ny; N->nz = nz;
+  return FSP_OK;
+}
+
+fsp_status net_features(fsp_net* N, hipStream_t stream, const float* r, const uint8_t* mask, const uint8_t* solid,
+                        const double* rr, const uint32_t* nwet, float sigma, uint16_t* x_out) {
+  const size_t V = N->V[0];
+  k_features<<<blocks_for(V), 256, 0, stream>>>(r, mask, solid, V, rr, nwet, sigma,
+                                               reinterpret_cast<_Float16*>(x_out), N->sigma, nullptr);
+  return hipGetLastError() == hipSuccess ? FSP_OK : FSP_DEVICE;
+}
+
+fsp_status net_apply(fsp_net* N, hipStream_t stream, const float* r, const uint8_t* mask, const uint8_t* solid,
+                     const double* rr, const uint32_t* nwet, const int32_t* gate, float* z) {
+  const size_t V = N->V[0];
+  k_features<<<blocks_for(V), 256, 0, stream>>>(r, mask, solid, V, rr, nwet, 0.f, N->x0, N->sigma, gate);
+  N->cur = stream;
+  N->gate = gate;
+  forward(N);
+  N->gate = nullptr;
+  k_unscale<<<blocks_for(V), 256, 0, stream>>>(N->y, mask, N->sigma, V, z, gate);
+  return hipGetLastError() == hipSuccess ? FSP_OK : FSP_DEVICE;
+}
+
+}  // namespace fsp
+
+namespace {
+
+fsp_status pack_all(fsp_net* N) {
+  for (int l = 0; l < NL; ++l) {
+    const LayerDesc& L = LAYERS[l];
+    const size_t total = (size_t)npad(L.cout) * kpad(L.cin) + (size_t)npad(L.cin) * kpad(gch(L.cout));
+    k_pack<<<blocks_for(total), 256, 0, N->cur>>>(N->theta + N->poff[l], L.cin, L.cout, N->wt[l], l > 0 ? N->wb[l] : nullptr);
+  }
+  return hipGetLastError() == hipSuccess ? FSP_OK : FSP_DEVICE;
+}
+
+// loss (host) and gradients into N->grad for the current x0 / target / wet
+fsp_status loss_and_grad(fsp_net* N, const float* target, const uint8_t* wet, double* loss, bool* bad) {
+  const size_t V = N->V[0];
+  NET_HIP(hipMemsetAsync(N->bad, 0, sizeof(int), N->cur));
+  forward(N);
+  // wet count for the mean
+  std::vector<uint8_t> hw;
+  size_t nwet = 0;
+  {
+    hw.resize(V);
+    NET_HIP(hipMemcpyAsync(hw.data(), wet, V, hipMemcpyDeviceToHost, N->cur));
+    NET_HIP(hipStreamSynchronize(N->cur));
+    for (uint8_t b : hw) nwet += b ? 1 : 0;
+  }
+  if (nwet == 0) nwet = 1;
+  if (N->gscale <= 0.f) N->gscale = std::ldexp(1.0f, (int)std::lround(std::log2((double)nwet)));
+  const unsigned blocks = std::min<unsigned>(4096, blocks_for(V));
+  k_loss<<<blocks, 256, 0, N->cur>>>(N->y, target, wet, V, 1.0 / (double)nwet, N->gscale, N->gy, N->lpart);
+  backward(N);
+  std::vector<double> lp(blocks);
+  NET_HIP(hipMemcpyAsync(lp.data(), N->lpart, blocks * sizeof(double), hipMemcpyDeviceToHost, N->cur));
+  NET_HIP(hipMemcpyAsync(N->hbad, N->bad, sizeof(int), hipMemcpyDeviceToHost, N->cur));
+  NET_HIP(hipStreamSynchronize(N->cur));
+  double s = 0.0;
+  for (double v : lp) s += v;
+  if (loss) *loss = s;
+  *bad = *N->hbad != 0;
+  return FSP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t fsp_net_param_count(void) { return (uint32_t)param_count(); }
+
+fsp_status fsp_net_create(int32_t device, fsp_net** out) {
+  if (!out) return FSP_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return FSP_DEVICE;
+  fsp_net* N = new (std::nothrow) fsp_net();
+  if (!N) return FSP_DEVICE;
+  N->device = device;
+  auto fail = [&](fsp_status s) { fsp_net_destroy(N); return s; };
+  if (hipSetDevice(device) != hipSuccess) return fail(FSP_DEVICE);
+  if (hipStreamCreateWithFlags(&N->stream, hipStreamNonBlocking) != hipSuccess) return fail(FSP_DEVICE);
+  N->cur = N->stream;
+  N->P = param_count();
+  size_t off = 0;
+  for (int l = 0; l < NL; ++l) {
+    N->poff[l] = off;
+    off += (size_t)LAYERS[l].cout * 27 * LAYERS[l].cin + LAYERS[l].cout;
+  }
+  fsp_status st;
+  if ((st = nalloc(N, &N->theta, N->P)) || (st = nalloc(N, &N->m1, N->P)) || (st = nalloc(N, &N->m2, N->P)) ||
+      (st = nalloc(N, &N->grad, N->P)) || (st = nalloc(N, &N->bad, 1)))
+    return fail(st);
+  for (int l = 0; l < NL; ++l) {
+    const LayerDesc& L = LAYERS[l];
+    if ((st = nalloc(N, &N->wt[l], (size_t)npad(L.cout) * kpad(L.cin))) ||
+        (st = nalloc(N, &N->wb[l], (size_t)npad(L.cin) * kpad(gch(L.cout)))))
+      return fail(st);
+  }
+  N->nparam_allocs = N->allocs.size();
+  if (hipHostMalloc((void**)&N->hbad, sizeof(int), hipHostMallocDefault) != hipSuccess) return fail(FSP_DEVICE);
+  if (hipStreamSynchronize(N->stream) != hipSuccess) return fail(FSP_DEVICE);
+  *out = N;
+  return FSP_OK;
+}
+
+void fsp_net_destroy(fsp_net* N) {
+  if (!N) return;
+  (void)hipSetDevice(N->device);
+  if (N->stream) (void)hipStreamSynchronize(N->stream);
+  free_shape(N, 0);
+  if (N->hbad) (void)hipHostFree(N->hbad);
+  if (N->stream) (void)hipStreamDestroy(N->stream);
+  delete N;
+}
+
+fsp_status fsp_net_set_params(fsp_net* N, const float* theta, size_t n) {
+  if (!N || !theta) return FSP_INVALID;
+  if (n != N->P) return FSP_SHAPE;
+  for (size_t q = 0; q < n; ++q)
+    if (!std::isfinite(theta[q])) return FSP_NOT_FINITE;
+  NET_HIP(hipSetDevice(N->device));
+  N->cur = N->stream;
+  NET_HIP(hipMemcpyAsync(N->theta, theta, n * sizeof(float), hipMemcpyHostToDevice, N->stream));
+  NET_HIP(hipMemsetAsync(N->m1, 0, n * sizeof(float), N->stream));
+  NET_HIP(hipMemsetAsync(N->m2, 0, n * sizeof(float), N->stream));
+  N->step = 0;
+  N->gscale = 0.f;
+  fsp_status st = pack_all(N);
+  if (st) return st;
+  NET_HIP(hipStreamSynchronize(N->stream));
+  return FSP_OK;
+}
+
+fsp_status fsp_net_get_params(fsp_net* N, float* theta, size_t n) {
+  if (!N || !theta) return FSP_INVALID;
+  if (n != N->P) return FSP_SHAPE;
+  NET_HIP(hipSetDevice(N->device));
+  NET_HIP(hipMemcpyAsync(theta, N->theta, n * sizeof(float), hipMemcpyDeviceToHost, N->stream));
+  NET_HIP(hipStreamSynchronize(N->stream));
+  return FSP_OK;
+}
+
+fsp_status fsp_net_reserve(fsp_net* N, int32_t nx, int32_t ny, int32_t nz) {
+  if (!N) return FSP_INVALID;
+  return fsp::net_reserve(N, nx, ny, nz);
+}
+
+fsp_status fsp_net_forward_device(fsp_net* N, int32_t nx, int32_t ny, int32_t nz, const uint16_t* d_x, float* d_y) {
+  if (!N || !d_x || !d_y) return FSP_INVALID;
+  NET_HIP(hipSetDevice(N->device));
+  fsp_status st = fsp::net_reserve(N, nx, ny, nz);
+  if (st) return st;
+  N->cur = N->stream;
+  NET_HIP(hipMemcpyAsync(N->x0, d_x, N->V[0] * 4 * sizeof(uint16_t), hipMemcpyDeviceToDevice, N->stream));
+  forward(N);
+  NET_HIP(hipMemcpyAsync(d_y, N->y, N->V[0] * sizeof(float), hipMemcpyDeviceToDevice, N->stream));
+  NET_HIP(hipStreamSynchronize(N->stream));
+  return FSP_OK;
+}
+
+fsp_status fsp_net_loss_grad_device(fsp_net* N, int32_t nx, int32_t ny, int32_t nz, const uint16_t* d_x,
+                                    const float* d_target, const uint8_t* d_wet, double* loss, float* grad) {
+  if (!N || !d_x || !d_target || !d_wet) return FSP_INVALID;
+  NET_HIP(hipSetDevice(N->device));
+  fsp_status st = fsp::net_reserve(N, nx, ny, nz);
+  if (st) return st;
+  N->cur = N->stream;
+  NET_HIP(hipMemcpyAsync(N->x0, d_x, N->V[0] * 4 * sizeof(uint16_t), hipMemcpyDeviceToDevice, N->stream));
+  bool bad = false;
+  if ((st = loss_and_grad(N, d_target, d_wet, loss, &bad))) return st;
+  if (bad) return FSP_NOT_FINITE;
+  if (grad) {
+    NET_HIP(hipMemcpyAsync(grad, N->grad, N->P * sizeof(float), hipMemcpyDeviceToHost, N->stream));
+    NET_HIP(hipStreamSynchronize(N->stream));
+  }
+  return FSP_OK;
+}
+
+fsp_status fsp_net_train_step_device(fsp_net* N, int32_t nx, int32_t ny, int32_t nz, const uint16_t* d_x,
+                                     const float* d_target, const uint8_t* d_wet, float lr, double* loss,
+                                     int32_t* skipped) {
+  if (!N || !d_x || !d_target || !d_wet || !(lr > 0.f)) return FSP_INVALID;
+  NET_HIP(hipSetDevice(N->device));
+  fsp_status st = fsp::net_reserve(N, nx, ny, nz);
+  if (st) return st;
+  N->cur = N->stream;
+  NET_HIP(hipMemcpyAsync(N->x0, d_x, N->V[0] * 4 * sizeof(uint16_t), hipMemcpyDeviceToDevice, N->stream));
+  bool bad = false;
+  if ((st = loss_and_grad(N, d_target, d_wet, loss, &bad))) return st;
+  if (skipped) *skipped = bad ? 1 : 0;
+  if (bad) {
+    N->gscale *= 0.5f;
+    return FSP_OK;
+  }
+  N->step += 1;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, (float)N->step), c2 = 1.f - std::pow(b2, (float)N->step);
+  k_adam<<<blocks_for(N->P), 256, 0, N->stream>>>(N->theta, N->m1, N->m2, N->grad, N->P, lr, b1, b2, 1e-8f, c1, c2,
+                                                  N->bad);
+  if ((st = pack_all(N))) return st;
+  NET_HIP(hipStreamSynchronize(N->stream));
+  return FSP_OK;
+}
+
+fsp_status fsp_net_mfma_probe(const uint16_t* d_a, const uint16_t* d_b, float* d_d) {
+  if (!d_a || !d_b || !d_d) return FSP_INVALID;
+  k_mfma_probe<<<1, 64>>>(reinterpret_cast<const _Float16*>(d_a), reinterpret_cast<const _Float16*>(d_b), d_d);
+  NET_HIP(hipDeviceSynchronize());
+  return FSP_OK;
+}
+
+}  // extern "C"

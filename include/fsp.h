@@ -43,7 +43,10 @@ typedef struct {
   float dx;
 } fsp_domain;
 
-enum { FSP_PRECOND_JACOBI = 0, FSP_PRECOND_MULTIGRID = 1 };
+/* FSP_PRECOND_LEARNED: the network of fsp_learned.h (fsp_attach_net) inside
+ * flexible CG, falling back to MGPCG (V-cycle as configured) when an FCG
+ * pass of learned_budget iterations ends unconverged. */
+enum { FSP_PRECOND_JACOBI = 0, FSP_PRECOND_MULTIGRID = 1, FSP_PRECOND_LEARNED = 2 };
 enum { FSP_SMOOTH_JACOBI = 0, FSP_SMOOTH_RBGS = 1 };
 enum { FSP_STORE_F32 = 0, FSP_STORE_F16 = 1, FSP_STORE_BF16 = 2 };
 /* Coarse operators: the reference's re-discretisation (averaged apertures,
@@ -66,6 +69,8 @@ typedef struct {
   int32_t refuse_degraded; /* 1: a degraded projection returns FSP_CONVERGENCE, outputs untouched */
   int32_t device;          /* HIP device ordinal */
   int32_t coarse_operator; /* FSP_COARSE_* */
+  int32_t graph;           /* 1: replay each CG iteration from a captured HIP graph */
+  int32_t learned_budget;  /* FSP_PRECOND_LEARNED: FCG iterations before the MGPCG fallback (0: 50) */
 } fsp_options;
 
 /* Surface state (Volume3 fields rho, g_eff, rest, homogeneous_ghost;

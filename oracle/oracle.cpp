@@ -201,5 +201,9 @@ uint64_t orc_fnv_bytes(const uint8_t* b, size_t n) {
   return h;
 }
 float orc_roundoff_3d() { return roundoff_3d<float>(); }
+// Storage rounding of the V-cycle vectors (Options::storage), in place.
+void orc_round_storage(int storage, float* x, size_t n) {
+  for (size_t i = 0; i < n; ++i) x[i] = round_storage(storage, x[i]);
+}
 
 }  // extern "C"

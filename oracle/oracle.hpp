@@ -60,9 +60,48 @@ struct Options {
   int32_t gauge;          // EXTENSION: 1 = remove the mean of b and of p on a pure-Neumann problem
   int32_t coarse_op;      // 0: re-discretised coarse levels (delta3d_multigrid.rs:141-255);
                           // 1: EXTENSION Galerkin P^T A P / 16 with piecewise-constant P
+  int32_t storage;        // EXTENSION: V-cycle vectors rounded to 0 f32 (the reference), 1 IEEE binary16,
+                          // 2 bfloat16 at every store; arithmetic stays in T
+  int32_t flexible;       // EXTENSION: 1 = flexible CG, beta = <z', r' - r> / <z, r> (Polak-Ribiere)
 };
 
-inline Options default_options() { return Options{1, 0, -1, 2, 2, 8, 1, 0}; }
+inline Options default_options() { return Options{1, 0, -1, 2, 2, 8, 1, 0, 0, 0}; }
+
+// Round-to-nearest-even of an f32 value to bfloat16 (8 significant bits,
+// f32 exponent range); the result is returned as the f32 it represents.
+inline float round_bf16(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7F800000u) == 0x7F800000u) return x;  // inf / nan unchanged
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  u &= 0xFFFF0000u;
+  float y;
+  std::memcpy(&y, &u, 4);
+  return y;
+}
+// Round-to-nearest-even of an f32 value to IEEE binary16 (11 significant
+// bits, normal range [2^-14, 65504], subnormal step 2^-24, overflow to inf).
+inline float round_f16(float x) {
+  const float ax = std::fabs(x);
+  if (!(ax < std::numeric_limits<float>::infinity())) return x;
+  if (ax < 6.103515625e-05f) {  // 2^-14: fixed step 2^-24
+    const float y = std::nearbyint(ax * 16777216.0f) * 5.9604644775390625e-08f;
+    return std::copysign(y, x);
+  }
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0xFFFu + ((u >> 13) & 1u);
+  u &= 0xFFFFE000u;
+  float y;
+  std::memcpy(&y, &u, 4);
+  if (std::fabs(y) > 65504.0f) return std::copysign(std::numeric_limits<float>::infinity(), x);
+  return y;
+}
+template <class T> inline T round_storage(int storage, T x) {
+  if (storage == 1) return (T)round_f16((float)x);
+  if (storage == 2) return (T)round_bf16((float)x);
+  return x;
+}
 
 // Geometry: the subset of Volume3's state that the projection reads.
 struct Geometry {
@@ -137,6 +176,7 @@ template <class T> struct Level3 {
   std::vector<T> open_u, open_v, open_w, kind, diag, x, r, t;
   std::vector<T> dd;   // Galerkin levels: Dirichlet part of the diagonal (units of inv)
   bool galerkin = false;
+  int storage = 0;     // Options::storage of the vectors x and r
   size_t cells() const { return nx * ny * nz; }
   size_t c(size_t i, size_t j, size_t k) const { return (k * ny + j) * nx + i; }
 };
@@ -523,6 +563,7 @@ template <class T> class Volume {
       size_t n = cx * cy * cz;
       L.kind.assign(n, T(0)); L.diag = L.kind; L.x = L.kind; L.r = L.kind; L.t = L.kind; L.dd = L.kind;
       L.galerkin = opt.coarse_op == 1;
+      L.storage = opt.storage;
       levels.push_back(std::move(L));
       x = cx; y = cy; z = cz;
     }
@@ -704,7 +745,7 @@ template <class T> class Volume {
         for (size_t i = 0; i < L.nx; ++i) {
           T s = T(0);
           for (auto& ch : CHILDREN) s += fine[((2 * k + ch[2]) * fny + 2 * j + ch[1]) * fnx + 2 * i + ch[0]];
-          L.r[L.c(i, j, k)] = T(0.125) * s;
+          L.r[L.c(i, j, k)] = round_storage(L.storage, T(0.125) * s);
         }
   }
   // delta3d_multigrid.rs:275-287
@@ -713,7 +754,10 @@ template <class T> class Volume {
       for (size_t j = 0; j < L.ny; ++j)
         for (size_t i = 0; i < L.nx; ++i) {
           T val = L.x[L.c(i, j, k)];
-          for (auto& ch : CHILDREN) fine[((2 * k + ch[2]) * fny + 2 * j + ch[1]) * fnx + 2 * i + ch[0]] += val;
+          for (auto& ch : CHILDREN) {
+            T& f = fine[((2 * k + ch[2]) * fny + 2 * j + ch[1]) * fnx + 2 * i + ch[0]];
+            f = round_storage(L.storage, f + val);
+          }
         }
   }
 
@@ -723,7 +767,7 @@ template <class T> class Volume {
   static void smooth3(Level3<T>& L) {
     row_sums(L, &L.x, L.t);
     for (size_t cc = 0; cc < L.cells(); ++cc)
-      if (L.diag[cc] > T(0)) L.x[cc] += damping() * (L.r[cc] - L.t[cc]) / L.diag[cc];
+      if (L.diag[cc] > T(0)) L.x[cc] = round_storage(L.storage, L.x[cc] + damping() * (L.r[cc] - L.t[cc]) / L.diag[cc]);
   }
   // EXTENSION: one colour of a Gauss-Seidel sweep on a coarse level. Cells of
   // one colour do not couple, so the update uses the other colour's values.
@@ -734,7 +778,7 @@ template <class T> class Volume {
         for (size_t i = 0; i < L.nx; ++i) {
           if ((int)((i + j + k) & 1) != colour) continue;
           size_t cc = L.c(i, j, k);
-          if (L.diag[cc] > T(0)) L.x[cc] += (L.r[cc] - L.t[cc]) / L.diag[cc];
+          if (L.diag[cc] > T(0)) L.x[cc] = round_storage(L.storage, L.x[cc] + (L.r[cc] - L.t[cc]) / L.diag[cc]);
         }
   }
   // A sweep: Jacobi (reference) or one red-black pass; `forward` = red then
@@ -783,7 +827,7 @@ template <class T> class Volume {
     if (opt.smoother == 0) {
       apply_mobile3(mz, mt);
       for (size_t cc = 0; cc < cells(); ++cc)
-        if (prec[cc] > T(0)) mz[cc] += damping() * prec[cc] * (res[cc] - mt[cc]);
+        if (prec[cc] > T(0)) mz[cc] = round_storage(opt.storage, mz[cc] + damping() * prec[cc] * (res[cc] - mt[cc]));
       return;
     }
     for (int pass = 0; pass < 2; ++pass) {
@@ -794,7 +838,7 @@ template <class T> class Volume {
           for (size_t i = 0; i < nx; ++i) {
             if ((int)((i + j + k) & 1) != colour) continue;
             size_t cc = c(i, j, k);
-            if (prec[cc] > T(0)) mz[cc] += prec[cc] * (res[cc] - mt[cc]);
+            if (prec[cc] > T(0)) mz[cc] = round_storage(opt.storage, mz[cc] + prec[cc] * (res[cc] - mt[cc]));
           }
     }
   }
@@ -910,7 +954,13 @@ template <class T> class Volume {
         if (mg_on) {
           T zn;
           if ((st = precondition_multigrid3(zn))) return st;
-          direction_multigrid3(zn / rz);
+          if (opt.flexible) {
+            // <z', r' - r> = -alpha <z', q> since r' = r - alpha q
+            const T qz = dot(tmp, mz);
+            direction_multigrid3(-(alpha * qz) / rz);
+          } else {
+            direction_multigrid3(zn / rz);
+          }
           rz = zn;
         } else {
           T zn;

@@ -33,7 +33,7 @@ class Options(C.Structure):
     _fields_ = [
         ("multigrid", C.c_int32), ("smoother", C.c_int32), ("max_levels", C.c_int32),
         ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32), ("coarse_sweeps", C.c_int32),
-        ("gauge", C.c_int32), ("coarse_op", C.c_int32),
+        ("gauge", C.c_int32), ("coarse_op", C.c_int32), ("storage", C.c_int32), ("flexible", C.c_int32),
     ]
 
 
@@ -64,6 +64,8 @@ def lib():
         vp, f32, u32, i32, sz = C.c_void_p, C.c_float, C.c_uint32, C.c_int, C.c_size_t
         L.orc_project.argtypes = [i32, vp, vp, vp, vp, vp, vp, f32, f32, u32, vp, vp, vp, vp, vp, vp]
         L.orc_apply.argtypes = [vp, vp, vp]
+        L.orc_round_storage.argtypes = [i32, vp, C.c_size_t]
+        L.orc_round_storage.restype = None
         L.orc_apply64.argtypes = [vp, vp, vp]
         L.orc_assemble.argtypes = [vp, vp, f32, vp, vp]
         L.orc_divergence.argtypes = [vp, vp, vp, vp, vp]
@@ -138,8 +140,9 @@ class Problem:
         return g
 
 
-def options(multigrid=1, smoother=0, max_levels=-1, pre=2, post=2, coarse=8, gauge=1, coarse_op=0):
-    return Options(multigrid, smoother, max_levels, pre, post, coarse, gauge, coarse_op)
+def options(multigrid=1, smoother=0, max_levels=-1, pre=2, post=2, coarse=8, gauge=1, coarse_op=0, storage=0,
+            flexible=0):
+    return Options(multigrid, smoother, max_levels, pre, post, coarse, gauge, coarse_op, storage, flexible)
 
 
 def project(prob: Problem, us, vs, ws, scale, k1, max_iters=20000, opts=None, p_warm=None, precision=32):
@@ -246,6 +249,13 @@ def level_count3(nx, ny, nz):
 
 def hierarchy_floats3(nx, ny, nz):
     return lib().orc_hierarchy_floats3(nx, ny, nz)
+
+
+def round_storage(storage, values):
+    """The oracle's rounding of a stored V-cycle value (0 f32, 1 binary16, 2 bfloat16)."""
+    a = np.array(values, np.float32, copy=True)
+    lib().orc_round_storage(storage, _ptr(a), a.size)
+    return a
 
 
 def fnv_bytes(data: bytes):

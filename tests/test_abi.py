@@ -1,5 +1,5 @@
-"""The C ABI library loads and exports every entry point include/fsp.h declares
-(CPU only: no compute call needs a device here)."""
+"""The C ABI library loads and exports every entry point include/fsp.h and
+include/fsp_learned.h declare (CPU only: no compute call needs a device here)."""
 import ctypes as C
 import os
 import re
@@ -10,9 +10,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_functions():
-    text = open(os.path.join(ROOT, "include", "fsp.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(fsp_\w+)\s*\(", text)))
+    names = set()
+    for header in ("fsp.h", "fsp_learned.h"):
+        text = open(os.path.join(ROOT, "include", header)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(fsp_\w+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_and_binding_list_agree():

@@ -71,6 +71,8 @@ WORKLOADS = {
     "s385_32": lambda: s385_workload(32),
     "box16": lambda: W.closed_box(16, seed=2),
     "tank34_generic": lambda: W.tank(34, 32, 32, seed=4),
+    # two compact coarse levels above the one-workgroup tail
+    "tank128x64x96": lambda: W.tank(128, 64, 96, seed=6),
 }
 
 
@@ -104,7 +106,8 @@ def test_divergence_is_the_reference_bit_for_bit(name):
     assert np.array_equal(bits(gpu), bits(ref))
 
 
-@pytest.mark.parametrize("name", ["tank32", "s385_32", "box16", "tank34_generic", "tank_flat48x32x40"])
+@pytest.mark.parametrize("name", ["tank32", "s385_32", "box16", "tank34_generic", "tank_flat48x32x40",
+                                  "tank128x64x96"])
 @pytest.mark.parametrize("smoother", [_lib.SMOOTH_JACOBI, _lib.SMOOTH_RBGS])
 def test_vcycle_matches_the_oracle(name, smoother):
     wl = WORKLOADS[name]()
@@ -333,3 +336,48 @@ def test_levels_follow_the_reference_rule():
         vol = Volume3.configure(Domain3(*dims, 0.25))
         assert vol.multigrid_levels() == levels
         vol.close()
+
+
+# ---------------------------------------------------------------- 16-bit V-cycle storage
+STORAGES = {"f16": _lib.STORE_F16, "bf16": _lib.STORE_BF16}
+
+
+@pytest.mark.parametrize("name", ["tank32", "tank_flat48x32x40", "box16", "tank128x64x96"])
+@pytest.mark.parametrize("storage", list(STORAGES))
+def test_vcycle_with_16bit_storage_matches_the_oracle(name, storage):
+    """Every stored V-cycle value rounded to the storage type, arithmetic in f32:
+    the card and the oracle round at the same stores."""
+    wl = WORKLOADS[name]()
+    wl.options["smoother"] = _lib.SMOOTH_RBGS
+    vol = W.build_volume(wl, vcycle_storage=STORAGES[storage])
+    prob = problem(wl)
+    _, prec = po.assemble(prob, np.zeros(wl.domain.cells, np.float32), 1.0)
+    r = noise(wl.domain.cells, 11)
+    r[prec == 0] = 0.0
+    z = host(vol.precondition(torch.from_numpy(r).cuda()))
+    opts = oracle_options(wl)
+    opts.storage = STORAGES[storage]
+    z_ref = po.vcycle(prob, r, opts).astype(np.float32)
+    z32 = po.vcycle(prob, r, oracle_options(wl)).astype(np.float32)
+    # the rounding is real: the result differs from the f32 V-cycle, by about the storage's precision
+    rel = np.abs(z_ref - z32).max() / np.abs(z32).max()
+    assert 0 < rel < (2e-2 if storage == "bf16" else 2e-3)
+    assert np.array_equal(bits(z), bits(z_ref)), f"max diff {np.abs(z - z_ref).max()}"
+
+
+@pytest.mark.parametrize("storage", list(STORAGES))
+def test_tank_with_16bit_storage_matches_the_f64_oracle(storage):
+    wl = W.tank(64, 48, 64, seed=7)
+    vol, rep, (oref, p32, u, v, w, p64) = run_both(wl, precision=64, vcycle_storage=STORAGES[storage])
+    print(rep, oref)
+    assert not rep.degraded and rep.residual <= 1e-6 and rep.divergence_plain <= 1e-5
+    assert rep.iterations <= oref["iterations"] + 1
+    assert_pressure_parity(wl, host(vol.pressure()), p64)
+
+
+def test_16bit_storage_is_refused_off_the_red_black_path():
+    wl = W.tank(32, 32, 32, seed=3)
+    with pytest.raises(ProjectionError):
+        W.build_volume(wl, smoother=_lib.SMOOTH_JACOBI, vcycle_storage=_lib.STORE_BF16)
+    with pytest.raises(ProjectionError):
+        W.build_volume(wl, preconditioner=_lib.PRECOND_JACOBI, vcycle_storage=_lib.STORE_F16)

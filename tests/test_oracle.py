@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 import scipy.fft
 
+from fluidisim_amd import workloads as W
 from oracle import pyoracle as po
 
 
@@ -299,3 +300,63 @@ def test_projection_leaves_a_divergence_free_field(dims):
     u3, v3 = u.reshape(nz, ny, nx + 1), v.reshape(nz, ny + 1, nx)
     for face in (u3[:, :, 0], u3[:, :, -1], v3[:, 0, :], v3[:, -1, :]):
         assert not np.any(face.view(np.uint32))
+
+
+# ---------------------------------------------------------------- 16-bit V-cycle storage
+def _storage_samples():
+    rng = np.random.default_rng(5)
+    x = np.concatenate([
+        rng.standard_normal(20000).astype(np.float32) * np.float32(10.0) ** rng.integers(-9, 6, 20000),
+        np.array([0.0, -0.0, 65504.0, 65519.99, 65520.0, 1e6, -1e6, 6.1035156e-05, 6.0e-05, 3e-8, 2.9e-8,
+                  5.9604645e-08, 1.0 + 2.0 ** -11, 1.0 + 3 * 2.0 ** -11, 1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8,
+                  3.4e38, -3.4e38, np.inf, -np.inf], np.float32)])
+    return x.astype(np.float32)
+
+
+def test_binary16_rounding_is_numpy_float16():
+    """IEEE binary16, nearest-even, subnormals and overflow to inf: numpy's
+    float16 conversion is the independent check."""
+    x = _storage_samples()
+    with np.errstate(over="ignore"):
+        want = x.astype(np.float16).astype(np.float32)
+    got = po.round_storage(1, x)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_bfloat16_rounding_is_torch_bfloat16():
+    torch = pytest.importorskip("torch")
+    x = _storage_samples()
+    want = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    got = po.round_storage(2, x)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_f32_storage_is_the_identity():
+    x = _storage_samples()
+    assert np.array_equal(po.round_storage(0, x).view(np.uint32), x.view(np.uint32))
+
+
+@pytest.mark.parametrize("storage", [1, 2])
+def test_16bit_storage_vcycle_stays_near_the_f32_one_and_projects_to_tolerance(storage):
+    """Rounding the stored V-cycle values perturbs the preconditioner by
+    about the storage precision; the projection still reaches 1e-6 in the
+    same number of iterations and the same pressure within 1e-5."""
+    wl = W.tank(32, 24, 32, seed=9)
+    d = wl.domain
+    prob = po.Problem(d.nx, d.ny, d.nz, d.dx, wl.eta, rho=wl.rho, g_eff=wl.g_eff, rest=wl.rest,
+                      homogeneous_ghost=wl.homogeneous_ghost, closed_lid=wl.closed_lid, solid=wl.solid)
+    o = wl.options
+    base = dict(multigrid=1, smoother=o["smoother"], max_levels=o["max_levels"], pre=o["pre_sweeps"],
+                post=o["post_sweeps"], coarse=o["coarse_sweeps"], coarse_op=o.get("coarse_operator", 0))
+    r0, *_, p0 = po.project(prob, wl.us, wl.vs, wl.ws, wl.scale, wl.k1, 500, po.options(**base), precision=64)
+    r1, *_, p1 = po.project(prob, wl.us, wl.vs, wl.ws, wl.scale, wl.k1, 500, po.options(storage=storage, **base),
+                            precision=64)
+    assert r1["iterations"] == r0["iterations"] and r1["residual"] <= 1e-6
+    assert np.linalg.norm(p1 - p0) <= 1e-5 * np.linalg.norm(p0)
+    _, prec = po.assemble(prob, np.zeros(wl.domain.cells, np.float32), 1.0)
+    r = np.random.default_rng(3).standard_normal(wl.domain.cells).astype(np.float32)
+    r[prec == 0] = 0
+    z0 = po.vcycle(prob, r, po.options(**base))
+    z1 = po.vcycle(prob, r, po.options(storage=storage, **base))
+    rel = np.abs(z1 - z0).max() / np.abs(z0).max()
+    assert 0 < rel < (2e-2 if storage == 2 else 2e-3)
