@@ -10,7 +10,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libfsp.so")
+# FSP_LIBRARY selects another build of the same library (an alternative
+# compile of the same sources, e.g. for tuning); the default is the in-tree one.
+LIB_PATH = os.environ.get("FSP_LIBRARY") or os.path.join(HERE, "libfsp.so")
 
 FSP_OK, FSP_DOMAIN, FSP_NOT_FINITE, FSP_SHAPE, FSP_CONVERGENCE, FSP_DEVICE, FSP_INVALID = range(7)
 PRECOND_JACOBI, PRECOND_MULTIGRID, PRECOND_LEARNED = 0, 1, 2
@@ -40,7 +42,7 @@ class Options(C.Structure):
         ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32), ("coarse_sweeps", C.c_int32),
         ("gauge", C.c_int32), ("vcycle_storage", C.c_int32), ("replace_every", C.c_int32),
         ("refuse_degraded", C.c_int32), ("device", C.c_int32), ("coarse_operator", C.c_int32),
-        ("graph", C.c_int32), ("learned_budget", C.c_int32),
+        ("graph", C.c_int32), ("learned_budget", C.c_int32), ("learned_smooth", C.c_int32),
     ]
 
 

@@ -27,14 +27,17 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose: bool = True) -> str:
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose: bool = True, out: str = LIB, build_dir: str = BUILD, defines=()) -> str:
+    """Compiles libfsp.so (to `out`, objects in `build_dir`; `defines` are
+    extra -D macros for tuning builds)."""
+    os.makedirs(build_dir, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    fsp_o = os.path.join(BUILD, "fsp.o")
-    net_o = os.path.join(BUILD, "learned.o")
-    gen_o = os.path.join(BUILD, "generate.o")
+    fsp_o = os.path.join(build_dir, "fsp.o")
+    net_o = os.path.join(build_dir, "learned.o")
+    gen_o = os.path.join(build_dir, "generate.o")
+    dflags = [f"-D{d}" for d in defines]
     jobs = [
-        [hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, "fsp.hip"), "-o", fsp_o],
+        [hipcc, *HIPCC_FLAGS, *dflags, "-c", os.path.join(CSRC, "fsp.hip"), "-o", fsp_o],
         [hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, "learned.hip"), "-o", net_o],
         ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", "-c",
          os.path.join(CSRC, "generate.cpp"), "-o", gen_o],
@@ -46,10 +49,10 @@ def build(verbose: bool = True) -> str:
     failed = [cmd for cmd, p in zip(jobs, procs) if p.wait() != 0]
     if failed:
         raise subprocess.CalledProcessError(1, failed[0])
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-o", tmp, fsp_o, net_o, gen_o, "-pthread"])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -107,6 +107,15 @@ namespace {
     }                                                  \
   } while (0)
 
+// One thread per item up to the partial-sum capacity (for kernels whose
+// per-item work is a chain of dependent loads).
+unsigned grid_wide(size_t n) {
+  size_t b = (n + BLOCK - 1) / BLOCK;
+  if (b > (size_t)MAX_PARTIALS) b = MAX_PARTIALS;
+  if (b == 0) b = 1;
+  return (unsigned)b;
+}
+
 unsigned grid_for(size_t n) {
   size_t b = (n + BLOCK - 1) / BLOCK;
   if (b > (size_t)MAX_BLOCKS) b = MAX_BLOCKS;
@@ -449,15 +458,17 @@ struct Launch {
     TS<ST>* z = reinterpret_cast<TS<ST>*>(S->z);
     TS<ST>* zt = reinterpret_cast<TS<ST>*>(S->zt);
     const TS<ST>* none = nullptr;
+    auto pass = [&](TS<ST>* x, int colour, int op) {
+      FSP_LAUNCH_N(S, "k_q_gs<0>", K_QGS(0), g, st, G, Q, (const TS<ST>*)x, x, S->r, colour, L0, none, op, S->ctl,
+                   S->partials);
+      if (op >= 0) FSP_FINISH(S, st, op, g, false, 1);
+    };
     FSP_LAUNCH_N(S, "k_q_gs<1>", K_QGS(1), g, st, G, Q, (const TS<ST>*)z, z, S->r, 0, L0, none, -1, S->ctl,
                  S->partials);
-    FSP_LAUNCH_N(S, "k_q_gs<0>", K_QGS(0), g, st, G, Q, (const TS<ST>*)z, z, S->r, 1, L0, none, -1, S->ctl,
-                 S->partials);
+    pass(z, 1, -1);
     for (int s = 1; s < S->opt.pre_sweeps; ++s) {
-      FSP_LAUNCH_N(S, "k_q_gs<0>", K_QGS(0), g, st, G, Q, (const TS<ST>*)z, z, S->r, 0, L0, none, -1, S->ctl,
-                   S->partials);
-      FSP_LAUNCH_N(S, "k_q_gs<0>", K_QGS(0), g, st, G, Q, (const TS<ST>*)z, z, S->r, 1, L0, none, -1, S->ctl,
-                   S->partials);
+      pass(z, 0, -1);
+      pass(z, 1, -1);
     }
     TS<ST>* out = z;
     int passes = 2 * S->opt.post_sweeps;
@@ -467,18 +478,14 @@ struct Launch {
       const dim3 gr((unsigned)(R.nbx * R.nby * R.planes));
       FSP_LAUNCH_N(S, "k_q_restrict", K_QRESTRICT, gr, st, G, R, (const TS<ST>*)z, (const float*)S->r, L0, S->ctl);
       const TS<ST>* x0 = coarse_cycle<ST>();
-      FSP_LAUNCH_N(S, "k_q_gs<2>", K_QGS(2), g, st, G, Q, (const TS<ST>*)z, zt, S->r, 1, L0, x0,
-                   passes == 1 ? dot_op : -1, S->ctl, S->partials);
-      if (passes == 1 && dot_op >= 0) FSP_FINISH(S, st, dot_op, g, false, 1);
+      const int op = passes == 1 ? dot_op : -1;
+      FSP_LAUNCH_N(S, "k_q_gs<2>", K_QGS(2), g, st, G, Q, (const TS<ST>*)z, zt, S->r, 1, L0, x0, op, S->ctl,
+                   S->partials);
+      if (op >= 0) FSP_FINISH(S, st, op, g, false, 1);
       out = zt;
       done = 1;
     }
-    for (int pz = done; pz < passes; ++pz) {
-      const int colour = (pz % 2 == 0) ? 1 : 0;
-      FSP_LAUNCH_N(S, "k_q_gs<0>", K_QGS(0), g, st, G, Q, (const TS<ST>*)out, out, S->r, colour, L0, none,
-                   pz == passes - 1 ? dot_op : -1, S->ctl, S->partials);
-      if (pz == passes - 1 && dot_op >= 0) FSP_FINISH(S, st, dot_op, g, false, 1);
-    }
+    for (int pz = done; pz < passes; ++pz) pass(out, (pz % 2 == 0) ? 1 : 0, pz == passes - 1 ? dot_op : -1);
     return out;
   }
 
@@ -487,6 +494,18 @@ struct Launch {
       // z = network(r) inside flexible CG; sigma from |r|^2 on the device
       const double* rr = op == OP_RZ ? &S->ctl->rn : &S->ctl->rr;
       (void)net_apply(S->net, st, S->r, S->mask, S->solid, rr, &S->ctl->wet_cells, &S->ctl->active, S->z);
+      // red-black sweeps of A z = r from the network's output: the network
+      // carries the smooth part of the error, the sweeps the rough part
+      if (S->quad) {
+        const QuadGrid& Q = S->qfine;
+        const dim3 g((unsigned)(Q.nbx * Q.nby * Q.planes));
+        const Level none_l{};
+        const float* none = nullptr;
+        for (int s = 0; s < S->opt.learned_smooth; ++s)
+          for (int c = 0; c < 2; ++c)
+            FSP_LAUNCH_N(S, "k_q_gs<0>", (k_q_gs<0, ST_F32>), g, st, G, Q, (const float*)S->z, S->z, S->r, c,
+                         none_l, none, -1, S->ctl, S->partials);
+      }
       FSP_LAUNCH(S, k_dot_rz_qz, dim3(grid_for(G.n)), st, G.n, (const float*)S->r, (const float*)S->z,
                  (const float*)S->q, op, S->ctl, S->partials);
       FSP_FINISH(S, st, op, dim3(grid_for(G.n)), false, 1);
@@ -596,9 +615,9 @@ fsp_status correct_and_measure(fsp_solver* S, const float* us, const float* vs, 
     FSP_LAUNCH(S, k_sub_mean<double>, dim3(grid_for(K.G.n)), K.st, K.G, p, 1, -1, S->ctl,
                        S->partials);
   }
-  FSP_LAUNCH(S, k_correct, dim3(grid_for(S->nu + S->nv + S->nw)), K.st, K.G, us, vs, ws,
+  FSP_LAUNCH(S, k_correct, dim3(grid_wide(K.G.n)), K.st, K.G, us, vs, ws,
                      (const double*)p, k1, u, v, w, S->ctl, S->partials);
-  FSP_FINISH(S, K.st, OP_UMAX, dim3(grid_for(S->nu + S->nv + S->nw)), true, 0);
+  FSP_FINISH(S, K.st, OP_UMAX, dim3(grid_wide(K.G.n)), true, 0);
   FSP_LAUNCH(S, k_divmetric, dim3(grid_for(K.G.n)), K.st, K.G, u, v, w, S->ctl, S->partials);
   FSP_FINISH(S, K.st, OP_DIVMAX, dim3(grid_for(K.G.n)), true, 0);
   fsp_status st = sync_ctl(S);
@@ -902,6 +921,7 @@ void fsp_default_options(fsp_options* o) {
   o->coarse_operator = FSP_COARSE_REDISCRETISED;
   o->graph = 1;
   o->learned_budget = 0;
+  o->learned_smooth = 1;
 }
 
 fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** out) {
@@ -914,7 +934,7 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
   fsp_default_options(&opt);
   if (o) opt = *o;
   if (opt.preconditioner < 0 || opt.preconditioner > 2 || opt.smoother < 0 || opt.smoother > 1) return FSP_INVALID;
-  if (opt.learned_budget < 0) return FSP_INVALID;
+  if (opt.learned_budget < 0 || opt.learned_smooth < 0 || opt.learned_smooth > 64) return FSP_INVALID;
   if (opt.pre_sweeps < 0 || opt.post_sweeps < 0 || opt.coarse_sweeps < 0 || opt.pre_sweeps > 64 ||
       opt.post_sweeps > 64 || opt.coarse_sweeps > 1024)
     return FSP_INVALID;
@@ -978,7 +998,7 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
   auto make_q = [](int nq, int rows, int planes) {
     QuadGrid Q{};
     int qx = 1;
-    while (qx < nq && qx < 64) qx <<= 1;
+    while (qx < nq && qx < FSP_QX_MAX) qx <<= 1;
     Q.nq = nq; Q.QX = qx; Q.RY = QBLOCK / qx;
     Q.nbx = (nq + qx - 1) / qx;
     Q.nby = (rows + Q.RY - 1) / Q.RY;

@@ -25,13 +25,19 @@ __device__ __forceinline__ size_t fwidx(const Fine& G, int i, int j, int k) {
   return ((size_t)k * G.ny + j) * G.nx + i;
 }
 
-// Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
-// walks a contiguous range of chunks and stencil neighbours share its L2.
-__device__ __forceinline__ unsigned logical_block() {
-  const unsigned G = gridDim.x, b = blockIdx.x;
-  if ((G & 7u) == 0u) return (b & 7u) * (G >> 3) + (b >> 3);
+// Blocks are dispatched round-robin over the 8 XCDs. FSP_XCD_REMAP = 1
+// remaps them so that each XCD walks a contiguous range of tiles (stencil
+// neighbours in its own L2); measured on the 256x128x256 tank, the plain
+// round-robin order is faster (the red-black pass runs at 5.2 TB/s instead
+// of 3.1 TB/s), so the remap is off by default.
+#ifndef FSP_XCD_REMAP
+#define FSP_XCD_REMAP 0
+#endif
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G) {
+  if (FSP_XCD_REMAP && (G & 7u) == 0u) return (b & 7u) * (G >> 3) + (b >> 3);
   return b;
 }
+__device__ __forceinline__ unsigned logical_block() { return xcd_block(blockIdx.x, gridDim.x); }
 #define FSP_FOR(idx, count)                                                         \
   for (size_t idx = (size_t)logical_block() * BLOCK + threadIdx.x; idx < (count); \
        idx += (size_t)gridDim.x * BLOCK)
@@ -976,7 +982,8 @@ __device__ __forceinline__ float corrected_side(const Fine& G, float us, const d
 #pragma clang fp contract(off)
   const size_t l = cidx(G, x, y, k), r = cidx(G, i, j, k);
   if (G.solid[l] || G.solid[r]) return us;
-  const bool wl = wet_at(G, x, y, k), wr = wet_at(G, i, j, k);
+  // wet3 of both cells, from the row codes built this projection
+  const bool wl = (G.mask[l] & M_WET) != 0, wr = (G.mask[r] & M_WET) != 0;
   const double kk = (double)k1, dx = (double)G.dx;
   double ch;
   if (wl && wr) ch = kk * (p[r] - p[l]) / dx;
@@ -986,50 +993,55 @@ __device__ __forceinline__ float corrected_side(const Fine& G, float us, const d
   return (float)((double)us - ch);
 }
 
+// One thread per cell: the cell's x-, y- and z- faces, and the closing face
+// of its row / column / pillar at the +x, +y, +z boundary (32-bit index
+// arithmetic only).
 __global__ void __launch_bounds__(BLOCK) k_correct(Fine G, const float* us, const float* vs, const float* ws,
                                                    const double* p, float k1, float* u, float* v, float* w, Ctl* ctl,
                                                    double* partials) {
 #pragma clang fp contract(off)
-  const size_t nx = G.nx, ny = G.ny, nz = G.nz;
-  const size_t nu = (nx + 1) * ny * nz, nv = nx * (ny + 1) * nz, nw = nx * ny * (nz + 1);
+  const int nx = G.nx, ny = G.ny, nz = G.nz;
   double mx = 0.0, unused = 0.0;
-  FSP_FOR(f, nu + nv + nw) {
-    float val;
-    if (f < nu) {
-      const int i = (int)(f % (nx + 1));
-      const size_t t = f / (nx + 1);
-      const int j = (int)(t % ny), k = (int)(t / ny);
-      val = us[f];
-      if (i > 0 && i < (int)nx && G.ou[f]) val = corrected_side(G, val, p, k1, i - 1, j, i, j, k);
-      u[f] = val;
-    } else if (f < nu + nv) {
-      const size_t g = f - nu;
-      const int i = (int)(g % nx);
-      const size_t t = g / nx;
-      const int j = (int)(t % (ny + 1)), k = (int)(t / (ny + 1));
-      val = vs[g];
-      if (j > 0 && j < (int)ny && G.ov[g]) val = corrected_side(G, val, p, k1, i, j - 1, i, j, k);
-      v[g] = val;
-    } else {
-      const size_t g = f - nu - nv;
-      const int i = (int)(g % nx);
-      const size_t t = g / nx;
-      const int j = (int)(t % ny), k = (int)(t / ny);
-      val = ws[g];
-      if (k >= 1 && G.ow[g] && wet_at(G, i, j, k - 1)) {
-        const size_t below = cidx(G, i, j, k - 1);
-        const double kk = (double)k1, dx = (double)G.dx;
-        if (k < (int)nz && wet_at(G, i, j, k)) {
-          val = (float)((double)val - kk * (p[cidx(G, i, j, k)] - p[below]) / dx);
-        } else {
-          float a, b;
-          ghost_up(G, i, j, k - 1, a, b);
-          val = (float)((double)val - kk * ((double)b - p[below]) * (double)a / dx);
-        }
-      }
-      w[g] = val;
-    }
+  auto face_u = [&](int i, int j, int k) {
+    const size_t f = fuidx(G, i, j, k);
+    float val = us[f];
+    if (i > 0 && i < nx && G.ou[f]) val = corrected_side(G, val, p, k1, i - 1, j, i, j, k);
+    u[f] = val;
     mx = fmax(mx, (double)fabsf(val));
+  };
+  auto face_v = [&](int i, int j, int k) {
+    const size_t g = fvidx(G, i, j, k);
+    float val = vs[g];
+    if (j > 0 && j < ny && G.ov[g]) val = corrected_side(G, val, p, k1, i, j - 1, i, j, k);
+    v[g] = val;
+    mx = fmax(mx, (double)fabsf(val));
+  };
+  auto face_w = [&](int i, int j, int k) {
+    const size_t g = fwidx(G, i, j, k);
+    float val = ws[g];
+    if (k >= 1 && G.ow[g] && (G.mask[cidx(G, i, j, k - 1)] & M_WET)) {
+      const size_t below = cidx(G, i, j, k - 1);
+      const double kk = (double)k1, dx = (double)G.dx;
+      if (k < nz && (G.mask[cidx(G, i, j, k)] & M_WET)) {
+        val = (float)((double)val - kk * (p[cidx(G, i, j, k)] - p[below]) / dx);
+      } else {
+        float a, b;
+        ghost_up(G, i, j, k - 1, a, b);
+        val = (float)((double)val - kk * ((double)b - p[below]) * (double)a / dx);
+      }
+    }
+    w[g] = val;
+    mx = fmax(mx, (double)fabsf(val));
+  };
+  FSP_FOR(c, G.n) {
+    int i, j, k;
+    decode(G, c, i, j, k);
+    face_u(i, j, k);
+    if (i == nx - 1) face_u(nx, j, k);
+    face_v(i, j, k);
+    if (j == ny - 1) face_v(i, ny, k);
+    face_w(i, j, k);
+    if (k == nz - 1) face_w(i, j, nz);
   }
   reduce_finish<true>(mx, unused, OP_UMAX, ctl, partials);
 }
@@ -1072,12 +1084,18 @@ __global__ void __launch_bounds__(BLOCK) k_backward(Fine G, const double* p, con
     decode(G, c, i, j, k);
     const double pc = fabs(p[c]);
     double acc = 0.0;
-    RowG R;
-    full_row(G, i, j, k, R);
     const size_t nb[6] = {c - 1, c + 1, c - G.nx, c + G.nx, c - plane, c + plane};
-    for (int f = 0; f < 6; ++f) {
-      if (R.type[f] == 1) acc += pc + fabs(p[nb[f]]);
-      else if (R.type[f] == 2) acc += pc * (double)R.it[f];
+    if (!(m & M_GHOST)) {
+      for (int f = 0; f < 6; ++f)
+        if (m & (1u << f)) acc += pc + fabs(p[nb[f]]);
+    } else {
+      uint8_t type[6];
+      float it[6];
+      ghost_row(G, i, j, k, m, type, it);
+      for (int f = 0; f < 6; ++f) {
+        if (type[f] == 1) acc += pc + fabs(p[nb[f]]);
+        else if (type[f] == 2) acc += pc * (double)it[f];
+      }
     }
     const double scale = fabs((double)b[c]) + acc * (double)G.inv;
     if (scale > 0.0) worst = fmax(worst, fabs((double)r[c]) / scale);

@@ -277,7 +277,7 @@ __device__ __forceinline__ void gct_pass(float* lds, const GcTailLvl& L, int col
   using SV = Store<ST>;
   float* x = lds + L.o_x;
   const float* r = lds + L.o_r;
-  for (int c = threadIdx.x; c < L.n; c += TAIL_THREADS) {
+  for (int c = (int)threadIdx.x; c < L.n; c += TAIL_THREADS) {
     const int t = c / L.nx, i = c - t * L.nx, k = t / L.ny, j = t - k * L.ny;
     if (((i + j + k) & 1) != col) {
       if (first) x[c] = 0.f;
@@ -297,7 +297,7 @@ __device__ __forceinline__ void gct_sweeps(float* lds, const GcTailLvl& L, int c
                                            bool from_zero) {
   if (count == 0) {
     if (from_zero) {
-      for (int c = threadIdx.x; c < L.n; c += TAIL_THREADS) lds[L.o_x + c] = 0.f;
+      for (int c = (int)threadIdx.x; c < L.n; c += TAIL_THREADS) lds[L.o_x + c] = 0.f;
       __syncthreads();
     }
     return;
@@ -316,6 +316,38 @@ __device__ __forceinline__ void gct_sweeps(float* lds, const GcTailLvl& L, int c
     fz = false;
     gct_pass<ST>(lds, L, 1 - c0, false);
   }
+}
+
+// r of level C = restriction of the residual of level L.
+template <int ST>
+__device__ __forceinline__ void gct_restrict(float* lds, const GcTailLvl& L, const GcTailLvl& C) {
+#pragma clang fp contract(off)
+  using SV = Store<ST>;
+  for (int c = (int)threadIdx.x; c < C.n; c += TAIL_THREADS) {
+    const int t = c / C.nx, i = c - t * C.nx, k = t / C.ny, j = t - k * C.ny;
+    float s = 0.f;
+    for (int ch = 0; ch < 8; ++ch) {
+      const int fi = 2 * i + (ch & 1), fj = 2 * j + ((ch >> 1) & 1), fk = 2 * k + ((ch >> 2) & 1);
+      const int f = (fk * L.ny + fj) * L.nx + fi;
+      float a[6];
+      gct_faces(lds, L, fi, fj, fk, f, a);
+      s += lds[L.o_r + f] - gct_row(lds, L, fi, fj, fk, f, a);
+    }
+    lds[C.o_r + c] = SV::rnd(0.125f * s);
+  }
+  __syncthreads();
+}
+
+// x of level L += x of its parent in C.
+template <int ST>
+__device__ __forceinline__ void gct_prolong(float* lds, const GcTailLvl& L, const GcTailLvl& C) {
+#pragma clang fp contract(off)
+  using SV = Store<ST>;
+  for (int c = (int)threadIdx.x; c < L.n; c += TAIL_THREADS) {
+    const int t = c / L.nx, i = c - t * L.nx, k = t / L.ny, j = t - k * L.ny;
+    lds[L.o_x + c] = SV::rnd(lds[L.o_x + c] + lds[C.o_x + ((k >> 1) * C.ny + (j >> 1)) * C.nx + (i >> 1)]);
+  }
+  __syncthreads();
 }
 
 // coarse_cycle3 (delta3d_multigrid.rs:303-332) over the tail levels, red-black
@@ -347,7 +379,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_gc_tail(TailSet T, Ctl* ctl) {
   for (int l = 0; l < T.count; ++l) {
     const Level& g = T.lv[l];
     const GcTailLvl L = tl[l];
-    for (int c = threadIdx.x; c < L.n; c += TAIL_THREADS) {
+    for (int c = (int)threadIdx.x; c < L.n; c += TAIL_THREADS) {
       const int t = c / L.nx, i = c - t * L.nx, k = t / L.ny, j = t - k * L.ny;
       lds[L.o_fx + c] = i > 0 ? g.ou[((size_t)k * L.ny + j) * (L.nx + 1) + i] : 0.f;
       lds[L.o_fy + c] = j > 0 ? g.ov[((size_t)k * (L.ny + 1) + j) * L.nx + i] : 0.f;
@@ -359,34 +391,12 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_gc_tail(TailSet T, Ctl* ctl) {
   __syncthreads();
   const int last = T.count - 1;
   for (int l = 0; l <= last; ++l) {
-    const GcTailLvl L = tl[l];
-    gct_sweeps<ST>(lds, L, l == last ? T.coarse : T.pre, true, l == last, true);
-    if (l < last) {
-      const GcTailLvl C = tl[l + 1];
-      for (int c = threadIdx.x; c < C.n; c += TAIL_THREADS) {
-        const int t = c / C.nx, i = c - t * C.nx, k = t / C.ny, j = t - k * C.ny;
-        float s = 0.f;
-        for (int ch = 0; ch < 8; ++ch) {
-          const int fi = 2 * i + (ch & 1), fj = 2 * j + ((ch >> 1) & 1), fk = 2 * k + ((ch >> 2) & 1);
-          const int f = (fk * L.ny + fj) * L.nx + fi;
-          float a[6];
-          gct_faces(lds, L, fi, fj, fk, f, a);
-          s += lds[L.o_r + f] - gct_row(lds, L, fi, fj, fk, f, a);
-        }
-        lds[C.o_r + c] = SV::rnd(0.125f * s);
-      }
-      __syncthreads();
-    }
+    gct_sweeps<ST>(lds, tl[l], l == last ? T.coarse : T.pre, true, l == last, true);
+    if (l < last) gct_restrict<ST>(lds, tl[l], tl[l + 1]);
   }
   for (int l = last - 1; l >= 0; --l) {
-    const GcTailLvl L = tl[l];
-    const GcTailLvl C = tl[l + 1];
-    for (int c = threadIdx.x; c < L.n; c += TAIL_THREADS) {
-      const int t = c / L.nx, i = c - t * L.nx, k = t / L.ny, j = t - k * L.ny;
-      lds[L.o_x + c] = SV::rnd(lds[L.o_x + c] + lds[C.o_x + ((k >> 1) * C.ny + (j >> 1)) * C.nx + (i >> 1)]);
-    }
-    __syncthreads();
-    gct_sweeps<ST>(lds, L, T.post, false, false, false);
+    gct_prolong<ST>(lds, tl[l], tl[l + 1]);
+    gct_sweeps<ST>(lds, tl[l], T.post, false, false, false);
   }
   const GcTailLvl L0 = tl[0];
   for (int q = threadIdx.x; q < L0.n; q += TAIL_THREADS)

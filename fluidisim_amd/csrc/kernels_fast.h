@@ -5,8 +5,7 @@
 // red-black pass and every access is a 16-byte vector access; x neighbours
 // across quads come from the adjacent lane by wavefront shuffle, y and z
 // neighbours are vector loads served mostly by L2. The 1-D grid is mapped to
-// (x-segment, y-block, z) tiles so that each XCD walks a contiguous range of
-// planes and its stencil neighbours stay in its own L2. Per-cell arithmetic
+// (x-segment, y-block, z) tiles in dispatch order (see xcd_block). Per-cell arithmetic
 // is the fine operator's (face order x-, x+, y-, y+, z-, z+, no contraction),
 // so results equal the generic kernels bit for bit.
 //
@@ -19,6 +18,15 @@
 namespace fsp {
 
 constexpr int QBLOCK = 256;
+// Minimum workgroups per CU the fine red-black kernel is compiled for (the
+// register budget follows from it).
+#ifndef FSP_QGS_MIN_BLOCKS
+#define FSP_QGS_MIN_BLOCKS 1
+#endif
+// Widest x extent of a workgroup tile, in quads (tile shapes for tuning).
+#ifndef FSP_QX_MAX
+#define FSP_QX_MAX 64
+#endif
 
 struct QuadGrid {
   int nq;   // quads per row (nx / 4)
@@ -34,7 +42,7 @@ struct QuadGrid {
 // (they still take part in shuffles).
 __device__ __forceinline__ bool quad_coords(const QuadGrid& Q, int& qx, int& y, int& z) {
   const unsigned G = gridDim.x, b = blockIdx.x;
-  const unsigned L = ((G & 7u) == 0u) ? (b & 7u) * (G >> 3) + (b >> 3) : b;
+  const unsigned L = xcd_block(b, G);
   const unsigned bx = L % (unsigned)Q.nbx;
   const unsigned t = L / (unsigned)Q.nbx;
   const unsigned by = t % (unsigned)Q.nby;
@@ -52,6 +60,23 @@ __device__ __forceinline__ float4 ld4(const float* p, size_t c) { return *reinte
 __device__ __forceinline__ void st4(float* p, size_t c, const float4& v) { *reinterpret_cast<float4*>(p + c) = v; }
 __device__ __forceinline__ float4 f4zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
 
+// The sum of a free-surface row (about one row in a hundred).
+__device__ __forceinline__ float ghost_row_sum(const Fine& G, int i, int j, int k, uint8_t m, float pc, const float nb[6],
+                                            float& prec) {
+#pragma clang fp contract(off)
+  uint8_t type[6];
+  float it[6];
+  ghost_row(G, i, j, k, m, type, it);
+  float acc = 0.f, diag = 0.f;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    if (type[f] == 1) { acc += pc - nb[f]; diag += 1.0f; }
+    else if (type[f] == 2) { acc += pc * it[f]; diag += it[f]; }
+  }
+  prec = diag > 0.f ? 1.0f / (diag * G.inv) : 0.f;
+  return acc;
+}
+
 // Row of the fine operator from neighbour values already in registers, in the
 // reference's face order; prec as rhs_mobile3 computes it.
 __device__ __forceinline__ float row_local(const Fine& G, int i, int j, int k, uint8_t m, float pc, const float nb[6],
@@ -66,16 +91,7 @@ __device__ __forceinline__ float row_local(const Fine& G, int i, int j, int k, u
       if (m & (1u << f)) acc += pc - nb[f];
     prec = G.prec_tab[__popc(m & 0x3Fu)];
   } else {
-    uint8_t type[6];
-    float it[6];
-    ghost_row(G, i, j, k, m, type, it);
-    float diag = 0.f;
-#pragma unroll
-    for (int f = 0; f < 6; ++f) {
-      if (type[f] == 1) { acc += pc - nb[f]; diag += 1.0f; }
-      else if (type[f] == 2) { acc += pc * it[f]; diag += it[f]; }
-    }
-    prec = diag > 0.f ? 1.0f / (diag * G.inv) : 0.f;
+    acc = ghost_row_sum(G, i, j, k, m, pc, nb, prec);
   }
   return acc * G.inv;
 }
@@ -107,7 +123,7 @@ __device__ __forceinline__ void x_neighbours(const QuadGrid& Q, bool valid, int 
 // z, zout and the parents xc are held in storage ST (r is the f32 CG
 // residual); every value entering the arithmetic is a stored value.
 template <int MODE, int ST>
-__global__ void __launch_bounds__(QBLOCK) k_q_gs(Fine G, QuadGrid Q, const typename Store<ST>::T* z,
+__global__ void __launch_bounds__(QBLOCK, FSP_QGS_MIN_BLOCKS) k_q_gs(Fine G, QuadGrid Q, const typename Store<ST>::T* z,
                                                  typename Store<ST>::T* zout, const float* r, int colour, Level L0,
                                                  const typename Store<ST>::T* xc, int dot_op, Ctl* ctl,
                                                  double* partials) {

@@ -3,11 +3,13 @@ network, its training pairs made by the plain MGPCG, training, and use
 inside the projection (FSP_PRECOND_LEARNED, flexible CG with MGPCG as the
 fallback).
 
-Training pair of a workload: input = features of the first residual r = b
-(cold start), x = (r*sigma, wet, ghost row, solid) with sigma = 1/rms(r);
-target = sigma * p / kappa on wet cells, p the MGPCG pressure, kappa =
-min(nx, ny, nz)^2 / 16 (an output scale near 1; flexible CG does not see a
-constant factor, so the projection uses z = y / sigma).
+Training pairs of a workload: for residuals r_k = b - A p_k of the plain
+MGPCG after k iterations (cold start, k = 0, 1, 2, 4), input x = (r_k*sigma,
+wet, ghost row, solid) with sigma = 1/rms(r_k), target sigma * (p - p_k) /
+kappa on wet cells, p the converged MGPCG pressure, kappa = min(nx, ny,
+nz)^2 / 16 (an output scale near 1; flexible CG does not see a constant
+factor, so the projection uses z = y / sigma). Inside the projection the
+network's output is followed by learned_smooth red-black sweeps.
 """
 from __future__ import annotations
 
@@ -128,19 +130,33 @@ def init_params(seed: int = 0):
     return np.concatenate(parts).astype(np.float32)
 
 
-def training_pair(net: Network, wl: W.Workload, max_iters: int = 2000):
-    """(x, target, wet, report) of one workload, solved by the plain MGPCG."""
+def training_pairs(net: Network, wl: W.Workload, iters=(0, 1, 2, 4), max_iters: int = 2000):
+    """Pairs (x, target, wet) of one workload: the residuals r_k = b - A p_k
+    of the plain MGPCG after k iterations (k in `iters`, p_0 = 0) and their
+    errors e_k = p - p_k, so the network learns A^-1 on the residuals flexible
+    CG meets, not on b alone. Returns (pairs, report of the full solve)."""
     torch = _torch()
     vol = W.build_volume(wl)
     rep = vol.project_mobile3(wl.scale, wl.k1, max_iters, warm=False)
+    p_star = vol.pressure().clone()
     div = vol.divergence(vol.us, vol.vs, vol.ws)
     rhs, prec = vol.assemble_pressure_problem_for_trials(div, wl.scale)
-    x, sigma = net.features(vol, rhs)
     d = wl.domain
-    wet = (x.view(-1, 4)[:, 1] != 0).to(torch.uint8)
-    target = (vol.pressure() * (sigma / kappa(d.nx, d.ny, d.nz))) * wet.to(torch.float32)
+    kap = kappa(d.nx, d.ny, d.nz)
+    pairs = []
+    for k in iters:
+        if k == 0:
+            p_k = torch.zeros_like(p_star)
+        else:
+            vol.project_mobile3(wl.scale, wl.k1, k, warm=False)
+            p_k = vol.pressure().clone()
+        r_k = rhs - vol.apply_pressure_operator_for_trials(p_k)
+        x, sigma = net.features(vol, r_k)
+        wet = (x.view(-1, 4)[:, 1] != 0).to(torch.uint8)
+        target = ((p_star - p_k) * (sigma / kap)) * wet.to(torch.float32)
+        pairs.append((x, target.contiguous(), wet.contiguous()))
     vol.close()
-    return x, target.contiguous(), wet.contiguous(), rep
+    return pairs, rep
 
 
 def tank_shape(X, Y, Z):
@@ -157,8 +173,8 @@ def train(net: Network, size=(128, 128, 128), samples=64, epochs=4, lr=1e-3, see
     pairs = []
     for s in range(samples):
         wl = W.tank(X, Y, Z, seed=seed0 + s)
-        x, t, w, rep = training_pair(net, wl)
-        pairs.append((x, t, w))
+        got, rep = training_pairs(net, wl)
+        pairs.extend(got)
         if s % 16 == 0:
             log(f"[learned] pair {s}: {rep.iterations} MGPCG iterations ({time.time() - t0:.1f} s)")
     history = []

@@ -71,6 +71,7 @@ typedef struct {
   int32_t coarse_operator; /* FSP_COARSE_* */
   int32_t graph;           /* 1: replay each CG iteration from a captured HIP graph */
   int32_t learned_budget;  /* FSP_PRECOND_LEARNED: FCG iterations before the MGPCG fallback (0: 50) */
+  int32_t learned_smooth;  /* FSP_PRECOND_LEARNED: red-black sweeps on the network's output (default 1) */
 } fsp_options;
 
 /* Surface state (Volume3 fields rho, g_eff, rest, homogeneous_ghost;

@@ -108,7 +108,9 @@ def test_mfma_operand_layout():
     A = rng.standard_normal((16, 16)).astype(np.float16)
     B = rng.standard_normal((16, 16)).astype(np.float16)
     D = t.empty(256, dtype=t.float32, device="cuda")
-    st = _lib.lib().fsp_net_mfma_probe(f16_tensor(A).data_ptr(), f16_tensor(B).data_ptr(), D.data_ptr())
+    a, b = f16_tensor(A), f16_tensor(B)  # kept alive across the call
+    t.cuda.synchronize()
+    st = _lib.lib().fsp_net_mfma_probe(a.data_ptr(), b.data_ptr(), D.data_ptr())
     assert st == 0
     want = A.astype(np.float64) @ B.astype(np.float64)
     got = D.cpu().numpy().reshape(16, 16)
