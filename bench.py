@@ -51,11 +51,14 @@ def parse():
     ap.add_argument("--kernel", default="k_q_gs<0>", help="kernel timed live for the roofline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", default="2,8", help="oracle sample: two iteration budgets")
+    ap.add_argument("--coarse-sweeps", type=int, default=2,
+                    help="red-black sweeps on the coarsest level (2 keeps the 9 iterations of 8 on this tank)")
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
                     help="V-cycle vector storage (arithmetic stays f32)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "fluidisim_amd", "learned_weights.npz"),
                     help="trained network for the learned-FCG line (skipped when the file is absent)")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--batch", type=int, default=48, help="patches of the batched line (0: skip)")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the 256 MiB device copy (counter calibration) and exit")
     return ap.parse_args()
@@ -128,6 +131,36 @@ def cpu_baseline(wl, gpu_iterations, iters_pair):
     }
 
 
+def batch_line(args, torch, W):
+    """Independent patches (edges 32..128 per axis), each autotuned, projected
+    one after another and four at a time on their own streams."""
+    from fluidisim_amd.volume import project_batch
+    wls = W.patches(args.batch, seed=args.seed)
+    vols = [W.build_volume(wl) for wl in wls]
+    tiles = [vol.autotune()[0] for vol in vols]
+    scales, k1s = [wl.scale for wl in wls], [wl.k1 for wl in wls]
+    out = {"patches": len(wls), "cells": int(sum(wl.domain.cells for wl in wls)),
+           "tiles": {str(t): tiles.count(t) for t in sorted(set(tiles))}}
+    for conc in (1, 4):
+        project_batch(vols, scales, k1s, 2000, concurrency=conc)  # warm-up (graphs, caches)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(max(1, args.steps)):
+            reps = project_batch(vols, scales, k1s, 2000, concurrency=conc)
+        torch.cuda.synchronize()
+        ms = 1000.0 * (time.perf_counter() - t0) / max(1, args.steps)
+        key = "ms_per_batch" if conc == 4 else "ms_sequential"
+        out[key] = round(ms, 3)
+        out["iterations_total"] = int(sum(r.iterations for r in reps))
+        out["degraded"] = int(sum(r.degraded for r in reps))
+        log(f"[bench] batch of {len(wls)} patches, concurrency {conc}: {ms:.3f} ms")
+    out["concurrency"] = 4
+    out["mcells_per_s"] = round(out["cells"] / (out["ms_per_batch"] * 1e-3) / 1e6, 1)
+    for vol in vols:
+        vol.close()
+    return out
+
+
 def learned_fcg(wl, args, torch, _lib, W):
     """The same projection with the trained network inside flexible CG
     (MGPCG fallback), timed like the plain one."""
@@ -173,6 +206,7 @@ def main():
     X, Y, Z = (int(v) for v in args.size.lower().split("x"))
     t0 = time.perf_counter()
     wl = W.tank(X, Y, Z, seed=args.seed)
+    wl.options["coarse_sweeps"] = args.coarse_sweeps
     log(f"[bench] workload {wl.name}: {wl.domain.cells} cells, generated in {time.perf_counter() - t0:.1f} s")
     storage = {"f32": _lib.STORE_F32, "f16": _lib.STORE_F16, "bf16": _lib.STORE_BF16}[args.storage]
     vol = W.build_volume(wl, vcycle_storage=storage)
@@ -232,6 +266,8 @@ def main():
                     "avg_us": round(avg_ms * 1000.0, 2), "bytes_per_launch": int(bytes_launch)}
         log(f"[bench] {args.kernel}: {eff.size} launches, {avg_ms * 1000:.1f} us avg, {achieved:.0f} GB/s")
 
+    batched = batch_line(args, torch, W) if args.batch > 0 else None
+
     learned = None
     if not args.no_learned and os.path.exists(args.weights):
         learned = learned_fcg(wl, args, torch, _lib, W)
@@ -247,7 +283,7 @@ def main():
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": wl.name, "domain": [wl.domain.nx, wl.domain.ny, wl.domain.nz],
-                   "cells": wl.domain.cells, "solver": "plain MGPCG (RBGS V-cycle 2+2, Galerkin coarse levels, 8 coarsest sweeps)",
+                   "cells": wl.domain.cells, "solver": f"plain MGPCG (RBGS V-cycle 2+2, Galerkin coarse levels, {args.coarse_sweeps} coarsest sweeps)",
                    "precision": "f32 CG vectors, f64 pressure/residual/reductions",
                    "vcycle_storage": args.storage, "rel_tol": 1e-6,
                    "iterations": last.iterations, "residual": last.residual,
@@ -256,6 +292,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "learned_fcg": learned,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
     vol.close()

@@ -25,6 +25,7 @@ EXPORTS = [
     "fsp_abi_version", "fsp_status_string", "fsp_default_options", "fsp_create", "fsp_destroy", "fsp_levels",
     "fsp_set_geometry", "fsp_project", "fsp_project_device", "fsp_apply_operator_device", "fsp_assemble_device",
     "fsp_precondition_device", "fsp_divergence_device", "fsp_profile_begin", "fsp_profile_end", "fsp_generate",
+    "fsp_project_batch", "fsp_autotune",
     # include/fsp_learned.h
     "fsp_net_param_count", "fsp_net_create", "fsp_net_destroy", "fsp_net_set_params", "fsp_net_get_params",
     "fsp_net_reserve", "fsp_net_features_device", "fsp_net_forward_device", "fsp_net_loss_grad_device",
@@ -44,6 +45,12 @@ class Options(C.Structure):
         ("refuse_degraded", C.c_int32), ("device", C.c_int32), ("coarse_operator", C.c_int32),
         ("graph", C.c_int32), ("learned_budget", C.c_int32), ("learned_smooth", C.c_int32),
     ]
+
+
+class PatchIO(C.Structure):
+    _fields_ = [("us", C.c_void_p), ("vs", C.c_void_p), ("ws", C.c_void_p), ("p", C.c_void_p),
+                ("p_warm", C.c_int32), ("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p),
+                ("scale", C.c_float), ("k1", C.c_float)]
 
 
 class Surface(C.Structure):
@@ -110,6 +117,10 @@ def lib():
     L.fsp_profile_end.restype = i32
     L.fsp_generate.argtypes = [i32, i32, i32, i32, C.c_uint64, vp, vp, vp, vp, vp]
     L.fsp_generate.restype = i32
+    L.fsp_project_batch.argtypes = [vp, vp, i32, f64, u32, i32, vp]
+    L.fsp_project_batch.restype = i32
+    L.fsp_autotune.argtypes = [vp, vp, vp]
+    L.fsp_autotune.restype = i32
     L.fsp_net_param_count.restype = u32
     L.fsp_net_create.argtypes = [i32, vp]
     L.fsp_net_create.restype = i32

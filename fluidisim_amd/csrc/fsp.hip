@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -19,6 +20,7 @@
 #include <limits>
 #include <new>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -1265,6 +1267,106 @@ fsp_status fsp_divergence_device(fsp_solver* S, const float* d_u, const float* d
   Launch K(S);
   FSP_LAUNCH(S, k_divergence, dim3(grid_for(K.G.n)), K.st, K.G, d_u, d_v, d_w, d_out);
   FSP_HIP(hipStreamSynchronize(S->stream));
+  return FSP_OK;
+}
+
+fsp_status fsp_project_batch(fsp_solver* const* solvers, const fsp_patch_io* io, int32_t n, double rel_tol,
+                             uint32_t max_iters, int32_t concurrency, fsp_report* reports) {
+  if (!solvers || !io || !reports || n < 0) return FSP_INVALID;
+  if (!(rel_tol > 0.0)) return FSP_NOT_FINITE;
+  for (int32_t i = 0; i < n; ++i) {
+    const fsp_patch_io& q = io[i];
+    if (!solvers[i] || !q.us || !q.vs || !q.ws || !q.p || !q.u || !q.v || !q.w) return FSP_INVALID;
+    if (!std::isfinite(q.scale) || !std::isfinite(q.k1)) return FSP_NOT_FINITE;
+    for (int32_t j = 0; j < i; ++j)
+      if (solvers[j] == solvers[i]) return FSP_INVALID;  // one patch per solver: they run concurrently
+  }
+  const int workers = std::max(1, std::min<int>(n, concurrency > 0 ? concurrency : 4));
+  std::atomic<int32_t> next{0};
+  std::vector<fsp_status> status(n, FSP_OK);
+  auto work = [&]() {
+    for (;;) {
+      const int32_t i = next.fetch_add(1);
+      if (i >= n) return;
+      fsp_solver* S = solvers[i];
+      const fsp_patch_io& q = io[i];
+      fsp_report r{};
+      fsp_status st = hipSetDevice(S->opt.device) == hipSuccess ? FSP_OK : FSP_DEVICE;
+      if (st == FSP_OK)
+        st = run(S, q.us, q.vs, q.ws, q.p, q.p_warm, q.u, q.v, q.w, q.scale, q.k1, rel_tol, max_iters, &r);
+      reports[i] = r;
+      status[i] = st;
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < workers; ++t) pool.emplace_back(work);
+  work();
+  for (std::thread& t : pool) t.join();
+  for (fsp_status st : status)
+    if (st != FSP_OK) return st;
+  return FSP_OK;
+}
+
+fsp_status fsp_autotune(fsp_solver* S, int32_t* tile_x, double* us) {
+  if (!S) return FSP_INVALID;
+  if (!S->geom) return FSP_DOMAIN;
+  if (!S->quad) {
+    if (tile_x) *tile_x = 0;
+    if (us) *us = 0.0;
+    return FSP_OK;
+  }
+  FSP_HIP(hipSetDevice(S->opt.device));
+  build_geometry(S, false);
+  Launch K(S);
+  const int nx = S->dom.nx, ny = S->dom.ny, nz = S->dom.nz, nq = nx / 4;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(S->n)), dim3(BLOCK), 0, K.st, S->z, S->n, 0.f);
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(S->n)), dim3(BLOCK), 0, K.st, S->r, S->n, 1.f);
+  K.set_pass(1, 0.0, 0.0, 0.0, 0, 1);
+  auto make_q = [](int nqv, int rows, int planes, int qx) {
+    QuadGrid Q{};
+    Q.nq = nqv; Q.QX = qx; Q.RY = QBLOCK / qx;
+    Q.nbx = (nqv + qx - 1) / qx;
+    Q.nby = (rows + Q.RY - 1) / Q.RY;
+    Q.rows = rows; Q.planes = planes;
+    return Q;
+  };
+  int widest = 1;
+  while (widest < nq && widest < 64) widest <<= 1;
+  int best = S->qfine.QX;
+  double best_us = 1e30;
+  const Level none_l{};
+  const float* none = nullptr;
+  for (int qx = widest; qx >= 4; qx >>= 1) {
+    const QuadGrid Q = make_q(nq, ny, nz, qx);
+    const size_t blocks = (size_t)Q.nbx * Q.nby * Q.planes;
+    if (blocks > (size_t)MAX_PARTIALS) continue;
+    const dim3 g((unsigned)blocks);
+    for (int w = 0; w < 3; ++w)
+      k_q_gs<0, ST_F32><<<g, dim3(BLOCK), 0, K.st>>>(K.G, Q, S->z, S->z, S->r, w & 1, none_l, none, -1, S->ctl,
+                                                      S->partials);
+    FSP_HIP(hipEventRecord(S->ev0, K.st));
+    const int reps = 10;
+    for (int w = 0; w < reps; ++w)
+      k_q_gs<0, ST_F32><<<g, dim3(BLOCK), 0, K.st>>>(K.G, Q, S->z, S->z, S->r, w & 1, none_l, none, -1, S->ctl,
+                                                      S->partials);
+    FSP_HIP(hipEventRecord(S->ev1, K.st));
+    FSP_HIP(hipEventSynchronize(S->ev1));
+    float ms = 0.f;
+    FSP_HIP(hipEventElapsedTime(&ms, S->ev0, S->ev1));
+    const double t = 1000.0 * ms / reps;
+    if (t < best_us) { best_us = t; best = qx; }
+  }
+  S->qfine = make_q(nq, ny, nz, best);
+  S->qrestr = make_q(nq, ny / 2, nz / 2, best);
+  // graphs hold the previous launch shapes
+  for (int gi = 0; gi < 2; ++gi) {
+    if (S->iter_exec[gi]) (void)hipGraphExecDestroy(S->iter_exec[gi]);
+    if (S->iter_graph[gi]) (void)hipGraphDestroy(S->iter_graph[gi]);
+    S->iter_exec[gi] = nullptr;
+    S->iter_graph[gi] = nullptr;
+  }
+  if (tile_x) *tile_x = best;
+  if (us) *us = best_us;
   return FSP_OK;
 }
 

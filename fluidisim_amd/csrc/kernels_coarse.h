@@ -246,8 +246,19 @@ namespace fsp {
 struct GcTailLvl {
   int nx, ny, nz, n;
   float inv;
+  float rnx, rny;  // 1/nx, 1/ny for the cell decode
   int o_x, o_r, o_fx, o_fy, o_fz, o_dd;
 };
+
+// Cell c -> (i, j, k) without integer division: a/b = floor((a + 1/2)/b),
+// exact in f32 for a < 2^22 (the error of the product stays below the 1/(2b)
+// margin); tail levels have at most 4096 cells.
+__device__ __forceinline__ void gct_decode(const GcTailLvl& L, int c, int& i, int& j, int& k) {
+  const int t = (int)(((float)c + 0.5f) * L.rnx);
+  i = c - t * L.nx;
+  k = (int)(((float)t + 0.5f) * L.rny);
+  j = t - k * L.ny;
+}
 
 __device__ __forceinline__ void gct_faces(const float* lds, const GcTailLvl& L, int i, int j, int k, int c,
                                           float a[6]) {
@@ -278,7 +289,8 @@ __device__ __forceinline__ void gct_pass(float* lds, const GcTailLvl& L, int col
   float* x = lds + L.o_x;
   const float* r = lds + L.o_r;
   for (int c = (int)threadIdx.x; c < L.n; c += TAIL_THREADS) {
-    const int t = c / L.nx, i = c - t * L.nx, k = t / L.ny, j = t - k * L.ny;
+    int i, j, k;
+    gct_decode(L, c, i, j, k);
     if (((i + j + k) & 1) != col) {
       if (first) x[c] = 0.f;
       continue;
@@ -324,7 +336,8 @@ __device__ __forceinline__ void gct_restrict(float* lds, const GcTailLvl& L, con
 #pragma clang fp contract(off)
   using SV = Store<ST>;
   for (int c = (int)threadIdx.x; c < C.n; c += TAIL_THREADS) {
-    const int t = c / C.nx, i = c - t * C.nx, k = t / C.ny, j = t - k * C.ny;
+    int i, j, k;
+    gct_decode(C, c, i, j, k);
     float s = 0.f;
     for (int ch = 0; ch < 8; ++ch) {
       const int fi = 2 * i + (ch & 1), fj = 2 * j + ((ch >> 1) & 1), fk = 2 * k + ((ch >> 2) & 1);
@@ -344,7 +357,8 @@ __device__ __forceinline__ void gct_prolong(float* lds, const GcTailLvl& L, cons
 #pragma clang fp contract(off)
   using SV = Store<ST>;
   for (int c = (int)threadIdx.x; c < L.n; c += TAIL_THREADS) {
-    const int t = c / L.nx, i = c - t * L.nx, k = t / L.ny, j = t - k * L.ny;
+    int i, j, k;
+    gct_decode(L, c, i, j, k);
     lds[L.o_x + c] = SV::rnd(lds[L.o_x + c] + lds[C.o_x + ((k >> 1) * C.ny + (j >> 1)) * C.nx + (i >> 1)]);
   }
   __syncthreads();
@@ -366,6 +380,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_gc_tail(TailSet T, Ctl* ctl) {
       const Level& g = T.lv[l];
       GcTailLvl& L = tl[l];
       L.nx = g.nx; L.ny = g.ny; L.nz = g.nz; L.n = (int)g.n; L.inv = g.inv;
+      L.rnx = 1.0f / (float)g.nx; L.rny = 1.0f / (float)g.ny;
       L.o_x = off; off += L.n;
       L.o_r = off; off += L.n;
       L.o_fx = off; off += L.n;
@@ -380,7 +395,8 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_gc_tail(TailSet T, Ctl* ctl) {
     const Level& g = T.lv[l];
     const GcTailLvl L = tl[l];
     for (int c = (int)threadIdx.x; c < L.n; c += TAIL_THREADS) {
-      const int t = c / L.nx, i = c - t * L.nx, k = t / L.ny, j = t - k * L.ny;
+      int i, j, k;
+    gct_decode(L, c, i, j, k);
       lds[L.o_fx + c] = i > 0 ? g.ou[((size_t)k * L.ny + j) * (L.nx + 1) + i] : 0.f;
       lds[L.o_fy + c] = j > 0 ? g.ov[((size_t)k * (L.ny + 1) + j) * L.nx + i] : 0.f;
       lds[L.o_fz + c] = k > 0 ? g.ow[((size_t)k * L.ny + j) * L.nx + i] : 0.f;

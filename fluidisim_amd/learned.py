@@ -164,9 +164,11 @@ def tank_shape(X, Y, Z):
     return X, Z, Y
 
 
-def train(net: Network, size=(128, 128, 128), samples=64, epochs=4, lr=1e-3, seed0=1000, log=print):
+def train(net: Network, size=(128, 128, 128), samples=64, epochs=4, lr=1e-3, seed0=1000, log=print,
+          on_epoch=None):
     """Trains on `samples` tank workloads of the given size (distinct seeds),
-    `epochs` passes; returns the per-epoch mean losses."""
+    `epochs` passes; returns the per-epoch mean losses. on_epoch(e, history)
+    runs after each epoch (e.g. to save the parameters)."""
     X, Y, Z = size
     shape = tank_shape(X, Y, Z)
     t0 = time.time()
@@ -189,6 +191,8 @@ def train(net: Network, size=(128, 128, 128), samples=64, epochs=4, lr=1e-3, see
                 losses.append(loss)
         history.append(float(np.mean(losses)) if losses else float("nan"))
         log(f"[learned] epoch {e}: loss {history[-1]:.4e} ({skipped} skipped steps, {time.time() - t0:.1f} s)")
+        if on_epoch is not None:
+            on_epoch(e, history)
     return history
 
 
@@ -203,11 +207,16 @@ def main():
     ap.add_argument("--out", default=WEIGHTS)
     a = ap.parse_args()
     size = tuple(int(v) for v in a.size.lower().split("x"))
+    torch = _torch()
+    torch.zeros(1, device="cuda")  # the device's runtime state, before the library's first call
     net = Network()
     net.set_params(init_params(a.seed))
-    hist = train(net, size, a.samples, a.epochs, a.lr)
-    net.save(a.out, size=np.array(size), samples=a.samples, epochs=a.epochs, lr=a.lr, history=np.array(hist))
-    print(f"[learned] saved {a.out}", flush=True)
+    def save(e, hist):
+        net.save(a.out, size=np.array(size), samples=a.samples, pairs=4 * a.samples, epochs=e + 1, lr=a.lr,
+                 history=np.array(hist))
+        print(f"[learned] saved {a.out} after epoch {e}", flush=True)
+
+    train(net, size, a.samples, a.epochs, a.lr, log=lambda m: print(m, flush=True), on_epoch=save)
 
 
 if __name__ == "__main__":

@@ -269,6 +269,15 @@ class Volume3:
         _raise(_lib.lib().fsp_precondition_device(self._h, _ptr(r), _ptr(z)), "precondition")
         return z
 
+    def autotune(self):
+        """Picks the fastest fine-level workgroup tile for this grid
+        (fsp_autotune); returns (tile width in quads, microseconds per pass)."""
+        self._upload_geometry()
+        self._sync()
+        tx, us = C.c_int32(), C.c_double()
+        _raise(_lib.lib().fsp_autotune(self._h, C.byref(tx), C.byref(us)), "autotune")
+        return tx.value, us.value
+
     def divergence(self, u, v, w):
         """delta3d.rs:747-763, :806-823."""
         import torch
@@ -278,6 +287,26 @@ class Volume3:
         _raise(_lib.lib().fsp_divergence_device(self._h, _ptr(u.contiguous()), _ptr(v.contiguous()),
                                                 _ptr(w.contiguous()), _ptr(out)), "divergence")
         return out
+
+
+def project_batch(volumes, scales, k1s, max_iters: int, rel_tol: float = 1e-6, concurrency: int = 0,
+                  warm: bool = False):
+    """project_mobile3 of independent patches, `concurrency` at a time on their
+    own streams (fsp_project_batch); returns the reports."""
+    n = len(volumes)
+    io = (_lib.PatchIO * max(n, 1))()
+    handles = (C.c_void_p * max(n, 1))()
+    for i, vol in enumerate(volumes):
+        vol._upload_geometry()
+        handles[i] = vol._h
+        io[i] = _lib.PatchIO(_ptr(vol.us), _ptr(vol.vs), _ptr(vol.ws), _ptr(vol.p), int(warm), _ptr(vol.u),
+                             _ptr(vol.v), _ptr(vol.w), float(scales[i]), float(k1s[i]))
+    if n:
+        volumes[0]._sync()
+    reps = (_lib.Report * max(n, 1))()
+    st = _lib.lib().fsp_project_batch(handles, io, n, rel_tol, max_iters, concurrency, reps)
+    _raise(st, "project_batch")
+    return [Report.from_c(reps[i]) for i in range(n)]
 
 
 def generate(cfg: int, nx: int, ny: int, nz: int, seed: int):

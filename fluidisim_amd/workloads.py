@@ -77,3 +77,19 @@ def build_volume(wl: Workload, **option_overrides):
     vol.set_free_surface(wl.eta, wl.rest)
     vol.set_velocity(wl.us, wl.vs, wl.ws)
     return vol
+
+
+PATCH_EDGES = (32, 48, 64, 96, 128)
+PATCH_WEIGHTS = (0.3, 0.25, 0.2, 0.15, 0.1)
+
+
+def patches(count: int = 48, seed: int = 1):
+    """Independent delta patches: edge lengths drawn per axis from 32..128
+    with weights 0.3/0.25/0.2/0.15/0.1, each the free-surface tank generator
+    at its own size and seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(count):
+        X, Y, Z = (int(v) for v in rng.choice(PATCH_EDGES, size=3, p=PATCH_WEIGHTS))
+        out.append(tank(X, Y, Z, seed=seed * 1000 + q))
+    return out

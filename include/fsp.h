@@ -140,6 +140,34 @@ fsp_status fsp_project_device(fsp_solver* s, const float* d_us, const float* d_v
                               float* d_p, int32_t p_warm, float* d_u, float* d_v, float* d_w,
                               float scale, float k1, double rel_tol, uint32_t max_iters, fsp_report* report);
 
+/* Batched projection of independent patches (no counterpart in the
+ * reference, which projects one volume per call): each patch is its own
+ * solver with its geometry set; `concurrency` patches run at a time, each on
+ * its solver's HIP stream from its own host thread (<= 0: 4, the hardware
+ * queues of a process). Device pointers. Every patch is projected; the
+ * first non-OK status is returned and reports[i] holds each patch's report. */
+typedef struct fsp_patch_io {
+  const float* us;
+  const float* vs;
+  const float* ws;
+  float* p;
+  int32_t p_warm;
+  float* u;
+  float* v;
+  float* w;
+  float scale;
+  float k1;
+} fsp_patch_io;
+
+fsp_status fsp_project_batch(fsp_solver* const* solvers, const fsp_patch_io* io, int32_t n, double rel_tol,
+                             uint32_t max_iters, int32_t concurrency, fsp_report* reports);
+
+/* Tile autotuner (no counterpart in the reference): times the fine-level
+ * red-black pass of `s` (geometry set) for each workgroup tile width in x
+ * (quads) that fits the rows and keeps the fastest. Writes the chosen width
+ * to *tile_x and its mean pass time to *us (either may be null). */
+fsp_status fsp_autotune(fsp_solver* s, int32_t* tile_x, double* us);
+
 /* Replaces apply_pressure_operator_for_trials (delta3d_mobile.rs:936-949):
  * out = A p with the current geometry. Device pointers. */
 fsp_status fsp_apply_operator_device(fsp_solver* s, const float* d_p, float* d_out);

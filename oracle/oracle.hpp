@@ -63,9 +63,11 @@ struct Options {
   int32_t storage;        // EXTENSION: V-cycle vectors rounded to 0 f32 (the reference), 1 IEEE binary16,
                           // 2 bfloat16 at every store; arithmetic stays in T
   int32_t flexible;       // EXTENSION: 1 = flexible CG, beta = <z', r' - r> / <z, r> (Polak-Ribiere)
+  int32_t level_pre;      // EXTENSION: sweeps before restriction on the coarse levels (<0: pre_sweeps)
+  int32_t level_post;     // EXTENSION: sweeps after prolongation on the coarse levels (<0: post_sweeps)
 };
 
-inline Options default_options() { return Options{1, 0, -1, 2, 2, 8, 1, 0, 0, 0}; }
+inline Options default_options() { return Options{1, 0, -1, 2, 2, 8, 1, 0, 0, 0, -1, -1}; }
 
 // Round-to-nearest-even of an f32 value to bfloat16 (8 significant bits,
 // f32 exponent range); the result is returned as the f32 it represents.
@@ -806,7 +808,8 @@ template <class T> class Volume {
           for (int s = 0; s < half; ++s) level_sweep(L, false);
         }
       } else {
-        for (int s = 0; s < opt.pre_sweeps; ++s) level_sweep(L, true);
+        const int pre = opt.level_pre >= 0 ? opt.level_pre : opt.pre_sweeps;
+        for (int s = 0; s < pre; ++s) level_sweep(L, true);
       }
       if (l < last) {
         row_sums(L, &L.x, L.t);
@@ -817,7 +820,8 @@ template <class T> class Volume {
     for (size_t l = last; l-- > 0;) {
       Level3<T>& L = levels[l];
       prolong_add3(L.nx, L.ny, L.x, levels[l + 1]);
-      for (int s = 0; s < opt.post_sweeps; ++s) level_sweep(L, false);
+      const int post = opt.level_post >= 0 ? opt.level_post : opt.post_sweeps;
+      for (int s = 0; s < post; ++s) level_sweep(L, false);
     }
   }
 

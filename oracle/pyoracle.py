@@ -34,6 +34,7 @@ class Options(C.Structure):
         ("multigrid", C.c_int32), ("smoother", C.c_int32), ("max_levels", C.c_int32),
         ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32), ("coarse_sweeps", C.c_int32),
         ("gauge", C.c_int32), ("coarse_op", C.c_int32), ("storage", C.c_int32), ("flexible", C.c_int32),
+        ("level_pre", C.c_int32), ("level_post", C.c_int32),
     ]
 
 
@@ -141,8 +142,9 @@ class Problem:
 
 
 def options(multigrid=1, smoother=0, max_levels=-1, pre=2, post=2, coarse=8, gauge=1, coarse_op=0, storage=0,
-            flexible=0):
-    return Options(multigrid, smoother, max_levels, pre, post, coarse, gauge, coarse_op, storage, flexible)
+            flexible=0, level_pre=-1, level_post=-1):
+    return Options(multigrid, smoother, max_levels, pre, post, coarse, gauge, coarse_op, storage, flexible,
+                   level_pre, level_post)
 
 
 def project(prob: Problem, us, vs, ws, scale, k1, max_iters=20000, opts=None, p_warm=None, precision=32):

@@ -381,3 +381,42 @@ def test_16bit_storage_is_refused_off_the_red_black_path():
         W.build_volume(wl, smoother=_lib.SMOOTH_JACOBI, vcycle_storage=_lib.STORE_BF16)
     with pytest.raises(ProjectionError):
         W.build_volume(wl, preconditioner=_lib.PRECOND_JACOBI, vcycle_storage=_lib.STORE_F16)
+
+
+# ---------------------------------------------------------------- batched patches and the autotuner
+def test_batched_patches_equal_one_by_one_projections():
+    """fsp_project_batch runs each patch's own solver concurrently: every
+    patch gets exactly the pressure of its own projection alone."""
+    from fluidisim_amd.volume import project_batch
+    wls = [w for w in W.patches(16, seed=4) if w.domain.cells <= 64 * 64 * 64][:6]
+    assert len(wls) >= 4
+    alone = []
+    for wl in wls:
+        vol = W.build_volume(wl)
+        rep = vol.project_mobile3(wl.scale, wl.k1, 2000, warm=False)
+        alone.append((rep, host(vol.pressure()).copy(), host(vol.velocity_u()).copy()))
+        vol.close()
+    vols = [W.build_volume(wl) for wl in wls]
+    reps = project_batch(vols, [wl.scale for wl in wls], [wl.k1 for wl in wls], 2000, concurrency=4)
+    for wl, vol, rep, (rep1, p1, u1) in zip(wls, vols, reps, alone):
+        print(wl.name, rep.iterations, rep1.iterations)
+        assert not rep.degraded and rep.iterations == rep1.iterations
+        assert np.array_equal(bits(host(vol.pressure())), bits(p1))
+        assert np.array_equal(bits(host(vol.velocity_u())), bits(u1))
+    for vol in vols:
+        vol.close()
+
+
+def test_autotuned_tiles_keep_the_projection():
+    wl = W.tank(96, 48, 64, seed=8)
+    vol = W.build_volume(wl)
+    rep0 = vol.project_mobile3(wl.scale, wl.k1, 2000, warm=False)
+    p0 = host(vol.pressure()).copy()
+    tile, us = vol.autotune()
+    print("tile", tile, "us", us)
+    assert tile in (4, 8, 16, 32) and us > 0
+    rep1 = vol.project_mobile3(wl.scale, wl.k1, 2000, warm=False)
+    p1 = host(vol.pressure())
+    assert not rep1.degraded and abs(rep1.iterations - rep0.iterations) <= 1
+    # the V-cycle is the same arithmetic; only the reduction order of the dots follows the tiles
+    assert np.linalg.norm(p1 - p0) <= 1e-6 * np.linalg.norm(p0)
