@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--kernel", default="k_q_gs<0>", help="kernel timed live for the roofline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-iters", default="2,8", help="oracle sample: two iteration budgets")
+    ap.add_argument("--cpu-iters", default="2,16", help="oracle sample: two iteration budgets")
     ap.add_argument("--coarse-sweeps", type=int, default=2,
                     help="red-black sweeps on the coarsest level (2 keeps the 9 iterations of 8 on this tank)")
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
@@ -59,6 +59,7 @@ def parse():
                     help="trained network for the learned-FCG line (skipped when the file is absent)")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--batch", type=int, default=48, help="patches of the batched line (0: skip)")
+    ap.add_argument("--batch-concurrency", default="1,4", help="concurrency levels timed for the batched line")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the 256 MiB device copy (counter calibration) and exit")
     return ap.parse_args()
@@ -141,7 +142,8 @@ def batch_line(args, torch, W):
     scales, k1s = [wl.scale for wl in wls], [wl.k1 for wl in wls]
     out = {"patches": len(wls), "cells": int(sum(wl.domain.cells for wl in wls)),
            "tiles": {str(t): tiles.count(t) for t in sorted(set(tiles))}}
-    for conc in (1, 4):
+    levels = [int(v) for v in args.batch_concurrency.split(",")]
+    for conc in levels:
         project_batch(vols, scales, k1s, 2000, concurrency=conc)  # warm-up (graphs, caches)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -149,12 +151,13 @@ def batch_line(args, torch, W):
             reps = project_batch(vols, scales, k1s, 2000, concurrency=conc)
         torch.cuda.synchronize()
         ms = 1000.0 * (time.perf_counter() - t0) / max(1, args.steps)
-        key = "ms_per_batch" if conc == 4 else "ms_sequential"
-        out[key] = round(ms, 3)
+        out[f"ms_concurrency_{conc}"] = round(ms, 3)
         out["iterations_total"] = int(sum(r.iterations for r in reps))
         out["degraded"] = int(sum(r.degraded for r in reps))
         log(f"[bench] batch of {len(wls)} patches, concurrency {conc}: {ms:.3f} ms")
-    out["concurrency"] = 4
+    best = min(levels, key=lambda c: out[f"ms_concurrency_{c}"])
+    out["concurrency"] = best
+    out["ms_per_batch"] = out[f"ms_concurrency_{best}"]
     out["mcells_per_s"] = round(out["cells"] / (out["ms_per_batch"] * 1e-3) / 1e6, 1)
     for vol in vols:
         vol.close()
