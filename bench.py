@@ -164,13 +164,14 @@ def batch_line(args, torch, W):
     return out
 
 
-def learned_fcg(wl, args, torch, _lib, W):
+def learned_fcg(wl, args, torch, _lib, W, vcycle):
     """The same projection with the trained network inside flexible CG
-    (MGPCG fallback), timed like the plain one."""
+    (MGPCG fallback), timed like the plain one. vcycle: the network's output
+    is the initial guess of one V-cycle (else learned_smooth sweeps follow)."""
     from fluidisim_amd.learned import Network
     net = Network()
     net.load(args.weights)
-    vol = W.build_volume(wl, preconditioner=_lib.PRECOND_LEARNED, learned_budget=200)
+    vol = W.build_volume(wl, preconditioner=_lib.PRECOND_LEARNED, learned_budget=200, learned_vcycle=int(vcycle))
     net.attach(vol)
 
     def step():
@@ -184,13 +185,16 @@ def learned_fcg(wl, args, torch, _lib, W):
     torch.cuda.synchronize()
     ms = 1000.0 * (time.perf_counter() - t0) / args.steps
     r = reps[-1]
-    log(f"[bench] learned FCG: {ms:.3f} ms/projection, {r.iterations} iterations, residual {r.residual:.2e}")
+    mode = "network + V-cycle" if vcycle else "network + red-black sweeps"
+    log(f"[bench] learned FCG ({mode}): {ms:.3f} ms/projection, {r.iterations} iterations, "
+        f"residual {r.residual:.2e}")
     vol.close()
     net.close()
     with np.load(args.weights, allow_pickle=False) as f:
         trained = {k: (f[k].tolist() if f[k].size < 16 else None) for k in f.files if k != "theta"}
-    return {"ms_per_projection": round(ms, 4), "iterations": r.iterations, "residual": r.residual,
-            "divergence_plain": r.divergence_plain, "degraded": r.degraded, "training": trained}
+    return {"preconditioner": mode, "ms_per_projection": round(ms, 4), "iterations": r.iterations,
+            "residual": r.residual, "divergence_plain": r.divergence_plain, "degraded": r.degraded,
+            "training": trained}
 
 
 def main():
@@ -271,9 +275,10 @@ def main():
 
     batched = batch_line(args, torch, W) if args.batch > 0 else None
 
-    learned = None
+    learned = learned_sweeps = None
     if not args.no_learned and os.path.exists(args.weights):
-        learned = learned_fcg(wl, args, torch, _lib, W)
+        learned = learned_fcg(wl, args, torch, _lib, W, vcycle=True)
+        learned_sweeps = learned_fcg(wl, args, torch, _lib, W, vcycle=False)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -295,6 +300,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "learned_fcg": learned,
+        "learned_fcg_sweeps": learned_sweeps,
         "batched": batched,
     }
     print(json.dumps(line), flush=True)

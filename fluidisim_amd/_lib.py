@@ -44,6 +44,7 @@ class Options(C.Structure):
         ("gauge", C.c_int32), ("vcycle_storage", C.c_int32), ("replace_every", C.c_int32),
         ("refuse_degraded", C.c_int32), ("device", C.c_int32), ("coarse_operator", C.c_int32),
         ("graph", C.c_int32), ("learned_budget", C.c_int32), ("learned_smooth", C.c_int32),
+        ("learned_vcycle", C.c_int32),
     ]
 
 

@@ -38,7 +38,7 @@ def build(verbose: bool = True, out: str = LIB, build_dir: str = BUILD, defines=
     dflags = [f"-D{d}" for d in defines]
     jobs = [
         [hipcc, *HIPCC_FLAGS, *dflags, "-c", os.path.join(CSRC, "fsp.hip"), "-o", fsp_o],
-        [hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, "learned.hip"), "-o", net_o],
+        [hipcc, *HIPCC_FLAGS, *dflags, "-c", os.path.join(CSRC, "learned.hip"), "-o", net_o],
         ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", "-c",
          os.path.join(CSRC, "generate.cpp"), "-o", gen_o],
     ]

@@ -448,12 +448,12 @@ struct Launch {
 
   bool quad_vcycle() const { return quad_vcycle_ok(S->quad, S->opt); }
 
-  // The V-cycle on quads with red-black smoothing: first pass from zero,
-  // residual fused with the restriction, prolongation fused with the first
-  // post-smoothing pass, <r, z> fused with the last one. Returns the buffer
-  // holding z (storage ST).
+  // The V-cycle on quads with red-black smoothing: first pass from zero (or,
+  // with from_guess, from the values z already holds), residual fused with
+  // the restriction, prolongation fused with the first post-smoothing pass,
+  // <r, z> fused with the last one. Returns the buffer holding z (storage ST).
   template <int ST>
-  TS<ST>* vcycle_quad(int dot_op) {
+  TS<ST>* vcycle_quad(int dot_op, bool from_guess = false) {
     const QuadGrid& Q = S->qfine;
     const dim3 g((unsigned)(Q.nbx * Q.nby * Q.planes));
     Level L0 = S->levels.empty() ? Level{} : S->levels[0];
@@ -465,8 +465,9 @@ struct Launch {
                    S->partials);
       if (op >= 0) FSP_FINISH(S, st, op, g, false, 1);
     };
-    FSP_LAUNCH_N(S, "k_q_gs<1>", K_QGS(1), g, st, G, Q, (const TS<ST>*)z, z, S->r, 0, L0, none, -1, S->ctl,
-                 S->partials);
+    if (from_guess) pass(z, 0, -1);
+    else FSP_LAUNCH_N(S, "k_q_gs<1>", K_QGS(1), g, st, G, Q, (const TS<ST>*)z, z, S->r, 0, L0, none, -1, S->ctl,
+                      S->partials);
     pass(z, 1, -1);
     for (int s = 1; s < S->opt.pre_sweeps; ++s) {
       pass(z, 0, -1);
@@ -496,9 +497,14 @@ struct Launch {
       // z = network(r) inside flexible CG; sigma from |r|^2 on the device
       const double* rr = op == OP_RZ ? &S->ctl->rn : &S->ctl->rr;
       (void)net_apply(S->net, st, S->r, S->mask, S->solid, rr, &S->ctl->wet_cells, &S->ctl->active, S->z);
-      // red-black sweeps of A z = r from the network's output: the network
-      // carries the smooth part of the error, the sweeps the rough part
-      if (S->quad) {
+      float* zout = S->z;
+      if (S->opt.learned_vcycle) {
+        // one V-cycle from the network's output: the network carries the
+        // error the multigrid hierarchy leaves, the V-cycle the rest
+        zout = reinterpret_cast<float*>(vcycle_quad<ST_F32>(-1, true));
+      } else if (S->quad) {
+        // red-black sweeps of A z = r from the network's output: the network
+        // carries the smooth part of the error, the sweeps the rough part
         const QuadGrid& Q = S->qfine;
         const dim3 g((unsigned)(Q.nbx * Q.nby * Q.planes));
         const Level none_l{};
@@ -508,10 +514,10 @@ struct Launch {
             FSP_LAUNCH_N(S, "k_q_gs<0>", (k_q_gs<0, ST_F32>), g, st, G, Q, (const float*)S->z, S->z, S->r, c,
                          none_l, none, -1, S->ctl, S->partials);
       }
-      FSP_LAUNCH(S, k_dot_rz_qz, dim3(grid_for(G.n)), st, G.n, (const float*)S->r, (const float*)S->z,
+      FSP_LAUNCH(S, k_dot_rz_qz, dim3(grid_for(G.n)), st, G.n, (const float*)S->r, (const float*)zout,
                  (const float*)S->q, op, S->ctl, S->partials);
       FSP_FINISH(S, st, op, dim3(grid_for(G.n)), false, 1);
-      S->zres = S->z;
+      S->zres = zout;
       return;
     }
     if (S->opt.preconditioner == FSP_PRECOND_JACOBI) {
@@ -924,6 +930,7 @@ void fsp_default_options(fsp_options* o) {
   o->graph = 1;
   o->learned_budget = 0;
   o->learned_smooth = 1;
+  o->learned_vcycle = 0;
 }
 
 fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** out) {
@@ -937,6 +944,11 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
   if (o) opt = *o;
   if (opt.preconditioner < 0 || opt.preconditioner > 2 || opt.smoother < 0 || opt.smoother > 1) return FSP_INVALID;
   if (opt.learned_budget < 0 || opt.learned_smooth < 0 || opt.learned_smooth > 64) return FSP_INVALID;
+  if (opt.learned_vcycle < 0 || opt.learned_vcycle > 1) return FSP_INVALID;
+  if (opt.preconditioner == FSP_PRECOND_LEARNED && opt.learned_vcycle &&
+      (opt.vcycle_storage != FSP_STORE_F32 || opt.smoother != FSP_SMOOTH_RBGS || opt.pre_sweeps < 1 ||
+       opt.post_sweeps < 1))
+    return FSP_INVALID;
   if (opt.pre_sweeps < 0 || opt.post_sweeps < 0 || opt.coarse_sweeps < 0 || opt.pre_sweeps > 64 ||
       opt.post_sweeps > 64 || opt.coarse_sweeps > 1024)
     return FSP_INVALID;
@@ -1013,6 +1025,8 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
     const size_t blocks = (size_t)S->qfine.nbx * S->qfine.nby * S->qfine.planes;
     S->quad = blocks <= (size_t)MAX_PARTIALS && (S->n / 4 + QBLOCK - 1) / QBLOCK <= (size_t)MAX_PARTIALS;
   }
+  if (opt.preconditioner == FSP_PRECOND_LEARNED && opt.learned_vcycle && !quad_vcycle_ok(S->quad, opt))
+    return fail(FSP_INVALID);
   S->tail_first = (int)S->levels.size();
   for (int t = (int)S->levels.size() - 1; t >= 0; --t) {
     if (S->levels[t].n > 4096 || (int)S->levels.size() - t > TAIL_MAX_LEVELS) break;

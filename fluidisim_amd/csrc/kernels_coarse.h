@@ -339,6 +339,8 @@ __device__ __forceinline__ void gct_restrict(float* lds, const GcTailLvl& L, con
     int i, j, k;
     gct_decode(C, c, i, j, k);
     float s = 0.f;
+    // a full unroll spills at the 128-VGPR budget of 1024-thread workgroups
+#pragma unroll 2
     for (int ch = 0; ch < 8; ++ch) {
       const int fi = 2 * i + (ch & 1), fj = 2 * j + ((ch >> 1) & 1), fk = 2 * k + ((ch >> 2) & 1);
       const int f = (fk * L.ny + fj) * L.nx + fi;

@@ -34,8 +34,16 @@ namespace net {
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int WAVES = 4;  // wavefronts per workgroup of the convolution kernels
+constexpr int WAVES = 4;  // wavefronts per workgroup of the weight-gradient kernels
 constexpr int THREADS = 64 * WAVES;
+#ifndef FSP_CONV_WAVES
+#define FSP_CONV_WAVES 8  // 8 measured faster than 4 at 256x256x128 (1.03 vs 1.31 ms per forward)
+#endif
+#ifndef FSP_CONV_MIN_WAVES
+#define FSP_CONV_MIN_WAVES 4  // wavefronts per SIMD the register budget of k_conv leaves room for
+#endif
+constexpr int CWAVES = FSP_CONV_WAVES;  // wavefronts per workgroup of the convolution kernels
+constexpr int CTHREADS = 64 * CWAVES;
 
 struct LayerDesc {
   int level, cin, cout, relu, skip, up;  // skip: layer index added before the ReLU (-1 none); up: input from level+1
@@ -72,66 +80,189 @@ struct ConvArgs {
   const int32_t* gate;   // return at once while *gate == 0 (null: always run)
 };
 
-// EPI 0: bias (+ skip) + ReLU -> binary16; 1: bias -> f32 channel 0 (head);
-// 2: plain product -> binary16 (backward data).
+// Output brick of one workgroup: 16 x-cells by BY rows by BZ planes; its
+// input halo (18 x (BY+2) x (BZ+2) cells, CIN channels) is staged in LDS
+// once, so each input value is read from memory once per brick instead of
+// once per tap.
+template <int CIN, int COUT>
+struct ConvBrick {
+  static constexpr int BX = 16, BY = 8, BZ = (CIN >= 32 || COUT >= 32) ? 4 : 8;
+  static constexpr int HX = BX + 2, HY = BY + 2, HZ = BZ + 2;
+  static constexpr int HCELLS = HX * HY * HZ;
+  static constexpr int TILES = BY * BZ;  // 16-cell rows of the brick
+  static constexpr int TPW = TILES / CWAVES;
+  static_assert(TILES % CWAVES == 0, "rows per wavefront");
+};
+constexpr int STAGE_LD = 20;  // floats per cell row of the epilogue stage (conflict-free writes)
+
+__host__ __device__ constexpr int tap_offset(int tap, int hx, int hy) {
+  return (tap / 9) * hy * hx + ((tap / 3) % 3) * hx + tap % 3;
+}
+
+// EPI 0: bias + ReLU -> binary16; 1: bias -> f32 channel 0 (head);
+// 2: plain product -> binary16 (backward data); 3: bias + skip + ReLU.
+//
+// Wavefront w owns rows w*TPW .. w*TPW+TPW-1 of the brick; for each K chunk
+// (4 channels of one tap per lane group) the weights are read from LDS once
+// and multiplied into every row's 16 x 16 accumulator tile.
 template <int CIN, int COUT, int UP, int EPI>
-__global__ void __launch_bounds__(THREADS) k_conv(ConvArgs a) {
+__global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv(ConvArgs a) {
+  using Bk = ConvBrick<CIN, COUT>;
   constexpr int NT = npad(COUT) / 16;
   constexpr int KP = kpad(CIN);
   constexpr int CHUNKS = KP / 16;
+  static_assert(COUT % 4 == 0 || (COUT == 1 && EPI == 1), "output channels in groups of 4");
+  constexpr int NP = npad(COUT);
+  constexpr int KPL = KP + 4;  // LDS row stride of the weights (halves), spreads the rows over the banks
+  __shared__ __attribute__((aligned(16))) _Float16 halo[Bk::HCELLS * CIN];
+  __shared__ __attribute__((aligned(16))) _Float16 wl[NP * KPL];
+  __shared__ __attribute__((aligned(16))) float stage[CWAVES][16 * STAGE_LD];
   if (a.gate && !*a.gate) return;
-  const int lane = threadIdx.x & 63;
-  const int wave = blockIdx.x * WAVES + (threadIdx.x >> 6);
-  const int segs = a.nx / 16;
-  if (wave >= segs * a.ny * a.nz) return;  // uniform per wavefront
-  const int seg = wave % segs, row = wave / segs;
-  const int y = row % a.ny, z = row / a.ny;
-  const int m = lane & 15, g = lane >> 4;
-  const int x = seg * 16 + m;  // this lane's A row (cell)
-  f4 acc[NT];
+  const int x0 = blockIdx.x * Bk::BX, y0 = blockIdx.y * Bk::BY, z0 = blockIdx.z * Bk::BZ;
+
+  // weights and halo (zero outside the domain) -> LDS, every load of the
+  // thread in flight before the first LDS write
+  {
+    constexpr int WV = NP * KP / 8;  // 16-byte vectors of the weights
+    constexpr int WPER = (WV + CTHREADS - 1) / CTHREADS;
+    uint4 wbuf[WPER];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
-  for (int ch = 0; ch < CHUNKS; ++ch) {
-    const int kk = ch * 16 + 4 * g;
-    const int tap = kk / CIN, ci = kk - tap * CIN;
-    h4 av = h4{0, 0, 0, 0};
-    if (tap < 27) {
-      const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-      const int xx = x + dx, yy = y + dy, zz = z + dz;
-      if (xx >= 0 && xx < a.nx && yy >= 0 && yy < a.ny && zz >= 0 && zz < a.nz) {
+    for (int r = 0; r < WPER; ++r) {
+      const int q = (int)threadIdx.x + r * CTHREADS;
+      if (q < WV) wbuf[r] = *reinterpret_cast<const uint4*>(a.wt + (size_t)q * 8);
+    }
+    constexpr int VH = CIN >= 8 ? 8 : 4;  // halves per vector
+    constexpr int VPC = CIN / VH;
+    constexpr int TOTAL = Bk::HCELLS * VPC;
+    constexpr int PER = (TOTAL + CTHREADS - 1) / CTHREADS;
+    typedef uint32_t hv __attribute__((ext_vector_type(VH / 2)));
+    hv buf[PER];
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      const int q = (int)threadIdx.x + r * CTHREADS;
+      const int cell = q / VPC, part = q - cell * VPC;
+      const int lx = cell % Bk::HX, t = cell / Bk::HX, ly = t % Bk::HY, lz = t / Bk::HY;
+      const int xx = x0 + lx - 1, yy = y0 + ly - 1, zz = z0 + lz - 1;
+      const bool in = q < TOTAL && xx >= 0 && xx < a.nx && yy >= 0 && yy < a.ny && zz >= 0 && zz < a.nz;
+      buf[r] = hv(0u);
+      if (in) {
         size_t v;
         if (UP) v = ((size_t)(zz >> 1) * a.cny + (yy >> 1)) * a.cnx + (xx >> 1);
         else v = ((size_t)zz * a.ny + yy) * a.nx + xx;
-        av = *reinterpret_cast<const h4*>(a.in + v * CIN + ci);
+        buf[r] = *reinterpret_cast<const hv*>(a.in + v * CIN + part * VH);
       }
     }
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const h4 bv = *reinterpret_cast<const h4*>(a.wt + (size_t)(t * 16 + m) * KP + kk);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, acc[t], 0, 0, 0);
+    for (int r = 0; r < WPER; ++r) {
+      const int q = (int)threadIdx.x + r * CTHREADS;
+      if (q < WV) {
+        const int row = q / (KP / 8), col = (q - row * (KP / 8)) * 8;
+        uint2* d = reinterpret_cast<uint2*>(wl + row * KPL + col);
+        d[0] = make_uint2(wbuf[r].x, wbuf[r].y);
+        d[1] = make_uint2(wbuf[r].z, wbuf[r].w);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      const int q = (int)threadIdx.x + r * CTHREADS;
+      if (q < TOTAL) *reinterpret_cast<hv*>(halo + (size_t)q * VH) = buf[r];
     }
   }
-  // lane holds D[cell 4g+v][channel t*16+m]
-  const size_t rowbase = ((size_t)z * a.ny + y) * a.nx + seg * 16;
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  f4 acc[Bk::TPW][NT];
+#pragma unroll
+  for (int i = 0; i < Bk::TPW; ++i)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[i][t] = f4{0.f, 0.f, 0.f, 0.f};
+  // epilogue ownership (see below): cell c of a row, channels q4..q4+3
+  const int c = lane >> 2, q4 = (lane & 3) * 4;
+  // skip values of the epilogue, loaded now so their latency hides under the products
+  h4 skv[EPI == 3 ? Bk::TPW : 1][NT];
+  if (EPI == 3) {
+#pragma unroll
+    for (int i = 0; i < Bk::TPW; ++i) {
+      const int tile = wave * Bk::TPW + i, y = y0 + tile % Bk::BY, z = z0 + tile / Bk::BY;
+      const size_t cell = ((size_t)z * a.ny + y) * a.nx + x0 + c;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int co = t * 16 + q4;
+        skv[i][t] = h4{0, 0, 0, 0};
+        if (y < a.ny && z < a.nz && co < COUT) skv[i][t] = *reinterpret_cast<const h4*>(a.skip + cell * COUT + co);
+      }
+    }
+  }
+  // halo index of this lane's cell (x = m) in row i of the wavefront, tap (-1,-1,-1)
+  auto row_base = [&](int i) {
+    const int tile = wave * Bk::TPW + i, ly = tile % Bk::BY, lz = tile / Bk::BY;
+    return ((lz * Bk::HY + ly) * Bk::HX + m) * CIN;
+  };
+#pragma unroll
+  for (int ch = 0; ch < CHUNKS; ++ch) {
+    const int kk = ch * 16 + 4 * g;
+    const int tap = kk / CIN, ci = kk - tap * CIN;
+    const bool live = tap < 27;  // K padding: zero operand
+    const int off = (live ? tap_offset(tap, Bk::HX, Bk::HY) : 0) * CIN + ci;
+    h4 b[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) b[t] = *reinterpret_cast<const h4*>(wl + (t * 16 + m) * KPL + kk);
+#pragma unroll
+    for (int i = 0; i < Bk::TPW; ++i) {
+      h4 av = *reinterpret_cast<const h4*>(halo + row_base(i) + off);
+      if (!live) av = h4{0, 0, 0, 0};
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, b[t], acc[i][t], 0, 0, 0);
+    }
+  }
+
+  // epilogue: lane (m, g) holds D[cell 4g+v][channel t*16+m]; through the
+  // wavefront's stage, lane L then owns cell L/4, channels 4*(L%4)..+3
+  float* st = stage[wave];
+  f4 bias[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    const int co = t * 16 + m;
-    if (co >= COUT) continue;
+    bias[t] = f4{0.f, 0.f, 0.f, 0.f};
+    if (EPI == 1 && t == 0) bias[t][0] = a.bias[0];
+    if ((EPI == 0 || EPI == 3) && t * 16 + q4 < COUT) bias[t] = *reinterpret_cast<const f4*>(a.bias + t * 16 + q4);
+  }
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const size_t cell = rowbase + 4 * g + v;
-      float val = acc[t][v];
-      if (EPI == 2) {
-        a.out[cell * COUT + co] = (_Float16)val;
-        continue;
-      }
-      val += a.bias[co];
+  for (int i = 0; i < Bk::TPW; ++i) {
+    const int tile = wave * Bk::TPW + i, y = y0 + tile % Bk::BY, z = z0 + tile / Bk::BY;
+    const bool rowok = y < a.ny && z < a.nz;  // uniform per wavefront
+    const size_t cell = ((size_t)z * a.ny + y) * a.nx + x0 + c;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) st[(4 * g + v) * STAGE_LD + m] = acc[i][t][v];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const f4 val = *reinterpret_cast<const f4*>(st + c * STAGE_LD + q4);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (!rowok) continue;
       if (EPI == 1) {
-        a.yout[cell] = val;
+        if (q4 == 0) a.yout[cell] = val[0] + bias[0][0];
         continue;
       }
-      if (a.skip) val += (float)a.skip[cell * COUT + co];
-      a.out[cell * COUT + co] = (_Float16)(val > 0.f ? val : 0.f);
+      const int co = t * 16 + q4;
+      if (co >= COUT) continue;
+      h4 o;
+      if (EPI == 2) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) o[e] = (_Float16)val[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          float u = val[e] + bias[t][e];
+          if (EPI == 3) u += (float)skv[i][t][e];
+          o[e] = (_Float16)(u > 0.f ? u : 0.f);
+        }
+      }
+      *reinterpret_cast<h4*>(a.out + cell * COUT + co) = o;
     }
   }
 }
@@ -489,9 +620,10 @@ void conv(hipStream_t st, int nx, int ny, int nz, const _Float16* in, const _Flo
   a.nx = nx; a.ny = ny; a.nz = nz;
   a.cnx = UP ? nx / 2 : nx; a.cny = UP ? ny / 2 : ny; a.cnz = UP ? nz / 2 : nz;
   a.in = in; a.wt = wt; a.bias = bias; a.skip = skip; a.out = out; a.yout = yout;
-  const size_t waves = (size_t)(nx / 16) * ny * nz;
-  const unsigned blocks = (unsigned)((waves + WAVES - 1) / WAVES);
-  k_conv<CIN, COUT, UP, EPI><<<blocks, THREADS, 0, st>>>(a);
+  using Bk = ConvBrick<CIN, COUT>;
+  const dim3 grid((unsigned)(nx / Bk::BX), (unsigned)((ny + Bk::BY - 1) / Bk::BY),
+                  (unsigned)((nz + Bk::BZ - 1) / Bk::BZ));
+  k_conv<CIN, COUT, UP, EPI><<<grid, CTHREADS, 0, st>>>(a);
 }
 
 // dW of layer l (g: gradient at its pre-activation with channel stride gs)
@@ -535,9 +667,9 @@ void forward(fsp_net* N) {
   k_pool<<<blocks_for(N->V[2] * 16), 256, 0, st>>>(N->act[3], N->p2, X2, Y2, Z2, 16, gt);
   conv<16, 32, 0, 0>(st, X2, Y2, Z2, N->p2, N->wt[4], B(4), nullptr, N->act[4], nullptr, gt);
   conv<32, 32, 0, 0>(st, X2, Y2, Z2, N->act[4], N->wt[5], B(5), nullptr, N->act[5], nullptr, gt);
-  conv<32, 16, 1, 0>(st, X1, Y1, Z1, N->act[5], N->wt[6], B(6), N->act[3], N->act[6], nullptr, gt);
+  conv<32, 16, 1, 3>(st, X1, Y1, Z1, N->act[5], N->wt[6], B(6), N->act[3], N->act[6], nullptr, gt);
   conv<16, 16, 0, 0>(st, X1, Y1, Z1, N->act[6], N->wt[7], B(7), nullptr, N->act[7], nullptr, gt);
-  conv<16, 8, 1, 0>(st, X0, Y0, Z0, N->act[7], N->wt[8], B(8), N->act[1], N->act[8], nullptr, gt);
+  conv<16, 8, 1, 3>(st, X0, Y0, Z0, N->act[7], N->wt[8], B(8), N->act[1], N->act[8], nullptr, gt);
   conv<8, 8, 0, 0>(st, X0, Y0, Z0, N->act[8], N->wt[9], B(9), nullptr, N->act[9], nullptr, gt);
   conv<8, 1, 0, 1>(st, X0, Y0, Z0, N->act[9], N->wt[10], B(10), nullptr, nullptr, N->y, gt);
 }

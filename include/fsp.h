@@ -72,6 +72,9 @@ typedef struct {
   int32_t graph;           /* 1: replay each CG iteration from a captured HIP graph */
   int32_t learned_budget;  /* FSP_PRECOND_LEARNED: FCG iterations before the MGPCG fallback (0: 50) */
   int32_t learned_smooth;  /* FSP_PRECOND_LEARNED: red-black sweeps on the network's output (default 1) */
+  int32_t learned_vcycle;  /* FSP_PRECOND_LEARNED: 1 = the network's output is the initial guess of one
+                              V-cycle (needs f32 storage and red-black smoothing) in place of the
+                              learned_smooth sweeps; default 0 */
 } fsp_options;
 
 /* Surface state (Volume3 fields rho, g_eff, rest, homogeneous_ghost;

@@ -50,3 +50,19 @@ def test_create_refuses_bad_domains_without_a_device():
     for d in [(0, 2, 2, 1.0), (2, 0, 2, 1.0), (2, 2, 0, 1.0), (2, 2, 2, 0.0), (2, 2, 2, float("nan"))]:
         dom = _lib.Domain(*d)
         assert L.fsp_create(C.byref(dom), None, C.byref(h)) == _lib.FSP_DOMAIN
+
+
+def test_create_refuses_inconsistent_learned_options_without_a_device():
+    """The network-then-V-cycle preconditioner needs f32 V-cycle storage and
+    red-black smoothing (the network writes f32 into the V-cycle's iterate);
+    the options are checked before any device call."""
+    L = _lib.lib()
+    h = C.c_void_p()
+    dom = _lib.Domain(64, 8, 8, 1.0)
+    base = dict(preconditioner=_lib.PRECOND_LEARNED, smoother=_lib.SMOOTH_RBGS, learned_vcycle=1)
+    for bad in [dict(vcycle_storage=_lib.STORE_F16), dict(vcycle_storage=_lib.STORE_BF16),
+                dict(smoother=_lib.SMOOTH_JACOBI), dict(pre_sweeps=0), dict(post_sweeps=0), dict(learned_vcycle=2),
+                dict(learned_vcycle=-1)]:
+        o = _lib.default_options(**{**base, **bad})
+        assert L.fsp_create(C.byref(dom), C.byref(o), C.byref(h)) == _lib.FSP_INVALID, bad
+    assert _lib.default_options().learned_vcycle == 0

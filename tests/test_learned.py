@@ -237,3 +237,38 @@ def test_trained_network_beats_the_jacobi_preconditioner(net):
     assert rl.iterations < rj.iterations
     vj.close()
     vl.close()
+
+
+@pytest.mark.gpu
+def test_network_then_vcycle_converges_like_mgpcg(net):
+    """learned_vcycle = 1: the trained network's output is the initial guess
+    of one V-cycle inside flexible CG. The projection reaches 1e-6 with the
+    reference's pressure, in no more iterations than the plain MGPCG plus
+    one (the V-cycle alone already gives the MGPCG rate)."""
+    from fluidisim_amd.learned import WEIGHTS
+    net.load(WEIGHTS)
+    wl = W.tank(64, 32, 64, seed=14)
+    vm = W.build_volume(wl)
+    rm = vm.project_mobile3(wl.scale, wl.k1, 2000, warm=False)
+    vl = W.build_volume(wl, preconditioner=_lib.PRECOND_LEARNED, learned_budget=200, learned_vcycle=1)
+    net.attach(vl)
+    rl = vl.project_mobile3(wl.scale, wl.k1, 2000, warm=False)
+    print("mgpcg", rm, "network + V-cycle", rl)
+    assert not rl.degraded and rl.residual <= 1e-6 and rl.divergence_plain <= 1e-5
+    assert rl.iterations <= rm.iterations + 1
+    d = wl.domain
+    prob = po.Problem(d.nx, d.ny, d.nz, d.dx, wl.eta, rho=wl.rho, g_eff=wl.g_eff, rest=wl.rest,
+                      homogeneous_ghost=wl.homogeneous_ghost, closed_lid=wl.closed_lid, solid=wl.solid)
+    o = wl.options
+    *_, p64 = po.project(prob, wl.us, wl.vs, wl.ws, wl.scale, wl.k1, 2000,
+                         po.options(multigrid=1, smoother=o["smoother"], pre=o["pre_sweeps"],
+                                    post=o["post_sweeps"], coarse=o["coarse_sweeps"],
+                                    coarse_op=o.get("coarse_operator", 0)), precision=64)
+    kind, _ = po.classify(prob)
+    wet = kind == 1
+    p = vl.pressure().cpu().numpy()[wet].astype(np.float64)
+    err = np.linalg.norm(p - p64[wet]) / np.linalg.norm(p64[wet])
+    print(f"pressure rel L2 {err:.2e}")
+    assert err <= 1e-4
+    vm.close()
+    vl.close()
