@@ -11,10 +11,11 @@
 // W[co][tap][ci], tap = (dz+1)*9 + (dy+1)*3 + (dx+1), zero padding at the
 // domain boundary.
 //
-// Convolution as a GEMM on v_mfma_f32_16x16x16_f16: a wavefront owns 16
-// consecutive x-cells of one row (M = 16) and all output channels (N, in
-// 16-channel tiles); K runs over (tap, input channel) in chunks of 16, each
-// lane loading 4 consecutive channels of one tap (8 bytes).
+// Convolution as an implicit GEMM on v_mfma_f32_16x16x32_f16 (k_conv): the
+// layer's weights (M = output channels, in 16-channel tiles) times a 16-cell
+// row of its input (N), K over (tap, input channel) in chunks of 32, the
+// input brick and the weights staged in LDS. The weight gradient (k_wgrad)
+// uses v_mfma_f32_16x16x16_f16 with K over cells.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -32,6 +33,7 @@ namespace fsp {
 namespace net {
 
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int WAVES = 4;  // wavefronts per workgroup of the weight-gradient kernels
@@ -56,7 +58,8 @@ constexpr LayerDesc LAYERS[NL] = {
     {0, 16, 8, 1, 1, 1},  {0, 8, 8, 1, -1, 0},   {0, 8, 1, 0, -1, 0},
 };
 
-__host__ __device__ constexpr int kpad(int cin) { return ((27 * cin + 15) / 16) * 16; }
+// K of a layer (27 taps x input channels) padded to the 32 of one product
+__host__ __device__ constexpr int kpad(int cin) { return ((27 * cin + 31) / 32) * 32; }
 __host__ __device__ constexpr int npad(int cout) { return ((cout + 15) / 16) * 16; }
 // channels of the gradient flowing into a layer's backward-data convolution
 __host__ __device__ constexpr int gch(int cout) { return cout < 4 ? 4 : cout; }
@@ -93,7 +96,6 @@ struct ConvBrick {
   static constexpr int TPW = TILES / CWAVES;
   static_assert(TILES % CWAVES == 0, "rows per wavefront");
 };
-constexpr int STAGE_LD = 20;  // floats per cell row of the epilogue stage (conflict-free writes)
 
 __host__ __device__ constexpr int tap_offset(int tap, int hx, int hy) {
   return (tap / 9) * hy * hx + ((tap / 3) % 3) * hx + tap % 3;
@@ -102,21 +104,23 @@ __host__ __device__ constexpr int tap_offset(int tap, int hx, int hy) {
 // EPI 0: bias + ReLU -> binary16; 1: bias -> f32 channel 0 (head);
 // 2: plain product -> binary16 (backward data); 3: bias + skip + ReLU.
 //
-// Wavefront w owns rows w*TPW .. w*TPW+TPW-1 of the brick; for each K chunk
-// (4 channels of one tap per lane group) the weights are read from LDS once
-// and multiplied into every row's 16 x 16 accumulator tile.
+// D[co][cell] = W[co][k] X[k][cell] on v_mfma_f32_16x16x32_f16: the weights
+// are the A operand (16 output channels), a 16-cell row of the brick the B
+// operand, K = 32 values of (tap, input channel) per product. Lane (m, g)
+// then holds output channels 4g..4g+3 of cell m, written as one 8-byte
+// store. Wavefront w owns rows w*TPW .. w*TPW+TPW-1; for each K chunk the
+// weights are read from LDS once and multiplied into every row's tile.
 template <int CIN, int COUT, int UP, int EPI>
 __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv(ConvArgs a) {
   using Bk = ConvBrick<CIN, COUT>;
   constexpr int NT = npad(COUT) / 16;
   constexpr int KP = kpad(CIN);
-  constexpr int CHUNKS = KP / 16;
+  constexpr int CHUNKS = KP / 32;
   static_assert(COUT % 4 == 0 || (COUT == 1 && EPI == 1), "output channels in groups of 4");
   constexpr int NP = npad(COUT);
-  constexpr int KPL = KP + 4;  // LDS row stride of the weights (halves), spreads the rows over the banks
+  constexpr int KPL = KP + 8;  // LDS row stride of the weights (halves): 16-byte rows spread over the banks
   __shared__ __attribute__((aligned(16))) _Float16 halo[Bk::HCELLS * CIN];
   __shared__ __attribute__((aligned(16))) _Float16 wl[NP * KPL];
-  __shared__ __attribute__((aligned(16))) float stage[CWAVES][16 * STAGE_LD];
   if (a.gate && !*a.gate) return;
   const int x0 = blockIdx.x * Bk::BX, y0 = blockIdx.y * Bk::BY, z0 = blockIdx.z * Bk::BZ;
 
@@ -129,6 +133,7 @@ __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv(ConvArgs 
 #pragma unroll
     for (int r = 0; r < WPER; ++r) {
       const int q = (int)threadIdx.x + r * CTHREADS;
+      wbuf[r] = make_uint4(0u, 0u, 0u, 0u);
       if (q < WV) wbuf[r] = *reinterpret_cast<const uint4*>(a.wt + (size_t)q * 8);
     }
     constexpr int VH = CIN >= 8 ? 8 : 4;  // halves per vector
@@ -157,9 +162,7 @@ __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv(ConvArgs 
       const int q = (int)threadIdx.x + r * CTHREADS;
       if (q < WV) {
         const int row = q / (KP / 8), col = (q - row * (KP / 8)) * 8;
-        uint2* d = reinterpret_cast<uint2*>(wl + row * KPL + col);
-        d[0] = make_uint2(wbuf[r].x, wbuf[r].y);
-        d[1] = make_uint2(wbuf[r].z, wbuf[r].w);
+        *reinterpret_cast<uint4*>(wl + row * KPL + col) = wbuf[r];
       }
     }
 #pragma unroll
@@ -172,83 +175,94 @@ __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv(ConvArgs 
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, g = lane >> 4;
-  f4 acc[Bk::TPW][NT];
-#pragma unroll
-  for (int i = 0; i < Bk::TPW; ++i)
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[i][t] = f4{0.f, 0.f, 0.f, 0.f};
-  // epilogue ownership (see below): cell c of a row, channels q4..q4+3
-  const int c = lane >> 2, q4 = (lane & 3) * 4;
-  // skip values of the epilogue, loaded now so their latency hides under the products
+  // skip values of the epilogue (channels 4g..4g+3 of cell m of every row),
+  // loaded now so their latency hides under the products
   h4 skv[EPI == 3 ? Bk::TPW : 1][NT];
   if (EPI == 3) {
 #pragma unroll
     for (int i = 0; i < Bk::TPW; ++i) {
       const int tile = wave * Bk::TPW + i, y = y0 + tile % Bk::BY, z = z0 + tile / Bk::BY;
-      const size_t cell = ((size_t)z * a.ny + y) * a.nx + x0 + c;
+      const size_t cell = ((size_t)z * a.ny + y) * a.nx + x0 + m;
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const int co = t * 16 + q4;
+        const int co = t * 16 + 4 * g;
         skv[i][t] = h4{0, 0, 0, 0};
         if (y < a.ny && z < a.nz && co < COUT) skv[i][t] = *reinterpret_cast<const h4*>(a.skip + cell * COUT + co);
       }
     }
   }
-  // halo index of this lane's cell (x = m) in row i of the wavefront, tap (-1,-1,-1)
+  f4 acc[Bk::TPW][NT];
+#pragma unroll
+  for (int i = 0; i < Bk::TPW; ++i)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[i][t] = f4{0.f, 0.f, 0.f, 0.f};
+  // halo index (halves) of this lane's cell (x = m) in row i of the wavefront, tap (-1,-1,-1)
   auto row_base = [&](int i) {
     const int tile = wave * Bk::TPW + i, ly = tile % Bk::BY, lz = tile / Bk::BY;
     return ((lz * Bk::HY + ly) * Bk::HX + m) * CIN;
   };
 #pragma unroll
   for (int ch = 0; ch < CHUNKS; ++ch) {
-    const int kk = ch * 16 + 4 * g;
-    const int tap = kk / CIN, ci = kk - tap * CIN;
-    const bool live = tap < 27;  // K padding: zero operand
-    const int off = (live ? tap_offset(tap, Bk::HX, Bk::HY) : 0) * CIN + ci;
-    h4 b[NT];
+    // this lane group's 8 K values: k = 32 ch + 8 g .. +7
+    const int kk = ch * 32 + 8 * g;
+    h8 w[NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) b[t] = *reinterpret_cast<const h4*>(wl + (t * 16 + m) * KPL + kk);
+    for (int t = 0; t < NT; ++t) w[t] = *reinterpret_cast<const h8*>(wl + (t * 16 + m) * KPL + kk);
+    if (CIN >= 8) {
+      // one tap, 8 consecutive channels
+      const int tap = kk / CIN, ci = kk - tap * CIN;
+      const bool live = tap < 27;  // K padding: zero operand
+      const int off = (live ? tap_offset(tap, Bk::HX, Bk::HY) : 0) * CIN + ci;
 #pragma unroll
-    for (int i = 0; i < Bk::TPW; ++i) {
-      h4 av = *reinterpret_cast<const h4*>(halo + row_base(i) + off);
-      if (!live) av = h4{0, 0, 0, 0};
+      for (int i = 0; i < Bk::TPW; ++i) {
+        h8 xv = *reinterpret_cast<const h8*>(halo + row_base(i) + off);
+        if (!live) xv = h8(0);
 #pragma unroll
-      for (int t = 0; t < NT; ++t) acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, b[t], acc[i][t], 0, 0, 0);
+        for (int t = 0; t < NT; ++t)
+          acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(w[t], xv, acc[i][t], 0, 0, 0);
+      }
+    } else {
+      // CIN == 4: two taps of 4 channels
+      const int tap0 = kk / 4, tap1 = tap0 + 1;
+      const bool live0 = tap0 < 27, live1 = tap1 < 27;
+      const int off0 = (live0 ? tap_offset(tap0, Bk::HX, Bk::HY) : 0) * CIN;
+      const int off1 = (live1 ? tap_offset(tap1, Bk::HX, Bk::HY) : 0) * CIN;
+#pragma unroll
+      for (int i = 0; i < Bk::TPW; ++i) {
+        h4 lo = *reinterpret_cast<const h4*>(halo + row_base(i) + off0);
+        h4 hi = *reinterpret_cast<const h4*>(halo + row_base(i) + off1);
+        if (!live0) lo = h4{0, 0, 0, 0};
+        if (!live1) hi = h4{0, 0, 0, 0};
+        const h8 xv = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(w[t], xv, acc[i][t], 0, 0, 0);
+      }
     }
   }
 
-  // epilogue: lane (m, g) holds D[cell 4g+v][channel t*16+m]; through the
-  // wavefront's stage, lane L then owns cell L/4, channels 4*(L%4)..+3
-  float* st = stage[wave];
+  // epilogue: lane (m, g) holds D[channel t*16 + 4g + v][cell m]
   f4 bias[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     bias[t] = f4{0.f, 0.f, 0.f, 0.f};
     if (EPI == 1 && t == 0) bias[t][0] = a.bias[0];
-    if ((EPI == 0 || EPI == 3) && t * 16 + q4 < COUT) bias[t] = *reinterpret_cast<const f4*>(a.bias + t * 16 + q4);
+    if ((EPI == 0 || EPI == 3) && t * 16 + 4 * g < COUT)
+      bias[t] = *reinterpret_cast<const f4*>(a.bias + t * 16 + 4 * g);
   }
 #pragma unroll
   for (int i = 0; i < Bk::TPW; ++i) {
     const int tile = wave * Bk::TPW + i, y = y0 + tile % Bk::BY, z = z0 + tile / Bk::BY;
-    const bool rowok = y < a.ny && z < a.nz;  // uniform per wavefront
-    const size_t cell = ((size_t)z * a.ny + y) * a.nx + x0 + c;
+    if (y >= a.ny || z >= a.nz) continue;  // uniform per wavefront
+    const size_t cell = ((size_t)z * a.ny + y) * a.nx + x0 + m;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) st[(4 * g + v) * STAGE_LD + m] = acc[i][t][v];
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const f4 val = *reinterpret_cast<const f4*>(st + c * STAGE_LD + q4);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      if (!rowok) continue;
+      const f4 val = acc[i][t];
       if (EPI == 1) {
-        if (q4 == 0) a.yout[cell] = val[0] + bias[0][0];
+        if (g == 0) a.yout[cell] = val[0] + bias[0][0];
         continue;
       }
-      const int co = t * 16 + q4;
+      const int co = t * 16 + 4 * g;
       if (co >= COUT) continue;
       h4 o;
       if (EPI == 2) {
