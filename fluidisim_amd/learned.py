@@ -165,10 +165,11 @@ def tank_shape(X, Y, Z):
 
 
 def train(net: Network, size=(128, 128, 128), samples=64, epochs=4, lr=1e-3, seed0=1000, log=print,
-          on_epoch=None):
+          on_epoch=None, lr_decay=1.0):
     """Trains on `samples` tank workloads of the given size (distinct seeds),
-    `epochs` passes; returns the per-epoch mean losses. on_epoch(e, history)
-    runs after each epoch (e.g. to save the parameters)."""
+    `epochs` passes, the step size multiplied by lr_decay after each pass;
+    returns the per-epoch mean losses. on_epoch(e, history) runs after each
+    epoch (e.g. to save the parameters)."""
     X, Y, Z = size
     shape = tank_shape(X, Y, Z)
     t0 = time.time()
@@ -185,7 +186,7 @@ def train(net: Network, size=(128, 128, 128), samples=64, epochs=4, lr=1e-3, see
         losses, skipped = [], 0
         for q in rng.permutation(len(pairs)):
             x, t, w = pairs[q]
-            loss, sk = net.train_step(x, t, w, shape, lr)
+            loss, sk = net.train_step(x, t, w, shape, lr * lr_decay ** e)
             skipped += int(sk)
             if not sk:
                 losses.append(loss)
@@ -203,6 +204,7 @@ def main():
     ap.add_argument("--samples", type=int, default=64)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr-decay", type=float, default=1.0, help="step-size factor per epoch")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=WEIGHTS)
     a = ap.parse_args()
@@ -213,10 +215,11 @@ def main():
     net.set_params(init_params(a.seed))
     def save(e, hist):
         net.save(a.out, size=np.array(size), samples=a.samples, pairs=4 * a.samples, epochs=e + 1, lr=a.lr,
-                 history=np.array(hist))
+                 lr_decay=a.lr_decay, history=np.array(hist))
         print(f"[learned] saved {a.out} after epoch {e}", flush=True)
 
-    train(net, size, a.samples, a.epochs, a.lr, log=lambda m: print(m, flush=True), on_epoch=save)
+    train(net, size, a.samples, a.epochs, a.lr, log=lambda m: print(m, flush=True), on_epoch=save,
+          lr_decay=a.lr_decay)
 
 
 if __name__ == "__main__":
