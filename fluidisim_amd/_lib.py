@@ -25,7 +25,8 @@ EXPORTS = [
     "fsp_abi_version", "fsp_status_string", "fsp_default_options", "fsp_create", "fsp_destroy", "fsp_levels",
     "fsp_set_geometry", "fsp_project", "fsp_project_device", "fsp_apply_operator_device", "fsp_assemble_device",
     "fsp_precondition_device", "fsp_divergence_device", "fsp_profile_begin", "fsp_profile_end", "fsp_generate",
-    "fsp_project_batch", "fsp_autotune",
+    "fsp_project_batch", "fsp_autotune", "fsp_step_surface_mobile_device", "fsp_get_surface", "fsp_advect_device",
+    "fsp_extrapolate_device", "fsp_transport_device", "fsp_surface_scalars",
     # include/fsp_learned.h
     "fsp_net_param_count", "fsp_net_create", "fsp_net_destroy", "fsp_net_set_params", "fsp_net_get_params",
     "fsp_net_reserve", "fsp_net_features_device", "fsp_net_forward_device", "fsp_net_loss_grad_device",
@@ -70,6 +71,14 @@ class Report(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class Balance(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("volume", "delta", "band_in", "perturbation_in", "sponge_out", "residual",
+                                          "outgoing")]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class FspError(RuntimeError):
     def __init__(self, status: int, what: str = ""):
         self.status = status
@@ -87,6 +96,10 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise FileNotFoundError(
             f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    # torch first: libfsp.so then binds to the HIP runtime torch already
+    # loaded. Loaded the other way round, the process holds two HIP
+    # runtimes, and the GPU is visible to only one of them.
+    import torch  # noqa: F401
     L = C.CDLL(LIB_PATH)
     vp, sz, f32, f64, i32, u32 = C.c_void_p, C.c_size_t, C.c_float, C.c_double, C.c_int32, C.c_uint32
     L.fsp_abi_version.restype = i32
@@ -122,6 +135,18 @@ def lib():
     L.fsp_project_batch.restype = i32
     L.fsp_autotune.argtypes = [vp, vp, vp]
     L.fsp_autotune.restype = i32
+    L.fsp_step_surface_mobile_device.argtypes = [vp, vp, vp, vp, vp, C.c_uint64, u32, vp, vp]
+    L.fsp_step_surface_mobile_device.restype = i32
+    L.fsp_get_surface.argtypes = [vp, vp, vp]
+    L.fsp_get_surface.restype = i32
+    L.fsp_advect_device.argtypes = [vp, vp, vp, vp, f32, vp, vp, vp]
+    L.fsp_advect_device.restype = i32
+    L.fsp_extrapolate_device.argtypes = [vp, vp, vp, vp]
+    L.fsp_extrapolate_device.restype = i32
+    L.fsp_transport_device.argtypes = [vp, vp, vp, f32, vp, vp]
+    L.fsp_transport_device.restype = i32
+    L.fsp_surface_scalars.argtypes = [vp, vp, vp]
+    L.fsp_surface_scalars.restype = i32
     L.fsp_net_param_count.restype = u32
     L.fsp_net_create.argtypes = [i32, vp]
     L.fsp_net_create.restype = i32

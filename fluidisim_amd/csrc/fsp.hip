@@ -29,6 +29,7 @@
 #include "kernels.h"
 #include "kernels_fast.h"
 #include "kernels_coarse.h"
+#include "kernels_step.h"
 #include "learned_internal.h"
 
 using namespace fsp;
@@ -72,6 +73,11 @@ struct fsp_solver {
   fsp_net* net = nullptr;             // learned preconditioner (not owned)
   int net_phase = 0;                  // 1 while the current pass uses the network
   uint32_t net_until = 0;             // iteration at which the learned phase ends
+  // surface step (allocated on its first call): saved state, discharges, scalars
+  float *sv_u = nullptr, *sv_v = nullptr, *sv_w = nullptr, *sv_p = nullptr, *sv_eta = nullptr, *sv_rt = nullptr;
+  float *flux_x = nullptr, *flux_y = nullptr;
+  StepScalars* step_sc = nullptr;
+  StepScalars* hstep = nullptr;
   bool prof_on = false;
   std::string prof_name;
   std::vector<hipEvent_t> prof_events;
@@ -606,6 +612,7 @@ void cleanup(fsp_solver* S) {
   S->allocs.clear();
   if (S->hctl) (void)hipHostFree(S->hctl);
   if (S->hhash) (void)hipHostFree(S->hhash);
+  if (S->hstep) (void)hipHostFree(S->hstep);
   if (S->ev0) (void)hipEventDestroy(S->ev0);
   if (S->ev1) (void)hipEventDestroy(S->ev1);
   if (S->stream) (void)hipStreamDestroy(S->stream);
@@ -1281,6 +1288,193 @@ fsp_status fsp_divergence_device(fsp_solver* S, const float* d_u, const float* d
   Launch K(S);
   FSP_LAUNCH(S, k_divergence, dim3(grid_for(K.G.n)), K.st, K.G, d_u, d_v, d_w, d_out);
   FSP_HIP(hipStreamSynchronize(S->stream));
+  return FSP_OK;
+}
+
+// ---------------------------------------------------------------- the surface step
+namespace {
+
+fsp_status step_reserve(fsp_solver* S) {
+  if (S->step_sc) return FSP_OK;
+  const size_t nx = S->dom.nx, ny = S->dom.ny;
+  fsp_status st;
+  if ((st = dalloc(S, &S->sv_u, S->nu)) || (st = dalloc(S, &S->sv_v, S->nv)) || (st = dalloc(S, &S->sv_w, S->nw)) ||
+      (st = dalloc(S, &S->sv_p, S->n)) || (st = dalloc(S, &S->sv_eta, S->ncols)) ||
+      (st = dalloc(S, &S->sv_rt, S->ncols)) || (st = dalloc(S, &S->flux_x, (nx + 1) * ny)) ||
+      (st = dalloc(S, &S->flux_y, nx * (ny + 1))))
+    return st;
+  StepScalars* sc = nullptr;
+  if ((st = dalloc(S, &sc, 1))) return st;
+  FSP_HIP(hipHostMalloc((void**)&S->hstep, sizeof(StepScalars), hipHostMallocDefault));
+  S->step_sc = sc;
+  return FSP_OK;
+}
+
+// The step's scalars of the current state (flux arrays: also the wall sum),
+// copied to S->hstep.
+fsp_status step_scalars(fsp_solver* S, bool with_flux) {
+  const Fine G = S->fine();
+  k_step_reset<<<1, 1, 0, S->stream>>>(S->step_sc);
+  k_step_scalars<<<1, STEP_THREADS, 0, S->stream>>>(G, with_flux ? S->flux_x : nullptr,
+                                                      with_flux ? S->flux_y : nullptr, S->step_sc);
+  FSP_HIP(hipGetLastError());
+  FSP_HIP(hipMemcpyAsync(S->hstep, S->step_sc, sizeof(StepScalars), hipMemcpyDeviceToHost, S->stream));
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  return FSP_OK;
+}
+
+fsp_status step_copy(fsp_solver* S, float* dst, const float* src, size_t n) {
+  FSP_HIP(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToDevice, S->stream));
+  return FSP_OK;
+}
+
+fsp_status advect(fsp_solver* S, const float* u, const float* v, const float* w, float dt, float* us, float* vs,
+                  float* ws) {
+  const Fine G = S->fine();
+  const float h = 0.5f / G.dx;
+  FSP_LAUNCH(S, k_advect_h<0>, dim3(grid_for(S->nu)), S->stream, G, u, v, w, dt, h, us);
+  FSP_LAUNCH(S, k_advect_h<1>, dim3(grid_for(S->nv)), S->stream, G, u, v, w, dt, h, vs);
+  FSP_LAUNCH(S, k_advect_w, dim3(grid_for(S->nw)), S->stream, G, u, v, w, dt, h, ws);
+  FSP_HIP(hipGetLastError());
+  return FSP_OK;
+}
+
+fsp_status extrapolate(fsp_solver* S, float* u, float* v, float* w) {
+  const Fine G = S->fine();
+  FSP_LAUNCH(S, k_extrapolate, dim3(grid_for(3 * S->ncols)), S->stream, G, u, v, w);
+  FSP_HIP(hipGetLastError());
+  return FSP_OK;
+}
+
+fsp_status transport(fsp_solver* S, const float* u, const float* v, float factor) {
+  const Fine G = S->fine();
+  const size_t nx = S->dom.nx, ny = S->dom.ny;
+  FSP_LAUNCH(S, k_flux, dim3(grid_for((nx + 1) * ny + nx * (ny + 1))), S->stream, G, u, v, S->flux_x, S->flux_y);
+  FSP_LAUNCH(S, k_heights, dim3(grid_for(S->ncols)), S->stream, G, (const float*)S->flux_x, (const float*)S->flux_y,
+             factor, S->eta, S->eta_rt);
+  FSP_HIP(hipGetLastError());
+  return FSP_OK;
+}
+
+}  // namespace
+
+fsp_status fsp_step_surface_mobile_device(fsp_solver* S, float* d_u, float* d_v, float* d_w, float* d_p,
+                                          uint64_t duration_us, uint32_t max_iters, fsp_report* report,
+                                          fsp_balance* balance) {
+  if (!S || !d_u || !d_v || !d_w || !d_p) return FSP_INVALID;
+  if (!S->geom) return FSP_DOMAIN;
+  if (duration_us == 0 || duration_us > (1ull << 53)) return FSP_NOT_FINITE;
+  const double dt = (double)duration_us * 1e-6;
+  const float dx = S->dom.dx, rho = S->surf.rho, g = S->surf.g_eff;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  fsp_status st = step_reserve(S);
+  if (st) return st;
+  if ((st = step_scalars(S, false))) return st;
+  if (g <= 0.f || dt * dt * (double)g / (double)dx > 1.0 || !S->hstep->in_bounds) return FSP_DOMAIN;
+  const float scale = (float)(-(double)rho / dt);
+  const float correction = (float)(dt / (double)rho);
+  const float factor = (float)(dt / (double)dx);
+  const float advection = (float)dt;
+  if (!std::isfinite(scale) || !std::isfinite(correction) || correction == 0.f || !std::isfinite(factor) ||
+      factor == 0.f || !(advection > 0.f))
+    return FSP_NOT_FINITE;
+  const double volume_before = S->hstep->volume;
+  // state a refusal restores
+  if ((st = step_copy(S, S->sv_u, d_u, S->nu)) || (st = step_copy(S, S->sv_v, d_v, S->nv)) ||
+      (st = step_copy(S, S->sv_w, d_w, S->nw)) || (st = step_copy(S, S->sv_p, d_p, S->n)) ||
+      (st = step_copy(S, S->sv_eta, S->eta, S->ncols)) || (st = step_copy(S, S->sv_rt, S->eta_rt, S->ncols)))
+    return st;
+  auto restore = [&](fsp_status why) -> fsp_status {
+    if (step_copy(S, d_u, S->sv_u, S->nu) || step_copy(S, d_v, S->sv_v, S->nv) || step_copy(S, d_w, S->sv_w, S->nw) ||
+        step_copy(S, d_p, S->sv_p, S->n) || step_copy(S, S->eta, S->sv_eta, S->ncols) ||
+        step_copy(S, S->eta_rt, S->sv_rt, S->ncols) || hipStreamSynchronize(S->stream) != hipSuccess)
+      return FSP_DEVICE;
+    return why;
+  };
+  if ((st = advect(S, d_u, d_v, d_w, advection, S->us, S->vs, S->ws))) return restore(st);
+  fsp_report R{};
+  st = run(S, S->us, S->vs, S->ws, d_p, 1, d_u, d_v, d_w, scale, correction, 1e-6, max_iters, &R);
+  if (report) *report = R;
+  if (st) return restore(st);
+  if (R.degraded) return restore(FSP_CONVERGENCE);
+  if ((st = extrapolate(S, d_u, d_v, d_w)) || (st = transport(S, d_u, d_v, factor))) return restore(st);
+  // scalars after the step, and the finiteness of every field the step wrote
+  const Fine G = S->fine();
+  k_step_reset<<<1, 1, 0, S->stream>>>(S->step_sc);
+  k_step_scalars<<<1, STEP_THREADS, 0, S->stream>>>(G, S->flux_x, S->flux_y, S->step_sc);
+  const float* fields[] = {d_u, d_v, d_w, d_p, S->eta, S->eta_rt, S->us, S->vs, S->ws, S->b, S->r};
+  const size_t lens[] = {S->nu, S->nv, S->nw, S->n, S->ncols, S->ncols, S->nu, S->nv, S->nw, S->n, S->n};
+  for (int q = 0; q < 11; ++q) k_finite<<<grid_for(lens[q]), BLOCK, 0, S->stream>>>(fields[q], lens[q], S->step_sc);
+  FSP_HIP(hipGetLastError());
+  FSP_HIP(hipMemcpyAsync(S->hstep, S->step_sc, sizeof(StepScalars), hipMemcpyDeviceToHost, S->stream));
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  const StepScalars after = *S->hstep;
+  if (!after.finite || !std::isfinite(R.residual) || !std::isfinite(R.divergence)) return restore(FSP_NOT_FINITE);
+  if (!after.in_bounds) return restore(FSP_DOMAIN);
+  if (balance) {
+    const double perturbation_in = after.boundary * dt * (double)dx;
+    const double delta = after.volume - volume_before;
+    *balance = fsp_balance{after.volume, delta, 0.0, perturbation_in, 0.0, delta - perturbation_in, 0.0};
+  }
+  return FSP_OK;
+}
+
+fsp_status fsp_get_surface(fsp_solver* S, float* eta, float* eta_roundoff) {
+  if (!S || !eta) return FSP_INVALID;
+  if (!S->geom) return FSP_DOMAIN;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  FSP_HIP(hipMemcpyAsync(eta, S->eta, S->ncols * sizeof(float), hipMemcpyDeviceToHost, S->stream));
+  if (eta_roundoff)
+    FSP_HIP(hipMemcpyAsync(eta_roundoff, S->eta_rt, S->ncols * sizeof(float), hipMemcpyDeviceToHost, S->stream));
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  return FSP_OK;
+}
+
+fsp_status fsp_advect_device(fsp_solver* S, const float* d_u, const float* d_v, const float* d_w, float dt,
+                             float* d_us, float* d_vs, float* d_ws) {
+  if (!S || !d_u || !d_v || !d_w || !d_us || !d_vs || !d_ws) return FSP_INVALID;
+  if (!S->geom) return FSP_DOMAIN;
+  if (!std::isfinite(dt)) return FSP_NOT_FINITE;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  fsp_status st = advect(S, d_u, d_v, d_w, dt, d_us, d_vs, d_ws);
+  if (st) return st;
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  return FSP_OK;
+}
+
+fsp_status fsp_extrapolate_device(fsp_solver* S, float* d_u, float* d_v, float* d_w) {
+  if (!S || !d_u || !d_v || !d_w) return FSP_INVALID;
+  if (!S->geom) return FSP_DOMAIN;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  fsp_status st = extrapolate(S, d_u, d_v, d_w);
+  if (st) return st;
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  return FSP_OK;
+}
+
+fsp_status fsp_transport_device(fsp_solver* S, const float* d_u, const float* d_v, float factor, float* d_flux_x,
+                                float* d_flux_y) {
+  if (!S || !d_u || !d_v) return FSP_INVALID;
+  if (!S->geom) return FSP_DOMAIN;
+  if (!std::isfinite(factor)) return FSP_NOT_FINITE;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  fsp_status st = step_reserve(S);
+  if (st || (st = transport(S, d_u, d_v, factor))) return st;
+  const size_t nx = S->dom.nx, ny = S->dom.ny;
+  if (d_flux_x) FSP_HIP(hipMemcpyAsync(d_flux_x, S->flux_x, (nx + 1) * ny * 4, hipMemcpyDeviceToDevice, S->stream));
+  if (d_flux_y) FSP_HIP(hipMemcpyAsync(d_flux_y, S->flux_y, nx * (ny + 1) * 4, hipMemcpyDeviceToDevice, S->stream));
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  return FSP_OK;
+}
+
+fsp_status fsp_surface_scalars(fsp_solver* S, double* volume, int32_t* in_bounds) {
+  if (!S) return FSP_INVALID;
+  if (!S->geom) return FSP_DOMAIN;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  fsp_status st = step_reserve(S);
+  if (st || (st = step_scalars(S, false))) return st;
+  if (volume) *volume = S->hstep->volume;
+  if (in_bounds) *in_bounds = S->hstep->in_bounds;
   return FSP_OK;
 }
 

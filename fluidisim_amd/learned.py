@@ -21,7 +21,7 @@ import numpy as np
 
 from . import _lib
 from . import workloads as W
-from .volume import _ptr, _raise
+from .volume import _ptr, _raise, _torch_device_first
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "learned_weights.npz")
@@ -40,6 +40,7 @@ class Network:
     """One network on one device (fsp_net)."""
 
     def __init__(self, device: int = 0):
+        _torch_device_first(device)
         h = C.c_void_p()
         _raise(_lib.lib().fsp_net_create(device, C.byref(h)), "fsp_net_create")
         self._h = h

@@ -83,6 +83,14 @@ class Report:
                       r.backward_error, r.divergence_plain, r.ms_device, r.ms_transfer)
 
 
+def _torch_device_first(device: int):
+    """Initialises torch's state on the device before the library's first
+    HIP call: torch reports no GPU when another HIP user initialised the
+    runtime before it in the same process."""
+    import torch
+    torch.zeros(1, device=torch.device("cuda", device))
+
+
 def _ptr(a):
     if a is None:
         return None
@@ -136,6 +144,7 @@ class Volume3:
         o = options if options is not None else _lib.default_options()
         for k, v in option_kw.items():
             setattr(o, k, v)
+        _torch_device_first(o.device)
         d = _lib.Domain(domain.nx, domain.ny, domain.nz, domain.dx)
         h = C.c_void_p()
         _raise(_lib.lib().fsp_create(C.byref(d), C.byref(o), C.byref(h)), "configure")
@@ -207,6 +216,19 @@ class Volume3:
                 raise ProjectionError(Error.Shape, "set_velocity")
             dst.copy_(t.reshape(-1).to(dst.device))
 
+    def set_step_velocity(self, u, v, w):
+        """The velocity the surface step advects (u, v, w of the reference's
+        Volume3, which its set_velocity sets; delta3d.rs)."""
+        import torch
+        nu, nv, nw = self.domain.faces
+        for src, dst, n in ((u, self.u, nu), (v, self.v, nv), (w, self.w, nw)):
+            t = torch.as_tensor(src, dtype=torch.float32)
+            if t.numel() != n:
+                raise ProjectionError(Error.Shape, "set_step_velocity")
+            if not bool(torch.isfinite(t).all()):
+                raise ProjectionError(Error.NotFinite, "set_step_velocity")
+            dst.copy_(t.reshape(-1).to(dst.device))
+
     def pressure(self):
         return self.p
 
@@ -230,6 +252,77 @@ class Volume3:
                                            max_iters, C.byref(rep))
         _raise(st, "project_mobile3")
         return Report.from_c(rep)
+
+    # ------------------------------------------------------------ the surface step
+    def step_surface_mobile(self, duration_us: int, max_iters: int) -> Report:
+        """delta3d_mobile.rs:819-927: advect u, v, w, project (warm start
+        from p), extrapolate, move the heights. Returns the projection's
+        Report; the step's Balance3 is in self.balance. A refusal raises
+        ProjectionError and leaves u, v, w, p and the surface as they were."""
+        self._upload_geometry()
+        self._sync()
+        rep, bal = _lib.Report(), _lib.Balance()
+        st = _lib.lib().fsp_step_surface_mobile_device(self._h, _ptr(self.u), _ptr(self.v), _ptr(self.w),
+                                                       _ptr(self.p), int(duration_us), int(max_iters),
+                                                       C.byref(rep), C.byref(bal))
+        self.last_report = Report.from_c(rep)
+        _raise(st, "step_surface_mobile")
+        self.balance = bal.as_dict()
+        self._eta, self._eta_roundoff = self.surface()
+        return self.last_report
+
+    def surface(self):
+        """(eta, eta_roundoff) of the device state, host arrays."""
+        self._upload_geometry()
+        eta = np.empty(self.domain.columns, np.float32)
+        ero = np.empty(self.domain.columns, np.float32)
+        _raise(_lib.lib().fsp_get_surface(self._h, _ptr(eta), _ptr(ero)), "surface")
+        return eta, ero
+
+    def surface_scalars(self):
+        """(perturbation_volume, mobile_in_bounds) of the current surface
+        (delta3d_balance.rs:105-113, delta3d_mobile.rs:801-813)."""
+        self._upload_geometry()
+        vol, ok = C.c_double(), C.c_int32()
+        _raise(_lib.lib().fsp_surface_scalars(self._h, C.byref(vol), C.byref(ok)), "surface_scalars")
+        return vol.value, bool(ok.value)
+
+    def advect_for_trials(self, u, v, w, dt: float):
+        """advect_mobile3 (delta3d_mobile.rs:615-732) alone: (us, vs, ws)."""
+        import torch
+        self._upload_geometry()
+        dev = self.p.device
+        u, v, w = (torch.as_tensor(a, dtype=torch.float32, device=dev).contiguous() for a in (u, v, w))
+        us, vs, ws = torch.empty_like(u), torch.empty_like(v), torch.empty_like(w)
+        self._sync()
+        _raise(_lib.lib().fsp_advect_device(self._h, _ptr(u), _ptr(v), _ptr(w), dt, _ptr(us), _ptr(vs), _ptr(ws)),
+               "advect")
+        return us, vs, ws
+
+    def extrapolate_for_trials(self, u, v, w):
+        """extrapolate_mobile3 (delta3d_mobile.rs:345-394) alone, on copies."""
+        import torch
+        self._upload_geometry()
+        dev = self.p.device
+        u, v, w = (torch.as_tensor(a, dtype=torch.float32, device=dev).clone().contiguous() for a in (u, v, w))
+        self._sync()
+        _raise(_lib.lib().fsp_extrapolate_device(self._h, _ptr(u), _ptr(v), _ptr(w)), "extrapolate")
+        return u, v, w
+
+    def transport_for_trials(self, u, v, factor: float):
+        """transport_mobile3 (delta3d_mobile.rs:734-782) alone: moves the
+        surface; returns the discharges (flux_x, flux_y)."""
+        import torch
+        self._upload_geometry()
+        dev = self.p.device
+        nx, ny = self.domain.nx, self.domain.ny
+        u, v = (torch.as_tensor(a, dtype=torch.float32, device=dev).contiguous() for a in (u, v))
+        fx = torch.empty((nx + 1) * ny, dtype=torch.float32, device=dev)
+        fy = torch.empty(nx * (ny + 1), dtype=torch.float32, device=dev)
+        self._sync()
+        _raise(_lib.lib().fsp_transport_device(self._h, _ptr(u), _ptr(v), factor, _ptr(fx), _ptr(fy)), "transport")
+        self._eta, self._eta_roundoff = self.surface()
+        return fx, fy
 
     def apply_pressure_operator_for_trials(self, p):
         """delta3d_mobile.rs:936-949."""

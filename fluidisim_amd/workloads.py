@@ -10,6 +10,11 @@ rho = dt = 1, so scale = -rho/dt = -1 and k1 = dt/rho = 1 (b = -div u*).
                 8 solid spheres, 16-mode smooth velocity of RMS 1;
                 multigrid over every coarse level the grid allows.
 Both use Galerkin coarse operators (P^T A P / 16).
+
+  slosh         the surface step's workload (physical units): an 8 m wide
+                tank, depth 2/3 of its height, a surface raised by a
+                cosine mode, solid spheres well below the surface, water at
+                rest; rho = 1025, g = 9.81, surface ghosts from the heights.
 """
 from __future__ import annotations
 
@@ -93,3 +98,38 @@ def patches(count: int = 48, seed: int = 1):
         X, Y, Z = (int(v) for v in rng.choice(PATCH_EDGES, size=3, p=PATCH_WEIGHTS))
         out.append(tank(X, Y, Z, seed=seed * 1000 + q))
     return out
+
+
+def slosh(X: int = 64, Y: int = 32, Z: int = 64, seed: int = 1, spheres: int = 4, amplitude: float = 0.02) -> Workload:
+    """Tank of X x Z columns, Y cells high (reference nx = X, ny = Z, nz = Y),
+    dx = 8 / X m; rest level at 2/3 of the height, surface rest + amplitude *
+    H * cos(pi x / Lx) cos(pi y / Ly) with H the rest depth; `spheres` solid
+    spheres of radius 2-4 cells whose tops stay below half the rest depth (so
+    the surface stays far above every column floor); zero velocity. All
+    arithmetic of the heights in f32."""
+    nx, ny, nz = X, Z, Y
+    f32 = np.float32
+    dx = f32(8.0) / f32(nx)
+    rest = f32(nz) * dx * f32(2.0) / f32(3.0)
+    i = (np.arange(nx, dtype=f32) + f32(0.5)) / f32(nx)
+    j = (np.arange(ny, dtype=f32) + f32(0.5)) / f32(ny)
+    mode = np.outer(np.cos(f32(np.pi) * j), np.cos(f32(np.pi) * i)).astype(f32)
+    eta = (rest + f32(amplitude) * rest * mode).astype(f32).ravel()
+    rng = np.random.default_rng(seed)
+    solid = np.zeros((nz, ny, nx), np.uint8)
+    kk, jj, ii = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    top_max = float(rest) / 2.0
+    for _ in range(spheres):
+        r = rng.uniform(2.0, 4.0)
+        cx, cy = rng.uniform(r + 1, nx - r - 1), rng.uniform(r + 1, ny - r - 1)
+        cz = rng.uniform(r + 1, max(r + 1.5, top_max / float(dx) - r - 1))
+        solid[(ii + 0.5 - cx) ** 2 + (jj + 0.5 - cy) ** 2 + (kk + 0.5 - cz) ** 2 <= r * r] = 1
+    nu, nv, nw = Domain3(nx, ny, nz, float(dx)).faces
+    z = lambda n: np.zeros(n, np.float32)
+    return Workload(f"slosh_{X}x{Y}x{Z}", Domain3(nx, ny, nz, float(dx)), eta, solid.ravel(), z(nu), z(nv), z(nw),
+                    homogeneous_ghost=False, closed_lid=False, scale=float(np.float32(-1025.0 / 1e-3)),
+                    k1=float(np.float32(1e-3 / 1025.0)), rho=1025.0, g_eff=9.81, rest=float(rest),
+                    options=dict(preconditioner=_lib.PRECOND_MULTIGRID, smoother=_lib.SMOOTH_RBGS,
+                                 max_levels=-1, pre_sweeps=2, post_sweeps=2, coarse_sweeps=8,
+                                 coarse_operator=_lib.COARSE_GALERKIN),
+                    seed=seed)

@@ -186,6 +186,52 @@ fsp_status fsp_precondition_device(fsp_solver* s, const float* d_r, float* d_z);
 /* Divergence of faces (delta3d.rs:747-763, :806-823). Device pointers. */
 fsp_status fsp_divergence_device(fsp_solver* s, const float* d_u, const float* d_v, const float* d_w, float* d_out);
 
+/* Balance3 of the uncoupled surface step (delta3d_balance.rs Balance3;
+ * delta3d_mobile.rs:866-877): perturbation volume after the step, its change,
+ * the wall discharge (zero: the walls are closed) and residual = delta -
+ * perturbation_in; band_in, sponge_out and outgoing are 0 on this path. m^3. */
+typedef struct {
+  double volume, delta, band_in, perturbation_in, sponge_out, residual, outgoing;
+} fsp_balance;
+
+/* Replaces Volume3::step_surface_mobile (delta3d_mobile.rs:819-927): advects
+ * (u, v, w) over duration_us microseconds into the predicted faces
+ * (advect_mobile3, :615-732), projects them with a warm start from p
+ * (project_mobile3 with scale = -rho/dt, k1 = dt/rho, relative tolerance
+ * 1e-6), extrapolates the solved faces up the columns (:345-394) and moves
+ * the column heights by the wetted discharges (transport_mobile3,
+ * :734-782). The solver's eta / eta_roundoff (set by fsp_set_geometry) are
+ * the surface state; fsp_get_surface reads them. Device pointers, updated in
+ * place. Refusals, as the reference's: duration 0 or above 2^53 us ->
+ * FSP_NOT_FINITE; g_eff <= 0, dt^2 g / dx > 1 or a surface out of bounds
+ * (below the column floor + 2 dx or above (nz-1) dx, before or after the
+ * step) -> FSP_DOMAIN; a degraded projection -> FSP_CONVERGENCE; a
+ * non-finite field -> FSP_NOT_FINITE. A refusal after the checks restores u,
+ * v, w, p, eta and eta_roundoff bit for bit. Scope: no sparse column set,
+ * the second-order advection term off (the reference's default). */
+fsp_status fsp_step_surface_mobile_device(fsp_solver* s, float* d_u, float* d_v, float* d_w, float* d_p,
+                                          uint64_t duration_us, uint32_t max_iters, fsp_report* report,
+                                          fsp_balance* balance);
+
+/* The solver's column heights and their roundoff (Volume3::surface,
+ * delta3d.rs; host arrays of nx*ny; eta_roundoff may be NULL). */
+fsp_status fsp_get_surface(fsp_solver* s, float* eta, float* eta_roundoff);
+
+/* Stages of the surface step on their own, for tests. advect_mobile3
+ * (delta3d_mobile.rs:615-732): predicted faces from (u, v, w), dt seconds. */
+fsp_status fsp_advect_device(fsp_solver* s, const float* d_u, const float* d_v, const float* d_w, float dt,
+                             float* d_us, float* d_vs, float* d_ws);
+/* extrapolate_mobile3 (delta3d_mobile.rs:345-394), in place, current geometry. */
+fsp_status fsp_extrapolate_device(fsp_solver* s, float* d_u, float* d_v, float* d_w);
+/* transport_mobile3 (delta3d_mobile.rs:734-782): updates the solver's eta and
+ * eta_roundoff; copies the discharges ((nx+1)*ny and nx*(ny+1)) out when the
+ * flux pointers are not NULL. factor = dt/dx. */
+fsp_status fsp_transport_device(fsp_solver* s, const float* d_u, const float* d_v, float factor, float* d_flux_x,
+                                float* d_flux_y);
+/* perturbation_volume (delta3d_balance.rs:105-113) and mobile_in_bounds
+ * (delta3d_mobile.rs:801-813) of the current surface. */
+fsp_status fsp_surface_scalars(fsp_solver* s, double* volume, int32_t* in_bounds);
+
 /* Live timing of one kernel (no reference counterpart; a measurement hook).
  * fsp_profile_begin reserves max_launches event pairs and brackets every
  * launch of the kernel named `kernel` (its symbol name, e.g. "k_fgs") with

@@ -53,9 +53,134 @@ int vcycle(const Geometry* g, const Options* o, const float* r, double* z) {
   return OK;
 }
 
+template <class T>
+int step_surface(const Geometry* g, const Options* o, float* u, float* v, float* w, float* p, float* eta,
+                 float* eta_roundoff, uint32_t steps, uint64_t duration_us, uint32_t max_iters, Report* reps,
+                 double* balances, uint32_t* done) {
+  Geometry g2 = *g;
+  g2.eta = eta;
+  g2.eta_roundoff = eta_roundoff;
+  Volume<T> vol;
+  Status st = setup(vol, &g2, o);
+  if (st) return st;
+  for (size_t q = 0; q < vol.nu(); ++q) vol.u[q] = (T)u[q];
+  for (size_t q = 0; q < vol.nv(); ++q) vol.v[q] = (T)v[q];
+  for (size_t q = 0; q < vol.nw(); ++q) vol.w[q] = (T)w[q];
+  for (size_t q = 0; q < vol.cells(); ++q) vol.p[q] = (T)p[q];
+  uint32_t n = 0;
+  for (; n < steps; ++n) {
+    Report r{};
+    typename Volume<T>::Balance b{};
+    st = vol.step_surface_mobile(duration_us, max_iters, r, b);
+    if (reps) reps[n] = r;
+    if (st) break;
+    if (balances) {
+      const double bb[7] = {b.volume, b.delta, b.band_in, b.perturbation_in, b.sponge_out, b.residual, b.outgoing};
+      std::copy(bb, bb + 7, balances + 7 * (size_t)n);
+    }
+  }
+  if (done) *done = n;
+  for (size_t q = 0; q < vol.nu(); ++q) u[q] = (float)vol.u[q];
+  for (size_t q = 0; q < vol.nv(); ++q) v[q] = (float)vol.v[q];
+  for (size_t q = 0; q < vol.nw(); ++q) w[q] = (float)vol.w[q];
+  for (size_t q = 0; q < vol.cells(); ++q) p[q] = (float)vol.p[q];
+  for (size_t q = 0; q < vol.nx * vol.ny; ++q) {
+    eta[q] = (float)vol.eta[q];
+    eta_roundoff[q] = (float)vol.eta_roundoff[q];
+  }
+  return st;
+}
+
+Options no_multigrid() {
+  Options opt = default_options();
+  opt.multigrid = 0;
+  return opt;
+}
+
 }  // namespace
 
 extern "C" {
+
+// Surface steps (delta3d_mobile.rs:819-927), `steps` of them in a row; u, v,
+// w, p, eta, eta_roundoff in and out (g's eta pointers are ignored). Returns
+// the status of the last step taken; *done = steps completed; balances: 7
+// doubles per step (Balance3 order: volume, delta, band_in,
+// perturbation_in, sponge_out, residual, outgoing).
+int orc_step_surface(int precision, const Geometry* g, const Options* o, float* u, float* v, float* w, float* p,
+                     float* eta, float* eta_roundoff, uint32_t steps, uint64_t duration_us, uint32_t max_iters,
+                     Report* reps, double* balances, uint32_t* done) {
+  if (precision == 64)
+    return step_surface<double>(g, o, u, v, w, p, eta, eta_roundoff, steps, duration_us, max_iters, reps, balances,
+                                done);
+  return step_surface<float>(g, o, u, v, w, p, eta, eta_roundoff, steps, duration_us, max_iters, reps, balances, done);
+}
+
+// Advection alone (delta3d_mobile.rs:615-732), f32.
+int orc_advect(const Geometry* g, const float* u, const float* v, const float* w, float dt, float* us, float* vs,
+               float* ws) {
+  Volume<float> vol;
+  const Options opt = no_multigrid();
+  Status st = setup(vol, g, &opt);
+  if (st) return st;
+  std::copy(u, u + vol.nu(), vol.u.begin());
+  std::copy(v, v + vol.nv(), vol.v.begin());
+  std::copy(w, w + vol.nw(), vol.w.begin());
+  vol.advect_mobile3(dt);
+  std::copy(vol.us.begin(), vol.us.end(), us);
+  std::copy(vol.vs.begin(), vol.vs.end(), vs);
+  std::copy(vol.ws.begin(), vol.ws.end(), ws);
+  return OK;
+}
+
+// Extrapolation alone (delta3d_mobile.rs:345-394), in place, f32.
+int orc_extrapolate(const Geometry* g, float* u, float* v, float* w) {
+  Volume<float> vol;
+  const Options opt = no_multigrid();
+  Status st = setup(vol, g, &opt);
+  if (st) return st;
+  std::copy(u, u + vol.nu(), vol.u.begin());
+  std::copy(v, v + vol.nv(), vol.v.begin());
+  std::copy(w, w + vol.nw(), vol.w.begin());
+  vol.extrapolate_mobile3();
+  std::copy(vol.u.begin(), vol.u.end(), u);
+  std::copy(vol.v.begin(), vol.v.end(), v);
+  std::copy(vol.w.begin(), vol.w.end(), w);
+  return OK;
+}
+
+// Height transport alone (delta3d_mobile.rs:734-782), f32: new eta and
+// eta_roundoff, the discharges ((nx+1)*ny and nx*(ny+1)).
+int orc_transport(const Geometry* g, const float* u, const float* v, float transport, float* eta_out,
+                  float* eta_roundoff_out, float* flux_x, float* flux_y) {
+  Volume<float> vol;
+  const Options opt = no_multigrid();
+  Status st = setup(vol, g, &opt);
+  if (st) return st;
+  std::copy(u, u + vol.nu(), vol.u.begin());
+  std::copy(v, v + vol.nv(), vol.v.begin());
+  vol.transport_mobile3(transport);
+  std::copy(vol.eta.begin(), vol.eta.end(), eta_out);
+  std::copy(vol.eta_roundoff.begin(), vol.eta_roundoff.end(), eta_roundoff_out);
+  if (flux_x) std::copy(vol.flux_x.begin(), vol.flux_x.end(), flux_x);
+  if (flux_y) std::copy(vol.flux_y.begin(), vol.flux_y.end(), flux_y);
+  return OK;
+}
+
+// delta3d_mobile.rs:801-813: 1 when the surface is within bounds.
+int orc_mobile_in_bounds(const Geometry* g) {
+  Volume<float> vol;
+  const Options opt = no_multigrid();
+  if (setup(vol, g, &opt)) return -1;
+  return vol.mobile_in_bounds() ? 1 : 0;
+}
+
+// delta3d_balance.rs:105-113
+double orc_perturbation_volume(const Geometry* g) {
+  Volume<float> vol;
+  const Options opt = no_multigrid();
+  if (setup(vol, g, &opt)) return std::nan("");
+  return vol.perturbation_volume();
+}
 
 int orc_project(int precision, const Geometry* g, const Options* o, const float* us, const float* vs, const float* ws,
                 const float* p_warm, float scale, float k1, uint32_t max_iters, float* p_out, float* u_out,

@@ -1040,6 +1040,219 @@ template <class T> class Volume {
     rep.divergence_plain = d_plain;
     return OK;
   }
+
+  // ---------------------------------------------------------------- the surface step
+  // The steps either side of the projection in step_surface_mobile
+  // (delta3d_mobile.rs:819-927). Scope: no sparse column set
+  // (sparse_outside3 is false, delta3d_sparse.rs:306-309) and the second-order
+  // advection term off (its default, delta3d_advection.rs:31-36).
+  std::vector<T> flux_x, flux_y;  // (nx+1)*ny and nx*(ny+1) wetted-column discharges
+
+  static T clamp01(T x) { return x < T(0) ? T(0) : (x > T(1) ? T(1) : x); }  // Rust f32::clamp (NaN passes)
+
+  // delta3d_mobile.rs:615-732: centred MAC advection of u, v, w into us, vs,
+  // ws (Euler step). Closed faces and the faces on the box walls keep their
+  // value; the top w face is not advected.
+  void advect_mobile3(T dt) {
+    const T h = T(0.5) / dx;
+    us = u; vs = v; ws = w;
+    for (int axis = 0; axis < 2; ++axis) {
+      const size_t na = axis == 0 ? nx : ny, nb = axis == 0 ? ny : nx;
+      auto val = [&](size_t a, size_t b, size_t k) { return axis == 0 ? u[fu(a, b, k)] : v[fv(b, a, k)]; };
+      auto vertical = [&](size_t a, size_t b, size_t k) { return axis == 0 ? w[fw(a, b, k)] : w[fw(b, a, k)]; };
+      auto transverse = [&](size_t a, size_t b, size_t k) { return axis == 0 ? v[fv(a, b, k)] : u[fu(b, a, k)]; };
+      for (size_t b = 0; b < nb; ++b)
+        for (size_t a = 1; a < na; ++a)
+          for (size_t k = 0; k < nz; ++k) {
+            const size_t i = axis == 0 ? a : b, j = axis == 0 ? b : a;
+            const size_t face = axis == 0 ? fu(i, j, k) : fv(i, j, k);
+            if (open3(axis, face) == T(0)) continue;
+            const T uc = val(a, b, k);
+            const T ux = (val(a + 1, b, k) - val(a - 1, b, k)) * h;
+            const T up = k + 1 < nz ? val(a, b, k + 1) : uc;
+            const T dn = k > 0 ? val(a, b, k - 1) : uc;
+            const T uz = (up - dn) * h;
+            const T wc = T(0.25) * (vertical(a - 1, b, k) + vertical(a, b, k) + vertical(a - 1, b, k + 1) +
+                                    vertical(a, b, k + 1));
+            const T rt = b + 1 < nb ? val(a, b + 1, k) : uc;
+            const T lf = b > 0 ? val(a, b - 1, k) : uc;
+            const T uy = (rt - lf) * h;
+            const T vc = T(0.25) * (transverse(a - 1, b, k) + transverse(a, b, k) + transverse(a - 1, b + 1, k) +
+                                    transverse(a, b + 1, k));
+            const T next = uc - dt * (uc * ux + wc * uz + vc * uy);
+            if (axis == 0) us[face] = next;
+            else vs[face] = next;
+          }
+    }
+    for (size_t j = 0; j < ny; ++j)
+      for (size_t i = 0; i < nx; ++i)
+        for (size_t k = 1; k < nz; ++k) {
+          const size_t f = fw(i, j, k);
+          if (open3(2, f) == T(0)) continue;
+          const T wc = w[f];
+          const T rt = i + 1 < nx ? w[fw(i + 1, j, k)] : wc;
+          const T lf = i > 0 ? w[fw(i - 1, j, k)] : wc;
+          const T wx = (rt - lf) * h;
+          const T wz = (w[fw(i, j, k + 1)] - w[fw(i, j, k - 1)]) * h;
+          const T uc = T(0.25) * (u[fu(i, j, k - 1)] + u[fu(i + 1, j, k - 1)] + u[fu(i, j, k)] + u[fu(i + 1, j, k)]);
+          const T bk = j + 1 < ny ? w[fw(i, j + 1, k)] : wc;
+          const T fr = j > 0 ? w[fw(i, j - 1, k)] : wc;
+          const T wy = (bk - fr) * h;
+          const T vc = T(0.25) * (v[fv(i, j, k - 1)] + v[fv(i, j + 1, k - 1)] + v[fv(i, j, k)] + v[fv(i, j + 1, k)]);
+          ws[f] = wc - dt * (uc * wx + wc * wz + vc * wy);
+        }
+  }
+
+  // delta3d_mobile.rs:345-394: up every column, each open face that is not
+  // solved (open, between two non-solid cells, at least one of them wet; a w
+  // face: open above a wet cell) takes the velocity of the last solved face
+  // below it. Closed faces are skipped and do not interrupt the run.
+  void extrapolate_mobile3() {
+    for (size_t j = 0; j < ny; ++j)
+      for (size_t i = 0; i < nx; ++i)
+        for (int axis = 0; axis < 3; ++axis) {
+          if ((axis == 0 && i == 0) || (axis == 1 && j == 0)) continue;
+          bool have = false;
+          T last = T(0);
+          const size_t k0 = axis == 2 ? 1 : 0, k1 = axis == 2 ? nz + 1 : nz;
+          for (size_t k = k0; k < k1; ++k) {
+            size_t f;
+            bool solved;
+            if (axis == 0) {
+              f = fu(i, j, k);
+              solved = open3(0, f) > T(0) && !solid3(c(i - 1, j, k)) && !solid3(c(i, j, k)) &&
+                       (wet3(i - 1, j, k) || wet3(i, j, k));
+            } else if (axis == 1) {
+              f = fv(i, j, k);
+              solved = open3(1, f) > T(0) && !solid3(c(i, j - 1, k)) && !solid3(c(i, j, k)) &&
+                       (wet3(i, j - 1, k) || wet3(i, j, k));
+            } else {
+              f = fw(i, j, k);
+              solved = open3(2, f) > T(0) && wet3(i, j, k - 1);
+            }
+            if (open3(axis, f) == T(0)) continue;
+            std::vector<T>& field = axis == 0 ? u : axis == 1 ? v : w;
+            if (solved) { last = field[f]; have = true; }
+            else if (have) field[f] = last;
+          }
+        }
+  }
+
+  // delta3d_mobile.rs:734-782: wetted discharges through the inner vertical
+  // faces from eta^n (every discharge before any height changes), then each
+  // column's height moves by -transport * (flux divergence) with the
+  // compensated (Kahan-style) roundoff carried in eta_roundoff.
+  void transport_mobile3(T transport) {
+    flux_x.assign((nx + 1) * ny, T(0));
+    flux_y.assign(nx * (ny + 1), T(0));
+    for (size_t j = 0; j < ny; ++j)
+      for (size_t i = 0; i < nx; ++i)
+        for (int axis = 0; axis < 2; ++axis) {
+          if ((axis == 0 && i == 0) || (axis == 1 && j == 0)) continue;
+          const size_t x = axis == 0 ? i - 1 : i, y = axis == 0 ? j : j - 1;
+          const T surface = T(0.5) * (eta[col(x, y)] + eta[col(i, j)]);
+          T q = T(0);
+          for (size_t k = 0; k < nz; ++k) {
+            const T wet = clamp01((surface - (T)k * dx) / dx);
+            if (wet == T(0)) break;
+            T vel, a;
+            if (axis == 0) { const size_t f = fu(i, j, k); vel = u[f]; a = open3(0, f); }
+            else { const size_t f = fv(i, j, k); vel = v[f]; a = open3(1, f); }
+            q += a * vel * dx * wet;
+          }
+          if (axis == 0) flux_x[j * (nx + 1) + i] = q;
+          else flux_y[j * nx + i] = q;
+        }
+    for (size_t j = 0; j < ny; ++j)
+      for (size_t i = 0; i < nx; ++i) {
+        const size_t cc = col(i, j);
+        const T x = flux_x[j * (nx + 1) + i + 1] - flux_x[j * (nx + 1) + i];
+        const T y = flux_y[(j + 1) * nx + i] - flux_y[j * nx + i];
+        const T increment = -transport * (x + y) - eta_roundoff[cc];
+        const T height = eta[cc] + increment;
+        eta_roundoff[cc] = (height - eta[cc]) - increment;
+        eta[cc] = height;
+      }
+  }
+
+  // delta3d_balance.rs:105-113
+  double perturbation_volume() const {
+    const double area = (double)dx * (double)dx, r = (double)rest;
+    double total = 0.0;
+    for (size_t q = 0; q < eta.size(); ++q) total += ((double)eta[q] - (double)eta_roundoff[q]) - r;
+    return total * area;
+  }
+
+  // delta3d_mobile.rs:784-800: discharge through the four outer walls,
+  // positive inwards, in m^3 over dt (zero by construction here).
+  double boundary_flux_mobile3(double dt) const {
+    double total = 0.0;
+    for (size_t j = 0; j < ny; ++j) total += (double)flux_x[j * (nx + 1)] - (double)flux_x[j * (nx + 1) + nx];
+    for (size_t i = 0; i < nx; ++i) total += (double)flux_y[i] - (double)flux_y[ny * nx + i];
+    return total * dt * (double)dx;
+  }
+
+  // delta3d_mobile.rs:801-813 with the floor of delta3d_cut.rs:429-447: the
+  // surface stays at least two cells above the top of the highest solid cell
+  // of its column (the flat bottom at 0 otherwise) and at most at (nz-1) dx.
+  bool mobile_in_bounds() const {
+    const T top = (T)(nz - 1) * dx;
+    for (size_t j = 0; j < ny; ++j)
+      for (size_t i = 0; i < nx; ++i) {
+        T floor = T(0);
+        for (size_t k = 0; k < nz; ++k)
+          if (solid3(c(i, j, k))) { const T t = (T)(k + 1) * dx; if (t > floor) floor = t; }
+        const T e = height3(i, j);
+        if (!(e >= floor + T(2) * dx && e <= top)) return false;
+      }
+    return true;
+  }
+
+  // delta3d_balance.rs Balance3 of the uncoupled step (delta3d_mobile.rs:866-877)
+  struct Balance { double volume, delta, band_in, perturbation_in, sponge_out, residual, outgoing; };
+
+  static bool all_finite(const std::vector<T>& f) {
+    for (const T& x : f) if (!std::isfinite(x)) return false;
+    return true;
+  }
+
+  // delta3d_mobile.rs:819-927: advect, project (warm start from p), extrapolate,
+  // transport, balance; a refusal restores u, v, w, p, eta and eta_roundoff.
+  Status step_surface_mobile(uint64_t duration_us, uint32_t max_iters, Report& rep, Balance& bal) {
+    if (duration_us == 0 || duration_us > (1ull << 53)) return ERR_NOT_FINITE;
+    const double dt = (double)duration_us * 1e-6;
+    const float dxf = (float)dx, rhof = (float)rho, gf = (float)g;
+    if (gf <= 0.f || dt * dt * (double)gf / (double)dxf > 1.0 || !mobile_in_bounds()) return ERR_DOMAIN;
+    const float scale = (float)(-(double)rhof / dt);
+    const float correction = (float)(dt / (double)rhof);
+    const float transport = (float)(dt / (double)dxf);
+    const float advection = (float)dt;
+    if (!std::isfinite(scale) || !std::isfinite(correction) || correction == 0.f || !std::isfinite(transport) ||
+        transport == 0.f || !(advection > 0.f))
+      return ERR_NOT_FINITE;
+    const std::vector<T> su = u, sv = v, sw = w, sp = p, se = eta, sr = eta_roundoff;
+    const double volume_before = perturbation_volume();
+    auto body = [&]() -> Status {
+      advect_mobile3((T)advection);
+      Status st = project_mobile3((T)scale, (T)correction, max_iters, rep);
+      if (st) return st;
+      if (rep.degraded) return ERR_CONVERGENCE;
+      extrapolate_mobile3();
+      transport_mobile3((T)transport);
+      const double volume = perturbation_volume();
+      const double delta = volume - volume_before;
+      const double pin = boundary_flux_mobile3(dt);
+      for (const std::vector<T>* f : {&u, &v, &w, &p, &eta, &eta_roundoff, &us, &vs, &ws, &rhs, &res, &dir, &tmp, &prec})
+        if (!all_finite(*f)) return ERR_NOT_FINITE;
+      if (!std::isfinite(rep.residual) || !std::isfinite(rep.divergence)) return ERR_NOT_FINITE;
+      if (!mobile_in_bounds()) return ERR_DOMAIN;
+      bal = Balance{volume, delta, 0.0, pin, 0.0, delta - pin, 0.0};
+      return OK;
+    };
+    const Status st = body();
+    if (st) { u = su; v = sv; w = sw; p = sp; eta = se; eta_roundoff = sr; }
+    return st;
+  }
 };
 
 }  // namespace orc

@@ -83,6 +83,13 @@ def lib():
         L.orc_roundoff_3d.restype = C.c_float
         L.orc_fnv_bytes.argtypes = [C.c_char_p, sz]
         L.orc_fnv_bytes.restype = C.c_uint64
+        L.orc_step_surface.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, u32, C.c_uint64, u32, vp, vp, vp]
+        L.orc_advect.argtypes = [vp, vp, vp, vp, f32, vp, vp, vp]
+        L.orc_extrapolate.argtypes = [vp, vp, vp, vp]
+        L.orc_transport.argtypes = [vp, vp, vp, f32, vp, vp, vp, vp]
+        L.orc_mobile_in_bounds.argtypes = [vp]
+        L.orc_perturbation_volume.argtypes = [vp]
+        L.orc_perturbation_volume.restype = C.c_double
         _lib = L
     return _lib
 
@@ -300,3 +307,65 @@ def s385_problem(nx: int) -> tuple[Problem, float, float]:
     scale = float(np.float32(-1025.0 / dt))
     k1 = float(np.float32(dt / 1025.0))
     return prob, scale, k1
+
+
+# ---------------------------------------------------------------- the surface step
+ERR_DOMAIN, ERR_NOT_FINITE, ERR_SHAPE, ERR_CONVERGENCE = 1, 2, 3, 4
+BALANCE_FIELDS = ("volume", "delta", "band_in", "perturbation_in", "sponge_out", "residual", "outgoing")
+
+
+def step_surface(prob: Problem, u, v, w, p, steps, duration_us, max_iters=4000, opts=None, precision=32):
+    """`steps` surface steps (delta3d_mobile.rs:819-927) from (u, v, w, p,
+    prob.eta, prob.eta_roundoff). Returns a dict: status of the last step,
+    steps done, per-step reports and balances, and the final fields."""
+    g = prob.geometry()
+    o = opts if opts is not None else options()
+    f = lambda a: np.array(a, np.float32, copy=True)
+    u, v, w, p = f(u), f(v), f(w), f(p)
+    eta = f(prob.eta)
+    ero = f(prob.eta_roundoff) if prob.eta_roundoff is not None else np.zeros_like(eta)
+    reps = (Report * max(1, steps))()
+    bal = np.zeros((max(1, steps), 7), np.float64)
+    done = C.c_uint32()
+    st = lib().orc_step_surface(precision, C.byref(g), C.byref(o), _ptr(u), _ptr(v), _ptr(w), _ptr(p), _ptr(eta),
+                                _ptr(ero), steps, duration_us, max_iters, reps, _ptr(bal), C.byref(done))
+    n = done.value
+    return {"status": st, "done": n, "reports": [reps[q].as_dict() for q in range(min(steps, n + (st != 0)))],
+            "balances": [dict(zip(BALANCE_FIELDS, bal[q])) for q in range(n)],
+            "u": u, "v": v, "w": w, "p": p, "eta": eta, "eta_roundoff": ero}
+
+
+def advect(prob: Problem, u, v, w, dt):
+    """delta3d_mobile.rs:615-732: (us, vs, ws)."""
+    g = prob.geometry()
+    u, v, w = (np.ascontiguousarray(a, np.float32) for a in (u, v, w))
+    us, vs, ws = np.empty_like(u), np.empty_like(v), np.empty_like(w)
+    assert lib().orc_advect(C.byref(g), _ptr(u), _ptr(v), _ptr(w), dt, _ptr(us), _ptr(vs), _ptr(ws)) == 0
+    return us, vs, ws
+
+
+def extrapolate(prob: Problem, u, v, w):
+    """delta3d_mobile.rs:345-394 on copies of (u, v, w)."""
+    g = prob.geometry()
+    u, v, w = (np.array(a, np.float32, copy=True) for a in (u, v, w))
+    assert lib().orc_extrapolate(C.byref(g), _ptr(u), _ptr(v), _ptr(w)) == 0
+    return u, v, w
+
+
+def transport(prob: Problem, u, v, factor):
+    """delta3d_mobile.rs:734-782: (eta, eta_roundoff, flux_x, flux_y)."""
+    g = prob.geometry()
+    u, v = (np.ascontiguousarray(a, np.float32) for a in (u, v))
+    nx, ny = prob.nx, prob.ny
+    eta, ero = np.zeros(nx * ny, np.float32), np.zeros(nx * ny, np.float32)
+    fx, fy = np.zeros((nx + 1) * ny, np.float32), np.zeros(nx * (ny + 1), np.float32)
+    assert lib().orc_transport(C.byref(g), _ptr(u), _ptr(v), factor, _ptr(eta), _ptr(ero), _ptr(fx), _ptr(fy)) == 0
+    return eta, ero, fx, fy
+
+
+def mobile_in_bounds(prob: Problem):
+    return lib().orc_mobile_in_bounds(C.byref(prob.geometry())) == 1
+
+
+def perturbation_volume(prob: Problem):
+    return lib().orc_perturbation_volume(C.byref(prob.geometry()))
