@@ -60,6 +60,7 @@ def parse():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--batch", type=int, default=48, help="patches of the batched line (0: skip)")
     ap.add_argument("--batch-concurrency", default="1,4", help="concurrency levels timed for the batched line")
+    ap.add_argument("--no-step", action="store_true", help="skip the surface-step line")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the 256 MiB device copy (counter calibration) and exit")
     return ap.parse_args()
@@ -197,6 +198,28 @@ def learned_fcg(wl, args, torch, _lib, W, vcycle):
             "training": trained}
 
 
+def surface_step_line(args, torch, W):
+    """The whole surface step (advect, project with a warm start,
+    extrapolate, transport, balance and checks) on the sloshing tank of the
+    same size, dt = 1 ms, timed over consecutive steps."""
+    X, Y, Z = (int(v) for v in args.size.lower().split("x"))
+    wl = W.slosh(X, Y, Z, seed=args.seed)
+    vol = W.build_volume(wl)
+    for _ in range(max(1, args.warmup)):
+        vol.step_surface_mobile(1000, 4000)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = [vol.step_surface_mobile(1000, 4000) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+    log(f"[bench] surface step ({wl.name}): {ms:.3f} ms/step, iterations {[r.iterations for r in reps]}")
+    out = {"workload": wl.name, "cells": wl.domain.cells, "dt_us": 1000, "ms_per_step": round(ms, 4),
+           "iterations": [r.iterations for r in reps], "balance_residual": vol.balance["residual"],
+           "steps_before_timing": max(1, args.warmup)}
+    vol.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -280,6 +303,8 @@ def main():
         learned = learned_fcg(wl, args, torch, _lib, W, vcycle=True)
         learned_sweeps = learned_fcg(wl, args, torch, _lib, W, vcycle=False)
 
+    step = None if args.no_step else surface_step_line(args, torch, W)
+
     cpu = None
     if not args.no_cpu_baseline:
         pair = tuple(int(v) for v in args.cpu_iters.split(","))
@@ -302,6 +327,7 @@ def main():
         "learned_fcg": learned,
         "learned_fcg_sweeps": learned_sweeps,
         "batched": batched,
+        "surface_step": step,
     }
     print(json.dumps(line), flush=True)
     vol.close()

@@ -45,6 +45,7 @@ struct fsp_solver {
   fsp_surface surf{};
   uint8_t *solid = nullptr, *ou = nullptr, *ov = nullptr, *ow = nullptr, *mask = nullptr;
   float *eta = nullptr, *eta_rt = nullptr;
+  float* col_floor = nullptr;  // per column: top of its highest solid cell (0 without one)
   uint16_t* gk0 = nullptr;  // per column: lowest ghost row
   float* gtab = nullptr;    // per column: 1/theta of G_SLOTS ghost rows
   float *b = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *zt = nullptr, *d0 = nullptr,
@@ -660,8 +661,9 @@ fsp_status backward_error(fsp_solver* S, const double* p, double& be) {
 }
 
 fsp_status fingerprint(fsp_solver* S, const double* p, uint64_t& h) {
-  const size_t chunks = (S->n + 1023) / 1024;
-  FSP_LAUNCH(S, k_hash_chunks, dim3(grid_for(chunks)), S->stream, p, S->n, S->hashes);
+  // one partial per block; S->hashes holds (n + 1023) / 1024 of them
+  const size_t chunks = std::min((size_t)grid_for(S->n), (S->n + 1023) / 1024);
+  FSP_LAUNCH(S, k_hash_chunks, dim3((unsigned)chunks), S->stream, p, S->n, S->hashes);
   hipLaunchKernelGGL(k_hash_combine, dim3(1), dim3(64), 0, S->stream, S->hashes, chunks, S->hash_out);
   FSP_HIP(hipMemcpyAsync(S->hhash, S->hash_out, sizeof(unsigned long long), hipMemcpyDeviceToHost, S->stream));
   FSP_HIP(hipStreamSynchronize(S->stream));
@@ -981,6 +983,7 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
   if ((st = dalloc(S, &S->solid, S->n)) || (st = dalloc(S, &S->ou, S->nu)) || (st = dalloc(S, &S->ov, S->nv)) ||
       (st = dalloc(S, &S->ow, S->nw)) || (st = dalloc(S, &S->mask, S->n)) || (st = dalloc(S, &S->eta, S->ncols)) ||
       (st = dalloc(S, &S->eta_rt, S->ncols)) || (st = dalloc(S, &S->gk0, S->ncols)) ||
+      (st = dalloc(S, &S->col_floor, S->ncols)) ||
       (st = dalloc(S, &S->gtab, S->ncols * (size_t)G_SLOTS * 8)))
     return fail(st);
   float** cellbufs[] = {&S->b, &S->r, &S->p, &S->q, &S->z, &S->zt, &S->d0, &S->d1};
@@ -1137,7 +1140,15 @@ fsp_status fsp_set_geometry(fsp_solver* S, const fsp_surface* surf, const float*
           hw[f] = val;
         }
   }
+  // column floors of the surface-step bounds (the reference's cut geometry
+  // raises a column's floor to the top of the highest cell a solid touches,
+  // delta3d_cut.rs:429-447)
+  std::vector<float> floors(S->ncols, 0.f);
+  for (size_t k = 0; k < nz; ++k)
+    for (size_t c = 0; c < S->ncols; ++c)
+      if (hs[k * S->ncols + c]) floors[c] = (float)(k + 1) * S->dom.dx;
   FSP_HIP(hipSetDevice(S->opt.device));
+  FSP_HIP(hipMemcpyAsync(S->col_floor, floors.data(), S->ncols * sizeof(float), hipMemcpyHostToDevice, S->stream));
   FSP_HIP(hipMemcpyAsync(S->solid, hs.data(), S->n, hipMemcpyHostToDevice, S->stream));
   FSP_HIP(hipMemcpyAsync(S->ou, hu.data(), S->nu, hipMemcpyHostToDevice, S->stream));
   FSP_HIP(hipMemcpyAsync(S->ov, hv.data(), S->nv, hipMemcpyHostToDevice, S->stream));
@@ -1315,7 +1326,7 @@ fsp_status step_reserve(fsp_solver* S) {
 fsp_status step_scalars(fsp_solver* S, bool with_flux) {
   const Fine G = S->fine();
   k_step_reset<<<1, 1, 0, S->stream>>>(S->step_sc);
-  k_step_scalars<<<1, STEP_THREADS, 0, S->stream>>>(G, with_flux ? S->flux_x : nullptr,
+  k_step_scalars<<<1, STEP_THREADS, 0, S->stream>>>(G, S->col_floor, with_flux ? S->flux_x : nullptr,
                                                       with_flux ? S->flux_y : nullptr, S->step_sc);
   FSP_HIP(hipGetLastError());
   FSP_HIP(hipMemcpyAsync(S->hstep, S->step_sc, sizeof(StepScalars), hipMemcpyDeviceToHost, S->stream));
@@ -1401,7 +1412,7 @@ fsp_status fsp_step_surface_mobile_device(fsp_solver* S, float* d_u, float* d_v,
   // scalars after the step, and the finiteness of every field the step wrote
   const Fine G = S->fine();
   k_step_reset<<<1, 1, 0, S->stream>>>(S->step_sc);
-  k_step_scalars<<<1, STEP_THREADS, 0, S->stream>>>(G, S->flux_x, S->flux_y, S->step_sc);
+  k_step_scalars<<<1, STEP_THREADS, 0, S->stream>>>(G, S->col_floor, S->flux_x, S->flux_y, S->step_sc);
   const float* fields[] = {d_u, d_v, d_w, d_p, S->eta, S->eta_rt, S->us, S->vs, S->ws, S->b, S->r};
   const size_t lens[] = {S->nu, S->nv, S->nw, S->n, S->ncols, S->ncols, S->nu, S->nv, S->nw, S->n, S->n};
   for (int q = 0; q < 11; ++q) k_finite<<<grid_for(lens[q]), BLOCK, 0, S->stream>>>(fields[q], lens[q], S->step_sc);

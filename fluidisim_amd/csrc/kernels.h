@@ -1106,32 +1106,36 @@ __global__ void __launch_bounds__(BLOCK) k_backward(Fine G, const double* p, con
 // Fingerprint of p: FNV-1a 64 over the bytes of chunks of 1024 values, then over the
 // chunk hashes in order. It detects a bitwise repeat of p like the
 // reference's sequential FNV-1a (delta3d.rs:1360-1369).
+// State fingerprint of the pressure for the stall test of the pass loop (a
+// repeated state ends it; the reference hashes p with FNV-1a,
+// delta3d_mobile.rs:575): every value's bits, salted with its index, go
+// through the splitmix64 finaliser (Steele, Lea and Flood 2014; Vigna's
+// constants) and the results are summed mod 2^64 -- order-independent, so
+// blocks reduce in parallel. Equal states give equal fingerprints. One
+// partial per block into hashes[blockIdx.x].
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
 __global__ void __launch_bounds__(BLOCK) k_hash_chunks(const double* p, size_t n, unsigned long long* hashes) {
-  const size_t chunks = (n + 1023) / 1024;
-  FSP_FOR(q, chunks) {
-    unsigned long long h = 0xcbf29ce484222325ull;
-    const size_t e = (q + 1) * 1024 < n ? (q + 1) * 1024 : n;
-    for (size_t s = q * 1024; s < e; ++s) {
-      const unsigned long long bits = (unsigned long long)__double_as_longlong(p[s]);
-      for (int b = 0; b < 8; ++b) {
-        h ^= (unsigned long long)((bits >> (8 * b)) & 0xffu);
-        h *= 0x100000001b3ull;
-      }
-    }
-    hashes[q] = h;
+  __shared__ unsigned long long part[BLOCK];
+  unsigned long long h = 0;
+  for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (size_t)gridDim.x * blockDim.x)
+    h += mix64((unsigned long long)__double_as_longlong(p[s]) ^ ((unsigned long long)s * 0x9e3779b97f4a7c15ull));
+  part[threadIdx.x] = h;
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) hashes[blockIdx.x] = part[0];
 }
 __global__ void k_hash_combine(const unsigned long long* hashes, size_t chunks, unsigned long long* out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  unsigned long long h = 0xcbf29ce484222325ull;
-  for (size_t q = 0; q < chunks; ++q) {
-    unsigned long long x = hashes[q];
-    for (int b = 0; b < 8; ++b) {
-      h ^= (x >> (8 * b)) & 0xffull;
-      h *= 0x100000001b3ull;
-    }
-  }
-  *out = h;
+  unsigned long long h = 0;
+  for (size_t q = 0; q < chunks; ++q) h += hashes[q];
+  *out = mix64(h);
 }
 
 // Host-side parameters of a CG pass, written on the stream.

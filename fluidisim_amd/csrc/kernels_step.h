@@ -236,11 +236,11 @@ struct StepScalars {
 
 constexpr int STEP_THREADS = 1024;
 
-// volume and in_bounds into s[slot]; with flux arrays, also the boundary sum.
-// The floor of a column is the top of its highest solid cell (0 without one),
-// as the reference's cut geometry raises it (delta3d_cut.rs:429-447).
-__global__ void __launch_bounds__(STEP_THREADS) k_step_scalars(Fine G, const float* flux_x, const float* flux_y,
-                                                               StepScalars* s) {
+// volume and in_bounds into *s; with flux arrays, also the wall sum. floor:
+// per column, the top of its highest solid cell (0 without one), as the
+// reference's cut geometry raises it (delta3d_cut.rs:429-447).
+__global__ void __launch_bounds__(STEP_THREADS) k_step_scalars(Fine G, const float* floor, const float* flux_x,
+                                                               const float* flux_y, StepScalars* s) {
 #pragma clang fp contract(off)
   __shared__ double red[STEP_THREADS];
   __shared__ int ok_all;
@@ -253,15 +253,8 @@ __global__ void __launch_bounds__(STEP_THREADS) k_step_scalars(Fine G, const flo
   int ok = 1;
   for (size_t c = threadIdx.x; c < cols; c += STEP_THREADS) {
     vol += ((double)G.eta[c] - (double)G.eta_rt[c]) - (double)G.rest;
-    float floor = 0.f;
-    if (G.cut)
-      for (int k = 0; k < nz; ++k)
-        if (G.solid[(size_t)k * cols + c]) {
-          const float t = (float)(k + 1) * dx;
-          if (t > floor) floor = t;
-        }
     const float e = G.eta[c];
-    if (!(e >= floor + 2.f * dx && e <= top)) ok = 0;
+    if (!(e >= floor[c] + 2.f * dx && e <= top)) ok = 0;
   }
   if (!ok) ok_all = 0;  // every writer stores the same value
   red[threadIdx.x] = vol;
