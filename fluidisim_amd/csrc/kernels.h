@@ -378,19 +378,48 @@ __device__ __forceinline__ float divergence_at(const Fine& G, const float* u, co
 }
 
 // ---------------------------------------------------------------- geometry kernels
+// Row codes (the face types of full_row, packed): bit f set for a wet
+// neighbour across face f, M_WET for a wet cell, M_GHOST when a face is a
+// ghost face. Every operand of a cell (its six apertures, seven solid flags,
+// five column heights) is loaded at once, indices clamped into the grid, and
+// the types are decided after; full_row gives the same types.
 __global__ void __launch_bounds__(BLOCK) k_mask(Fine G, uint8_t* mask, Ctl* ctl, double* partials) {
   double wet = 0.0, ghosts = 0.0;
+  const int nx = G.nx, ny = G.ny, nz = G.nz;
   FSP_FOR(c, G.n) {
     int i, j, k;
     decode(G, c, i, j, k);
+    const int im = i > 0 ? i - 1 : i, ip = i + 1 < nx ? i + 1 : i;
+    const int jm = j > 0 ? j - 1 : j, jp = j + 1 < ny ? j + 1 : j;
+    const int km = k > 0 ? k - 1 : k, kp = k + 1 < nz ? k + 1 : k;
+    // apertures: x-, x+, y-, y+, z- (face k), z+ (face k + 1)
+    const uint8_t a0 = G.ou[fuidx(G, i, j, k)], a1 = G.ou[fuidx(G, i + 1, j, k)];
+    const uint8_t a2 = G.ov[fvidx(G, i, j, k)], a3 = G.ov[fvidx(G, i, j + 1, k)];
+    const uint8_t a4 = G.ow[fwidx(G, i, j, k)], a5 = G.ow[fwidx(G, i, j, k + 1)];
+    const uint8_t s_c = G.solid[cidx(G, i, j, k)];
+    const uint8_t s0 = G.solid[cidx(G, im, j, k)], s1 = G.solid[cidx(G, ip, j, k)];
+    const uint8_t s2 = G.solid[cidx(G, i, jm, k)], s3 = G.solid[cidx(G, i, jp, k)];
+    const uint8_t s4 = G.solid[cidx(G, i, j, km)], s5 = G.solid[cidx(G, i, j, kp)];
+    const float h = G.eta[(size_t)j * nx + i];
+    const float h0 = G.eta[(size_t)j * nx + im], h1 = G.eta[(size_t)j * nx + ip];
+    const float h2 = G.eta[(size_t)jm * nx + i], h3 = G.eta[(size_t)jp * nx + i];
+    const float zc = zc_of(G, k);
     uint8_t m = 0;
-    if (wet_at(G, i, j, k)) {
+    if (!s_c && zc < h) {
       m = M_WET;
-      RowG R;
-      full_row(G, i, j, k, R);
-      for (int f = 0; f < 6; ++f) {
-        if (R.type[f] == 1) m |= (uint8_t)(1u << f);
-        else if (R.type[f] == 2) { m |= M_GHOST; ghosts += 1.0; }
+      const bool has[4] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny};
+      const uint8_t ap[4] = {a0, a1, a2, a3}, sn[4] = {s0, s1, s2, s3};
+      const float hn[4] = {h0, h1, h2, h3};
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        if (!has[f] || ap[f] == 0 || sn[f]) continue;
+        if (zc < hn[f]) m |= (uint8_t)(1u << f);  // wet neighbour (not solid, below its surface)
+        else { m |= M_GHOST; ghosts += 1.0; }
+      }
+      if (k > 0 && a4 && !s4) m |= M_ZM;
+      if (a5) {
+        if (k + 1 < nz && !s5 && zc_of(G, k + 1) < h) m |= M_ZP;
+        else { m |= M_GHOST; ghosts += 1.0; }
       }
       wet += 1.0;
     }
@@ -977,57 +1006,65 @@ __global__ void __launch_bounds__(BLOCK) k_cprolong(Level F, Level L, const type
 // Gradient subtract (delta3d_mobile.rs:284-343), one thread per face, from
 // the f64 pressure (the gradient is formed in f64, the face rounded to f32),
 // and max |u| over all faces (:422-427).
-__device__ __forceinline__ float corrected_side(const Fine& G, float us, const double* p, float k1, int x, int y, int i,
-                                                int j, int k) {
+__device__ __forceinline__ float corrected_side(const Fine& G, float us, double pl, double pr, bool wl, bool wr,
+                                                float k1, int x, int y, int i, int j, int k) {
 #pragma clang fp contract(off)
-  const size_t l = cidx(G, x, y, k), r = cidx(G, i, j, k);
-  if (G.solid[l] || G.solid[r]) return us;
-  // wet3 of both cells, from the row codes built this projection
-  const bool wl = (G.mask[l] & M_WET) != 0, wr = (G.mask[r] & M_WET) != 0;
   const double kk = (double)k1, dx = (double)G.dx;
   double ch;
-  if (wl && wr) ch = kk * (p[r] - p[l]) / dx;
-  else if (wl) { float a, b; ghost_side(G, x, y, k, i, j, a, b); ch = kk * ((double)b - p[l]) * (double)a / dx; }
-  else if (wr) { float a, b; ghost_side(G, i, j, k, x, y, a, b); ch = kk * (p[r] - (double)b) * (double)a / dx; }
+  if (wl && wr) ch = kk * (pr - pl) / dx;
+  else if (wl) { float a, b; ghost_side(G, x, y, k, i, j, a, b); ch = kk * ((double)b - pl) * (double)a / dx; }
+  else if (wr) { float a, b; ghost_side(G, i, j, k, x, y, a, b); ch = kk * (pr - (double)b) * (double)a / dx; }
   else return us;
   return (float)((double)us - ch);
 }
 
 // One thread per cell: the cell's x-, y- and z- faces, and the closing face
 // of its row / column / pillar at the +x, +y, +z boundary (32-bit index
-// arithmetic only).
+// arithmetic only). Every operand of a face (aperture, both cells' solid
+// flags and row codes, both pressures) is loaded at once, with indices
+// clamped into the grid on the walls, and the face's case is selected after:
+// one memory round trip per face instead of a chain of four.
 __global__ void __launch_bounds__(BLOCK) k_correct(Fine G, const float* us, const float* vs, const float* ws,
                                                    const double* p, float k1, float* u, float* v, float* w, Ctl* ctl,
                                                    double* partials) {
 #pragma clang fp contract(off)
   const int nx = G.nx, ny = G.ny, nz = G.nz;
   double mx = 0.0, unused = 0.0;
+  // horizontal face between cells l (x, y) and r (i, j) of plane k; inner = not on a wall
+  auto side = [&](float val, uint8_t open, bool inner, size_t l, size_t r, int x, int y, int i, int j, int k) {
+    const uint8_t sl = G.solid[l], sr = G.solid[r], ml = G.mask[l], mr = G.mask[r];
+    const double pl = p[l], pr = p[r];
+    if (!inner || !open || sl || sr) return val;
+    return corrected_side(G, val, pl, pr, (ml & M_WET) != 0, (mr & M_WET) != 0, k1, x, y, i, j, k);
+  };
   auto face_u = [&](int i, int j, int k) {
     const size_t f = fuidx(G, i, j, k);
-    float val = us[f];
-    if (i > 0 && i < nx && G.ou[f]) val = corrected_side(G, val, p, k1, i - 1, j, i, j, k);
+    const int x = i > 0 ? i - 1 : 0, ii = i < nx ? i : nx - 1;
+    const float val = side(us[f], G.ou[f], i > 0 && i < nx, cidx(G, x, j, k), cidx(G, ii, j, k), i - 1, j, i, j, k);
     u[f] = val;
     mx = fmax(mx, (double)fabsf(val));
   };
   auto face_v = [&](int i, int j, int k) {
     const size_t g = fvidx(G, i, j, k);
-    float val = vs[g];
-    if (j > 0 && j < ny && G.ov[g]) val = corrected_side(G, val, p, k1, i, j - 1, i, j, k);
+    const int y = j > 0 ? j - 1 : 0, jj = j < ny ? j : ny - 1;
+    const float val = side(vs[g], G.ov[g], j > 0 && j < ny, cidx(G, i, y, k), cidx(G, i, jj, k), i, j - 1, i, j, k);
     v[g] = val;
     mx = fmax(mx, (double)fabsf(val));
   };
   auto face_w = [&](int i, int j, int k) {
     const size_t g = fwidx(G, i, j, k);
+    const size_t below = cidx(G, i, j, k >= 1 ? k - 1 : 0), above = cidx(G, i, j, k < nz ? k : nz - 1);
+    const uint8_t open = G.ow[g], mb = G.mask[below], ma = G.mask[above];
+    const double pb = p[below], pa = p[above];
     float val = ws[g];
-    if (k >= 1 && G.ow[g] && (G.mask[cidx(G, i, j, k - 1)] & M_WET)) {
-      const size_t below = cidx(G, i, j, k - 1);
+    if (k >= 1 && open && (mb & M_WET)) {
       const double kk = (double)k1, dx = (double)G.dx;
-      if (k < nz && (G.mask[cidx(G, i, j, k)] & M_WET)) {
-        val = (float)((double)val - kk * (p[cidx(G, i, j, k)] - p[below]) / dx);
+      if (k < nz && (ma & M_WET)) {
+        val = (float)((double)val - kk * (pa - pb) / dx);
       } else {
         float a, b;
         ghost_up(G, i, j, k - 1, a, b);
-        val = (float)((double)val - kk * ((double)b - p[below]) * (double)a / dx);
+        val = (float)((double)val - kk * ((double)b - pb) * (double)a / dx);
       }
     }
     w[g] = val;
