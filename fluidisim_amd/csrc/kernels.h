@@ -1115,34 +1115,38 @@ __global__ void __launch_bounds__(BLOCK) k_backward(Fine G, const double* p, con
   const size_t plane = (size_t)G.nx * G.ny;
   double worst = 0.0, unused = 0.0;
   FSP_FOR(c, G.n) {
-    const uint8_t m = G.mask[c];
-    if (!(m & M_WET)) continue;
     int i, j, k;
     decode(G, c, i, j, k);
+    // every operand loaded at once (neighbour indices clamped into the grid),
+    // the row's case decided after
+    const size_t nb[6] = {i > 0 ? c - 1 : c, i + 1 < G.nx ? c + 1 : c, j > 0 ? c - G.nx : c,
+                          j + 1 < G.ny ? c + G.nx : c, k > 0 ? c - plane : c, k + 1 < G.nz ? c + plane : c};
+    const uint8_t m = G.mask[c];
     const double pc = fabs(p[c]);
+    double pn[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) pn[f] = p[nb[f]];
+    const float bc = b[c], rc = r[c];
+    if (!(m & M_WET)) continue;
     double acc = 0.0;
-    const size_t nb[6] = {c - 1, c + 1, c - G.nx, c + G.nx, c - plane, c + plane};
     if (!(m & M_GHOST)) {
       for (int f = 0; f < 6; ++f)
-        if (m & (1u << f)) acc += pc + fabs(p[nb[f]]);
+        if (m & (1u << f)) acc += pc + fabs(pn[f]);
     } else {
       uint8_t type[6];
       float it[6];
       ghost_row(G, i, j, k, m, type, it);
       for (int f = 0; f < 6; ++f) {
-        if (type[f] == 1) acc += pc + fabs(p[nb[f]]);
+        if (type[f] == 1) acc += pc + fabs(pn[f]);
         else if (type[f] == 2) acc += pc * (double)it[f];
       }
     }
-    const double scale = fabs((double)b[c]) + acc * (double)G.inv;
-    if (scale > 0.0) worst = fmax(worst, fabs((double)r[c]) / scale);
+    const double scale = fabs((double)bc) + acc * (double)G.inv;
+    if (scale > 0.0) worst = fmax(worst, fabs((double)rc) / scale);
   }
   reduce_finish<true>(worst, unused, OP_BACKWARD, ctl, partials);
 }
 
-// Fingerprint of p: FNV-1a 64 over the bytes of chunks of 1024 values, then over the
-// chunk hashes in order. It detects a bitwise repeat of p like the
-// reference's sequential FNV-1a (delta3d.rs:1360-1369).
 // State fingerprint of the pressure for the stall test of the pass loop (a
 // repeated state ends it; the reference hashes p with FNV-1a,
 // delta3d_mobile.rs:575): every value's bits, salted with its index, go

@@ -119,8 +119,12 @@ __global__ void __launch_bounds__(256) k_advect_w(Fine G, const float* u, const 
 
 // extrapolate_mobile3 (delta3d_mobile.rs:345-394): one thread per (column,
 // axis), x fastest across threads so the faces of one plane are read
-// together; each marches up its column carrying the last solved velocity.
+// together; each marches up its column carrying the last solved velocity,
+// EX levels at a time: the loads of a group of levels are all issued before
+// the carried value walks through them.
+constexpr int EX = 8;
 __global__ void __launch_bounds__(256) k_extrapolate(Fine G, float* u, float* v, float* w) {
+#pragma clang fp contract(off)
   const int nx = G.nx, ny = G.ny, nz = G.nz;
   const size_t cols = (size_t)nx * ny;
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < 3 * cols; t += (size_t)gridDim.x * blockDim.x) {
@@ -130,30 +134,57 @@ __global__ void __launch_bounds__(256) k_extrapolate(Fine G, float* u, float* v,
     if ((axis == 0 && i == 0) || (axis == 1 && j == 0)) continue;
     float* field = axis == 0 ? u : axis == 1 ? v : w;
     const uint8_t* open = axis == 0 ? G.ou : axis == 1 ? G.ov : G.ow;
+    // the two cells a horizontal face joins (this column and the one at -x or -y);
+    // a w face's cell below is in this column
+    const int x = axis == 0 ? i - 1 : i, y = axis == 1 ? j - 1 : j;
+    const float h_here = G.eta[(size_t)j * nx + i], h_prev = G.eta[(size_t)y * nx + x];
     bool have = false;
     float last = 0.f;
     const int k0 = axis == 2 ? 1 : 0, k1 = axis == 2 ? nz + 1 : nz;
-    for (int k = k0; k < k1; ++k) {
-      size_t f;
-      bool solved;
-      if (axis == 0) {
-        f = st_fu(G, i, j, k);
-        solved = open[f] != 0 && !G.solid[st_c(G, i - 1, j, k)] && !G.solid[st_c(G, i, j, k)] &&
-                 (st_wet(G, i - 1, j, k) || st_wet(G, i, j, k));
-      } else if (axis == 1) {
-        f = st_fv(G, i, j, k);
-        solved = open[f] != 0 && !G.solid[st_c(G, i, j - 1, k)] && !G.solid[st_c(G, i, j, k)] &&
-                 (st_wet(G, i, j - 1, k) || st_wet(G, i, j, k));
-      } else {
-        f = st_fw(G, i, j, k);
-        solved = open[f] != 0 && st_wet(G, i, j, k - 1);
+    for (int kb = k0; kb < k1; kb += EX) {
+      uint8_t op[EX], sa[EX], sb[EX];
+      float val[EX];
+#pragma unroll
+      for (int e = 0; e < EX; ++e) {
+        const int k = kb + e < k1 ? kb + e : k1 - 1;
+        size_t f;
+        if (axis == 0) {
+          f = st_fu(G, i, j, k);
+          sa[e] = G.solid[st_c(G, x, y, k)];
+          sb[e] = G.solid[st_c(G, i, j, k)];
+        } else if (axis == 1) {
+          f = st_fv(G, i, j, k);
+          sa[e] = G.solid[st_c(G, x, y, k)];
+          sb[e] = G.solid[st_c(G, i, j, k)];
+        } else {
+          f = st_fw(G, i, j, k);
+          sa[e] = G.solid[st_c(G, i, j, k - 1)];
+          sb[e] = 0;
+        }
+        op[e] = open[f];
+        val[e] = field[f];
       }
-      if (open[f] == 0) continue;
-      if (solved) {
-        last = field[f];
-        have = true;
-      } else if (have) {
-        field[f] = last;
+#pragma unroll
+      for (int e = 0; e < EX; ++e) {
+        const int k = kb + e;
+        if (k >= k1) break;
+        if (op[e] == 0) continue;
+        bool solved;
+        if (axis < 2) {
+          // wet3 of both cells (delta3d_mobile.rs:97-103): not solid, centre below the surface
+          const float zc = ((float)k + 0.5f) * G.dx;
+          solved = !sa[e] && !sb[e] && (zc < h_prev || zc < h_here);
+        } else {
+          const float zc = ((float)(k - 1) + 0.5f) * G.dx;
+          solved = !sa[e] && zc < h_here;
+        }
+        size_t f = axis == 0 ? st_fu(G, i, j, k) : axis == 1 ? st_fv(G, i, j, k) : st_fw(G, i, j, k);
+        if (solved) {
+          last = val[e];
+          have = true;
+        } else if (have) {
+          field[f] = last;
+        }
       }
     }
   }
