@@ -61,6 +61,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=48, help="patches of the batched line (0: skip)")
     ap.add_argument("--batch-concurrency", default="1,4", help="concurrency levels timed for the batched line")
     ap.add_argument("--no-step", action="store_true", help="skip the surface-step line")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-pointer (PCIe-inclusive) line")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the 256 MiB device copy (counter calibration) and exit")
     return ap.parse_args()
@@ -198,6 +199,36 @@ def learned_fcg(wl, args, torch, _lib, W, vcycle):
             "training": trained}
 
 
+def host_pointer_line(vol, wl, steps):
+    """The same projection through the host-pointer entry point (fsp_project):
+    the predicted faces copied in and p, u, v, w copied out over PCIe from
+    pageable host arrays; reported beside the device-resident value."""
+    import ctypes as C
+    from fluidisim_amd import _lib
+    d = wl.domain
+    nu, nv, nw = d.faces
+    us, vs, ws = (np.ascontiguousarray(a, np.float32) for a in (wl.us, wl.vs, wl.ws))
+    p = np.zeros(d.cells, np.float32)
+    u, v, w = np.zeros(nu, np.float32), np.zeros(nv, np.float32), np.zeros(nw, np.float32)
+    rep = _lib.Report()
+    ms, transfer = [], []
+    for q in range(steps + 1):
+        t0 = time.perf_counter()
+        st = _lib.lib().fsp_project(vol._h, us.ctypes.data, nu, vs.ctypes.data, nv, ws.ctypes.data, nw, p.ctypes.data,
+                                    d.cells, 0, u.ctypes.data, nu, v.ctypes.data, nv, w.ctypes.data, nw, wl.scale,
+                                    wl.k1, 1e-6, 2000, C.byref(rep))
+        if st != 0:
+            raise RuntimeError(f"fsp_project: status {st}")
+        if q:  # the first call warms the path
+            ms.append(1000.0 * (time.perf_counter() - t0))
+            transfer.append(rep.ms_transfer)
+    out = {"ms_per_projection": round(float(np.mean(ms)), 4), "ms_transfer": round(float(np.mean(transfer)), 4),
+           "bytes_moved": int(4 * (nu + nv + nw) * 2 + 4 * d.cells), "host_memory": "pageable numpy arrays"}
+    log(f"[bench] host pointers: {out['ms_per_projection']:.3f} ms/projection, of which {out['ms_transfer']:.3f} ms "
+        f"PCIe transfer")
+    return out
+
+
 def surface_step_line(args, torch, W):
     """The whole surface step (advect, project with a warm start,
     extrapolate, transport, balance and checks) on the sloshing tank of the
@@ -296,6 +327,8 @@ def main():
                     "avg_us": round(avg_ms * 1000.0, 2), "bytes_per_launch": int(bytes_launch)}
         log(f"[bench] {args.kernel}: {eff.size} launches, {avg_ms * 1000:.1f} us avg, {achieved:.0f} GB/s")
 
+    host = None if args.no_host else host_pointer_line(vol, wl, max(1, min(args.steps, 3)))
+
     batched = batch_line(args, torch, W) if args.batch > 0 else None
 
     learned = learned_sweeps = None
@@ -328,6 +361,7 @@ def main():
         "learned_fcg_sweeps": learned_sweeps,
         "batched": batched,
         "surface_step": step,
+        "host_pointers": host,
     }
     print(json.dumps(line), flush=True)
     vol.close()
