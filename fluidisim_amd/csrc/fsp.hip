@@ -134,18 +134,31 @@ unsigned grid_for(size_t n) {
 
 // Launch on the solver's stream; when the kernel is the one being timed,
 // bracket the launch with a pair of events.
-// Launch with BLOCK threads; `name` is what fsp_profile_begin matches (the
-// kernel without its storage argument).
-#define FSP_LAUNCH_N(S, name, kern, grid, stream, ...)                                             \
-  do {                                                                                         \
-    const bool prof_ = (S)->prof_on && (S)->prof_name == name &&                               \
-                       (S)->prof_used + 2 <= (S)->prof_events.size();                           \
-    if (prof_) (void)hipEventRecord((S)->prof_events[(S)->prof_used], stream);                 \
-    kern<<<grid, dim3(BLOCK), 0, stream>>>(__VA_ARGS__);                                       \
-    if (prof_) {                                                                               \
-      (void)hipEventRecord((S)->prof_events[(S)->prof_used + 1], stream);                     \
-      (S)->prof_used += 2;                                                                     \
-    }                                                                                          \
+// Event pair around the launch of the kernel being timed.
+inline bool prof_begin(fsp_solver* S, const char* name, hipStream_t stream) {
+  const bool on = S->prof_on && S->prof_name == name && S->prof_used + 2 <= S->prof_events.size();
+  if (on) (void)hipEventRecord(S->prof_events[S->prof_used], stream);
+  return on;
+}
+inline void prof_end(fsp_solver* S, bool on, hipStream_t stream) {
+  if (!on) return;
+  (void)hipEventRecord(S->prof_events[S->prof_used + 1], stream);
+  S->prof_used += 2;
+}
+
+// Launch with BLOCK threads (FSP_LAUNCH_NB: with `block`); `name` is what
+// fsp_profile_begin matches (the kernel without its storage argument).
+#define FSP_LAUNCH_N(S, name, kern, grid, stream, ...)            \
+  do {                                                          \
+    const bool prof_ = prof_begin(S, name, stream);             \
+    kern<<<grid, dim3(BLOCK), 0, stream>>>(__VA_ARGS__);        \
+    prof_end(S, prof_, stream);                                 \
+  } while (0)
+#define FSP_LAUNCH_NB(S, name, kern, grid, block, stream, ...)    \
+  do {                                                          \
+    const bool prof_ = prof_begin(S, name, stream);             \
+    kern<<<grid, dim3(block), 0, stream>>>(__VA_ARGS__);        \
+    prof_end(S, prof_, stream);                                 \
   } while (0)
 #define FSP_LAUNCH(S, kern, grid, stream, ...) FSP_LAUNCH_N(S, #kern, kern, grid, stream, __VA_ARGS__)
 
@@ -370,6 +383,16 @@ struct Launch {
     return t;
   }
 
+  // A compact level with a coarser level below it, small enough to be bound by
+  // launch latency, is smoothed in tiles (k_gb_pre / k_gb_post) when its sweep
+  // counts fit the tile's halo.
+  bool tiled(int l) const {
+    const Level& L = S->levels[l];
+    return L.compact && l + 1 < (int)S->levels.size() && L.n <= (size_t)FSP_GB_MAX_CELLS &&
+           S->opt.pre_sweeps >= 1 && S->opt.pre_sweeps <= GB_MAX_SWEEPS && S->opt.post_sweeps >= 1 &&
+           S->opt.post_sweeps <= GB_MAX_SWEEPS;
+  }
+
   // coarse_cycle3 (delta3d_multigrid.rs:303-332); returns the level-0 iterate.
   template <int ST>
   TS<ST>* coarse_cycle() {
@@ -378,6 +401,16 @@ struct Launch {
     std::vector<TS<ST>*> xcur(S->levels.size(), nullptr);
     for (int l = 0; l < tf; ++l) {
       Level& L = S->levels[l];
+      if (tiled(l)) {
+        if (S->opt.pre_sweeps == 2)
+          FSP_LAUNCH_NB(S, "k_gb_pre", (k_gb_pre<ST, 2>), dim3(gb_blocks(L)), GB_THREADS, st, L, S->levels[l + 1],
+                        S->ctl);
+        else
+          FSP_LAUNCH_NB(S, "k_gb_pre", (k_gb_pre<ST, 1>), dim3(gb_blocks(L)), GB_THREADS, st, L, S->levels[l + 1],
+                        S->ctl);
+        xcur[l] = reinterpret_cast<TS<ST>*>(L.x);
+        continue;
+      }
       const int sweeps = (l == last) ? S->opt.coarse_sweeps : S->opt.pre_sweeps;
       if (L.compact) xcur[l] = gc_sweeps<ST>(l, sweeps, l == last);
       else xcur[l] = coarse_sweeps<ST>(l, nullptr, sweeps, true, l == last);
@@ -397,6 +430,17 @@ struct Launch {
     for (int l = std::min(tf, last) - 1; l >= 0; --l) {
       Level& L = S->levels[l];
       Level& C = S->levels[l + 1];
+      if (tiled(l)) {
+        TS<ST>* t = reinterpret_cast<TS<ST>*>(L.t);
+        if (S->opt.post_sweeps == 2)
+          FSP_LAUNCH_NB(S, "k_gb_post", (k_gb_post<ST, 2>), dim3(gb_blocks(L)), GB_THREADS, st, L, C,
+                        (const TS<ST>*)xcur[l], (const TS<ST>*)xcur[l + 1], t, S->ctl);
+        else
+          FSP_LAUNCH_NB(S, "k_gb_post", (k_gb_post<ST, 1>), dim3(gb_blocks(L)), GB_THREADS, st, L, C,
+                        (const TS<ST>*)xcur[l], (const TS<ST>*)xcur[l + 1], t, S->ctl);
+        xcur[l] = t;
+        continue;
+      }
       if (L.compact) {
         xcur[l] = gc_ascend<ST>(l, xcur[l + 1]);
         continue;

@@ -235,6 +235,247 @@ __global__ void __launch_bounds__(QBLOCK) k_gc_restrict(Level F, QuadGrid Q, con
   SV::st2(reinterpret_cast<T*>(C.r), lidx(C, 2 * qx, J, K), out);
 }
 
+// ---------------------------------------------------------------- tiled compact levels
+// On a compact level small enough that its one-pass launches are bound by
+// launch latency rather than by memory, one workgroup takes an 8x8x8 tile
+// with a halo of 4 cells and runs every colour pass of the pre-smoothing and
+// the restriction of the residual (k_gb_pre), or the prolongation and every
+// pass of the post-smoothing (k_gb_post), in LDS. The halo's values are
+// recomputed by each workgroup: pass m of n runs on the tile grown by n - m
+// cells (n - m + 1 before a restriction), which is exactly the region whose
+// next-pass neighbours it feeds, so every interior value is computed from the
+// operands k_gc_gs / k_gc_restrict use, in their order, and is bit-identical.
+// Four halo cells cover two sweeps of two colours. Box cells outside the
+// level hold zero couplings, Dirichlet part, r and x, which the row excludes
+// exactly as the edge tests of the quad kernels do.
+#ifndef FSP_GB_MAX_CELLS
+#define FSP_GB_MAX_CELLS (1 << 18)  // largest compact level smoothed in tiles
+#endif
+constexpr int GB_T = 8;                    // tile interior per dimension (even: colours and children align)
+constexpr int GB_H = 4;                    // halo
+constexpr int GB_B = GB_T + 2 * GB_H + 1;  // box per dimension (+1: the + faces of the outermost halo cells)
+constexpr int GB_B2 = GB_B * GB_B;
+constexpr int GB_N = GB_B2 * GB_B;
+constexpr int GB_I = GB_T * GB_T * GB_T;   // interior cells
+constexpr int GB_THREADS = 1024;
+constexpr int GB_MAX_SWEEPS = GB_H / 2;
+static_assert(GB_THREADS >= GB_I, "one thread per interior cell");
+
+struct GbLds {
+  float x[GB_N], r[GB_N], dd[GB_N];
+  _Float16 fx[GB_N], fy[GB_N], fz[GB_N];
+  float res[GB_I];
+};
+
+struct GbTile {
+  int i0, j0, k0;  // level coordinates of box cell (0, 0, 0)
+};
+
+__device__ __forceinline__ GbTile gb_tile(const Level& L) {
+  const int tx = (L.nx + GB_T - 1) / GB_T, ty = (L.ny + GB_T - 1) / GB_T;
+  const int b = (int)blockIdx.x;
+  GbTile t;
+  t.i0 = (b % tx) * GB_T - GB_H;
+  t.j0 = ((b / tx) % ty) * GB_T - GB_H;
+  t.k0 = (b / (tx * ty)) * GB_T - GB_H;
+  return t;
+}
+
+__device__ __forceinline__ bool gb_inside(const Level& L, int i, int j, int k) {
+  return (unsigned)i < (unsigned)L.nx && (unsigned)j < (unsigned)L.ny && (unsigned)k < (unsigned)L.nz;
+}
+
+// Couplings, Dirichlet parts and r of the box, and its starting x:
+// pre-smoothing (POST false): the first pass from x = 0 (k_gc_gs<1>, colour 0
+// cells r/diag, the others 0), for which each thread also loads its cells' +
+// faces; post-smoothing: the level's x prolonged from its parents in C as
+// k_gc_gs<2> prolongs it. One round of loads: cells outside the level (or
+// past the box) load cell 0 and are replaced by zeros.
+template <int ST, bool POST>
+__device__ __forceinline__ void gb_load(GbLds& s, const Level& L, const GbTile& t, const typename Store<ST>::T* x,
+                                        const Level& C, const typename Store<ST>::T* xp) {
+#pragma clang fp contract(off)
+  using SV = Store<ST>;
+  const typename SV::T* lr = reinterpret_cast<const typename SV::T*>(L.r);
+  const size_t plane = (size_t)L.nx * L.ny;
+  constexpr int ITER = (GB_N + GB_THREADS - 1) / GB_THREADS;
+  float ax[ITER], ay[ITER], az[ITER], bx_[ITER], by_[ITER], bz_[ITER], dd[ITER], r[ITER], v[ITER];
+  bool in[ITER], colour0[ITER];
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    const int q = (int)threadIdx.x + it * GB_THREADS;
+    const int bz = q / GB_B2, by = (q / GB_B) % GB_B, bx = q % GB_B;
+    const int i = t.i0 + bx, j = t.j0 + by, k = t.k0 + bz;
+    in[it] = q < GB_N && gb_inside(L, i, j, k);
+    colour0[it] = ((bx + by + bz) & 1) == 0;  // box and level parities agree: i0, j0, k0 are even
+    const size_t c = in[it] ? lidx(L, i, j, k) : 0;
+    ax[it] = gc_ld(L.cx, c);
+    ay[it] = gc_ld(L.cy, c);
+    az[it] = gc_ld(L.cz, c);
+    dd[it] = L.dd[c];
+    r[it] = SV::ld(lr, c);
+    if (POST) {
+      v[it] = SV::rnd(SV::ld(x, c) + SV::ld(xp, in[it] ? lidx(C, i >> 1, j >> 1, k >> 1) : 0));
+    } else {
+      bx_[it] = in[it] && i + 1 < L.nx ? gc_ld(L.cx, c + 1) : 0.f;
+      by_[it] = in[it] && j + 1 < L.ny ? gc_ld(L.cy, c + L.nx) : 0.f;
+      bz_[it] = in[it] && k + 1 < L.nz ? gc_ld(L.cz, c + plane) : 0.f;
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    const int q = (int)threadIdx.x + it * GB_THREADS;
+    if (q >= GB_N) break;
+    const bool o = in[it];
+    const float a[6] = {o ? ax[it] : 0.f, POST ? 0.f : bx_[it], o ? ay[it] : 0.f,
+                        POST ? 0.f : by_[it], o ? az[it] : 0.f, POST ? 0.f : bz_[it]};
+    const float d = o ? dd[it] : 0.f, rr = o ? r[it] : 0.f;
+    s.fx[q] = (_Float16)a[0];
+    s.fy[q] = (_Float16)a[2];
+    s.fz[q] = (_Float16)a[4];
+    s.dd[q] = d;
+    s.r[q] = rr;
+    float x0 = 0.f;
+    if (POST) {
+      if (o) x0 = v[it];
+    } else if (o && colour0[it]) {
+      const float dg = gc_diag(a, d, L.inv);
+      if (dg > 0.f) x0 = SV::rnd(rr / dg);
+    }
+    s.x[q] = x0;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void gb_faces(const GbLds& s, int c, float a[6]) {
+  a[0] = (float)s.fx[c];
+  a[1] = (float)s.fx[c + 1];
+  a[2] = (float)s.fy[c];
+  a[3] = (float)s.fy[c + GB_B];
+  a[4] = (float)s.fz[c];
+  a[5] = (float)s.fz[c + GB_B2];
+}
+
+__device__ __forceinline__ float gb_row(const GbLds& s, int c, const float a[6], float inv) {
+  const float nb[6] = {s.x[c - 1], s.x[c + 1], s.x[c - GB_B], s.x[c + GB_B], s.x[c - GB_B2], s.x[c + GB_B2]};
+  return gc_row(a, s.dd[c], inv, s.x[c], nb);
+}
+
+// One colour (k_gc_gs<0>) on the tile grown by GROW cells (1 <= GROW < GB_H:
+// every neighbour read stays in the box).
+template <int ST, int GROW>
+__device__ __forceinline__ void gb_pass(GbLds& s, float inv, int colour) {
+#pragma clang fp contract(off)
+  using SV = Store<ST>;
+  static_assert(GROW >= 0 && GROW < GB_H, "pass region inside the box");
+  constexpr int e = GB_T + 2 * GROW, o = GB_H - GROW, half = e / 2, cnt = half * e * e;
+#pragma unroll
+  for (int it = 0; it < (cnt + GB_THREADS - 1) / GB_THREADS; ++it) {
+    const int q = (int)threadIdx.x + it * GB_THREADS;
+    if (q >= cnt) break;
+    const int u = q % half, row = q / half;
+    const int by = o + row % e, bz = o + row / e;
+    const int bx = o + 2 * u + ((colour + by + bz + o) & 1);
+    const int c = (bz * GB_B + by) * GB_B + bx;
+    float a[6];
+    gb_faces(s, c, a);
+    const float dg = gc_diag(a, s.dd[c], inv);
+    if (!(dg > 0.f)) continue;
+    const float pc = s.x[c];
+    s.x[c] = SV::rnd(pc + (s.r[c] - gb_row(s, c, a, inv)) / dg);
+  }
+  __syncthreads();
+}
+
+// Passes 2 .. 2 SWEEPS of the pre-smoothing (colours 1, 0, 1, ...), pass m on
+// the tile grown by 2 SWEEPS - m + 1.
+template <int ST, int SWEEPS>
+__device__ __forceinline__ void gb_pre_passes(GbLds& s, float inv) {
+  if (SWEEPS >= 2) {
+    gb_pass<ST, 2 * SWEEPS - 1>(s, inv, 1);
+    gb_pass<ST, 2 * SWEEPS - 2>(s, inv, 0);
+  }
+  gb_pass<ST, 1>(s, inv, 1);
+}
+
+// Passes 1 .. 2 SWEEPS of the post-smoothing (colours 1, 0, 1, 0), pass m on
+// the tile grown by 2 SWEEPS - m.
+template <int ST, int SWEEPS>
+__device__ __forceinline__ void gb_post_passes(GbLds& s, float inv) {
+  if (SWEEPS >= 2) {
+    gb_pass<ST, 2 * SWEEPS - 1>(s, inv, 1);
+    gb_pass<ST, 2 * SWEEPS - 2>(s, inv, 0);
+  }
+  gb_pass<ST, 1>(s, inv, 1);
+  gb_pass<ST, 0>(s, inv, 0);
+}
+
+// Pre-smoothing from zero (SWEEPS forward red-black sweeps, as gc_sweeps
+// launches them) and the restriction of the residual to C.r (k_gc_restrict).
+// Writes the smoothed x of the tile's interior to L.x.
+template <int ST, int SWEEPS>
+__global__ void __launch_bounds__(GB_THREADS) k_gb_pre(Level L, Level C, Ctl* ctl) {
+#pragma clang fp contract(off)
+  using SV = Store<ST>;
+  using T = typename SV::T;
+  static_assert(SWEEPS >= 1 && SWEEPS <= GB_MAX_SWEEPS, "halo");
+  if (!ctl->active) return;
+  __shared__ GbLds s;
+  const GbTile t = gb_tile(L);
+  gb_load<ST, false>(s, L, t, nullptr, C, nullptr);
+  gb_pre_passes<ST, SWEEPS>(s, L.inv);
+  const int tid = (int)threadIdx.x;
+  if (tid < GB_I) {
+    const int ix = tid % GB_T, iy = (tid / GB_T) % GB_T, iz = tid / (GB_T * GB_T);
+    const int c = ((iz + GB_H) * GB_B + iy + GB_H) * GB_B + ix + GB_H;
+    const int i = t.i0 + GB_H + ix, j = t.j0 + GB_H + iy, k = t.k0 + GB_H + iz;
+    float a[6];
+    gb_faces(s, c, a);
+    s.res[tid] = s.r[c] - gb_row(s, c, a, L.inv);
+    if (gb_inside(L, i, j, k)) SV::st(reinterpret_cast<T*>(L.x), lidx(L, i, j, k), s.x[c]);
+  }
+  __syncthreads();
+  constexpr int HT = GB_T / 2;
+  if (tid >= HT * HT * HT) return;
+  const int I = tid % HT, J = (tid / HT) % HT, K = tid / (HT * HT);
+  const int ci = (t.i0 + GB_H) / 2 + I, cj = (t.j0 + GB_H) / 2 + J, ck = (t.k0 + GB_H) / 2 + K;
+  if (!gb_inside(C, ci, cj, ck)) return;
+  float sum = 0.f;
+#pragma unroll
+  for (int ch = 0; ch < 8; ++ch) {  // children in z, y, x order, x fastest
+    const int fx = 2 * I + (ch & 1), fy = 2 * J + ((ch >> 1) & 1), fz = 2 * K + (ch >> 2);
+    sum += s.res[(fz * GB_T + fy) * GB_T + fx];
+  }
+  SV::st(reinterpret_cast<T*>(C.r), lidx(C, ci, cj, ck), 0.125f * sum);
+}
+
+// Prolongation of the parents xp (level C) onto x, then SWEEPS backward
+// red-black sweeps (colour 1 first, as gc_ascend launches them); the tile's
+// interior goes to xout.
+template <int ST, int SWEEPS>
+__global__ void __launch_bounds__(GB_THREADS) k_gb_post(Level L, Level C, const typename Store<ST>::T* x,
+                                                        const typename Store<ST>::T* xp,
+                                                        typename Store<ST>::T* xout, Ctl* ctl) {
+#pragma clang fp contract(off)
+  using SV = Store<ST>;
+  static_assert(SWEEPS >= 1 && SWEEPS <= GB_MAX_SWEEPS, "halo");
+  if (!ctl->active) return;
+  __shared__ GbLds s;
+  const GbTile t = gb_tile(L);
+  gb_load<ST, true>(s, L, t, x, C, xp);
+  gb_post_passes<ST, SWEEPS>(s, L.inv);
+  const int tid = (int)threadIdx.x;
+  if (tid >= GB_I) return;
+  const int ix = tid % GB_T, iy = (tid / GB_T) % GB_T, iz = tid / (GB_T * GB_T);
+  const int i = t.i0 + GB_H + ix, j = t.j0 + GB_H + iy, k = t.k0 + GB_H + iz;
+  if (gb_inside(L, i, j, k))
+    SV::st(xout, lidx(L, i, j, k), s.x[((iz + GB_H) * GB_B + iy + GB_H) * GB_B + ix + GB_H]);
+}
+
+inline unsigned gb_blocks(const Level& L) {
+  return (unsigned)(((L.nx + GB_T - 1) / GB_T) * ((L.ny + GB_T - 1) / GB_T) * ((L.nz + GB_T - 1) / GB_T));
+}
+
 }  // namespace fsp
 
 namespace fsp {

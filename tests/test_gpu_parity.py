@@ -126,6 +126,29 @@ def test_vcycle_matches_the_oracle(name, smoother):
     assert np.array_equal(bits(z), bits(z_ref)), f"max diff {np.abs(z - z_ref).max()}"
 
 
+@pytest.mark.parametrize("name,sweeps", [
+    ("tank_flat48x32x40", (1, 1)), ("tank_flat48x32x40", (2, 1)), ("tank_flat48x32x40", (3, 2)),
+    ("tank128x64x96", (1, 2)), ("tank128x64x96", (3, 3)),
+    # a compact level above the tiled size (one pass per launch) over two tiled ones
+    ("tank160x128x128", (2, 2)), ("tank160x128x128", (1, 1))])
+def test_vcycle_sweep_counts_match_the_oracle(name, sweeps):
+    """Compact levels smoothed in tiles (one or two sweeps, tiles cut by the
+    level's edge) and one pass per launch (three sweeps, large levels): the
+    oracle's V-cycle bit for bit."""
+    wl = W.tank(160, 128, 128, seed=9) if name == "tank160x128x128" else WORKLOADS[name]()
+    wl.options["pre_sweeps"], wl.options["post_sweeps"] = sweeps
+    vol = W.build_volume(wl)
+    prob = problem(wl)
+    _, prec = po.assemble(prob, np.zeros(wl.domain.cells, np.float32), 1.0)
+    r = np.random.default_rng(13).uniform(-1.0, 1.0, wl.domain.cells).astype(np.float32)
+    r[prec == 0] = 0.0
+    z = host(vol.precondition(torch.from_numpy(r).cuda()))
+    z_ref = po.vcycle(prob, r, oracle_options(wl)).astype(np.float32)
+    assert np.abs(z_ref).max() > 0
+    assert np.array_equal(bits(z), bits(z_ref)), f"max diff {np.abs(z - z_ref).max()}"
+    vol.close()
+
+
 @pytest.mark.parametrize("name", ["tank32", "tank_flat48x32x40"])
 def test_vcycle_with_the_reference_coarse_operator_matches_the_oracle(name):
     wl = WORKLOADS[name]()
