@@ -60,6 +60,7 @@ struct fsp_solver {
   std::vector<QuadGrid> cq, cqr;  // compact levels: quads of the level, and of its restriction
   int tail_first = 0;
   size_t tail_lds = 0;
+  float* tail_pack = nullptr;  // Galerkin tail: couplings of the tail levels in LDS order
   float* zres = nullptr;  // buffer holding M^-1 r after precondition()
   Ctl* ctl = nullptr;
   double* partials = nullptr;
@@ -185,8 +186,8 @@ auto with_storage(int storage, F&& f) {
 
 // Fold the per-block partials of a reducing kernel (k_finish) right after it.
 void launch_finish(hipStream_t st, int op, unsigned nblocks, bool is_max, int gated, Ctl* ctl, double* partials) {
-  if (is_max) hipLaunchKernelGGL(k_finish<true>, dim3(1), dim3(BLOCK), 0, st, op, nblocks, gated, ctl, partials);
-  else hipLaunchKernelGGL(k_finish<false>, dim3(1), dim3(BLOCK), 0, st, op, nblocks, gated, ctl, partials);
+  if (is_max) hipLaunchKernelGGL(k_finish<true>, dim3(1), dim3(FIN_THREADS), 0, st, op, nblocks, gated, ctl, partials);
+  else hipLaunchKernelGGL(k_finish<false>, dim3(1), dim3(FIN_THREADS), 0, st, op, nblocks, gated, ctl, partials);
 }
 #define FSP_FINISH(S, stream, op, grid, is_max, gated) \
   launch_finish(stream, op, (grid).x, is_max, gated, (S)->ctl, (S)->partials)
@@ -273,6 +274,22 @@ struct Launch {
       FSP_LAUNCH(S, k_cdiag, dim3(grid_for(L.n)), st, L);
       if (L.compact) FSP_LAUNCH(S, k_gc_build, dim3(grid_for(L.n)), st, L);
     }
+    if (S->tail_pack) {
+      const TailSet T = tail_set();
+      FSP_LAUNCH(S, k_gct_pack, dim3(gct_pack_blocks(T)), st, T, S->tail_pack);
+    }
+  }
+
+  TailSet tail_set() const {
+    TailSet T{};
+    T.count = (int)S->levels.size() - S->tail_first;
+    for (int l = 0; l < T.count; ++l) T.lv[l] = S->levels[S->tail_first + l];
+    T.pack = S->tail_pack;
+    T.smoother = S->opt.smoother;
+    T.pre = S->opt.pre_sweeps;
+    T.post = S->opt.post_sweeps;
+    T.coarse = S->opt.coarse_sweeps;
+    return T;
   }
 
   void fill_gated(float* x, size_t n, float v) {
@@ -325,13 +342,7 @@ struct Launch {
   // leaves its x in levels[tail_first].x.
   template <int ST>
   void tail() {
-    TailSet T{};
-    T.count = (int)S->levels.size() - S->tail_first;
-    for (int l = 0; l < T.count; ++l) T.lv[l] = S->levels[S->tail_first + l];
-    T.smoother = S->opt.smoother;
-    T.pre = S->opt.pre_sweeps;
-    T.post = S->opt.post_sweeps;
-    T.coarse = S->opt.coarse_sweeps;
+    const TailSet T = tail_set();
     if (gc_tail_ok(S->opt)) k_gc_tail<ST><<<dim3(1), dim3(TAIL_THREADS), S->tail_lds, st>>>(T, S->ctl);
     else k_tail<ST><<<dim3(1), dim3(TAIL_THREADS), S->tail_lds, st>>>(T, S->ctl);
   }
@@ -1087,9 +1098,14 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
     const int count = (int)S->levels.size() - t;
     const size_t bytes = gc_tail_ok(opt) ? gc_tail_lds_bytes(&S->levels[t], count)
                                          : tail_lds_bytes(&S->levels[t], count, opt.smoother);
-    if (bytes > 64 * 1024) break;
+    if (bytes > TAIL_LDS_FLOATS * sizeof(float)) break;
     S->tail_first = t;
     S->tail_lds = bytes;
+  }
+  if (gc_tail_ok(opt) && S->tail_first < (int)S->levels.size()) {
+    size_t floats = 0;
+    for (int l = S->tail_first; l < (int)S->levels.size(); ++l) floats += 4 * S->levels[l].n;
+    if ((st = dalloc(S, &S->tail_pack, floats))) return fail(st);
   }
   // Compact Galerkin levels: every level above the tail whose couplings are
   // exact in binary16 (level <= 5) and whose rows split into quads.

@@ -152,18 +152,23 @@ __device__ __forceinline__ void reduce_finish(double a, double b, int op, Ctl* c
 
 // Folds the partials of `nblocks` blocks in block order and applies the
 // finish operation. `gated`: skipped like its producer once the loop stopped.
+// FIN_THREADS threads with eight loads each fold 8192 partials (the fine
+// kernels' grid) in one round of memory latency.
+constexpr int FIN_THREADS = 1024;
+
 template <bool MAX>
-__global__ void __launch_bounds__(BLOCK) k_finish(int op, unsigned nblocks, int gated, Ctl* ctl, double* partials) {
+__global__ void __launch_bounds__(FIN_THREADS) k_finish(int op, unsigned nblocks, int gated, Ctl* ctl,
+                                                        double* partials) {
   if (gated && !ctl->active) return;
-  __shared__ double sa[BLOCK / 64], sb[BLOCK / 64];
+  __shared__ double sa[FIN_THREADS / 64], sb[FIN_THREADS / 64];
   const double2* part = reinterpret_cast<const double2*>(partials + (size_t)op * MAX_PARTIALS * 2);
   // eight independent loads in flight per step, folded in a fixed order
   double a = 0.0, b = 0.0;
-  for (unsigned base = threadIdx.x; base < nblocks; base += 8 * BLOCK) {
+  for (unsigned base = threadIdx.x; base < nblocks; base += 8 * FIN_THREADS) {
     double2 v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const unsigned q = base + (unsigned)u * BLOCK;
+      const unsigned q = base + (unsigned)u * FIN_THREADS;
       v[u] = q < nblocks ? part[q] : make_double2(0.0, 0.0);
     }
 #pragma unroll
@@ -180,7 +185,7 @@ __global__ void __launch_bounds__(BLOCK) k_finish(int op, unsigned nblocks, int 
   if (threadIdx.x != 0) return;
   a = sa[0];
   b = sb[0];
-  for (int w = 1; w < BLOCK / 64; ++w) {
+  for (int w = 1; w < FIN_THREADS / 64; ++w) {
     a = MAX ? fmax(a, sa[w]) : a + sa[w];
     b = MAX ? fmax(b, sb[w]) : b + sb[w];
   }

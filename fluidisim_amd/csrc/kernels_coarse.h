@@ -482,14 +482,56 @@ namespace fsp {
 
 // ---------------------------------------------------------------- compact tail
 // Every Galerkin level below the compact ones in one workgroup, in LDS:
-// per cell x, r (values rounded to storage ST), the x-, y-, z- couplings in
-// f32 and dd (6 floats). Same arithmetic as k_gc_gs / k_gc_restrict.
+// per cell the x-, y-, z- couplings in f32 and dd, then x and r (values
+// rounded to storage ST): 6 floats. Same arithmetic as k_gc_gs /
+// k_gc_restrict. The couplings of all levels lie first, level after level
+// ([fx, fy, fz, dd] blocks of n floats each), exactly as k_gct_pack lays
+// them out in global memory, so that the kernel loads them as one
+// contiguous copy.
 struct GcTailLvl {
   int nx, ny, nz, n;
   float inv;
-  float rnx, rny;  // 1/nx, 1/ny for the cell decode
+  float rnx, rny, rhx;  // 1/nx, 1/ny, 1/(nx/2) for the cell decodes
   int o_x, o_r, o_fx, o_fy, o_fz, o_dd;
 };
+
+// Float offset of level l's coupling block in the pack (and in LDS).
+__host__ __device__ inline size_t gct_pack_offset(const TailSet& T, int l) {
+  size_t off = 0;
+  for (int m = 0; m < l; ++m) off += 4 * T.lv[m].n;
+  return off;
+}
+
+// The couplings of the tail levels gathered from their face arrays (run with
+// the hierarchy). Blocks are handed out level by level, so a block's level
+// is uniform.
+__global__ void __launch_bounds__(BLOCK) k_gct_pack(TailSet T, float* pack) {
+  int b = (int)blockIdx.x, l = 0;
+  size_t off = 0;
+  for (; l < T.count; ++l) {
+    const int nb = (int)((T.lv[l].n + BLOCK - 1) / BLOCK);
+    if (b < nb) break;
+    b -= nb;
+    off += 4 * T.lv[l].n;
+  }
+  if (l >= T.count) return;
+  const Level& g = T.lv[l];
+  const size_t c = (size_t)b * BLOCK + threadIdx.x;
+  if (c >= g.n) return;
+  int i, j, k;
+  ldecode(g, c, i, j, k);
+  const size_t n = g.n;
+  pack[off + c] = i > 0 ? g.ou[((size_t)k * g.ny + j) * (g.nx + 1) + i] : 0.f;
+  pack[off + n + c] = j > 0 ? g.ov[((size_t)k * (g.ny + 1) + j) * g.nx + i] : 0.f;
+  pack[off + 2 * n + c] = k > 0 ? g.ow[((size_t)k * g.ny + j) * g.nx + i] : 0.f;
+  pack[off + 3 * n + c] = g.dd[c];
+}
+
+inline unsigned gct_pack_blocks(const TailSet& T) {
+  unsigned b = 0;
+  for (int l = 0; l < T.count; ++l) b += (unsigned)((T.lv[l].n + BLOCK - 1) / BLOCK);
+  return b;
+}
 
 // Cell c -> (i, j, k) without integer division: a/b = floor((a + 1/2)/b),
 // exact in f32 for a < 2^22 (the error of the product stays below the 1/(2b)
@@ -529,6 +571,27 @@ __device__ __forceinline__ void gct_pass(float* lds, const GcTailLvl& L, int col
   using SV = Store<ST>;
   float* x = lds + L.o_x;
   const float* r = lds + L.o_r;
+  if ((L.nx & 1) == 0) {
+    // even rows: every pair (2u, 2u + 1) holds one cell of each colour, so
+    // the threads walk the colour's cells only
+    const int hx = L.nx >> 1, m = L.n >> 1;
+    for (int q = (int)threadIdx.x; q < m; q += TAIL_THREADS) {
+      const int row = (int)(((float)q + 0.5f) * L.rhx);
+      const int u = q - row * hx;
+      const int k = (int)(((float)row + 0.5f) * L.rny);
+      const int j = row - k * L.ny;
+      const int i = 2 * u + ((col + j + k) & 1);
+      const int c = row * L.nx + i;
+      if (first) x[c ^ 1] = 0.f;  // the pair's other cell (row * nx is even)
+      float a[6];
+      gct_faces(lds, L, i, j, k, c, a);
+      const float dg = gc_diag(a, lds[L.o_dd + c], L.inv);
+      if (first) { x[c] = dg > 0.f ? SV::rnd(r[c] / dg) : 0.f; continue; }
+      if (dg > 0.f) x[c] = SV::rnd(x[c] + (r[c] - gct_row(lds, L, i, j, k, c, a)) / dg);
+    }
+    __syncthreads();
+    return;
+  }
   for (int c = (int)threadIdx.x; c < L.n; c += TAIL_THREADS) {
     int i, j, k;
     gct_decode(L, c, i, j, k);
@@ -545,6 +608,59 @@ __device__ __forceinline__ void gct_pass(float* lds, const GcTailLvl& L, int col
   __syncthreads();
 }
 
+// The palindromic sweeps of a coarsest level of at most 64 cells in the
+// first wavefront: each lane holds one cell's x in a register and reads its
+// neighbours' by lane shuffle, so the passes need no workgroup barrier.
+// Per cell the arithmetic of gct_pass, in the same order.
+template <int ST>
+__device__ __forceinline__ void gct_coarsest_wave(float* lds, const GcTailLvl& L, int count, bool from_zero) {
+#pragma clang fp contract(off)
+  using SV = Store<ST>;
+  if (threadIdx.x < 64) {
+    const int c = (int)threadIdx.x;
+    const bool in = c < L.n;
+    int i = 0, j = 0, k = 0;
+    if (in) gct_decode(L, c, i, j, k);
+    const int plane = L.nx * L.ny;
+    float a[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float dd = 0.f, r = 0.f, x = 0.f;
+    if (in) {
+      gct_faces(lds, L, i, j, k, c, a);
+      dd = lds[L.o_dd + c];
+      r = lds[L.o_r + c];
+      if (!from_zero) x = lds[L.o_x + c];
+    }
+    const bool has[6] = {in && i > 0, in && i + 1 < L.nx, in && j > 0, in && j + 1 < L.ny, in && k > 0,
+                         in && k + 1 < L.nz};
+    const int src[6] = {has[0] ? c - 1 : c, has[1] ? c + 1 : c, has[2] ? c - L.nx : c,
+                        has[3] ? c + L.nx : c, has[4] ? c - plane : c, has[5] ? c + plane : c};
+    const float dg = gc_diag(a, dd, L.inv);
+    const int mine = (i + j + k) & 1;
+    auto pass = [&](int col, bool first) {
+      float nb[6];
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        const float v = __shfl(x, src[f]);
+        nb[f] = has[f] ? v : 0.f;
+      }
+      if (!in) return;
+      if (mine != col) {
+        if (first) x = 0.f;
+        return;
+      }
+      if (first) { x = dg > 0.f ? SV::rnd(r / dg) : 0.f; return; }
+      if (dg > 0.f) x = SV::rnd(x + (r - gc_row(a, dd, L.inv, x, nb)) / dg);
+    };
+    bool fz = from_zero;
+    const int half = count / 2;
+    for (int s = 0; s < half; ++s) { pass(0, fz); fz = false; pass(1, false); }
+    if (count & 1) { pass(0, fz); fz = false; }
+    for (int s = 0; s < half; ++s) { pass(1, fz); fz = false; pass(0, false); }
+    if (in) lds[L.o_x + c] = x;
+  }
+  __syncthreads();
+}
+
 template <int ST>
 __device__ __forceinline__ void gct_sweeps(float* lds, const GcTailLvl& L, int count, bool forward, bool coarsest,
                                            bool from_zero) {
@@ -556,6 +672,10 @@ __device__ __forceinline__ void gct_sweeps(float* lds, const GcTailLvl& L, int c
     return;
   }
   bool fz = from_zero;
+  if (coarsest && L.n <= 64) {
+    gct_coarsest_wave<ST>(lds, L, count, from_zero);
+    return;
+  }
   if (coarsest) {
     const int half = count / 2;
     for (int s = 0; s < half; ++s) { gct_pass<ST>(lds, L, 0, fz); fz = false; gct_pass<ST>(lds, L, 1, false); }
@@ -571,25 +691,29 @@ __device__ __forceinline__ void gct_sweeps(float* lds, const GcTailLvl& L, int c
   }
 }
 
-// r of level C = restriction of the residual of level L.
+// r of level C = restriction of the residual of level L. Eight consecutive
+// lanes take the eight children of one coarse cell, one residual each, and
+// every lane of the group sums them in child order (z, y, x; x fastest).
+// Groups never straddle the loop's end (8 C.n threads), so the lanes read by
+// a shuffle are active.
 template <int ST>
 __device__ __forceinline__ void gct_restrict(float* lds, const GcTailLvl& L, const GcTailLvl& C) {
 #pragma clang fp contract(off)
   using SV = Store<ST>;
-  for (int c = (int)threadIdx.x; c < C.n; c += TAIL_THREADS) {
+  const int lane = (int)(threadIdx.x & 63u), base = lane & ~7;
+  for (int q = (int)threadIdx.x; q < 8 * C.n; q += TAIL_THREADS) {
+    const int cc = q >> 3, ch = q & 7;
     int i, j, k;
-    gct_decode(C, c, i, j, k);
+    gct_decode(C, cc, i, j, k);
+    const int fi = 2 * i + (ch & 1), fj = 2 * j + ((ch >> 1) & 1), fk = 2 * k + (ch >> 2);
+    const int f = (fk * L.ny + fj) * L.nx + fi;
+    float a[6];
+    gct_faces(lds, L, fi, fj, fk, f, a);
+    const float t = lds[L.o_r + f] - gct_row(lds, L, fi, fj, fk, f, a);
     float s = 0.f;
-    // a full unroll spills at the 128-VGPR budget of 1024-thread workgroups
-#pragma unroll 2
-    for (int ch = 0; ch < 8; ++ch) {
-      const int fi = 2 * i + (ch & 1), fj = 2 * j + ((ch >> 1) & 1), fk = 2 * k + ((ch >> 2) & 1);
-      const int f = (fk * L.ny + fj) * L.nx + fi;
-      float a[6];
-      gct_faces(lds, L, fi, fj, fk, f, a);
-      s += lds[L.o_r + f] - gct_row(lds, L, fi, fj, fk, f, a);
-    }
-    lds[C.o_r + c] = SV::rnd(0.125f * s);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) s += __shfl(t, base + m);
+    if (ch == 0) lds[C.o_r + cc] = SV::rnd(0.125f * s);
   }
   __syncthreads();
 }
@@ -615,37 +739,53 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_gc_tail(TailSet T, Ctl* ctl) {
 #pragma clang fp contract(off)
   using SV = Store<ST>;
   if (!ctl->active) return;
-  extern __shared__ float lds[];
+  extern __shared__ __attribute__((aligned(16))) float lds[];
   __shared__ GcTailLvl tl[TAIL_MAX_LEVELS];
+  int total = 0;  // cells of all tail levels
+  for (int l = 0; l < T.count; ++l) total += (int)T.lv[l].n;
   if (threadIdx.x == 0) {
-    int off = 0;
+    int off = 0, xr = 4 * total;
     for (int l = 0; l < T.count; ++l) {
       const Level& g = T.lv[l];
       GcTailLvl& L = tl[l];
       L.nx = g.nx; L.ny = g.ny; L.nz = g.nz; L.n = (int)g.n; L.inv = g.inv;
-      L.rnx = 1.0f / (float)g.nx; L.rny = 1.0f / (float)g.ny;
-      L.o_x = off; off += L.n;
-      L.o_r = off; off += L.n;
+      L.rnx = 1.0f / (float)g.nx; L.rny = 1.0f / (float)g.ny; L.rhx = 1.0f / (float)(g.nx >> 1 > 0 ? g.nx >> 1 : 1);
       L.o_fx = off; off += L.n;
       L.o_fy = off; off += L.n;
       L.o_fz = off; off += L.n;
       L.o_dd = off; off += L.n;
+      L.o_x = xr; xr += L.n;
+      L.o_r = xr; xr += L.n;
     }
   }
-  __syncthreads();
-  // every level's couplings in one round of independent loads
-  for (int l = 0; l < T.count; ++l) {
-    const Level& g = T.lv[l];
-    const GcTailLvl L = tl[l];
-    for (int c = (int)threadIdx.x; c < L.n; c += TAIL_THREADS) {
-      int i, j, k;
-    gct_decode(L, c, i, j, k);
-      lds[L.o_fx + c] = i > 0 ? g.ou[((size_t)k * L.ny + j) * (L.nx + 1) + i] : 0.f;
-      lds[L.o_fy + c] = j > 0 ? g.ov[((size_t)k * (L.ny + 1) + j) * L.nx + i] : 0.f;
-      lds[L.o_fz + c] = k > 0 ? g.ow[((size_t)k * L.ny + j) * L.nx + i] : 0.f;
-      lds[L.o_dd + c] = g.dd[c];
-      if (l == 0) lds[L.o_r + c] = SV::ld(reinterpret_cast<const typename SV::T*>(g.r), c);
+  // the couplings of every level: one contiguous copy of 4 * total floats
+  // (a multiple of four, 16-byte aligned at both ends), one round of loads
+  {
+    const float4* src = reinterpret_cast<const float4*>(T.pack);
+    float4* dst = reinterpret_cast<float4*>(lds);
+    constexpr int MAXQ = (TAIL_LDS_FLOATS / 6 + TAIL_THREADS - 1) / TAIL_THREADS;  // float4s per thread
+    float4 v[MAXQ];
+#pragma unroll
+    for (int u = 0; u < MAXQ; ++u) {
+      const int q = (int)threadIdx.x + u * TAIL_THREADS;
+      v[u] = q < total ? src[q] : f4zero();
     }
+    const int c0 = (int)threadIdx.x;
+    const float r0 = c0 < (int)T.lv[0].n ? SV::ld(reinterpret_cast<const typename SV::T*>(T.lv[0].r), c0) : 0.f;
+    const float r1 = c0 + TAIL_THREADS < (int)T.lv[0].n
+                         ? SV::ld(reinterpret_cast<const typename SV::T*>(T.lv[0].r), c0 + TAIL_THREADS) : 0.f;
+    const float r2 = c0 + 2 * TAIL_THREADS < (int)T.lv[0].n
+                         ? SV::ld(reinterpret_cast<const typename SV::T*>(T.lv[0].r), c0 + 2 * TAIL_THREADS) : 0.f;
+#pragma unroll
+    for (int u = 0; u < MAXQ; ++u) {
+      const int q = (int)threadIdx.x + u * TAIL_THREADS;
+      if (q < total) dst[q] = v[u];
+    }
+    __syncthreads();  // tl
+    const int orr = tl[0].o_r, n0 = tl[0].n;
+    if (c0 < n0) lds[orr + c0] = r0;
+    if (c0 + TAIL_THREADS < n0) lds[orr + c0 + TAIL_THREADS] = r1;
+    if (c0 + 2 * TAIL_THREADS < n0) lds[orr + c0 + 2 * TAIL_THREADS] = r2;
   }
   __syncthreads();
   const int last = T.count - 1;
@@ -667,5 +807,6 @@ inline size_t gc_tail_lds_bytes(const Level* lv, int count) {
   for (int l = 0; l < count; ++l) floats += 6 * lv[l].n;
   return floats * sizeof(float);
 }
+static_assert(TAIL_LDS_FLOATS / 6 <= 3 * TAIL_THREADS, "the first tail level's r in three loads per thread");
 
 }  // namespace fsp

@@ -364,9 +364,11 @@ __global__ void __launch_bounds__(QBLOCK) k_q_update(size_t nquads, double* p, f
 // ---------------------------------------------------------------- small coarse levels in one workgroup
 constexpr int TAIL_THREADS = 1024;
 constexpr int TAIL_MAX_LEVELS = 8;
+constexpr int TAIL_LDS_FLOATS = 64 * 1024 / 4;  // LDS of the tail workgroup
 
 struct TailSet {
   Level lv[TAIL_MAX_LEVELS];  // global arrays of the tail levels (first .. last)
+  const float* pack;          // Galerkin tail: every level's couplings in LDS order (k_gct_pack)
   int count;
   int smoother;               // FSP_SMOOTH_*
   int pre, post, coarse;
