@@ -48,6 +48,7 @@ struct fsp_solver {
   float* col_floor = nullptr;  // per column: top of its highest solid cell (0 without one)
   uint16_t* gk0 = nullptr;  // per column: lowest ghost row
   float* gtab = nullptr;    // per column: 1/theta of G_SLOTS ghost rows
+  float* precv = nullptr;   // per cell: row_prec (quad V-cycle)
   float *b = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *zt = nullptr, *d0 = nullptr,
         *d1 = nullptr;
   float *us = nullptr, *vs = nullptr, *ws = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
@@ -99,6 +100,7 @@ struct fsp_solver {
     G.mask = mask;
     G.gk0 = gk0;
     G.gtab = gtab;
+    G.precv = precv;
     // prec of a plain row with n faces, 1/(n * inv), as rhs_mobile3 computes it
     G.prec_tab[0] = 0.f;
     for (int q = 1; q < 7; ++q) G.prec_tab[q] = 1.0f / ((float)q * G.inv);
@@ -250,6 +252,7 @@ struct Launch {
     // ghost-row table from the fresh row codes (k_mask's finish does not touch it)
     Fine Gm = G;
     FSP_LAUNCH(S, k_ghost_table, dim3(grid_for(S->ncols)), st, Gm, S->gk0, S->gtab);
+    if (S->precv) FSP_LAUNCH(S, k_prec, g, st, Gm, S->precv);
   }
 
   // Coarse geometry of every level (prepare_multigrid3, delta3d_multigrid.rs:370-387).
@@ -527,10 +530,12 @@ struct Launch {
                    S->partials);
       if (op >= 0) FSP_FINISH(S, st, op, g, false, 1);
     };
-    if (from_guess) pass(z, 0, -1);
-    else FSP_LAUNCH_N(S, "k_q_gs<1>", K_QGS(1), g, st, G, Q, (const TS<ST>*)z, z, S->r, 0, L0, none, -1, S->ctl,
-                      S->partials);
-    pass(z, 1, -1);
+    if (from_guess) {
+      pass(z, 0, -1);
+      pass(z, 1, -1);
+    } else {
+      FSP_LAUNCH_N(S, "k_q_gs_first", k_q_gs_first<ST>, g, st, G, Q, z, S->r, S->ctl);
+    }
     for (int s = 1; s < S->opt.pre_sweeps; ++s) {
       pass(z, 0, -1);
       pass(z, 1, -1);
@@ -1092,6 +1097,8 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
   }
   if (opt.preconditioner == FSP_PRECOND_LEARNED && opt.learned_vcycle && !quad_vcycle_ok(S->quad, opt))
     return fail(FSP_INVALID);
+  if (opt.preconditioner != FSP_PRECOND_JACOBI && quad_vcycle_ok(S->quad, opt) && (st = dalloc(S, &S->precv, S->n)))
+    return fail(st);
   S->tail_first = (int)S->levels.size();
   for (int t = (int)S->levels.size() - 1; t >= 0; --t) {
     if (S->levels[t].n > 4096 || (int)S->levels.size() - t > TAIL_MAX_LEVELS) break;

@@ -74,6 +74,7 @@ struct Fine {
   const uint8_t* mask;   // cells (row codes)
   const uint16_t* gk0;   // columns: lowest ghost row (0xFFFF: none)
   const float* gtab;     // columns x G_SLOTS x 8: 1/theta of the ghost faces
+  const float* precv;    // cells: row_prec of every row (k_prec)
   float prec_tab[7];     // 1/(n * inv) for a plain row with n faces
 };
 

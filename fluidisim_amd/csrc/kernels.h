@@ -365,6 +365,16 @@ __device__ __forceinline__ float row_prec(const Fine& G, int i, int j, int k, ui
 // prec == 0 exactly on dry cells and on wet cells without any face.
 __device__ __forceinline__ bool prec_positive(uint8_t m) { return (m & M_WET) && (m & (0x3F | M_GHOST)); }
 
+// row_prec of every cell, for kernels that need the factor of neighbouring
+// rows (the first red-black pass recomputed beside the second).
+__global__ void __launch_bounds__(BLOCK) k_prec(Fine G, float* out) {
+  FSP_FOR(c, G.n) {
+    int i, j, k;
+    decode(G, c, i, j, k);
+    out[c] = row_prec(G, i, j, k, G.mask[c]);
+  }
+}
+
 // delta3d.rs:747-763 (plain) and :806-823 (cut with fixed solids).
 __device__ __forceinline__ float divergence_at(const Fine& G, const float* u, const float* v, const float* w, int i,
                                                int j, int k, size_t c) {

@@ -113,8 +113,8 @@ __device__ __forceinline__ void x_neighbours(const QuadGrid& Q, bool valid, int 
 }
 
 // ---------------------------------------------------------------- red-black pass
-// One colour of a red-black Gauss-Seidel sweep on quads, in place.
-// MODE 1: the first pass from z = 0 (colour cells get prec*r, the others 0).
+// One colour of a red-black Gauss-Seidel sweep on quads, in place (the first
+// pass from z = 0 is fused with the second in k_q_gs_first).
 // MODE 2: z is first prolonged from the first coarse level (z += x_c of the
 //         parent, 0 on rows with prec == 0), then the colour is updated;
 //         neighbours are prolonged on the fly, so this mode reads z and
@@ -135,24 +135,7 @@ __global__ void __launch_bounds__(QBLOCK, FSP_QGS_MIN_BLOCKS) k_q_gs(Fine G, Qua
   const size_t plane = (size_t)G.nx * G.ny;
   const size_t c0 = valid ? ((size_t)k * G.ny + y) * G.nx + 4 * (size_t)qx : 0;
   double s = 0.0, unused = 0.0;
-  if (MODE == 1) {
-    if (valid) {
-      const uint32_t mq = *reinterpret_cast<const uint32_t*>(G.mask + c0);
-      const float4 rq = ld4(r, c0);
-      float4 out = f4zero();
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        if (((e + y + k) & 1) != colour) continue;
-        const uint8_t m = (uint8_t)(mq >> (8 * e));
-        const float prec = row_prec(G, 4 * qx + e, y, k, m);
-        q_set(out, e, prec > 0.f ? SV::rnd(prec * q_get(rq, e)) : 0.f);
-      }
-      SV::st4(zout, c0, out);
-      if (dot_op >= 0) s = (double)rq.x * out.x + (double)rq.y * out.y + (double)rq.z * out.z + (double)rq.w * out.w;
-    }
-    if (dot_op >= 0) reduce_finish<false>(s, unused, dot_op, ctl, partials);
-    return;
-  }
+  static_assert(MODE == 0 || MODE == 2, "in place, or prolongation then update");
   // parent values of the prolongation
   auto parent = [&](int yy, int kk, int xq) -> float2 { return SV::ld2(xc, lidx(L0, 2 * xq, yy >> 1, kk >> 1)); };
   // prolongation z + parent, rounded to storage (prolong_add3 stores it)
@@ -208,6 +191,76 @@ __global__ void __launch_bounds__(QBLOCK, FSP_QGS_MIN_BLOCKS) k_q_gs(Fine G, Qua
     if (dot_op >= 0) s = (double)rq.x * out.x + (double)rq.y * out.y + (double)rq.z * out.z + (double)rq.w * out.w;
   }
   if (dot_op >= 0) reduce_finish<false>(s, unused, dot_op, ctl, partials);
+}
+
+// The first two passes of the pre-smoothing from z = 0 in one launch: the
+// colour-0 pass from zero is local (colour-0 cells row_prec * r, colour-1
+// cells 0), so the colour-1 pass recomputes those values for its neighbours
+// from r and the per-cell row_prec (k_prec) instead of reading them back.
+// Every value is the one the two single passes store, so the result is
+// theirs bit for bit, for one pass's traffic plus the row_prec reads (a
+// variant deriving plain rows' factor from the row code ran slower).
+template <int ST>
+__global__ void __launch_bounds__(QBLOCK, FSP_QGS_MIN_BLOCKS) k_q_gs_first(Fine G, QuadGrid Q,
+                                                                         typename Store<ST>::T* zout, const float* r,
+                                                                         Ctl* ctl) {
+#pragma clang fp contract(off)
+  using SV = Store<ST>;
+  if (!ctl->active) return;
+  int qx, y, k;
+  const bool valid = quad_coords(Q, qx, y, k);
+  const size_t plane = (size_t)G.nx * G.ny;
+  const size_t c0 = valid ? ((size_t)k * G.ny + y) * G.nx + 4 * (size_t)qx : 0;
+  // the first pass's value of cell (x, yy, kk): row_prec * r on colour 0
+  auto first = [&](int x, int yy, int kk, float prec, float rv) -> float {
+    if (((x + yy + kk) & 1) != 0) return 0.f;
+    return prec > 0.f ? SV::rnd(prec * rv) : 0.f;
+  };
+  auto first4 = [&](const float4& pq, const float4& rq4, int yy, int kk) {
+    float4 v;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) q_set(v, e, first(4 * qx + e, yy, kk, q_get(pq, e), q_get(rq4, e)));
+    return v;
+  };
+  float4 rq = f4zero(), zc = f4zero(), ym = f4zero(), yp = f4zero(), zm = f4zero(), zp = f4zero();
+  uint32_t mq = 0;
+  if (valid) {
+    // every load first, then the values
+    mq = *reinterpret_cast<const uint32_t*>(G.mask + c0);
+    rq = ld4(r, c0);
+    const float4 pc = ld4(G.precv, c0);
+    const bool hym = y > 0, hyp = y + 1 < G.ny, hzm = k > 0, hzp = k + 1 < G.nz;
+    const size_t cym = c0 - G.nx, cyp = c0 + G.nx, czm = c0 - plane, czp = c0 + plane;
+    const float4 r_ym = hym ? ld4(r, cym) : f4zero(), p_ym = hym ? ld4(G.precv, cym) : f4zero();
+    const float4 r_yp = hyp ? ld4(r, cyp) : f4zero(), p_yp = hyp ? ld4(G.precv, cyp) : f4zero();
+    const float4 r_zm = hzm ? ld4(r, czm) : f4zero(), p_zm = hzm ? ld4(G.precv, czm) : f4zero();
+    const float4 r_zp = hzp ? ld4(r, czp) : f4zero(), p_zp = hzp ? ld4(G.precv, czp) : f4zero();
+    zc = first4(pc, rq, y, k);
+    if (hym) ym = first4(p_ym, r_ym, y - 1, k);
+    if (hyp) yp = first4(p_yp, r_yp, y + 1, k);
+    if (hzm) zm = first4(p_zm, r_zm, y, k - 1);
+    if (hzp) zp = first4(p_zp, r_zp, y, k + 1);
+  }
+  float left, right;
+  x_neighbours(Q, valid, qx, zc, left, right, [&](int off) {
+    const size_t c = c0 + off;
+    return first(4 * qx + off, y, k, G.precv[c], r[c]);
+  });
+  if (!valid) return;
+  float4 out = zc;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (((e + y + k) & 1) != 1) continue;
+    const uint8_t m = (uint8_t)(mq >> (8 * e));
+    const float pc = q_get(zc, e);
+    const float xl = e == 0 ? left : q_get(zc, e - 1);
+    const float xr = e == 3 ? right : q_get(zc, e + 1);
+    const float nb[6] = {xl, xr, q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
+    float prec;
+    const float az = row_local(G, 4 * qx + e, y, k, m, pc, nb, prec);
+    if (prec > 0.f) q_set(out, e, SV::rnd(pc + prec * (q_get(rq, e) - az)));
+  }
+  SV::st4(zout, c0, out);
 }
 
 // ---------------------------------------------------------------- residual + restriction
