@@ -373,7 +373,10 @@ struct Launch {
       if (count & 1) colour(0);
       for (int s = 0; s < half; ++s) { colour(1); colour(0); }
     } else {
-      for (int s = 0; s < count; ++s) { colour(0); colour(1); }
+      // the first sweep's two passes in one launch
+      FSP_LAUNCH_N(S, "k_gc_gs_first", k_gc_gs_first<ST>, g, st, L, S->cq[l], x, S->ctl);
+      first = false;
+      for (int s = 1; s < count; ++s) { colour(0); colour(1); }
     }
     return x;
   }
@@ -1126,7 +1129,8 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
       QuadGrid q = make_q(L.nx / 4, L.ny, L.nz);
       const size_t blocks = (size_t)q.nbx * q.nby * q.planes;
       if (blocks > 0xFFFFFFFFu) continue;
-      if ((st = dalloc(S, &L.cx, L.n)) || (st = dalloc(S, &L.cy, L.n)) || (st = dalloc(S, &L.cz, L.n)))
+      if ((st = dalloc(S, &L.cx, L.n)) || (st = dalloc(S, &L.cy, L.n)) || (st = dalloc(S, &L.cz, L.n)) ||
+          (st = dalloc(S, &L.dg, L.n)))
         return fail(st);
       L.compact = 1;
       S->cq[l] = q;

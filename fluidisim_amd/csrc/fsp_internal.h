@@ -97,6 +97,7 @@ struct Level {
   uint16_t* cx;          // compact levels: binary16 coupling of each cell's x- face
   uint16_t* cy;          // ... y- face
   uint16_t* cz;          // ... z- face
+  float* dg;             // compact levels: diagonal of each row (gc_diag of the binary16 couplings)
 };
 
 // ---------------------------------------------------------------- V-cycle storage

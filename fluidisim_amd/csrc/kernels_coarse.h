@@ -31,17 +31,29 @@ __device__ __forceinline__ float gc_ld(const uint16_t* a, size_t c) {
   return (float)*reinterpret_cast<const _Float16*>(a + c);
 }
 
-// Cell-aligned binary16 couplings of a Galerkin level from its face arrays.
+__device__ __forceinline__ float gc_diag(const float a[6], float dd, float inv);
+
+// Cell-aligned binary16 couplings of a Galerkin level from its face arrays,
+// and each row's diagonal from those binary16 values (what the passes compute).
 __global__ void __launch_bounds__(BLOCK) k_gc_build(Level L) {
   FSP_FOR(c, L.n) {
     int i, j, k;
     ldecode(L, c, i, j, k);
-    const float ax = i > 0 ? L.ou[((size_t)k * L.ny + j) * (L.nx + 1) + i] : 0.f;
-    const float ay = j > 0 ? L.ov[((size_t)k * (L.ny + 1) + j) * L.nx + i] : 0.f;
-    const float az = k > 0 ? L.ow[((size_t)k * L.ny + j) * L.nx + i] : 0.f;
+    const size_t ux = ((size_t)k * L.ny + j) * (L.nx + 1) + i;
+    const size_t vy = ((size_t)k * (L.ny + 1) + j) * L.nx + i;
+    const size_t wz = ((size_t)k * L.ny + j) * L.nx + i;
+    const float ax = i > 0 ? L.ou[ux] : 0.f;
+    const float ay = j > 0 ? L.ov[vy] : 0.f;
+    const float az = k > 0 ? L.ow[wz] : 0.f;
+    const float bx = i + 1 < L.nx ? L.ou[ux + 1] : 0.f;
+    const float by = j + 1 < L.ny ? L.ov[vy + L.nx] : 0.f;
+    const float bz = k + 1 < L.nz ? L.ow[wz + (size_t)L.nx * L.ny] : 0.f;
     reinterpret_cast<_Float16*>(L.cx)[c] = (_Float16)ax;
     reinterpret_cast<_Float16*>(L.cy)[c] = (_Float16)ay;
     reinterpret_cast<_Float16*>(L.cz)[c] = (_Float16)az;
+    const float a[6] = {(float)(_Float16)ax, (float)(_Float16)bx, (float)(_Float16)ay,
+                        (float)(_Float16)by, (float)(_Float16)az, (float)(_Float16)bz};
+    L.dg[c] = gc_diag(a, L.dd[c], L.inv);
   }
 }
 
@@ -172,6 +184,74 @@ __global__ void __launch_bounds__(QBLOCK) k_gc_gs(Level L, QuadGrid Q, const typ
                          q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
     const float row = gc_row(a, dd, L.inv, pc, nb);
     q_set(out, e, SV::rnd(pc + (q_get(rq, e) - row) / dg));
+  }
+  SV::st4(xout, c0, out);
+}
+
+// The first two passes of a compact level's pre-smoothing from x = 0 in one
+// launch (k_gc_gs<1> colour 0, then k_gc_gs<0> colour 1): the first pass is
+// local (colour-0 cells r/diag, the others 0), so the second recomputes its
+// neighbours' values from their r and diagonal (L.dg) instead of reading them
+// back. Same values, same order: bit-identical to the two launches.
+template <int ST>
+__global__ void __launch_bounds__(QBLOCK) k_gc_gs_first(Level L, QuadGrid Q, typename Store<ST>::T* xout,
+                                                        Ctl* ctl) {
+#pragma clang fp contract(off)
+  using SV = Store<ST>;
+  using T = typename SV::T;
+  if (!ctl->active) return;
+  int qx, y, k;
+  const bool valid = quad_coords(Q, qx, y, k);
+  const size_t plane = (size_t)L.nx * L.ny;
+  const size_t c0 = valid ? ((size_t)k * L.ny + y) * L.nx + 4 * (size_t)qx : 0;
+  const T* lr = reinterpret_cast<const T*>(L.r);
+  auto first = [&](int x, int yy, int kk, float dg, float rv) -> float {
+    if (((x + yy + kk) & 1) != 0) return 0.f;
+    return dg > 0.f ? SV::rnd(rv / dg) : 0.f;
+  };
+  auto first4 = [&](const float4& dq, const float4& rq4, int yy, int kk) {
+    float4 v;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) q_set(v, e, first(4 * qx + e, yy, kk, q_get(dq, e), q_get(rq4, e)));
+    return v;
+  };
+  GcQuad g{};
+  float4 rq = f4zero(), xc = f4zero(), ym = f4zero(), yp = f4zero(), zm = f4zero(), zp = f4zero();
+  if (valid) {
+    g = gc_quad(L, c0, 4 * qx, y, k);
+    rq = SV::ld4(lr, c0);
+    const float4 dq = ld4(L.dg, c0);
+    const bool hym = y > 0, hyp = y + 1 < L.ny, hzm = k > 0, hzp = k + 1 < L.nz;
+    const size_t cym = c0 - L.nx, cyp = c0 + L.nx, czm = c0 - plane, czp = c0 + plane;
+    const float4 r_ym = hym ? SV::ld4(lr, cym) : f4zero(), d_ym = hym ? ld4(L.dg, cym) : f4zero();
+    const float4 r_yp = hyp ? SV::ld4(lr, cyp) : f4zero(), d_yp = hyp ? ld4(L.dg, cyp) : f4zero();
+    const float4 r_zm = hzm ? SV::ld4(lr, czm) : f4zero(), d_zm = hzm ? ld4(L.dg, czm) : f4zero();
+    const float4 r_zp = hzp ? SV::ld4(lr, czp) : f4zero(), d_zp = hzp ? ld4(L.dg, czp) : f4zero();
+    xc = first4(dq, rq, y, k);
+    if (hym) ym = first4(d_ym, r_ym, y - 1, k);
+    if (hyp) yp = first4(d_yp, r_yp, y + 1, k);
+    if (hzm) zm = first4(d_zm, r_zm, y, k - 1);
+    if (hzp) zp = first4(d_zp, r_zp, y, k + 1);
+  }
+  float left, right;
+  x_neighbours(Q, valid, qx, xc, left, right, [&](int off) {
+    const size_t c = c0 + off;
+    return first(4 * qx + off, y, k, L.dg[c], SV::ld(lr, c));
+  });
+  if (!valid) return;
+  float4 out = xc;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (((4 * qx + e + y + k) & 1) != 1) continue;
+    float a[6];
+    gc_faces(g, e, a);
+    const float dd = q_get(g.dd, e);
+    const float dg = gc_diag(a, dd, L.inv);
+    if (!(dg > 0.f)) continue;
+    const float pc = q_get(xc, e);
+    const float nb[6] = {e == 0 ? left : q_get(xc, e - 1), e == 3 ? right : q_get(xc, e + 1),
+                         q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
+    q_set(out, e, SV::rnd(pc + (q_get(rq, e) - gc_row(a, dd, L.inv, pc, nb)) / dg));
   }
   SV::st4(xout, c0, out);
 }
