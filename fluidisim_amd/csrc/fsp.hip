@@ -255,14 +255,26 @@ struct Launch {
     if (S->precv) FSP_LAUNCH(S, k_prec, g, st, Gm, S->precv);
   }
 
+  // A level whose V-cycle kernels read only couplings and Dirichlet parts:
+  // a compact level, or a level of the Galerkin tail.
+  bool kindless(int l) const {
+    return S->levels[l].compact || (gc_tail_ok(S->opt) && l >= S->tail_first);
+  }
+
   // Coarse geometry of every level (prepare_multigrid3, delta3d_multigrid.rs:370-387).
+  // Cell kinds and diagonals are built only up to the last level whose
+  // kernels read them (kinds of a level derive from those of the level above).
   void hierarchy() {
+    int last_generic = -1;
+    for (int l = 0; l < (int)S->levels.size(); ++l)
+      if (!kindless(l)) last_generic = l;
     for (size_t l = 0; l < S->levels.size(); ++l) {
       Level& L = S->levels[l];
+      const bool kinds = (int)l <= last_generic, diag = !kindless((int)l);
       const size_t nf = (size_t)(L.nx + 1) * L.ny * L.nz + (size_t)L.nx * (L.ny + 1) * L.nz +
                         (size_t)L.nx * L.ny * (L.nz + 1);
       if (l == 0) {
-        FSP_LAUNCH(S, k_ckind_fine, dim3(grid_for(L.n)), st, G, L);
+        if (kinds) FSP_LAUNCH(S, k_ckind_fine, dim3(grid_for(L.n)), st, G, L);
         if (L.galerkin) {
           FSP_LAUNCH(S, k_cfaces_fine_gal, dim3(grid_for(nf)), st, G, L);
           FSP_LAUNCH(S, k_cdd_fine, dim3(grid_for(L.n)), st, G, L);
@@ -270,11 +282,11 @@ struct Launch {
           FSP_LAUNCH(S, k_cfaces_fine, dim3(grid_for(nf)), st, G, L);
         }
       } else {
-        FSP_LAUNCH(S, k_ckind, dim3(grid_for(L.n)), st, S->levels[l - 1], L);
+        if (kinds) FSP_LAUNCH(S, k_ckind, dim3(grid_for(L.n)), st, S->levels[l - 1], L);
         FSP_LAUNCH(S, k_cfaces, dim3(grid_for(nf)), st, S->levels[l - 1], L);
         if (L.galerkin) FSP_LAUNCH(S, k_cdd, dim3(grid_for(L.n)), st, S->levels[l - 1], L);
       }
-      FSP_LAUNCH(S, k_cdiag, dim3(grid_for(L.n)), st, L);
+      if (diag) FSP_LAUNCH(S, k_cdiag, dim3(grid_for(L.n)), st, L);
       if (L.compact) FSP_LAUNCH(S, k_gc_build, dim3(grid_for(L.n)), st, L);
     }
     if (S->tail_pack) {
