@@ -247,8 +247,14 @@ struct Launch {
 
   void mask() {
     const dim3 g(grid_for(G.n));
-    FSP_LAUNCH(S, k_mask, g, st, G, S->mask, S->ctl, S->partials);
-    FSP_FINISH(S, st, OP_COUNT, g, false, 0);
+    if (S->quad) {
+      const dim3 gq((unsigned)(S->qfine.nbx * S->qfine.nby * S->qfine.planes));
+      FSP_LAUNCH(S, k_mask_q, gq, st, G, S->qfine, S->mask, S->ctl, S->partials);
+      FSP_FINISH(S, st, OP_COUNT, gq, false, 0);
+    } else {
+      FSP_LAUNCH(S, k_mask, g, st, G, S->mask, S->ctl, S->partials);
+      FSP_FINISH(S, st, OP_COUNT, g, false, 0);
+    }
     // ghost-row table from the fresh row codes (k_mask's finish does not touch it)
     Fine Gm = G;
     FSP_LAUNCH(S, k_ghost_table, dim3(grid_for(S->ncols)), st, Gm, S->gk0, S->gtab);
@@ -702,7 +708,7 @@ fsp_status correct_and_measure(fsp_solver* S, const float* us, const float* vs, 
   if (S->opt.gauge) {
     FSP_LAUNCH(S, k_sum_active<double>, dim3(grid_for(K.G.n)), K.st, K.G, (const double*)p,
                        OP_MEAN_P, S->ctl, S->partials);
-    FSP_FINISH(S, K.st, OP_MEAN_P, dim3(grid_for(K.G.n)), false, 0);
+    FSP_FINISH(S, K.st, OP_MEAN_P, dim3(grid_for(K.G.n)), false, 2);
     FSP_LAUNCH(S, k_sub_mean<double>, dim3(grid_for(K.G.n)), K.st, K.G, p, 1, -1, S->ctl,
                        S->partials);
   }
@@ -802,10 +808,10 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
   if (S->opt.gauge) {
     FSP_LAUNCH(S, k_sum_active<float>, dim3(grid_for(G.n)), K.st, G, (const float*)S->b,
                        OP_MEAN_B, S->ctl, S->partials);
-    FSP_FINISH(S, K.st, OP_MEAN_B, dim3(grid_for(G.n)), false, 0);
+    FSP_FINISH(S, K.st, OP_MEAN_B, dim3(grid_for(G.n)), false, 2);
     FSP_LAUNCH(S, k_sub_mean<float>, dim3(grid_for(G.n)), K.st, G, S->b, 0, OP_B2, S->ctl,
                        S->partials);
-    FSP_FINISH(S, K.st, OP_B2, dim3(grid_for(G.n)), false, 0);
+    FSP_FINISH(S, K.st, OP_B2, dim3(grid_for(G.n)), false, 2);
   }
   double* p64 = S->p64;
   FSP_LAUNCH(S, k_load_p, dim3(grid_for(G.n)), K.st, G, (const float*)p, p64, warm);

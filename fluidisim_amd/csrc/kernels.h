@@ -151,7 +151,9 @@ __device__ __forceinline__ void reduce_finish(double a, double b, int op, Ctl* c
 }
 
 // Folds the partials of `nblocks` blocks in block order and applies the
-// finish operation. `gated`: skipped like its producer once the loop stopped.
+// finish operation. `gated`: 1, skipped like its producer once the loop
+// stopped; 2, skipped like the gauge kernels unless the problem is pure
+// Neumann.
 // FIN_THREADS threads with eight loads each fold 8192 partials (the fine
 // kernels' grid) in one round of memory latency.
 constexpr int FIN_THREADS = 1024;
@@ -159,7 +161,7 @@ constexpr int FIN_THREADS = 1024;
 template <bool MAX>
 __global__ void __launch_bounds__(FIN_THREADS) k_finish(int op, unsigned nblocks, int gated, Ctl* ctl,
                                                         double* partials) {
-  if (gated && !ctl->active) return;
+  if ((gated == 1 && !ctl->active) || (gated == 2 && !ctl->pure_neumann)) return;
   __shared__ double sa[FIN_THREADS / 64], sb[FIN_THREADS / 64];
   const double2* part = reinterpret_cast<const double2*>(partials + (size_t)op * MAX_PARTIALS * 2);
   // eight independent loads in flight per step, folded in a fixed order
@@ -398,6 +400,34 @@ __device__ __forceinline__ float divergence_at(const Fine& G, const float* u, co
 // ghost face. Every operand of a cell (its six apertures, seven solid flags,
 // five column heights) is loaded at once, indices clamped into the grid, and
 // the types are decided after; full_row gives the same types.
+// Row code of cell (i, j, k) from its solid flag s_c, its apertures ap
+// (x-, x+, y-, y+, z-, z+), the solid flags sn of its neighbours (same order,
+// clamped into the grid) and the surface heights of its column (h) and of the
+// four side columns (hn, clamped); counts wet rows and ghost faces.
+__device__ __forceinline__ uint8_t row_code(const Fine& G, int i, int j, int k, uint8_t s_c, const uint8_t sn[6],
+                                            const uint8_t ap[6], float h, const float hn[4], double& wet,
+                                            double& ghosts) {
+  const float zc = zc_of(G, k);
+  uint8_t m = 0;
+  if (!s_c && zc < h) {
+    m = M_WET;
+    const bool has[4] = {i > 0, i + 1 < G.nx, j > 0, j + 1 < G.ny};
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      if (!has[f] || ap[f] == 0 || sn[f]) continue;
+      if (zc < hn[f]) m |= (uint8_t)(1u << f);  // wet neighbour (not solid, below its surface)
+      else { m |= M_GHOST; ghosts += 1.0; }
+    }
+    if (k > 0 && ap[4] && !sn[4]) m |= M_ZM;
+    if (ap[5]) {
+      if (k + 1 < G.nz && !sn[5] && zc_of(G, k + 1) < h) m |= M_ZP;
+      else { m |= M_GHOST; ghosts += 1.0; }
+    }
+    wet += 1.0;
+  }
+  return m;
+}
+
 __global__ void __launch_bounds__(BLOCK) k_mask(Fine G, uint8_t* mask, Ctl* ctl, double* partials) {
   double wet = 0.0, ghosts = 0.0;
   const int nx = G.nx, ny = G.ny, nz = G.nz;
@@ -418,27 +448,9 @@ __global__ void __launch_bounds__(BLOCK) k_mask(Fine G, uint8_t* mask, Ctl* ctl,
     const float h = G.eta[(size_t)j * nx + i];
     const float h0 = G.eta[(size_t)j * nx + im], h1 = G.eta[(size_t)j * nx + ip];
     const float h2 = G.eta[(size_t)jm * nx + i], h3 = G.eta[(size_t)jp * nx + i];
-    const float zc = zc_of(G, k);
-    uint8_t m = 0;
-    if (!s_c && zc < h) {
-      m = M_WET;
-      const bool has[4] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny};
-      const uint8_t ap[4] = {a0, a1, a2, a3}, sn[4] = {s0, s1, s2, s3};
-      const float hn[4] = {h0, h1, h2, h3};
-#pragma unroll
-      for (int f = 0; f < 4; ++f) {
-        if (!has[f] || ap[f] == 0 || sn[f]) continue;
-        if (zc < hn[f]) m |= (uint8_t)(1u << f);  // wet neighbour (not solid, below its surface)
-        else { m |= M_GHOST; ghosts += 1.0; }
-      }
-      if (k > 0 && a4 && !s4) m |= M_ZM;
-      if (a5) {
-        if (k + 1 < nz && !s5 && zc_of(G, k + 1) < h) m |= M_ZP;
-        else { m |= M_GHOST; ghosts += 1.0; }
-      }
-      wet += 1.0;
-    }
-    mask[c] = m;
+    const uint8_t ap[6] = {a0, a1, a2, a3, a4, a5}, sn[6] = {s0, s1, s2, s3, s4, s5};
+    const float hn[4] = {h0, h1, h2, h3};
+    mask[c] = row_code(G, i, j, k, s_c, sn, ap, h, hn, wet, ghosts);
   }
   reduce_finish<false>(wet, ghosts, OP_COUNT, ctl, partials);
 }
@@ -450,18 +462,29 @@ __global__ void __launch_bounds__(BLOCK) k_mask(Fine G, uint8_t* mask, Ctl* ctl,
 __global__ void __launch_bounds__(BLOCK) k_ghost_table(Fine G, uint16_t* gk0, float* gtab) {
   const size_t cols = (size_t)G.nx * G.ny;
   const size_t plane = cols;
+  constexpr int SCAN = 16;  // rows per round of independent loads
   FSP_FOR(col, cols) {
     const int i = (int)(col % (size_t)G.nx), j = (int)(col / (size_t)G.nx);
+    // the lowest ghost row, SCAN rows per round
     int k0 = -1;
-    for (int k = 0; k < G.nz; ++k)
-      if (G.mask[(size_t)k * plane + col] & M_GHOST) { k0 = k; break; }
+    for (int kb = 0; kb < G.nz && k0 < 0; kb += SCAN) {
+      uint8_t mm[SCAN];
+#pragma unroll
+      for (int u = 0; u < SCAN; ++u) mm[u] = kb + u < G.nz ? G.mask[(size_t)(kb + u) * plane + col] : 0;
+#pragma unroll
+      for (int u = 0; u < SCAN; ++u)
+        if (k0 < 0 && (mm[u] & M_GHOST)) k0 = kb + u;
+    }
     gk0[col] = (uint16_t)(k0 < 0 ? 0xFFFF : k0);
     if (k0 < 0) continue;
+    uint8_t ms[G_SLOTS];
+#pragma unroll
+    for (int s = 0; s < G_SLOTS; ++s) ms[s] = k0 + s < G.nz ? G.mask[(size_t)(k0 + s) * plane + col] : 0;
     for (int s = 0; s < G_SLOTS && k0 + s < G.nz; ++s) {
       const int k = k0 + s;
       float* e = gtab + (col * G_SLOTS + s) * 8;
       float it[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      if (G.mask[(size_t)k * plane + col] & M_GHOST) {
+      if (ms[s] & M_GHOST) {
         RowG R;
         full_row(G, i, j, k, R);
         for (int f = 0; f < 6; ++f) it[f] = R.type[f] == 2 ? R.it[f] : 0.f;
@@ -511,8 +534,10 @@ __global__ void __launch_bounds__(BLOCK) k_assemble(Fine G, const float* us, con
 }
 
 // Sum of x and count over rows with prec > 0 (gauge of a pure-Neumann problem).
+// Skipped (with its fold, k_finish gated == 2) unless the problem is pure Neumann.
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_sum_active(Fine G, const T* x, int op, Ctl* ctl, double* partials) {
+  if (!ctl->pure_neumann) return;
   double s = 0.0, n = 0.0;
   FSP_FOR(c, G.n) {
     if (prec_positive(G.mask[c])) { s += (double)x[c]; n += 1.0; }
@@ -521,10 +546,11 @@ __global__ void __launch_bounds__(BLOCK) k_sum_active(Fine G, const T* x, int op
 }
 
 // x -= mean on active rows when the problem is pure Neumann; for b also
-// recompute |b|^2.
+// recompute |b|^2 (otherwise x and |b|^2 stand: skipped, with its fold).
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_sub_mean(Fine G, T* x, int which, int op, Ctl* ctl, double* partials) {
   const int pure = ctl->pure_neumann;
+  if (!pure) return;
   const T mean = (T)(which == 0 ? ctl->mean_b : ctl->mean_p);
   double s2 = 0.0, unused = 0.0;
   FSP_FOR(c, G.n) {

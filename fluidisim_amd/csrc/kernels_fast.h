@@ -263,6 +263,48 @@ __global__ void __launch_bounds__(QBLOCK, FSP_QGS_MIN_BLOCKS) k_q_gs_first(Fine 
   SV::st4(zout, c0, out);
 }
 
+// Row codes on quads (k_mask with four x-cells per thread: one decode, and
+// the solid flags, apertures and surface heights of the quad and of its y and
+// z neighbours as 4-byte and 16-byte loads). Same row_code per cell.
+__global__ void __launch_bounds__(QBLOCK) k_mask_q(Fine G, QuadGrid Q, uint8_t* mask, Ctl* ctl, double* partials) {
+  double wet = 0.0, ghosts = 0.0;
+  int qx, j, k;
+  const bool valid = quad_coords(Q, qx, j, k);
+  if (valid) {
+    const int nx = G.nx, ny = G.ny, nz = G.nz, i0 = 4 * qx;
+    const int jm = j > 0 ? j - 1 : j, jp = j + 1 < ny ? j + 1 : j;
+    const int km = k > 0 ? k - 1 : k, kp = k + 1 < nz ? k + 1 : k;
+    const int im0 = i0 > 0 ? i0 - 1 : i0, ip3 = i0 + 4 < nx ? i0 + 4 : i0 + 3;
+    auto u32 = [](const uint8_t* p, size_t c) { return *reinterpret_cast<const uint32_t*>(p + c); };
+    const uint32_t sc = u32(G.solid, cidx(G, i0, j, k));
+    const uint32_t sym = u32(G.solid, cidx(G, i0, jm, k)), syp = u32(G.solid, cidx(G, i0, jp, k));
+    const uint32_t szm = u32(G.solid, cidx(G, i0, j, km)), szp = u32(G.solid, cidx(G, i0, j, kp));
+    const uint8_t sxm = G.solid[cidx(G, im0, j, k)], sxp = G.solid[cidx(G, ip3, j, k)];
+    const size_t fu = fuidx(G, i0, j, k);
+    uint8_t au[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) au[e] = G.ou[fu + e];
+    const uint32_t av0 = u32(G.ov, fvidx(G, i0, j, k)), av1 = u32(G.ov, fvidx(G, i0, j + 1, k));
+    const uint32_t aw0 = u32(G.ow, fwidx(G, i0, j, k)), aw1 = u32(G.ow, fwidx(G, i0, j, k + 1));
+    const float4 h4 = ld4(G.eta, (size_t)j * nx + i0);
+    const float4 hym = ld4(G.eta, (size_t)jm * nx + i0), hyp = ld4(G.eta, (size_t)jp * nx + i0);
+    const float hxm = G.eta[(size_t)j * nx + im0], hxp = G.eta[(size_t)j * nx + ip3];
+    auto byte = [](uint32_t w, int e) { return (uint8_t)(w >> (8 * e)); };
+    uint32_t out = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const uint8_t ap[6] = {au[e], au[e + 1], byte(av0, e), byte(av1, e), byte(aw0, e), byte(aw1, e)};
+      const uint8_t sn[6] = {e > 0 ? byte(sc, e - 1) : sxm, e < 3 ? byte(sc, e + 1) : sxp, byte(sym, e),
+                             byte(syp, e), byte(szm, e), byte(szp, e)};
+      const float hn[4] = {e > 0 ? q_get(h4, e - 1) : hxm, e < 3 ? q_get(h4, e + 1) : hxp, q_get(hym, e),
+                           q_get(hyp, e)};
+      out |= (uint32_t)row_code(G, i0 + e, j, k, byte(sc, e), sn, ap, q_get(h4, e), hn, wet, ghosts) << (8 * e);
+    }
+    *reinterpret_cast<uint32_t*>(mask + cidx(G, i0, j, k)) = out;
+  }
+  reduce_finish<false>(wet, ghosts, OP_COUNT, ctl, partials);
+}
+
 // ---------------------------------------------------------------- residual + restriction
 // One thread per (fine quad column qx, coarse row J, coarse plane K): the
 // residual t = r - A z of its 16 fine cells, summed per coarse cell in the
