@@ -313,6 +313,19 @@ struct Launch {
     return T;
   }
 
+  // r = b - A p64 and |r|^2 into `op`, on quads where the grid has them.
+  void residual(const double* p64, int op) {
+    if (S->quad) {
+      const dim3 gq((unsigned)(S->qfine.nbx * S->qfine.nby * S->qfine.planes));
+      FSP_LAUNCH(S, k_residual_q, gq, st, G, S->qfine, p64, S->b, S->r, op, S->ctl, S->partials);
+      FSP_FINISH(S, st, op, gq, false, 0);
+    } else {
+      const dim3 g(grid_for(G.n));
+      FSP_LAUNCH(S, k_residual, g, st, G, p64, S->b, S->r, op, S->ctl, S->partials);
+      FSP_FINISH(S, st, op, g, false, 0);
+    }
+  }
+
   void fill_gated(float* x, size_t n, float v) {
     FSP_LAUNCH(S, k_fill_gated, dim3(grid_for(n)), st, x, n, v, S->ctl);
   }
@@ -815,9 +828,7 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
   }
   double* p64 = S->p64;
   FSP_LAUNCH(S, k_load_p, dim3(grid_for(G.n)), K.st, G, (const float*)p, p64, warm);
-  FSP_LAUNCH(S, k_residual, dim3(grid_for(G.n)), K.st, G, (const double*)p64, S->b, S->r, OP_RR,
-                     S->ctl, S->partials);
-  FSP_FINISH(S, K.st, OP_RR, dim3(grid_for(G.n)), false, 0);
+  K.residual(p64, OP_RR);
   fsp_status st = sync_ctl(S);
   if (st) return st;
   if (S->hctl->nonfinite) return FSP_NOT_FINITE;
@@ -893,9 +904,7 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
       if (net_failed) K.set_pass(0, rr, tol_b2, physical_target, it, max_iters);  // clears the flag
     }
     // True residual, recomputed: it decides, never the recurrence (:546-553).
-    FSP_LAUNCH(S, k_residual, dim3(grid_for(G.n)), K.st, G, (const double*)p64, S->b, S->r,
-                       OP_RR, S->ctl, S->partials);
-    FSP_FINISH(S, K.st, OP_RR, dim3(grid_for(G.n)), false, 0);
+    K.residual(p64, OP_RR);
     if ((st = sync_ctl(S))) return st;
     if (S->hctl->nonfinite) return FSP_NOT_FINITE;
     const double actual = S->hctl->actual;

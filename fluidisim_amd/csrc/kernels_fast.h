@@ -305,6 +305,65 @@ __global__ void __launch_bounds__(QBLOCK) k_mask_q(Fine G, QuadGrid Q, uint8_t* 
   reduce_finish<false>(wet, ghosts, OP_COUNT, ctl, partials);
 }
 
+// r = b - A p, |r|^2 on quads (k_residual with four x-cells per thread: the
+// pressures of the quad and of its y and z neighbours as 16-byte loads with
+// indices clamped into the grid, the row's case decided after; apply_row_t's
+// arithmetic per cell).
+__global__ void __launch_bounds__(QBLOCK) k_residual_q(Fine G, QuadGrid Q, const double* p, const float* b, float* r,
+                                                       int op, Ctl* ctl, double* partials) {
+#pragma clang fp contract(off)
+  double s = 0.0, unused = 0.0;
+  int qx, y, k;
+  const bool valid = quad_coords(Q, qx, y, k);
+  if (valid) {
+    const size_t plane = (size_t)G.nx * G.ny;
+    const size_t c0 = ((size_t)k * G.ny + y) * G.nx + 4 * (size_t)qx;
+    const uint32_t mq = *reinterpret_cast<const uint32_t*>(G.mask + c0);
+    const float4 bq = ld4(b, c0);
+    auto quad = [&](size_t c, double v[4]) {
+      const double2 a = *reinterpret_cast<const double2*>(p + c), e = *reinterpret_cast<const double2*>(p + c + 2);
+      v[0] = a.x; v[1] = a.y; v[2] = e.x; v[3] = e.y;
+    };
+    double pc[4], pym[4], pyp[4], pzm[4], pzp[4];
+    quad(c0, pc);
+    quad(y > 0 ? c0 - G.nx : c0, pym);
+    quad(y + 1 < G.ny ? c0 + G.nx : c0, pyp);
+    quad(k > 0 ? c0 - plane : c0, pzm);
+    quad(k + 1 < G.nz ? c0 + plane : c0, pzp);
+    const double pxm = p[qx > 0 ? c0 - 1 : c0], pxp = p[4 * qx + 4 < G.nx ? c0 + 4 : c0 + 3];
+    float4 out;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const uint8_t m = (uint8_t)(mq >> (8 * e));
+      double acc = 0.0;
+      if (m & M_WET) {
+        const double pe = pc[e];
+        const double nb[6] = {e > 0 ? pc[e - 1] : pxm, e < 3 ? pc[e + 1] : pxp, pym[e], pyp[e], pzm[e], pzp[e]};
+        if (!(m & M_GHOST)) {
+#pragma unroll
+          for (int f = 0; f < 6; ++f)
+            if (m & (1u << f)) acc += pe - nb[f];
+        } else {
+          uint8_t type[6];
+          float it[6];
+          ghost_row(G, 4 * qx + e, y, k, m, type, it);
+#pragma unroll
+          for (int f = 0; f < 6; ++f) {
+            if (type[f] == 1) acc += pe - nb[f];
+            else if (type[f] == 2) acc += pe * (double)it[f];
+          }
+        }
+        acc = acc * (double)G.inv;
+      }
+      const double rc = (double)q_get(bq, e) - acc;
+      q_set(out, e, (float)rc);
+      s += rc * rc;
+    }
+    st4(r, c0, out);
+  }
+  if (op >= 0) reduce_finish<false>(s, unused, op, ctl, partials);
+}
+
 // ---------------------------------------------------------------- residual + restriction
 // One thread per (fine quad column qx, coarse row J, coarse plane K): the
 // residual t = r - A z of its 16 fine cells, summed per coarse cell in the
