@@ -827,8 +827,15 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
     FSP_FINISH(S, K.st, OP_B2, dim3(grid_for(G.n)), false, 2);
   }
   double* p64 = S->p64;
-  FSP_LAUNCH(S, k_load_p, dim3(grid_for(G.n)), K.st, G, (const float*)p, p64, warm);
-  K.residual(p64, OP_RR);
+  if (!warm && S->quad) {
+    const size_t nquads = G.n / 4;
+    const dim3 gs((unsigned)((nquads + QBLOCK - 1) / QBLOCK));
+    FSP_LAUNCH(S, k_start_cold, gs, K.st, nquads, (const float*)S->b, p64, S->r, OP_RR, S->ctl, S->partials);
+    FSP_FINISH(S, K.st, OP_RR, gs, false, 0);
+  } else {
+    FSP_LAUNCH(S, k_load_p, dim3(grid_for(G.n)), K.st, G, (const float*)p, p64, warm);
+    K.residual(p64, OP_RR);
+  }
   fsp_status st = sync_ctl(S);
   if (st) return st;
   if (S->hctl->nonfinite) return FSP_NOT_FINITE;

@@ -364,6 +364,24 @@ __global__ void __launch_bounds__(QBLOCK) k_residual_q(Fine G, QuadGrid Q, const
   if (op >= 0) reduce_finish<false>(s, unused, op, ctl, partials);
 }
 
+// Cold start, p = 0: p64 = 0, r = b - A 0 = b exactly, |r|^2 into `op`
+// (k_load_p and k_residual in one pass over b).
+__global__ void __launch_bounds__(QBLOCK) k_start_cold(size_t nquads, const float* b, double* p, float* r, int op,
+                                                       Ctl* ctl, double* partials) {
+  double s = 0.0, unused = 0.0;
+  const size_t t = (size_t)blockIdx.x * QBLOCK + threadIdx.x;
+  if (t < nquads) {
+    const size_t c = 4 * t;
+    const float4 bq = ld4(b, c);
+    double2* pp = reinterpret_cast<double2*>(p + c);
+    pp[0] = make_double2(0.0, 0.0);
+    pp[1] = make_double2(0.0, 0.0);
+    st4(r, c, bq);
+    s = (double)bq.x * bq.x + (double)bq.y * bq.y + (double)bq.z * bq.z + (double)bq.w * bq.w;
+  }
+  reduce_finish<false>(s, unused, op, ctl, partials);
+}
+
 // ---------------------------------------------------------------- residual + restriction
 // One thread per (fine quad column qx, coarse row J, coarse plane K): the
 // residual t = r - A z of its 16 fine cells, summed per coarse cell in the
