@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 def bytes_per_cell(kernel, zb):
     table = {
         "k_q_gs<0>": 1 + zb + 4 + zb,              # red-black pass: mask, z, r -> z (whole quads)
-        "k_q_gs_first": 1 + 4 + zb,                # first two passes from zero: mask, r -> z
+        "k_q_gs_first": 1 + 4 + 4 + zb,            # first two passes from zero: mask, r, row_prec -> z
         "k_q_gs<2>": 1 + zb + zb / 8 + 4 + zb,     # prolongation + pass: mask, z, coarse x, r -> z'
         "k_q_spmv_dir": 1 + zb + 4 + 4 + 4,        # mask, z, d_old -> d_new, q
         "k_q_update": 8 + 8 + 4 + 4 + 4 + 4,       # p64 rw, d, q, r rw
