@@ -229,6 +229,9 @@ __global__ void __launch_bounds__(BLOCK) k_copy_gated(const float* a, float* b, 
 
 // The tail of Galerkin levels with red-black smoothing runs in the compact
 // form (couplings and dd only).
+// Caller buffers read or written with 16-byte vector accesses must be 16-byte aligned.
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
 bool gc_tail_ok(const fsp_options& o) {
   return o.coarse_operator == FSP_COARSE_GALERKIN && o.smoother == FSP_SMOOTH_RBGS;
 }
@@ -815,9 +818,16 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
   const Fine& G = K.G;
   FSP_HIP(hipEventRecord(S->ev0, S->stream));
   build_geometry(S, true);
-  FSP_LAUNCH(S, k_assemble, dim3(grid_for(G.n)), K.st, G, us, vs, ws, (const float*)nullptr,
-                     scale, S->b, (float*)nullptr, S->ctl, S->partials);
-  FSP_FINISH(S, K.st, OP_B2, dim3(grid_for(G.n)), false, 0);
+  if (S->quad && aligned16(vs) && aligned16(ws)) {
+    const dim3 gq((unsigned)(S->qfine.nbx * S->qfine.nby * S->qfine.planes));
+    FSP_LAUNCH(S, k_assemble_q, gq, K.st, G, S->qfine, us, vs, ws, (const float*)nullptr, scale, S->b,
+               (float*)nullptr, S->ctl, S->partials);
+    FSP_FINISH(S, K.st, OP_B2, gq, false, 0);
+  } else {
+    FSP_LAUNCH(S, k_assemble, dim3(grid_for(G.n)), K.st, G, us, vs, ws, (const float*)nullptr,
+               scale, S->b, (float*)nullptr, S->ctl, S->partials);
+    FSP_FINISH(S, K.st, OP_B2, dim3(grid_for(G.n)), false, 0);
+  }
   if (S->opt.gauge) {
     FSP_LAUNCH(S, k_sum_active<float>, dim3(grid_for(G.n)), K.st, G, (const float*)S->b,
                        OP_MEAN_B, S->ctl, S->partials);
@@ -1329,9 +1339,16 @@ fsp_status fsp_assemble_device(fsp_solver* S, const float* d_div, float scale, f
   FSP_HIP(hipSetDevice(S->opt.device));
   build_geometry(S, false);
   Launch K(S);
-  FSP_LAUNCH(S, k_assemble, dim3(grid_for(K.G.n)), K.st, K.G, (const float*)nullptr,
-                     (const float*)nullptr, (const float*)nullptr, d_div, scale, d_rhs, d_prec, S->ctl, S->partials);
-  FSP_FINISH(S, K.st, OP_B2, dim3(grid_for(K.G.n)), false, 0);
+  if (S->quad && aligned16(d_div) && aligned16(d_rhs) && aligned16(d_prec)) {
+    const dim3 gq((unsigned)(S->qfine.nbx * S->qfine.nby * S->qfine.planes));
+    FSP_LAUNCH(S, k_assemble_q, gq, K.st, K.G, S->qfine, (const float*)nullptr, (const float*)nullptr,
+               (const float*)nullptr, d_div, scale, d_rhs, d_prec, S->ctl, S->partials);
+    FSP_FINISH(S, K.st, OP_B2, gq, false, 0);
+  } else {
+    FSP_LAUNCH(S, k_assemble, dim3(grid_for(K.G.n)), K.st, K.G, (const float*)nullptr,
+               (const float*)nullptr, (const float*)nullptr, d_div, scale, d_rhs, d_prec, S->ctl, S->partials);
+    FSP_FINISH(S, K.st, OP_B2, dim3(grid_for(K.G.n)), false, 0);
+  }
   FSP_HIP(hipStreamSynchronize(S->stream));
   return FSP_OK;
 }

@@ -364,6 +364,78 @@ __global__ void __launch_bounds__(QBLOCK) k_residual_q(Fine G, QuadGrid Q, const
   if (op >= 0) reduce_finish<false>(s, unused, op, ctl, partials);
 }
 
+// b = scale * div + ghost terms, prec (optional) and |b|^2 on quads
+// (k_assemble with four x-cells per thread: the faces of the quad as 16-byte
+// loads, its x faces as five scalars, or the given divergence; the row's
+// arithmetic is k_assemble's, dry cells get 0).
+__global__ void __launch_bounds__(QBLOCK) k_assemble_q(Fine G, QuadGrid Q, const float* us, const float* vs,
+                                                       const float* ws, const float* div_in, float scale, float* b,
+                                                       float* prec_out, Ctl* ctl, double* partials) {
+#pragma clang fp contract(off)
+  double s2 = 0.0, unused = 0.0;
+  int qx, j, k;
+  const bool valid = quad_coords(Q, qx, j, k);
+  if (valid) {
+    const int i0 = 4 * qx;
+    const size_t c0 = cidx(G, i0, j, k);
+    const uint32_t mq = *reinterpret_cast<const uint32_t*>(G.mask + c0);
+    const size_t fu = fuidx(G, i0, j, k), fv0 = fvidx(G, i0, j, k), fv1 = fvidx(G, i0, j + 1, k);
+    const size_t fw0 = fwidx(G, i0, j, k), fw1 = fwidx(G, i0, j, k + 1);
+    float u5[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    float4 v0 = f4zero(), v1 = f4zero(), w0 = f4zero(), w1 = f4zero(), dq = f4zero();
+    if (div_in) {
+      dq = ld4(div_in, c0);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 5; ++e) u5[e] = us[fu + e];
+      v0 = ld4(vs, fv0); v1 = ld4(vs, fv1); w0 = ld4(ws, fw0); w1 = ld4(ws, fw1);
+    }
+    float4 out, pout = f4zero();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const uint8_t m = (uint8_t)(mq >> (8 * e));
+      float bb = 0.f;
+      if (m & M_WET) {
+        float div;
+        if (div_in) {
+          div = q_get(dq, e);
+        } else if (G.cut && G.solid[c0 + e]) {
+          div = 0.f;
+        } else {
+          float fl = u5[e], fr = u5[e + 1], ff = q_get(v0, e), fk = q_get(v1, e), fb = q_get(w0, e), ft = q_get(w1, e);
+          if (G.cut) {
+            fl = (float)G.ou[fu + e] * fl; fr = (float)G.ou[fu + e + 1] * fr;
+            ff = (float)G.ov[fv0 + e] * ff; fk = (float)G.ov[fv1 + e] * fk;
+            fb = (float)G.ow[fw0 + e] * fb; ft = (float)G.ow[fw1 + e] * ft;
+          }
+          div = ((fr - fl + ft - fb) + (fk - ff)) / G.dx;
+        }
+        bb = scale * div;
+        float diag = 0.f;
+        if (!(m & M_GHOST)) {
+          for (int f = 0; f < 6; ++f) if (m & (1u << f)) diag += 1.0f;
+        } else {
+          RowG R;
+          full_row(G, i0 + e, j, k, R);
+          for (int f = 0; f < 6; ++f) {
+            if (R.type[f] == 1) diag += 1.0f;
+            else if (R.type[f] == 2) {
+              diag += R.it[f];
+              bb += R.val[f] * R.it[f] * G.inv;
+            }
+          }
+        }
+        if (prec_out) q_set(pout, e, diag > 0.f ? 1.0f / (diag * G.inv) : 0.f);
+      }
+      q_set(out, e, bb);
+      s2 += (double)bb * (double)bb;
+    }
+    st4(b, c0, out);
+    if (prec_out) st4(prec_out, c0, pout);
+  }
+  reduce_finish<false>(s2, unused, OP_B2, ctl, partials);
+}
+
 // Cold start, p = 0: p64 = 0, r = b - A 0 = b exactly, |r|^2 into `op`
 // (k_load_p and k_residual in one pass over b).
 __global__ void __launch_bounds__(QBLOCK) k_start_cold(size_t nquads, const float* b, double* p, float* r, int op,
