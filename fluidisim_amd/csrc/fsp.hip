@@ -728,11 +728,24 @@ fsp_status correct_and_measure(fsp_solver* S, const float* us, const float* vs, 
     FSP_LAUNCH(S, k_sub_mean<double>, dim3(grid_for(K.G.n)), K.st, K.G, p, 1, -1, S->ctl,
                        S->partials);
   }
-  FSP_LAUNCH(S, k_correct, dim3(grid_wide(K.G.n)), K.st, K.G, us, vs, ws,
-                     (const double*)p, k1, u, v, w, S->ctl, S->partials);
-  FSP_FINISH(S, K.st, OP_UMAX, dim3(grid_wide(K.G.n)), true, 0);
-  FSP_LAUNCH(S, k_divmetric, dim3(grid_for(K.G.n)), K.st, K.G, u, v, w, S->ctl, S->partials);
-  FSP_FINISH(S, K.st, OP_DIVMAX, dim3(grid_for(K.G.n)), true, 0);
+  if (S->quad && aligned16(vs) && aligned16(ws) && aligned16(v) && aligned16(w)) {
+    const dim3 gq((unsigned)(S->qfine.nbx * S->qfine.nby * S->qfine.planes));
+    FSP_LAUNCH(S, k_correct_q, gq, K.st, K.G, S->qfine, us, vs, ws, (const double*)p, k1, u, v, w, S->ctl,
+               S->partials);
+    FSP_FINISH(S, K.st, OP_UMAX, gq, true, 0);
+  } else {
+    FSP_LAUNCH(S, k_correct, dim3(grid_wide(K.G.n)), K.st, K.G, us, vs, ws,
+               (const double*)p, k1, u, v, w, S->ctl, S->partials);
+    FSP_FINISH(S, K.st, OP_UMAX, dim3(grid_wide(K.G.n)), true, 0);
+  }
+  if (S->quad && aligned16(v) && aligned16(w)) {
+    const dim3 gq((unsigned)(S->qfine.nbx * S->qfine.nby * S->qfine.planes));
+    FSP_LAUNCH(S, k_divmetric_q, gq, K.st, K.G, S->qfine, u, v, w, S->ctl, S->partials);
+    FSP_FINISH(S, K.st, OP_DIVMAX, gq, true, 0);
+  } else {
+    FSP_LAUNCH(S, k_divmetric, dim3(grid_for(K.G.n)), K.st, K.G, u, v, w, S->ctl, S->partials);
+    FSP_FINISH(S, K.st, OP_DIVMAX, dim3(grid_for(K.G.n)), true, 0);
+  }
   fsp_status st = sync_ctl(S);
   if (st) return st;
   // delta3d_mobile.rs:422-432 (f32 arithmetic)

@@ -436,6 +436,167 @@ __global__ void __launch_bounds__(QBLOCK) k_assemble_q(Fine G, QuadGrid Q, const
   reduce_finish<false>(s2, unused, OP_B2, ctl, partials);
 }
 
+// Velocity correction on quads (k_correct with four x-cells per thread: the
+// x-, y- and z- faces of the quad and the closing faces on the +x, +y, +z
+// walls; the solid flags, row codes and pressures of the quad and of the cells
+// below it in x, y and z as 4-, 16- and 32-byte loads with indices clamped into
+// the grid). Each face's arithmetic is k_correct's (corrected_side, ghost_up).
+__global__ void __launch_bounds__(QBLOCK) k_correct_q(Fine G, QuadGrid Q, const float* us, const float* vs,
+                                                      const float* ws, const double* p, float k1, float* u, float* v,
+                                                      float* w, Ctl* ctl, double* partials) {
+#pragma clang fp contract(off)
+  double mx = 0.0, unused = 0.0;
+  int qx, j, k;
+  const bool valid = quad_coords(Q, qx, j, k);
+  if (valid) {
+    const int nx = G.nx, ny = G.ny, nz = G.nz, i0 = 4 * qx;
+    const size_t plane = (size_t)nx * ny;
+    const size_t c0 = cidx(G, i0, j, k);
+    const size_t cy = j > 0 ? c0 - nx : c0, cz = k > 0 ? c0 - plane : c0, cx = i0 > 0 ? c0 - 1 : c0;
+    auto u32 = [](const uint8_t* a, size_t c) { return *reinterpret_cast<const uint32_t*>(a + c); };
+    auto byte = [](uint32_t wv, int e) { return (uint8_t)(wv >> (8 * e)); };
+    auto pquad = [&](size_t c, double o[4]) {
+      const double2 a = *reinterpret_cast<const double2*>(p + c), b = *reinterpret_cast<const double2*>(p + c + 2);
+      o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+    };
+    const uint32_t sq = u32(G.solid, c0), mq = u32(G.mask, c0);
+    const uint32_t sy = u32(G.solid, cy), my = u32(G.mask, cy), sz = u32(G.solid, cz), mz = u32(G.mask, cz);
+    const uint8_t sx = G.solid[cx], mxl = G.mask[cx];
+    double pq[4], py[4], pz[4];
+    pquad(c0, pq);
+    pquad(cy, py);
+    pquad(cz, pz);
+    const double px = p[cx];
+    const size_t fu = fuidx(G, i0, j, k), fv = fvidx(G, i0, j, k), fw = fwidx(G, i0, j, k);
+    float ux[5];
+    uint8_t ax[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      ux[e] = e < 4 || i0 + 4 == nx ? us[fu + e] : 0.f;
+      ax[e] = e < 4 || i0 + 4 == nx ? G.ou[fu + e] : 0;
+    }
+    const float4 vq = ld4(vs, fv), wq = ld4(ws, fw);
+    const uint32_t avq = u32(G.ov, fv), awq = u32(G.ow, fw);
+    const bool top_y = j == ny - 1, top_z = k == nz - 1;
+    const size_t fvt = fvidx(G, i0, ny, k), fwt = fwidx(G, i0, j, nz);
+    const float4 vt = top_y ? ld4(vs, fvt) : f4zero(), wt = top_z ? ld4(ws, fwt) : f4zero();
+    const uint32_t awt = top_z ? u32(G.ow, fwt) : 0u;
+    const double kk = (double)k1, dx = (double)G.dx;
+    // x- faces of the four cells, and the closing +x face
+    float uo[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int i = i0 + e;
+      const uint8_t sl = e > 0 ? byte(sq, e - 1) : sx, ml = e > 0 ? byte(mq, e - 1) : mxl;
+      const double pl = e > 0 ? pq[e - 1] : px;
+      const bool inner = i > 0;  // i < nx holds inside the quad
+      float val = ux[e];
+      if (inner && ax[e] && !sl && !byte(sq, e))
+        val = corrected_side(G, val, pl, pq[e], (ml & M_WET) != 0, (byte(mq, e) & M_WET) != 0, k1, i - 1, j, i, j, k);
+      uo[e] = val;
+      mx = fmax(mx, (double)fabsf(val));
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) u[fu + e] = uo[e];
+    if (i0 + 4 == nx) {
+      u[fu + 4] = ux[4];
+      mx = fmax(mx, (double)fabsf(ux[4]));
+    }
+    // y- faces, and the closing +y faces
+    float4 vo;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int i = i0 + e;
+      float val = q_get(vq, e);
+      if (j > 0 && byte(avq, e) && !byte(sy, e) && !byte(sq, e))
+        val = corrected_side(G, val, py[e], pq[e], (byte(my, e) & M_WET) != 0, (byte(mq, e) & M_WET) != 0, k1, i,
+                             j - 1, i, j, k);
+      q_set(vo, e, val);
+      mx = fmax(mx, (double)fabsf(val));
+    }
+    st4(v, fv, vo);
+    if (top_y) {
+      st4(v, fvt, vt);
+      mx = fmax(mx, (double)fmaxf(fmaxf(fabsf(vt.x), fabsf(vt.y)), fmaxf(fabsf(vt.z), fabsf(vt.w))));
+    }
+    // z- faces, and the closing +z faces
+    auto zface = [&](float val, uint8_t open, int kf, uint8_t mb, double pb, uint8_t ma, double pa, int i) {
+      if (kf >= 1 && open && (mb & M_WET)) {
+        if (kf < nz && (ma & M_WET)) {
+          val = (float)((double)val - kk * (pa - pb) / dx);
+        } else {
+          float a, b;
+          ghost_up(G, i, j, kf - 1, a, b);
+          val = (float)((double)val - kk * ((double)b - pb) * (double)a / dx);
+        }
+      }
+      return val;
+    };
+    float4 wo;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float val = zface(q_get(wq, e), byte(awq, e), k, byte(mz, e), pz[e], byte(mq, e), pq[e], i0 + e);
+      q_set(wo, e, val);
+      mx = fmax(mx, (double)fabsf(val));
+    }
+    st4(w, fw, wo);
+    if (top_z) {
+      float4 wt2;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        // face nz: below and (clamped) above are the top cell itself
+        const float val = zface(q_get(wt, e), byte(awt, e), nz, byte(mq, e), pq[e], byte(mq, e), pq[e], i0 + e);
+        q_set(wt2, e, val);
+        mx = fmax(mx, (double)fabsf(val));
+      }
+      st4(w, fwt, wt2);
+    }
+  }
+  reduce_finish<true>(mx, unused, OP_UMAX, ctl, partials);
+}
+
+// max |div u| over wet rows and over plain rows on quads (k_divmetric with
+// four x-cells per thread; divergence_at's arithmetic per cell).
+__global__ void __launch_bounds__(QBLOCK) k_divmetric_q(Fine G, QuadGrid Q, const float* u, const float* v,
+                                                        const float* w, Ctl* ctl, double* partials) {
+#pragma clang fp contract(off)
+  double all = 0.0, plain = 0.0;
+  int qx, j, k;
+  const bool valid = quad_coords(Q, qx, j, k);
+  if (valid) {
+    const int i0 = 4 * qx;
+    const size_t c0 = cidx(G, i0, j, k);
+    const uint32_t mq = *reinterpret_cast<const uint32_t*>(G.mask + c0);
+    if (mq & 0x40404040u) {  // a wet cell in the quad
+      const size_t fu = fuidx(G, i0, j, k), fv0 = fvidx(G, i0, j, k), fv1 = fvidx(G, i0, j + 1, k);
+      const size_t fw0 = fwidx(G, i0, j, k), fw1 = fwidx(G, i0, j, k + 1);
+      float u5[5];
+#pragma unroll
+      for (int e = 0; e < 5; ++e) u5[e] = u[fu + e];
+      const float4 v0 = ld4(v, fv0), v1 = ld4(v, fv1), w0 = ld4(w, fw0), w1 = ld4(w, fw1);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const uint8_t m = (uint8_t)(mq >> (8 * e));
+        if (!(m & M_WET)) continue;
+        float div = 0.f;
+        if (!(G.cut && G.solid[c0 + e])) {
+          float fl = u5[e], fr = u5[e + 1], ff = q_get(v0, e), fk = q_get(v1, e), fb = q_get(w0, e), ft = q_get(w1, e);
+          if (G.cut) {
+            fl = (float)G.ou[fu + e] * fl; fr = (float)G.ou[fu + e + 1] * fr;
+            ff = (float)G.ov[fv0 + e] * ff; fk = (float)G.ov[fv1 + e] * fk;
+            fb = (float)G.ow[fw0 + e] * fb; ft = (float)G.ow[fw1 + e] * ft;
+          }
+          div = ((fr - fl + ft - fb) + (fk - ff)) / G.dx;
+        }
+        const double d = (double)fabsf(div);
+        all = fmax(all, d);
+        if (!(m & M_GHOST)) plain = fmax(plain, d);
+      }
+    }
+  }
+  reduce_finish<true>(all, plain, OP_DIVMAX, ctl, partials);
+}
+
 // Cold start, p = 0: p64 = 0, r = b - A 0 = b exactly, |r|^2 into `op`
 // (k_load_p and k_residual in one pass over b).
 __global__ void __launch_bounds__(QBLOCK) k_start_cold(size_t nquads, const float* b, double* p, float* r, int op,
