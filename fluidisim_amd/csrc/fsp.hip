@@ -167,7 +167,6 @@ inline void prof_end(fsp_solver* S, bool on, hipStream_t stream) {
 
 // Kernels templated on the V-cycle storage ST of the enclosing function.
 #define K_QGS(M) k_q_gs<M, ST>
-#define K_QRESTRICT k_q_restrict<ST>
 #define K_QSPMV k_q_spmv_dir<ST>
 #define K_CGS k_cgs<ST>
 #define K_CRESTRICT k_crestrict<ST>
@@ -581,9 +580,13 @@ struct Launch {
     int passes = 2 * S->opt.post_sweeps;
     int done = 0;
     if (!S->levels.empty()) {
-      const QuadGrid& R = S->qrestr;
-      const dim3 gr((unsigned)(R.nbx * R.nby * R.planes));
-      FSP_LAUNCH_N(S, "k_q_restrict", K_QRESTRICT, gr, st, G, R, (const TS<ST>*)z, (const float*)S->r, L0, S->ctl);
+      // the restriction's grid for 128-thread halves (k_q_restrict2)
+      QuadGrid H = S->qrestr;
+      H.RY = (QBLOCK / 2) / H.QX;
+      H.nby = (H.rows + H.RY - 1) / H.RY;
+      const dim3 gr((unsigned)(H.nbx * H.nby * H.planes));
+      FSP_LAUNCH_N(S, "k_q_restrict", (k_q_restrict2<ST>), gr, st, G, H, (const TS<ST>*)z, (const float*)S->r, L0,
+                   S->ctl);
       const TS<ST>* x0 = coarse_cycle<ST>();
       const int op = passes == 1 ? dot_op : -1;
       FSP_LAUNCH_N(S, "k_q_gs<2>", K_QGS(2), g, st, G, Q, (const TS<ST>*)z, zt, S->r, 1, L0, x0, op, S->ctl,

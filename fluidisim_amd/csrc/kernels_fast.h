@@ -616,59 +616,82 @@ __global__ void __launch_bounds__(QBLOCK) k_start_cold(size_t nquads, const floa
 }
 
 // ---------------------------------------------------------------- residual + restriction
-// One thread per (fine quad column qx, coarse row J, coarse plane K): the
-// residual t = r - A z of its 16 fine cells, summed per coarse cell in the
-// reference's child order and scaled by 1/8 (restrict3, delta3d_multigrid.rs:258-272).
+// The residual t = r - A z of the 16 fine cells of (fine quad column qx,
+// coarse row J, coarse plane K), summed per coarse cell in the reference's
+// child order and scaled by 1/8 (restrict3, delta3d_multigrid.rs:258-272),
+// with the work split over two threads: the first 128 threads of a
+// workgroup take the fine plane 2K, the last 128 the plane 2K + 1. The first half's
+// partial sums (children of plane 2K, in the reference's order) reach the
+// second half through LDS, which adds its own children in order: the sum,
+// and so the coarse r, is the one-thread sum bit for bit, with twice the
+// waves in flight to overlap the loads with the arithmetic (38 -> 28 us at
+// 256x256x128). H: the restriction's quad grid for 128-thread halves.
 template <int ST>
-__global__ void __launch_bounds__(QBLOCK) k_q_restrict(Fine G, QuadGrid Q, const typename Store<ST>::T* z,
-                                                       const float* r, Level L0, Ctl* ctl) {
+__global__ void __launch_bounds__(QBLOCK) k_q_restrict2(Fine G, QuadGrid H, const typename Store<ST>::T* z,
+                                                        const float* r, Level L0, Ctl* ctl) {
 #pragma clang fp contract(off)
   using SV = Store<ST>;
+  __shared__ float2 carry[QBLOCK / 2];
   if (!ctl->active) return;
-  int qx, J, K;
-  const bool valid = quad_coords(Q, qx, J, K);
+  const int local = (int)(threadIdx.x & (QBLOCK / 2 - 1)), dz = (int)(threadIdx.x / (QBLOCK / 2));
+  const unsigned Lb = xcd_block(blockIdx.x, gridDim.x);
+  const unsigned bx = Lb % (unsigned)H.nbx, tb = Lb / (unsigned)H.nbx;
+  const int qx = (int)bx * H.QX + (local & (H.QX - 1));
+  const int J = (int)(tb % (unsigned)H.nby) * H.RY + local / H.QX;
+  const int K = (int)(tb / (unsigned)H.nby);
+  const bool valid = qx < H.nq && J < H.rows && K < H.planes;
   const size_t plane = (size_t)G.nx * G.ny;
-  float t[2][2][4];  // [dz][dy][e]
+  const int k = 2 * K + dz;
+  float t[2][4];  // [dy][e]
 #pragma unroll
-  for (int dz = 0; dz < 2; ++dz)
-#pragma unroll
-    for (int dy = 0; dy < 2; ++dy) {
-      const int y = 2 * J + dy, k = 2 * K + dz;
-      const size_t c0 = valid ? ((size_t)k * G.ny + y) * G.nx + 4 * (size_t)qx : 0;
-      float4 zc = f4zero(), rq = f4zero(), ym = f4zero(), yp = f4zero(), zm = f4zero(), zp = f4zero();
-      uint32_t mq = 0;
-      if (valid) {
-        mq = *reinterpret_cast<const uint32_t*>(G.mask + c0);
-        zc = SV::ld4(z, c0);
-        rq = ld4(r, c0);
-        if (y > 0) ym = SV::ld4(z, c0 - G.nx);
-        if (y + 1 < G.ny) yp = SV::ld4(z, c0 + G.nx);
-        if (k > 0) zm = SV::ld4(z, c0 - plane);
-        if (k + 1 < G.nz) zp = SV::ld4(z, c0 + plane);
-      }
-      float left, right;
-      x_neighbours(Q, valid, qx, zc, left, right, [&](int off) { return SV::ld(z, c0 + off); });
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const uint8_t m = (uint8_t)(mq >> (8 * e));
-        const float pc = q_get(zc, e);
-        const float xl = e == 0 ? left : q_get(zc, e - 1);
-        const float xr = e == 3 ? right : q_get(zc, e + 1);
-        const float nb[6] = {xl, xr, q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
-        float prec;
-        const float az = valid ? row_local(G, 4 * qx + e, y, k, m, pc, nb, prec) : 0.f;
-        t[dz][dy][e] = q_get(rq, e) - az;
-      }
+  for (int dy = 0; dy < 2; ++dy) {
+    const int y = 2 * J + dy;
+    const size_t c0 = valid ? ((size_t)k * G.ny + y) * G.nx + 4 * (size_t)qx : 0;
+    float4 zc = f4zero(), rq = f4zero(), ym = f4zero(), yp = f4zero(), zm = f4zero(), zp = f4zero();
+    uint32_t mq = 0;
+    if (valid) {
+      mq = *reinterpret_cast<const uint32_t*>(G.mask + c0);
+      zc = SV::ld4(z, c0);
+      rq = ld4(r, c0);
+      if (y > 0) ym = SV::ld4(z, c0 - G.nx);
+      if (y + 1 < G.ny) yp = SV::ld4(z, c0 + G.nx);
+      if (k > 0) zm = SV::ld4(z, c0 - plane);
+      if (k + 1 < G.nz) zp = SV::ld4(z, c0 + plane);
     }
-  if (!valid) return;
+    float left, right;
+    x_neighbours(H, valid, qx, zc, left, right, [&](int off) { return SV::ld(z, c0 + off); });
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const uint8_t m = (uint8_t)(mq >> (8 * e));
+      const float pc = q_get(zc, e);
+      const float xl = e == 0 ? left : q_get(zc, e - 1);
+      const float xr = e == 3 ? right : q_get(zc, e + 1);
+      const float nb[6] = {xl, xr, q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
+      float prec;
+      const float az = valid ? row_local(G, 4 * qx + e, y, k, m, pc, nb, prec) : 0.f;
+      t[dy][e] = q_get(rq, e) - az;
+    }
+  }
+  float2 sum = make_float2(0.f, 0.f);
+  if (dz == 0) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      float s = 0.f;
+      s += t[0][2 * h]; s += t[0][2 * h + 1];
+      s += t[1][2 * h]; s += t[1][2 * h + 1];
+      if (h == 0) sum.x = s; else sum.y = s;
+    }
+    carry[local] = sum;
+  }
+  __syncthreads();
+  if (dz == 0 || !valid) return;
+  sum = carry[local];
   float2 out;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
-    float s = 0.f;
-    s += t[0][0][2 * h]; s += t[0][0][2 * h + 1];
-    s += t[0][1][2 * h]; s += t[0][1][2 * h + 1];
-    s += t[1][0][2 * h]; s += t[1][0][2 * h + 1];
-    s += t[1][1][2 * h]; s += t[1][1][2 * h + 1];
+    float s = h == 0 ? sum.x : sum.y;
+    s += t[0][2 * h]; s += t[0][2 * h + 1];
+    s += t[1][2 * h]; s += t[1][2 * h + 1];
     if (h == 0) out.x = 0.125f * s; else out.y = 0.125f * s;
   }
   SV::st2(reinterpret_cast<typename SV::T*>(L0.r), lidx(L0, 2 * qx, J, K), out);
