@@ -764,9 +764,14 @@ fsp_status correct_and_measure(fsp_solver* S, const float* us, const float* vs, 
 
 fsp_status backward_error(fsp_solver* S, const double* p, double& be) {
   Launch K(S);
-  FSP_LAUNCH(S, k_backward, dim3(grid_for(K.G.n)), K.st, K.G, p, S->b, S->r, S->ctl,
-                     S->partials);
-  FSP_FINISH(S, K.st, OP_BACKWARD, dim3(grid_for(K.G.n)), true, 0);
+  if (S->quad) {
+    const dim3 gq((unsigned)(S->qfine.nbx * S->qfine.nby * S->qfine.planes));
+    FSP_LAUNCH(S, k_backward_q, gq, K.st, K.G, S->qfine, (const double*)p, S->b, S->r, S->ctl, S->partials);
+    FSP_FINISH(S, K.st, OP_BACKWARD, gq, true, 0);
+  } else {
+    FSP_LAUNCH(S, k_backward, dim3(grid_for(K.G.n)), K.st, K.G, p, S->b, S->r, S->ctl, S->partials);
+    FSP_FINISH(S, K.st, OP_BACKWARD, dim3(grid_for(K.G.n)), true, 0);
+  }
   fsp_status st = sync_ctl(S);
   if (st) return st;
   be = S->hctl->red[OP_BACKWARD][0];

@@ -870,12 +870,18 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_gc_tail(TailSet T, Ctl* ctl) {
   __syncthreads();
   const int last = T.count - 1;
   for (int l = 0; l <= last; ++l) {
-    gct_sweeps<ST>(lds, tl[l], l == last ? T.coarse : T.pre, true, l == last, true);
-    if (l < last) gct_restrict<ST>(lds, tl[l], tl[l + 1]);
+    // level descriptors in registers: the passes do not reload them after each barrier
+    const GcTailLvl L = tl[l];
+    gct_sweeps<ST>(lds, L, l == last ? T.coarse : T.pre, true, l == last, true);
+    if (l < last) {
+      const GcTailLvl C = tl[l + 1];
+      gct_restrict<ST>(lds, L, C);
+    }
   }
   for (int l = last - 1; l >= 0; --l) {
-    gct_prolong<ST>(lds, tl[l], tl[l + 1]);
-    gct_sweeps<ST>(lds, tl[l], T.post, false, false, false);
+    const GcTailLvl L = tl[l], C = tl[l + 1];
+    gct_prolong<ST>(lds, L, C);
+    gct_sweeps<ST>(lds, L, T.post, false, false, false);
   }
   const GcTailLvl L0 = tl[0];
   for (int q = threadIdx.x; q < L0.n; q += TAIL_THREADS)

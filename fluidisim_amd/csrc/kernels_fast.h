@@ -597,6 +597,57 @@ __global__ void __launch_bounds__(QBLOCK) k_divmetric_q(Fine G, QuadGrid Q, cons
   reduce_finish<true>(all, plain, OP_DIVMAX, ctl, partials);
 }
 
+// Oettli-Prager backward error on quads (k_backward with four x-cells per
+// thread, the pressures of the quad and of its y and z neighbours as 32-byte
+// loads with indices clamped into the grid; k_backward's arithmetic per cell).
+__global__ void __launch_bounds__(QBLOCK) k_backward_q(Fine G, QuadGrid Q, const double* p, const float* b,
+                                                       const float* r, Ctl* ctl, double* partials) {
+#pragma clang fp contract(off)
+  double worst = 0.0, unused = 0.0;
+  int qx, y, k;
+  const bool valid = quad_coords(Q, qx, y, k);
+  if (valid) {
+    const size_t plane = (size_t)G.nx * G.ny;
+    const size_t c0 = ((size_t)k * G.ny + y) * G.nx + 4 * (size_t)qx;
+    const uint32_t mq = *reinterpret_cast<const uint32_t*>(G.mask + c0);
+    auto quad = [&](size_t c, double v[4]) {
+      const double2 a = *reinterpret_cast<const double2*>(p + c), e = *reinterpret_cast<const double2*>(p + c + 2);
+      v[0] = a.x; v[1] = a.y; v[2] = e.x; v[3] = e.y;
+    };
+    double pc[4], pym[4], pyp[4], pzm[4], pzp[4];
+    quad(c0, pc);
+    quad(y > 0 ? c0 - G.nx : c0, pym);
+    quad(y + 1 < G.ny ? c0 + G.nx : c0, pyp);
+    quad(k > 0 ? c0 - plane : c0, pzm);
+    quad(k + 1 < G.nz ? c0 + plane : c0, pzp);
+    const double pxm = p[qx > 0 ? c0 - 1 : c0], pxp = p[4 * qx + 4 < G.nx ? c0 + 4 : c0 + 3];
+    const float4 bq = ld4(b, c0), rq = ld4(r, c0);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const uint8_t m = (uint8_t)(mq >> (8 * e));
+      if (!(m & M_WET)) continue;
+      const double pe = fabs(pc[e]);
+      const double pn[6] = {e > 0 ? pc[e - 1] : pxm, e < 3 ? pc[e + 1] : pxp, pym[e], pyp[e], pzm[e], pzp[e]};
+      double acc = 0.0;
+      if (!(m & M_GHOST)) {
+        for (int f = 0; f < 6; ++f)
+          if (m & (1u << f)) acc += pe + fabs(pn[f]);
+      } else {
+        uint8_t type[6];
+        float it[6];
+        ghost_row(G, 4 * qx + e, y, k, m, type, it);
+        for (int f = 0; f < 6; ++f) {
+          if (type[f] == 1) acc += pe + fabs(pn[f]);
+          else if (type[f] == 2) acc += pe * (double)it[f];
+        }
+      }
+      const double scale = fabs((double)q_get(bq, e)) + acc * (double)G.inv;
+      if (scale > 0.0) worst = fmax(worst, fabs((double)q_get(rq, e)) / scale);
+    }
+  }
+  reduce_finish<true>(worst, unused, OP_BACKWARD, ctl, partials);
+}
+
 // Cold start, p = 0: p64 = 0, r = b - A 0 = b exactly, |r|^2 into `op`
 // (k_load_p and k_residual in one pass over b).
 __global__ void __launch_bounds__(QBLOCK) k_start_cold(size_t nquads, const float* b, double* p, float* r, int op,
