@@ -719,10 +719,10 @@ void cleanup(fsp_solver* S) {
   if (S->stream) (void)hipStreamDestroy(S->stream);
 }
 
-// Correction and acceptance metric (delta3d_mobile.rs:560-561); returns
-// (divergence over all wet rows, over plain rows), normalised as the reference.
-fsp_status correct_and_measure(fsp_solver* S, const float* us, const float* vs, const float* ws, double* p, float k1,
-                               float* u, float* v, float* w, double& all, double& plain) {
+// The correction and its metric, launched (read_measure reads the metric
+// after the next sync of the control block).
+void launch_correct(fsp_solver* S, const float* us, const float* vs, const float* ws, double* p, float k1, float* u,
+                    float* v, float* w) {
   Launch K(S);
   if (S->opt.gauge) {
     FSP_LAUNCH(S, k_sum_active<double>, dim3(grid_for(K.G.n)), K.st, K.G, (const double*)p,
@@ -749,8 +749,9 @@ fsp_status correct_and_measure(fsp_solver* S, const float* us, const float* vs, 
     FSP_LAUNCH(S, k_divmetric, dim3(grid_for(K.G.n)), K.st, K.G, u, v, w, S->ctl, S->partials);
     FSP_FINISH(S, K.st, OP_DIVMAX, dim3(grid_for(K.G.n)), true, 0);
   }
-  fsp_status st = sync_ctl(S);
-  if (st) return st;
+}
+
+void read_measure(fsp_solver* S, double& all, double& plain) {
   // delta3d_mobile.rs:422-432 (f32 arithmetic)
   const float dmax_all = (float)S->hctl->red[OP_DIVMAX][0];
   const float dmax_plain = (float)S->hctl->red[OP_DIVMAX][1];
@@ -759,10 +760,22 @@ fsp_status correct_and_measure(fsp_solver* S, const float* us, const float* vs, 
   const float ratio = mx > 0.f ? S->dom.dx / echelle : 0.f;
   all = (double)(dmax_all * ratio);
   plain = (double)(dmax_plain * ratio);
+}
+
+// Correction and acceptance metric (delta3d_mobile.rs:560-561); returns
+// (divergence over all wet rows, over plain rows), normalised as the reference.
+fsp_status correct_and_measure(fsp_solver* S, const float* us, const float* vs, const float* ws, double* p, float k1,
+                               float* u, float* v, float* w, double& all, double& plain) {
+  launch_correct(S, us, vs, ws, p, k1, u, v, w);
+  fsp_status st = sync_ctl(S);
+  if (st) return st;
+  read_measure(S, all, plain);
   return FSP_OK;
 }
 
-fsp_status backward_error(fsp_solver* S, const double* p, double& be) {
+// The backward error of the true residual, launched (read from the control
+// block after the next sync).
+void launch_backward(fsp_solver* S, const double* p) {
   Launch K(S);
   if (S->quad) {
     const dim3 gq((unsigned)(S->qfine.nbx * S->qfine.nby * S->qfine.planes));
@@ -772,6 +785,10 @@ fsp_status backward_error(fsp_solver* S, const double* p, double& be) {
     FSP_LAUNCH(S, k_backward, dim3(grid_for(K.G.n)), K.st, K.G, p, S->b, S->r, S->ctl, S->partials);
     FSP_FINISH(S, K.st, OP_BACKWARD, dim3(grid_for(K.G.n)), true, 0);
   }
+}
+
+fsp_status backward_error(fsp_solver* S, const double* p, double& be) {
+  launch_backward(S, p);
   fsp_status st = sync_ctl(S);
   if (st) return st;
   be = S->hctl->red[OP_BACKWARD][0];
@@ -882,6 +899,7 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
   uint32_t power = 1, since = 0;
   double actual_rr = 0.0;
   bool settled = false;
+  bool wrapped = false;  // the loop ended on a pass whose correction and backward error were launched with it
   double set_all = 0.0, set_plain = 0.0;
   // The floor certificate gamma_{m+4} (delta3d.rs:65-72, delta_projection.rs:194-203)
   // with the unit roundoff of the pressure's precision (f64 here).
@@ -943,6 +961,15 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
     }
     // True residual, recomputed: it decides, never the recurrence (:546-553).
     K.residual(p64, OP_RR);
+    // The correction with its metric and the backward error ride on the same
+    // sync as the true residual (read below if the pass ends in them; the
+    // correction writes only u, v, w, which a later one overwrites). Not with
+    // the gauge of a pure-Neumann problem, whose correction shifts p first.
+    const bool spec = !(S->opt.gauge && S->hctl->pure_neumann);
+    if (spec) {
+      launch_correct(S, us, vs, ws, p64, k1, u, v, w);
+      launch_backward(S, p64);
+    }
     if ((st = sync_ctl(S))) return st;
     if (S->hctl->nonfinite) return FSP_NOT_FINITE;
     const double actual = S->hctl->actual;
@@ -958,23 +985,27 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
     bool have_measured = false;
     double m_all = 0.0, m_plain = 0.0;
     if (actual <= tol_b2) {
-      if (exhausted) { actual_rr = actual; break; }
+      if (exhausted) { actual_rr = actual; wrapped = spec; break; }
       double a, pl;
-      if ((st = correct_and_measure(S, us, vs, ws, p64, k1, u, v, w, a, pl))) return st;
-      if (pl <= 1e-5) { actual_rr = actual; settled = true; set_all = a; set_plain = pl; break; }
+      if (spec) read_measure(S, a, pl);
+      else if ((st = correct_and_measure(S, us, vs, ws, p64, k1, u, v, w, a, pl))) return st;
+      if (pl <= 1e-5) { actual_rr = actual; settled = true; set_all = a; set_plain = pl; wrapped = spec; break; }
       have_measured = true; m_all = a; m_plain = pl;
       const float ratio = (float)(1e-5 / pl);
       physical_target = (double)((float)actual * ratio * ratio);
     } else if (exhausted) {
       actual_rr = actual;
+      wrapped = spec;
       break;
     }
     double be;
-    if ((st = backward_error(S, p64, be))) return st;
+    if (spec) be = S->hctl->red[OP_BACKWARD][0];
+    else if ((st = backward_error(S, p64, be))) return st;
     if (be <= roundoff) {
       floor_stop = true;
       actual_rr = actual;
       if (have_measured) { settled = true; set_all = m_all; set_plain = m_plain; }
+      wrapped = spec;
       break;
     }
     uint64_t state;
@@ -1001,9 +1032,11 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
   }
   double d_all, d_plain;
   if (settled) { d_all = set_all; d_plain = set_plain; }
+  else if (wrapped) read_measure(S, d_all, d_plain);
   else if ((st = correct_and_measure(S, us, vs, ws, p64, k1, u, v, w, d_all, d_plain))) return st;
   double be;
-  if ((st = backward_error(S, p64, be))) return st;
+  if (wrapped) be = S->hctl->red[OP_BACKWARD][0];
+  else if ((st = backward_error(S, p64, be))) return st;
   FSP_LAUNCH(S, k_store_p, dim3(grid_for(G.n)), K.st, G.n, (const double*)p64, p);
   const bool accepted = (actual_rr <= tol_b2 || floor_stop) && d_plain <= 1e-5;
   FSP_HIP(hipEventRecord(S->ev1, S->stream));
