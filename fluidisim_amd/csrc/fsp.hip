@@ -291,8 +291,8 @@ struct Launch {
         }
       } else {
         if (kinds) FSP_LAUNCH(S, k_ckind, dim3(grid_for(L.n)), st, S->levels[l - 1], L);
-        FSP_LAUNCH(S, k_cfaces, dim3(grid_for(nf)), st, S->levels[l - 1], L);
-        if (L.galerkin) FSP_LAUNCH(S, k_cdd, dim3(grid_for(L.n)), st, S->levels[l - 1], L);
+        if (L.galerkin) FSP_LAUNCH(S, k_cgal, dim3(grid_for(std::max(nf, L.n))), st, S->levels[l - 1], L);
+        else FSP_LAUNCH(S, k_cfaces, dim3(grid_for(nf)), st, S->levels[l - 1], L);
       }
       if (diag) FSP_LAUNCH(S, k_cdiag, dim3(grid_for(L.n)), st, L);
       if (L.compact) FSP_LAUNCH(S, k_gc_build, dim3(grid_for(L.n)), st, L);
