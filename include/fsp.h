@@ -138,7 +138,11 @@ fsp_status fsp_project(fsp_solver* s,
                        float* u, size_t n_u, float* v, size_t n_v, float* w, size_t n_w,
                        float scale, float k1, double rel_tol, uint32_t max_iters, fsp_report* report);
 
-/* The same on device pointers (resident use; no host copies). */
+/* The same on device pointers (resident use; no host copies). On a non-OK
+ * status d_p is left as it was; d_u, d_v, d_w may hold an intermediate
+ * correction. Face arrays that are 16-byte aligned (as every hipMalloc or
+ * torch allocation is) are read and written four cells at a time; others
+ * take the per-cell kernels (same results, slower). */
 fsp_status fsp_project_device(fsp_solver* s, const float* d_us, const float* d_vs, const float* d_ws,
                               float* d_p, int32_t p_warm, float* d_u, float* d_v, float* d_w,
                               float scale, float k1, double rel_tol, uint32_t max_iters, fsp_report* report);
