@@ -2,9 +2,9 @@
 
 Reference of the learned preconditioner's network in numpy (float64), with
 its exact backward pass, so the card's forward and backward kernels can be
-checked. The network has no counterpart in the reference solver: it is the
-north-star extension "3-level 3D conv encoder-decoder, 8/16/32 channels,
-3x3x3 kernels, ReLU, skip connections" applied inside flexible CG.
+checked. The network has no counterpart in the reference solver: it is a
+3-level 3D convolutional encoder-decoder (8/16/32 channels, 3x3x3 kernels,
+ReLU, skip connections) applied as the preconditioner of flexible CG.
 
 Layout: activations (nz, ny, nx, C) channels last, cells (k*ny+j)*nx+i.
 Weights of a layer: W[co, tap, ci], tap = (dz+1)*9 + (dy+1)*3 + (dx+1), zero
