@@ -130,12 +130,15 @@ def test_vcycle_matches_the_oracle(name, smoother):
     ("tank_flat48x32x40", (1, 1)), ("tank_flat48x32x40", (2, 1)), ("tank_flat48x32x40", (3, 2)),
     ("tank128x64x96", (1, 2)), ("tank128x64x96", (3, 3)),
     # a compact level above the tiled size (one pass per launch) over two tiled ones
-    ("tank160x128x128", (2, 2)), ("tank160x128x128", (1, 1))])
+    ("tank160x128x128", (2, 2)), ("tank160x128x128", (1, 1)),
+    # a thin tiled level (32x32x4: tiles cut in z) over a 16x16x2 tail
+    ("tank64x8x64", (2, 2))])
 def test_vcycle_sweep_counts_match_the_oracle(name, sweeps):
     """Compact levels smoothed in tiles (one or two sweeps, tiles cut by the
     level's edge) and one pass per launch (three sweeps, large levels): the
     oracle's V-cycle bit for bit."""
-    wl = W.tank(160, 128, 128, seed=9) if name == "tank160x128x128" else WORKLOADS[name]()
+    extra = {"tank160x128x128": lambda: W.tank(160, 128, 128, seed=9), "tank64x8x64": lambda: W.tank(64, 8, 64, seed=10)}
+    wl = extra[name]() if name in extra else WORKLOADS[name]()
     wl.options["pre_sweeps"], wl.options["post_sweeps"] = sweeps
     vol = W.build_volume(wl)
     prob = problem(wl)
