@@ -342,7 +342,7 @@ constexpr int GB_MAX_SWEEPS = GB_H / 2;
 static_assert(GB_THREADS >= GB_I, "one thread per interior cell");
 
 struct GbLds {
-  float x[GB_N], r[GB_N], dd[GB_N];
+  float x[GB_N], r[GB_N], dd[GB_N], dg[GB_N];
   _Float16 fx[GB_N], fy[GB_N], fz[GB_N];
   float res[GB_I];
 };
@@ -365,10 +365,10 @@ __device__ __forceinline__ bool gb_inside(const Level& L, int i, int j, int k) {
   return (unsigned)i < (unsigned)L.nx && (unsigned)j < (unsigned)L.ny && (unsigned)k < (unsigned)L.nz;
 }
 
-// Couplings, Dirichlet parts and r of the box, and its starting x:
-// pre-smoothing (POST false): the first pass from x = 0 (k_gc_gs<1>, colour 0
-// cells r/diag, the others 0), for which each thread also loads its cells' +
-// faces; post-smoothing: the level's x prolonged from its parents in C as
+// Couplings, Dirichlet parts, diagonals (L.dg, gc_diag of the same binary16
+// couplings) and r of the box, and its starting x: pre-smoothing (POST
+// false): the first pass from x = 0 (k_gc_gs<1>, colour 0 cells r/diag, the
+// others 0); post-smoothing: the level's x prolonged from its parents in C as
 // k_gc_gs<2> prolongs it. One round of loads: cells outside the level (or
 // past the box) load cell 0 and are replaced by zeros.
 template <int ST, bool POST>
@@ -377,9 +377,8 @@ __device__ __forceinline__ void gb_load(GbLds& s, const Level& L, const GbTile& 
 #pragma clang fp contract(off)
   using SV = Store<ST>;
   const typename SV::T* lr = reinterpret_cast<const typename SV::T*>(L.r);
-  const size_t plane = (size_t)L.nx * L.ny;
   constexpr int ITER = (GB_N + GB_THREADS - 1) / GB_THREADS;
-  float ax[ITER], ay[ITER], az[ITER], bx_[ITER], by_[ITER], bz_[ITER], dd[ITER], r[ITER], v[ITER];
+  float ax[ITER], ay[ITER], az[ITER], dgv[ITER], dd[ITER], r[ITER], v[ITER];
   bool in[ITER], colour0[ITER];
 #pragma unroll
   for (int it = 0; it < ITER; ++it) {
@@ -393,33 +392,26 @@ __device__ __forceinline__ void gb_load(GbLds& s, const Level& L, const GbTile& 
     ay[it] = gc_ld(L.cy, c);
     az[it] = gc_ld(L.cz, c);
     dd[it] = L.dd[c];
+    if (!POST) dgv[it] = L.dg[c];
     r[it] = SV::ld(lr, c);
-    if (POST) {
-      v[it] = SV::rnd(SV::ld(x, c) + SV::ld(xp, in[it] ? lidx(C, i >> 1, j >> 1, k >> 1) : 0));
-    } else {
-      bx_[it] = in[it] && i + 1 < L.nx ? gc_ld(L.cx, c + 1) : 0.f;
-      by_[it] = in[it] && j + 1 < L.ny ? gc_ld(L.cy, c + L.nx) : 0.f;
-      bz_[it] = in[it] && k + 1 < L.nz ? gc_ld(L.cz, c + plane) : 0.f;
-    }
+    if (POST) v[it] = SV::rnd(SV::ld(x, c) + SV::ld(xp, in[it] ? lidx(C, i >> 1, j >> 1, k >> 1) : 0));
   }
 #pragma unroll
   for (int it = 0; it < ITER; ++it) {
     const int q = (int)threadIdx.x + it * GB_THREADS;
     if (q >= GB_N) break;
     const bool o = in[it];
-    const float a[6] = {o ? ax[it] : 0.f, POST ? 0.f : bx_[it], o ? ay[it] : 0.f,
-                        POST ? 0.f : by_[it], o ? az[it] : 0.f, POST ? 0.f : bz_[it]};
-    const float d = o ? dd[it] : 0.f, rr = o ? r[it] : 0.f;
-    s.fx[q] = (_Float16)a[0];
-    s.fy[q] = (_Float16)a[2];
-    s.fz[q] = (_Float16)a[4];
+    const float d = o ? dd[it] : 0.f, rr = o ? r[it] : 0.f, dg = (o && !POST) ? dgv[it] : 0.f;
+    s.fx[q] = (_Float16)(o ? ax[it] : 0.f);
+    s.fy[q] = (_Float16)(o ? ay[it] : 0.f);
+    s.fz[q] = (_Float16)(o ? az[it] : 0.f);
     s.dd[q] = d;
+    if (!POST) s.dg[q] = dg;
     s.r[q] = rr;
     float x0 = 0.f;
     if (POST) {
       if (o) x0 = v[it];
     } else if (o && colour0[it]) {
-      const float dg = gc_diag(a, d, L.inv);
       if (dg > 0.f) x0 = SV::rnd(rr / dg);
     }
     s.x[q] = x0;
@@ -442,8 +434,10 @@ __device__ __forceinline__ float gb_row(const GbLds& s, int c, const float a[6],
 }
 
 // One colour (k_gc_gs<0>) on the tile grown by GROW cells (1 <= GROW < GB_H:
-// every neighbour read stays in the box).
-template <int ST, int GROW>
+// every neighbour read stays in the box). DG: the diagonals were loaded into
+// LDS (pre-smoothing: faster); else they are formed from the couplings
+// (post-smoothing: the extra load cost more than the arithmetic it saves).
+template <int ST, int GROW, bool DG>
 __device__ __forceinline__ void gb_pass(GbLds& s, float inv, int colour) {
 #pragma clang fp contract(off)
   using SV = Store<ST>;
@@ -459,7 +453,7 @@ __device__ __forceinline__ void gb_pass(GbLds& s, float inv, int colour) {
     const int c = (bz * GB_B + by) * GB_B + bx;
     float a[6];
     gb_faces(s, c, a);
-    const float dg = gc_diag(a, s.dd[c], inv);
+    const float dg = DG ? s.dg[c] : gc_diag(a, s.dd[c], inv);
     if (!(dg > 0.f)) continue;
     const float pc = s.x[c];
     s.x[c] = SV::rnd(pc + (s.r[c] - gb_row(s, c, a, inv)) / dg);
@@ -472,10 +466,10 @@ __device__ __forceinline__ void gb_pass(GbLds& s, float inv, int colour) {
 template <int ST, int SWEEPS>
 __device__ __forceinline__ void gb_pre_passes(GbLds& s, float inv) {
   if (SWEEPS >= 2) {
-    gb_pass<ST, 2 * SWEEPS - 1>(s, inv, 1);
-    gb_pass<ST, 2 * SWEEPS - 2>(s, inv, 0);
+    gb_pass<ST, 2 * SWEEPS - 1, true>(s, inv, 1);
+    gb_pass<ST, 2 * SWEEPS - 2, true>(s, inv, 0);
   }
-  gb_pass<ST, 1>(s, inv, 1);
+  gb_pass<ST, 1, true>(s, inv, 1);
 }
 
 // Passes 1 .. 2 SWEEPS of the post-smoothing (colours 1, 0, 1, 0), pass m on
@@ -483,11 +477,11 @@ __device__ __forceinline__ void gb_pre_passes(GbLds& s, float inv) {
 template <int ST, int SWEEPS>
 __device__ __forceinline__ void gb_post_passes(GbLds& s, float inv) {
   if (SWEEPS >= 2) {
-    gb_pass<ST, 2 * SWEEPS - 1>(s, inv, 1);
-    gb_pass<ST, 2 * SWEEPS - 2>(s, inv, 0);
+    gb_pass<ST, 2 * SWEEPS - 1, false>(s, inv, 1);
+    gb_pass<ST, 2 * SWEEPS - 2, false>(s, inv, 0);
   }
-  gb_pass<ST, 1>(s, inv, 1);
-  gb_pass<ST, 0>(s, inv, 0);
+  gb_pass<ST, 1, false>(s, inv, 1);
+  gb_pass<ST, 0, false>(s, inv, 0);
 }
 
 // Pre-smoothing from zero (SWEEPS forward red-black sweeps, as gc_sweeps
