@@ -172,7 +172,6 @@ inline void prof_end(fsp_solver* S, bool on, hipStream_t stream) {
 #define K_CRESTRICT k_crestrict<ST>
 #define K_CPROLONG k_cprolong<ST>
 #define K_GCGS(M) k_gc_gs<M, ST>
-#define K_GCRESTRICT k_gc_restrict<ST>
 template <int ST> using TS = typename Store<ST>::T;
 
 // Calls f with the storage type of `storage` (FSP_STORE_*) as a constant.
@@ -466,11 +465,16 @@ struct Launch {
       else xcur[l] = coarse_sweeps<ST>(l, nullptr, sweeps, true, l == last);
       if (l < last) {
         Level& C = S->levels[l + 1];
-        if (L.compact)
-          FSP_LAUNCH_N(S, "k_gc_restrict", K_GCRESTRICT, qgrid(S->cqr[l]), st, L, S->cqr[l], (const TS<ST>*)xcur[l], C,
+        if (L.compact) {
+          // the restriction's grid for 128-thread halves (k_gc_restrict2)
+          QuadGrid H = S->cqr[l];
+          H.RY = (QBLOCK / 2) / H.QX;
+          H.nby = (H.rows + H.RY - 1) / H.RY;
+          FSP_LAUNCH_N(S, "k_gc_restrict", (k_gc_restrict2<ST>), qgrid(H), st, L, H, (const TS<ST>*)xcur[l], C,
                        S->ctl);
-        else
+        } else {
           FSP_LAUNCH_N(S, "k_crestrict", K_CRESTRICT, dim3(grid_for(C.n)), st, L, C, (const TS<ST>*)xcur[l], S->ctl);
+        }
       }
     }
     if (tf <= last) {

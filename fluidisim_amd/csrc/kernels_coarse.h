@@ -257,59 +257,80 @@ __global__ void __launch_bounds__(QBLOCK) k_gc_gs_first(Level L, QuadGrid Q, typ
 }
 
 // Residual r - row(x) of a compact level restricted to the next level
-// (delta3d_multigrid.rs:312-321, restrict3 :258-272): one thread per (quad
-// of the fine level, coarse row J, coarse plane K), 16 fine cells -> 2
-// coarse cells, children summed in the reference's order.
+// (delta3d_multigrid.rs:312-321, restrict3 :258-272): per (quad of the fine
+// level, coarse row J, coarse plane K), 16 fine cells -> 2 coarse cells,
+// children summed in the reference's order, the work split over two threads
+// (fine plane 2K in the first 128 threads of a workgroup, 2K + 1 in the last
+// 128) with the first half's partial sums carried to the second through LDS
+// and continued in child order: bit-identical to one thread per pair, with
+// twice the waves (10 -> 6 us on the 128x128x64 level). H: the restriction's
+// quad grid for 128-thread halves.
 template <int ST>
-__global__ void __launch_bounds__(QBLOCK) k_gc_restrict(Level F, QuadGrid Q, const typename Store<ST>::T* xf, Level C,
-                                                        Ctl* ctl) {
+__global__ void __launch_bounds__(QBLOCK) k_gc_restrict2(Level F, QuadGrid H, const typename Store<ST>::T* xf,
+                                                         Level C, Ctl* ctl) {
 #pragma clang fp contract(off)
   using SV = Store<ST>;
   using T = typename SV::T;
+  __shared__ float2 carry[QBLOCK / 2];
   if (!ctl->active) return;
-  int qx, J, K;
-  const bool valid = quad_coords(Q, qx, J, K);
+  const int local = (int)(threadIdx.x & (QBLOCK / 2 - 1)), dz = (int)(threadIdx.x / (QBLOCK / 2));
+  const unsigned Lb = xcd_block(blockIdx.x, gridDim.x);
+  const unsigned bx = Lb % (unsigned)H.nbx, tb = Lb / (unsigned)H.nbx;
+  const int qx = (int)bx * H.QX + (local & (H.QX - 1));
+  const int J = (int)(tb % (unsigned)H.nby) * H.RY + local / H.QX;
+  const int K = (int)(tb / (unsigned)H.nby);
+  const bool valid = qx < H.nq && J < H.rows && K < H.planes;
   const size_t plane = (size_t)F.nx * F.ny;
   const T* fr = reinterpret_cast<const T*>(F.r);
-  float t[2][2][4];
+  const int k = 2 * K + dz;
+  float t[2][4];  // [dy][e]
 #pragma unroll
-  for (int dz = 0; dz < 2; ++dz)
-#pragma unroll
-    for (int dy = 0; dy < 2; ++dy) {
-      const int y = 2 * J + dy, k = 2 * K + dz;
-      const size_t c0 = valid ? ((size_t)k * F.ny + y) * F.nx + 4 * (size_t)qx : 0;
-      GcQuad g{};
-      float4 xc = f4zero(), rq = f4zero(), ym = f4zero(), yp = f4zero(), zm = f4zero(), zp = f4zero();
-      if (valid) {
-        g = gc_quad(F, c0, 4 * qx, y, k);
-        rq = SV::ld4(fr, c0);
-        xc = SV::ld4(xf, c0);
-        if (y > 0) ym = SV::ld4(xf, c0 - F.nx);
-        if (y + 1 < F.ny) yp = SV::ld4(xf, c0 + F.nx);
-        if (k > 0) zm = SV::ld4(xf, c0 - plane);
-        if (k + 1 < F.nz) zp = SV::ld4(xf, c0 + plane);
-      }
-      float left, right;
-      x_neighbours(Q, valid, qx, xc, left, right, [&](int off) { return SV::ld(xf, c0 + off); });
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        float a[6];
-        gc_faces(g, e, a);
-        const float pc = q_get(xc, e);
-        const float nb[6] = {e == 0 ? left : q_get(xc, e - 1), e == 3 ? right : q_get(xc, e + 1),
-                             q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
-        t[dz][dy][e] = q_get(rq, e) - gc_row(a, q_get(g.dd, e), F.inv, pc, nb);
-      }
+  for (int dy = 0; dy < 2; ++dy) {
+    const int y = 2 * J + dy;
+    const size_t c0 = valid ? ((size_t)k * F.ny + y) * F.nx + 4 * (size_t)qx : 0;
+    GcQuad g{};
+    float4 xc = f4zero(), rq = f4zero(), ym = f4zero(), yp = f4zero(), zm = f4zero(), zp = f4zero();
+    if (valid) {
+      g = gc_quad(F, c0, 4 * qx, y, k);
+      rq = SV::ld4(fr, c0);
+      xc = SV::ld4(xf, c0);
+      if (y > 0) ym = SV::ld4(xf, c0 - F.nx);
+      if (y + 1 < F.ny) yp = SV::ld4(xf, c0 + F.nx);
+      if (k > 0) zm = SV::ld4(xf, c0 - plane);
+      if (k + 1 < F.nz) zp = SV::ld4(xf, c0 + plane);
     }
-  if (!valid) return;
+    float left, right;
+    x_neighbours(H, valid, qx, xc, left, right, [&](int off) { return SV::ld(xf, c0 + off); });
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      float a[6];
+      gc_faces(g, e, a);
+      const float pc = q_get(xc, e);
+      const float nb[6] = {e == 0 ? left : q_get(xc, e - 1), e == 3 ? right : q_get(xc, e + 1),
+                           q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
+      t[dy][e] = q_get(rq, e) - gc_row(a, q_get(g.dd, e), F.inv, pc, nb);
+    }
+  }
+  float2 sum = make_float2(0.f, 0.f);
+  if (dz == 0) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      float s = 0.f;
+      s += t[0][2 * h]; s += t[0][2 * h + 1];
+      s += t[1][2 * h]; s += t[1][2 * h + 1];
+      if (h == 0) sum.x = s; else sum.y = s;
+    }
+    carry[local] = sum;
+  }
+  __syncthreads();
+  if (dz == 0 || !valid) return;
+  sum = carry[local];
   float2 out;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
-    float s = 0.f;
-    s += t[0][0][2 * h]; s += t[0][0][2 * h + 1];
-    s += t[0][1][2 * h]; s += t[0][1][2 * h + 1];
-    s += t[1][0][2 * h]; s += t[1][0][2 * h + 1];
-    s += t[1][1][2 * h]; s += t[1][1][2 * h + 1];
+    float s = h == 0 ? sum.x : sum.y;
+    s += t[0][2 * h]; s += t[0][2 * h + 1];
+    s += t[1][2 * h]; s += t[1][2 * h + 1];
     if (h == 0) out.x = 0.125f * s; else out.y = 0.125f * s;
   }
   SV::st2(reinterpret_cast<T*>(C.r), lidx(C, 2 * qx, J, K), out);
