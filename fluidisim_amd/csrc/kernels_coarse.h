@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(QBLOCK) k_gc_restrict2(Level F, QuadGrid H, co
 // recomputed by each workgroup: pass m of n runs on the tile grown by n - m
 // cells (n - m + 1 before a restriction), which is exactly the region whose
 // next-pass neighbours it feeds, so every interior value is computed from the
-// operands k_gc_gs / k_gc_restrict use, in their order, and is bit-identical.
+// operands k_gc_gs / k_gc_restrict2 use, in their order, and is bit-identical.
 // Four halo cells cover two sweeps of two colours. Box cells outside the
 // level hold zero couplings, Dirichlet part, r and x, which the row excludes
 // exactly as the edge tests of the quad kernels do.
@@ -506,7 +506,7 @@ __device__ __forceinline__ void gb_post_passes(GbLds& s, float inv) {
 }
 
 // Pre-smoothing from zero (SWEEPS forward red-black sweeps, as gc_sweeps
-// launches them) and the restriction of the residual to C.r (k_gc_restrict).
+// launches them) and the restriction of the residual to C.r (k_gc_restrict2).
 // Writes the smoothed x of the tile's interior to L.x.
 template <int ST, int SWEEPS>
 __global__ void __launch_bounds__(GB_THREADS) k_gb_pre(Level L, Level C, Ctl* ctl) {
@@ -579,7 +579,7 @@ namespace fsp {
 // Every Galerkin level below the compact ones in one workgroup, in LDS:
 // per cell the x-, y-, z- couplings in f32 and dd, then x and r (values
 // rounded to storage ST): 6 floats. Same arithmetic as k_gc_gs /
-// k_gc_restrict. The couplings of all levels lie first, level after level
+// k_gc_restrict2. The couplings of all levels lie first, level after level
 // ([fx, fy, fz, dd] blocks of n floats each), exactly as k_gct_pack lays
 // them out in global memory, so that the kernel loads them as one
 // contiguous copy.
