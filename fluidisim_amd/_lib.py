@@ -127,7 +127,7 @@ def lib():
     L.fsp_divergence_device.restype = i32
     L.fsp_profile_begin.argtypes = [vp, C.c_char_p, u32]
     L.fsp_profile_begin.restype = i32
-    L.fsp_profile_end.argtypes = [vp, vp, vp, vp, u32]
+    L.fsp_profile_end.argtypes = [vp, vp, vp, vp, vp, u32]
     L.fsp_profile_end.restype = i32
     L.fsp_generate.argtypes = [i32, i32, i32, i32, C.c_uint64, vp, vp, vp, vp, vp]
     L.fsp_generate.restype = i32

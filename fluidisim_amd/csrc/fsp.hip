@@ -74,6 +74,7 @@ struct fsp_solver {
   hipGraph_t iter_graph[2] = {};      // one CG iteration: [0] as configured, [1] learned phase
   hipGraphExec_t iter_exec[2] = {};
   fsp_net* net = nullptr;             // learned preconditioner (not owned)
+  uint64_t net_gen = 0;               // the network's buffer generation iter_graph[1] captured
   int net_phase = 0;                  // 1 while the current pass uses the network
   uint32_t net_until = 0;             // iteration at which the learned phase ends
   // surface step (allocated on its first call): saved state, discharges, scalars
@@ -85,6 +86,7 @@ struct fsp_solver {
   std::string prof_name;
   std::vector<hipEvent_t> prof_events;
   size_t prof_used = 0;
+  int32_t* prof_active = nullptr;  // per timed launch: the loop-active flag it started with
 
   Fine fine() const {
     Fine G;
@@ -138,9 +140,18 @@ unsigned grid_for(size_t n) {
 // Launch on the solver's stream; when the kernel is the one being timed,
 // bracket the launch with a pair of events.
 // Event pair around the launch of the kernel being timed.
+// The loop-active flag a timed launch starts with (stream order: the value
+// the kernel itself reads), recorded before its start event.
+__global__ void k_prof_flag(const Ctl* ctl, int32_t* out) {
+  if (threadIdx.x == 0) *out = ctl->active;
+}
+
 inline bool prof_begin(fsp_solver* S, const char* name, hipStream_t stream) {
   const bool on = S->prof_on && S->prof_name == name && S->prof_used + 2 <= S->prof_events.size();
-  if (on) (void)hipEventRecord(S->prof_events[S->prof_used], stream);
+  if (on) {
+    k_prof_flag<<<1, 64, 0, stream>>>(S->ctl, S->prof_active + S->prof_used / 2);
+    (void)hipEventRecord(S->prof_events[S->prof_used], stream);
+  }
   return on;
 }
 inline void prof_end(fsp_solver* S, bool on, hipStream_t stream) {
@@ -315,15 +326,18 @@ struct Launch {
   }
 
   // r = b - A p64 and |r|^2 into `op`, on quads where the grid has them.
-  void residual(const double* p64, int op) {
+  // replacement: inside the CG loop, only in the iterations that replace the
+  // recurrence residual by the true one (replace_every).
+  void residual(const double* p64, int op, bool replacement = false) {
+    const int gated = replacement ? 1 : 0, fin_gate = replacement ? 3 : 0;
     if (S->quad) {
       const dim3 gq((unsigned)(S->qfine.nbx * S->qfine.nby * S->qfine.planes));
-      FSP_LAUNCH(S, k_residual_q, gq, st, G, S->qfine, p64, S->b, S->r, op, S->ctl, S->partials);
-      FSP_FINISH(S, st, op, gq, false, 0);
+      FSP_LAUNCH(S, k_residual_q, gq, st, G, S->qfine, p64, S->b, S->r, op, S->ctl, S->partials, gated);
+      FSP_FINISH(S, st, op, gq, false, fin_gate);
     } else {
       const dim3 g(grid_for(G.n));
-      FSP_LAUNCH(S, k_residual, g, st, G, p64, S->b, S->r, op, S->ctl, S->partials);
-      FSP_FINISH(S, st, op, g, false, 0);
+      FSP_LAUNCH(S, k_residual, g, st, G, p64, S->b, S->r, op, S->ctl, S->partials, gated);
+      FSP_FINISH(S, st, op, g, false, fin_gate);
     }
   }
 
@@ -684,17 +698,19 @@ struct Launch {
                  S->partials);
       FSP_FINISH(S, st, OP_RN, dim3(grid_for(G.n)), false, 1);
     }
+    // true-residual replacement: every replace_every-th iteration r (and
+    // |r|^2, which the loop test reads) becomes b - A p64 formed in f64; the
+    // preconditioned residual, beta and the direction then follow from it
+    if (S->opt.replace_every > 0) residual(S->p64, OP_RN_TRUE, true);
     precondition(OP_RZ);
   }
 
   void set_pass(int active, double rr, double tol_b2, double target, uint32_t it, uint32_t max_iters) {
-    // A pass ends after replace_every iterations so the recurrence residual
-    // is replaced by the true one (f64) at least that often.
     const uint32_t every = S->opt.replace_every > 0 ? (uint32_t)S->opt.replace_every : 0u;
-    uint32_t limit = every ? (it + every < it ? 0xffffffffu : it + every) : 0xffffffffu;
+    uint32_t limit = 0xffffffffu;
     if (S->net_phase) limit = std::min(limit, S->net_until);
     hipLaunchKernelGGL(k_set_pass, dim3(1), dim3(64), 0, st, S->ctl, active, rr, tol_b2, target, it, max_iters, limit,
-                       S->net_phase);
+                       S->net_phase, every);
   }
 };
 
@@ -713,6 +729,8 @@ void cleanup(fsp_solver* S) {
   }
   for (hipEvent_t e : S->prof_events) (void)hipEventDestroy(e);
   S->prof_events.clear();
+  if (S->prof_active) (void)hipFree(S->prof_active);
+  S->prof_active = nullptr;
   for (void* p : S->allocs) (void)hipFree(p);
   S->allocs.clear();
   if (S->hctl) (void)hipHostFree(S->hctl);
@@ -832,7 +850,48 @@ fsp_status capture_iteration(fsp_solver* S) {
   }
   S->iter_graph[slot] = g;
   FSP_HIP(hipGraphInstantiate(&S->iter_exec[slot], g, nullptr, nullptr, 0));
+  // device resources of the graph now, not at its first replay
+  FSP_HIP(hipGraphUpload(S->iter_exec[slot], S->stream));
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  if (slot == 1) S->net_gen = net_generation(S->net);
   return FSP_OK;
+}
+
+void drop_graph(fsp_solver* S, int slot) {
+  if (S->iter_exec[slot]) (void)hipGraphExecDestroy(S->iter_exec[slot]);
+  if (S->iter_graph[slot]) (void)hipGraphDestroy(S->iter_graph[slot]);
+  S->iter_exec[slot] = nullptr;
+  S->iter_graph[slot] = nullptr;
+}
+
+// Captures and instantiates the iteration graph of phase `phase` (0: as
+// configured, 1: the learned phase) ahead of the projections, so that a
+// projection allocates nothing (the reference reserves every buffer before
+// seal(), delta3d.rs:225-241). A throwaway capture of the prime first sets
+// which buffer holds M^-1 r, as the prime of a projection does before its
+// first iteration.
+fsp_status prepare_graph(fsp_solver* S, int phase) {
+  if (!S->opt.graph || S->iter_exec[phase]) return FSP_OK;
+  if (phase == 1 && !S->net) return FSP_OK;
+  const int saved = S->net_phase;
+  const bool prof = S->prof_on;
+  S->net_phase = phase;
+  S->prof_on = false;
+  hipError_t e = hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    {
+      Launch K(S);
+      K.precondition(OP_RZ_PRIME);
+    }
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(S->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+  }
+  S->prof_on = prof;
+  const fsp_status st = e == hipSuccess ? capture_iteration(S) : FSP_DEVICE;
+  S->net_phase = saved;
+  if (e != hipSuccess) std::fprintf(stderr, "fsp: prime capture failed: %s\n", hipGetErrorString(e));
+  return st;
 }
 
 // Geometry rebuilt from the surface at the start of every projection
@@ -853,6 +912,12 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
     if (!S->net) return FSP_INVALID;
     const fsp_status ns = net_reserve(S->net, S->dom.nx, S->dom.ny, S->dom.nz);
     if (ns) return ns;
+    // the network's buffers moved since the learned graph was captured (a
+    // forward pass or another solver at another shape): capture it again
+    if (S->iter_exec[1] && S->net_gen != net_generation(S->net)) {
+      drop_graph(S, 1);
+      if (const fsp_status gs = prepare_graph(S, 1)) return gs;
+    }
     S->net_phase = 1;
     S->net_until = S->opt.learned_budget > 0 ? (uint32_t)S->opt.learned_budget : 50u;
   }
@@ -1063,6 +1128,8 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
   return FSP_OK;
 }
 
+fsp_status step_reserve(fsp_solver* S);
+
 bool all_finite(const float* x, size_t n) {
   for (size_t q = 0; q < n; ++q) if (!std::isfinite(x[q])) return false;
   return true;
@@ -1246,6 +1313,9 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
   // the network's output is an f32 vector; it shares z with the V-cycle
   if (opt.preconditioner == FSP_PRECOND_LEARNED && opt.vcycle_storage != FSP_STORE_F32) return fail(FSP_INVALID);
   S->zres = S->z;
+  // the surface step's buffers and the iteration graph: nothing is reserved
+  // after this call (I-06)
+  if ((st = step_reserve(S)) || (st = prepare_graph(S, 0))) return fail(st);
   *out = S;
   return FSP_OK;
 }
@@ -1418,12 +1488,11 @@ fsp_status fsp_attach_net(fsp_solver* S, fsp_net* net) {
   FSP_HIP(hipStreamSynchronize(S->stream));
   S->net = net;
   // graphs of the learned phase hold the previous network's buffers
-  if (S->iter_exec[1]) (void)hipGraphExecDestroy(S->iter_exec[1]);
-  if (S->iter_graph[1]) (void)hipGraphDestroy(S->iter_graph[1]);
-  S->iter_exec[1] = nullptr;
-  S->iter_graph[1] = nullptr;
-  if (net) return net_reserve(net, S->dom.nx, S->dom.ny, S->dom.nz);
-  return FSP_OK;
+  drop_graph(S, 1);
+  if (!net) return FSP_OK;
+  fsp_status st = net_reserve(net, S->dom.nx, S->dom.ny, S->dom.nz);
+  if (st == FSP_OK && S->opt.preconditioner == FSP_PRECOND_LEARNED) st = prepare_graph(S, 1);
+  return st;
 }
 
 fsp_status fsp_net_features_device(fsp_net* net, fsp_solver* S, const float* d_r, uint16_t* d_x, float sigma,
@@ -1677,8 +1746,14 @@ fsp_status fsp_project_batch(fsp_solver* const* solvers, const fsp_patch_io* io,
     const fsp_patch_io& q = io[i];
     if (!solvers[i] || !q.us || !q.vs || !q.ws || !q.p || !q.u || !q.v || !q.w) return FSP_INVALID;
     if (!std::isfinite(q.scale) || !std::isfinite(q.k1)) return FSP_NOT_FINITE;
-    for (int32_t j = 0; j < i; ++j)
+    for (int32_t j = 0; j < i; ++j) {
       if (solvers[j] == solvers[i]) return FSP_INVALID;  // one patch per solver: they run concurrently
+      // a network's activations belong to one forward pass at a time
+      if (solvers[i]->net && solvers[j]->net == solvers[i]->net &&
+          solvers[i]->opt.preconditioner == FSP_PRECOND_LEARNED &&
+          solvers[j]->opt.preconditioner == FSP_PRECOND_LEARNED)
+        return FSP_INVALID;
+    }
   }
   const int workers = std::max(1, std::min<int>(n, concurrency > 0 ? concurrency : 4));
   std::atomic<int32_t> next{0};
@@ -1757,13 +1832,11 @@ fsp_status fsp_autotune(fsp_solver* S, int32_t* tile_x, double* us) {
   }
   S->qfine = make_q(nq, ny, nz, best);
   S->qrestr = make_q(nq, ny / 2, nz / 2, best);
-  // graphs hold the previous launch shapes
-  for (int gi = 0; gi < 2; ++gi) {
-    if (S->iter_exec[gi]) (void)hipGraphExecDestroy(S->iter_exec[gi]);
-    if (S->iter_graph[gi]) (void)hipGraphDestroy(S->iter_graph[gi]);
-    S->iter_exec[gi] = nullptr;
-    S->iter_graph[gi] = nullptr;
-  }
+  // graphs hold the previous launch shapes: capture them again
+  for (int gi = 0; gi < 2; ++gi) drop_graph(S, gi);
+  fsp_status gst = prepare_graph(S, 0);
+  if (gst == FSP_OK && S->net && S->opt.preconditioner == FSP_PRECOND_LEARNED) gst = prepare_graph(S, 1);
+  if (gst) return gst;
   if (tile_x) *tile_x = best;
   if (us) *us = best_us;
   return FSP_OK;
@@ -1774,6 +1847,10 @@ fsp_status fsp_profile_begin(fsp_solver* S, const char* kernel, uint32_t max_lau
   FSP_HIP(hipSetDevice(S->opt.device));
   FSP_HIP(hipStreamSynchronize(S->stream));
   for (hipEvent_t e : S->prof_events) (void)hipEventDestroy(e);
+  S->prof_events.clear();
+  if (S->prof_active) (void)hipFree(S->prof_active);
+  S->prof_active = nullptr;
+  FSP_HIP(hipMalloc((void**)&S->prof_active, (size_t)max_launches * sizeof(int32_t)));
   S->prof_events.assign(2 * (size_t)max_launches, nullptr);
   for (auto& e : S->prof_events) FSP_HIP(hipEventCreate(&e));
   S->prof_name = kernel;
@@ -1783,17 +1860,21 @@ fsp_status fsp_profile_begin(fsp_solver* S, const char* kernel, uint32_t max_lau
 }
 
 fsp_status fsp_profile_end(fsp_solver* S, uint32_t* launches, double* total_ms, float* durations_ms,
-                           uint32_t max_durations) {
+                           int32_t* active, uint32_t max_durations) {
   if (!S) return FSP_INVALID;
   FSP_HIP(hipStreamSynchronize(S->stream));
+  const size_t count = S->prof_used / 2;
   double total = 0.0;
-  for (size_t q = 0; q + 1 < S->prof_used; q += 2) {
+  for (size_t q = 0; q < count; ++q) {
     float ms = 0.f;
-    FSP_HIP(hipEventElapsedTime(&ms, S->prof_events[q], S->prof_events[q + 1]));
+    FSP_HIP(hipEventElapsedTime(&ms, S->prof_events[2 * q], S->prof_events[2 * q + 1]));
     total += ms;
-    if (durations_ms && q / 2 < max_durations) durations_ms[q / 2] = ms;
+    if (durations_ms && q < max_durations) durations_ms[q] = ms;
   }
-  if (launches) *launches = (uint32_t)(S->prof_used / 2);
+  if (active && count > 0)
+    FSP_HIP(hipMemcpy(active, S->prof_active, std::min<size_t>(count, max_durations) * sizeof(int32_t),
+                      hipMemcpyDeviceToHost));
+  if (launches) *launches = (uint32_t)count;
   if (total_ms) *total_ms = total;
   S->prof_on = false;
   S->prof_used = 0;

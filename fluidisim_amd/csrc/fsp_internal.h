@@ -36,6 +36,7 @@ enum Op : int {
   OP_BACKWARD = 10, // backward error max
   OP_COUNT = 11,    // wet cells, ghost faces
   OP_GENERIC = 12,  // plain sum (tests)
+  OP_RN_TRUE = 13,  // |r|^2 of the true residual replacing the recurrence inside the loop
 };
 
 // Device-resident control block of the conjugate gradient.
@@ -45,6 +46,7 @@ struct Ctl {
   double red[NUM_SLOTS][2];  // raw totals of every reduction slot
   double mean_b, mean_p;
   uint32_t it, max_iters, pass_limit;
+  uint32_t replace_every;  // every replace_every-th iteration r becomes the true residual b - A p (0: never)
   int32_t active;        // 1 while the CG loop runs; every iteration kernel returns early at 0
   int32_t first_dir;     // 1: the next direction is d = z
   int32_t dsel;          // which direction buffer holds the current d
@@ -53,6 +55,8 @@ struct Ctl {
   int32_t pure_neumann;  // no Dirichlet face on the fine level (and gauge on)
   int32_t gauge;
   int32_t flexible;      // 1: beta = -alpha <z', q> / <z, r> (flexible CG), else <z', r'> / <z, r>
+  int32_t stalled;       // a replaced true residual did not fall below the previous one: the pass ends
+  double rr_replaced;    // |r|^2 of the last replaced true residual of the pass
   uint32_t wet_cells, ghost_faces;
   uint32_t counters[NUM_SLOTS];
 };

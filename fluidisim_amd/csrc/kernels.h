@@ -113,6 +113,15 @@ __device__ void finish(int op, Ctl* ctl, double a, double b) {
       ctl->rn = a;
       if (!isfinite(a)) { ctl->nonfinite = 1; ctl->active = 0; }
       break;
+    case OP_RN_TRUE:
+      // the true residual replaces the recurrence; no progress since the
+      // previous replacement ends the pass, so that the driver's round-off
+      // floor checks run on it
+      ctl->rn = a;
+      if (!isfinite(a)) { ctl->nonfinite = 1; ctl->active = 0; break; }
+      if (!(a < ctl->rr_replaced)) ctl->stalled = 1;
+      ctl->rr_replaced = a;
+      break;
     case OP_RZ:
       if (!isfinite(a) || (ctl->flexible && !isfinite(b))) { ctl->nonfinite = 1; ctl->active = 0; break; }
       // flexible CG (Polak-Ribiere): <z', r' - r> = -alpha <z', q> since r' = r - alpha q
@@ -121,7 +130,7 @@ __device__ void finish(int op, Ctl* ctl, double a, double b) {
       ctl->rr = ctl->rn;
       ctl->it += 1;
       ctl->active = ((ctl->rr > ctl->tol_b2 || ctl->rr > ctl->physical_target) && ctl->it < ctl->max_iters &&
-                     ctl->it < ctl->pass_limit) ? 1 : 0;
+                     ctl->it < ctl->pass_limit && !ctl->stalled) ? 1 : 0;
       break;
     case OP_RZ_PRIME:
       if (!isfinite(a)) { ctl->nonfinite = 1; ctl->active = 0; break; }
@@ -144,6 +153,13 @@ __device__ void finish(int op, Ctl* ctl, double a, double b) {
   }
 }
 
+// The iteration in progress replaces the recurrence residual by the true
+// one: every replace_every-th iteration of the CG loop (the direction and
+// <r, z> recurrences then continue from the replaced r).
+__device__ __forceinline__ bool replacing(const Ctl* ctl) {
+  return ctl->active && ctl->replace_every > 0u && (ctl->it + 1u) % ctl->replace_every == 0u;
+}
+
 template <bool MAX>
 __device__ __forceinline__ void reduce_finish(double a, double b, int op, Ctl* ctl, double* partials) {
   (void)ctl;
@@ -161,7 +177,7 @@ constexpr int FIN_THREADS = 1024;
 template <bool MAX>
 __global__ void __launch_bounds__(FIN_THREADS) k_finish(int op, unsigned nblocks, int gated, Ctl* ctl,
                                                         double* partials) {
-  if ((gated == 1 && !ctl->active) || (gated == 2 && !ctl->pure_neumann)) return;
+  if ((gated == 1 && !ctl->active) || (gated == 2 && !ctl->pure_neumann) || (gated == 3 && !replacing(ctl))) return;
   __shared__ double sa[FIN_THREADS / 64], sb[FIN_THREADS / 64];
   const double2* part = reinterpret_cast<const double2*>(partials + (size_t)op * MAX_PARTIALS * 2);
   // eight independent loads in flight per step, folded in a fixed order
@@ -576,9 +592,11 @@ __global__ void __launch_bounds__(BLOCK) k_fill(float* x, size_t n, float value)
 
 // r = b - A p, |r|^2 (delta3d_mobile.rs:497-503, :547-553), with the
 // pressure held and the residual formed in f64; r is stored in f32.
+// gated: only in an iteration that replaces the residual.
 __global__ void __launch_bounds__(BLOCK) k_residual(Fine G, const double* p, const float* b, float* r, int op, Ctl* ctl,
-                                                    double* partials) {
+                                                    double* partials, int gated) {
 #pragma clang fp contract(off)
+  if (gated && !replacing(ctl)) return;
   double s = 0.0, unused = 0.0;
   FSP_FOR(c, G.n) {
     int i, j, k;
@@ -1245,10 +1263,13 @@ __global__ void k_hash_combine(const unsigned long long* hashes, size_t chunks, 
 
 // Host-side parameters of a CG pass, written on the stream.
 __global__ void k_set_pass(Ctl* ctl, int active, double rr, double tol_b2, double physical_target, uint32_t it,
-                           uint32_t max_iters, uint32_t pass_limit, int flexible) {
+                           uint32_t max_iters, uint32_t pass_limit, int flexible, uint32_t replace_every) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   ctl->active = active;
   ctl->flexible = flexible;
+  ctl->replace_every = replace_every;
+  ctl->stalled = 0;
+  ctl->rr_replaced = rr;
   ctl->rr = rr;
   ctl->tol_b2 = tol_b2;
   ctl->physical_target = physical_target;

@@ -310,8 +310,9 @@ __global__ void __launch_bounds__(QBLOCK) k_mask_q(Fine G, QuadGrid Q, uint8_t* 
 // indices clamped into the grid, the row's case decided after; apply_row_t's
 // arithmetic per cell).
 __global__ void __launch_bounds__(QBLOCK) k_residual_q(Fine G, QuadGrid Q, const double* p, const float* b, float* r,
-                                                       int op, Ctl* ctl, double* partials) {
+                                                       int op, Ctl* ctl, double* partials, int gated) {
 #pragma clang fp contract(off)
+  if (gated && !replacing(ctl)) return;
   double s = 0.0, unused = 0.0;
   int qx, y, k;
   const bool valid = quad_coords(Q, qx, y, k);

@@ -590,6 +590,7 @@ struct fsp_net {
   float gscale = 0.f;
   // activations for one shape
   int nx = 0, ny = 0, nz = 0;
+  uint64_t generation = 0;  // bumped whenever the shape buffers are reallocated
   size_t V[3] = {};
   _Float16* x0 = nullptr;
   _Float16* act[NL] = {};
@@ -769,8 +770,11 @@ fsp_status net_reserve(fsp_net* N, int nx, int ny, int nz) {
   N->partial_floats = std::max(most * 64, (size_t)8192 * 256);
   if ((st = nalloc(N, &N->partial, N->partial_floats))) return st;
   N->nx = nx; N->ny = ny; N->nz = nz;
+  ++N->generation;
   return FSP_OK;
 }
+
+uint64_t net_generation(const fsp_net* N) { return N ? N->generation : 0; }
 
 fsp_status net_features(fsp_net* N, hipStream_t stream, const float* r, const uint8_t* mask, const uint8_t* solid,
                         const double* rr, const uint32_t* nwet, float sigma, uint16_t* x_out) {

@@ -240,11 +240,13 @@ fsp_status fsp_surface_scalars(fsp_solver* s, double* volume, int32_t* in_bounds
  * fsp_profile_begin reserves max_launches event pairs and brackets every
  * launch of the kernel named `kernel` (its symbol name, e.g. "k_fgs") with
  * them; fsp_profile_end synchronises and returns the launch count, the
- * summed device time and (if durations_ms is not NULL) up to max_durations
- * per-launch times in launch order. */
+ * summed device time and, when the arrays are not NULL, up to max_durations
+ * per-launch times and loop-active flags in launch order. active[i] is the
+ * CG loop's active flag launch i started with: the iteration kernels return
+ * at once when it is 0 (launches queued past convergence). */
 fsp_status fsp_profile_begin(fsp_solver* s, const char* kernel, uint32_t max_launches);
 fsp_status fsp_profile_end(fsp_solver* s, uint32_t* launches, double* total_ms, float* durations_ms,
-                           uint32_t max_durations);
+                           int32_t* active, uint32_t max_durations);
 
 /* Seeded synthetic inputs (host memory). cfg: 0 = closed box, i.i.d.
  * U[-1,1] faces; 1 = free-surface tank with 8 spheres and a 16-mode smooth

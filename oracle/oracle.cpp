@@ -91,6 +91,40 @@ int step_surface(const Geometry* g, const Options* o, float* u, float* v, float*
   return st;
 }
 
+template <class T>
+int backward_error(const Geometry* g, const double* p, const double* rhs, const double* res, double* out) {
+  Volume<T> vol;
+  Options opt = default_options();
+  opt.multigrid = 0;
+  Status st = vol.configure(*g, opt);
+  if (st) return st;
+  for (size_t q = 0; q < vol.cells(); ++q) {
+    vol.p[q] = (T)p[q];
+    vol.rhs[q] = (T)rhs[q];
+    vol.res[q] = (T)res[q];
+  }
+  *out = (double)vol.backward_error_mobile3();
+  return OK;
+}
+
+template <class T>
+int coarse_apply(const Geometry* g, const Options* o, int level, const double* x, double* out) {
+  Volume<T> vol;
+  Options opt = o ? *o : default_options();
+  opt.multigrid = 1;
+  Status st = vol.configure(*g, opt);
+  if (st) return st;
+  if (level < 0 || (size_t)level >= vol.levels.size()) return ERR_DOMAIN;
+  vol.prepare_multigrid3();
+  const Level3<T>& L = vol.levels[level];
+  const size_t n = L.nx * L.ny * L.nz;
+  std::vector<T> xx(n), oo(n);
+  for (size_t q = 0; q < n; ++q) xx[q] = (T)x[q];
+  Volume<T>::row_sums(L, &xx, oo);
+  for (size_t q = 0; q < n; ++q) out[q] = (double)oo[q];
+  return OK;
+}
+
 Options no_multigrid() {
   Options opt = default_options();
   opt.multigrid = 0;
@@ -240,6 +274,38 @@ int orc_divergence(const Geometry* g, const float* u, const float* v, const floa
   std::vector<float> uu(u, u + vol.nu()), vv(v, v + vol.nv()), ww(w, w + vol.nw()), oo(vol.cells());
   vol.divergence(uu, vv, ww, oo);
   std::copy(oo.begin(), oo.end(), out);
+  return OK;
+}
+
+// Oettli-Prager backward error (delta3d_mobile.rs:243-271) of a given
+// pressure, right-hand side and residual, in f32 (32) or f64 (64).
+int orc_backward_error(int precision, const Geometry* g, const double* p, const double* rhs, const double* res,
+                       double* out) {
+  if (precision == 64) return backward_error<double>(g, p, rhs, res, out);
+  return backward_error<float>(g, p, rhs, res, out);
+}
+
+// row_sums (delta3d_multigrid.rs:216-255) of coarse level `level` applied to x,
+// after the hierarchy is prepared from the geometry (options as given).
+int orc_coarse_apply(int precision, const Geometry* g, const Options* o, int level, const double* x, double* out) {
+  if (precision == 64) return coarse_apply<double>(g, o, level, x, out);
+  return coarse_apply<float>(g, o, level, x, out);
+}
+
+// The acceptance metric (delta3d_mobile.rs:398-433) of given faces: the
+// maximum |div| over wet rows and over plain rows, normalised by the largest
+// face speed (f32, the reference's arithmetic).
+int orc_divergence_metric(const Geometry* g, const float* u, const float* v, const float* w, double* div_all,
+                          double* div_plain) {
+  Volume<float> vol;
+  Options o = default_options();
+  o.multigrid = 0;
+  Status st = vol.configure(*g, o);
+  if (st) return st;
+  std::copy(u, u + vol.nu(), vol.u.begin());
+  std::copy(v, v + vol.nv(), vol.v.begin());
+  std::copy(w, w + vol.nw(), vol.w.begin());
+  vol.divergence_mobile3(*div_all, *div_plain);
   return OK;
 }
 

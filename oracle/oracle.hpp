@@ -65,9 +65,10 @@ struct Options {
   int32_t flexible;       // EXTENSION: 1 = flexible CG, beta = <z', r' - r> / <z, r> (Polak-Ribiere)
   int32_t level_pre;      // EXTENSION: sweeps before restriction on the coarse levels (<0: pre_sweeps)
   int32_t level_post;     // EXTENSION: sweeps after prolongation on the coarse levels (<0: post_sweeps)
+  double rel_tol;         // EXTENSION: relative residual target; 1e-6 is the reference's (delta3d_mobile.rs:507)
 };
 
-inline Options default_options() { return Options{1, 0, -1, 2, 2, 8, 1, 0, 0, 0, -1, -1}; }
+inline Options default_options() { return Options{1, 0, -1, 2, 2, 8, 1, 0, 0, 0, -1, -1, 1e-6}; }
 
 // Round-to-nearest-even of an f32 value to bfloat16 (8 significant bits,
 // f32 exponent range); the result is returned as the f32 it represents.
@@ -929,7 +930,8 @@ template <class T> class Volume {
     T rz = T(0);
     bool primed = false;
     uint32_t it = 0;
-    const T tol = (T)1e-12;
+    // rr > 1e-12 b2 (:507, :515); another target only through the rel_tol extension
+    const T tol = opt.rel_tol == 1e-6 ? (T)1e-12 : (T)(opt.rel_tol * opt.rel_tol);
     bool floor_stop = false;
     T physical_target = std::numeric_limits<T>::infinity();
     bool have_checkpoint = false;

@@ -34,7 +34,7 @@ class Options(C.Structure):
         ("multigrid", C.c_int32), ("smoother", C.c_int32), ("max_levels", C.c_int32),
         ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32), ("coarse_sweeps", C.c_int32),
         ("gauge", C.c_int32), ("coarse_op", C.c_int32), ("storage", C.c_int32), ("flexible", C.c_int32),
-        ("level_pre", C.c_int32), ("level_post", C.c_int32),
+        ("level_pre", C.c_int32), ("level_post", C.c_int32), ("rel_tol", C.c_double),
     ]
 
 
@@ -71,6 +71,9 @@ def lib():
         L.orc_assemble.argtypes = [vp, vp, f32, vp, vp]
         L.orc_divergence.argtypes = [vp, vp, vp, vp, vp]
         L.orc_correct.argtypes = [vp, vp, vp, vp, vp, f32, vp, vp, vp, vp, vp]
+        L.orc_divergence_metric.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.orc_backward_error.argtypes = [i32, vp, vp, vp, vp, vp]
+        L.orc_coarse_apply.argtypes = [i32, vp, vp, i32, vp, vp]
         L.orc_vcycle.argtypes = [i32, vp, vp, vp, vp]
         L.orc_classify.argtypes = [vp, vp, vp]
         L.orc_coarse_level.argtypes = [vp, vp, i32, vp, vp]
@@ -149,9 +152,9 @@ class Problem:
 
 
 def options(multigrid=1, smoother=0, max_levels=-1, pre=2, post=2, coarse=8, gauge=1, coarse_op=0, storage=0,
-            flexible=0, level_pre=-1, level_post=-1):
+            flexible=0, level_pre=-1, level_post=-1, rel_tol=1e-6):
     return Options(multigrid, smoother, max_levels, pre, post, coarse, gauge, coarse_op, storage, flexible,
-                   level_pre, level_post)
+                   level_pre, level_post, rel_tol)
 
 
 def project(prob: Problem, us, vs, ws, scale, k1, max_iters=20000, opts=None, p_warm=None, precision=32):
@@ -219,6 +222,36 @@ def correct(prob: Problem, us, vs, ws, p, k1):
     assert lib().orc_correct(C.byref(g), _ptr(us), _ptr(vs), _ptr(ws), _ptr(p), k1, _ptr(u), _ptr(v), _ptr(w),
                              C.byref(a), C.byref(b)) == 0
     return u, v, w, a.value, b.value
+
+
+def divergence_metric(prob: Problem, u, v, w):
+    """(all, plain): the acceptance metric of delta3d_mobile.rs:398-433 on the given faces."""
+    g = prob.geometry()
+    u, v, w = (np.ascontiguousarray(a, np.float32) for a in (u, v, w))
+    a, b = C.c_double(), C.c_double()
+    assert lib().orc_divergence_metric(C.byref(g), _ptr(u), _ptr(v), _ptr(w), C.byref(a), C.byref(b)) == 0
+    return a.value, b.value
+
+
+def backward_error(prob: Problem, p, rhs, res, precision=64):
+    """max |r| / (|b| + |A| |p|) over wet rows (delta3d_mobile.rs:243-271)."""
+    g = prob.geometry()
+    p, rhs, res = (np.ascontiguousarray(a, np.float64) for a in (p, rhs, res))
+    out = C.c_double()
+    assert lib().orc_backward_error(precision, C.byref(g), _ptr(p), _ptr(rhs), _ptr(res), C.byref(out)) == 0
+    return out.value
+
+
+def coarse_apply(prob: Problem, level, x, opts=None, precision=64):
+    """row_sums of coarse level `level` applied to x (delta3d_multigrid.rs:216-255)."""
+    g = prob.geometry()
+    o = opts if opts is not None else options()
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros_like(x)
+    st = lib().orc_coarse_apply(precision, C.byref(g), C.byref(o), level, _ptr(x), _ptr(out))
+    if st != 0:
+        raise RuntimeError(f"no coarse level {level}")
+    return out
 
 
 def vcycle(prob: Problem, r, opts=None, precision=32):
