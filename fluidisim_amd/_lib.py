@@ -19,6 +19,7 @@ PRECOND_JACOBI, PRECOND_MULTIGRID, PRECOND_LEARNED = 0, 1, 2
 SMOOTH_JACOBI, SMOOTH_RBGS = 0, 1
 STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
 COARSE_REDISCRETISED, COARSE_GALERKIN = 0, 1
+SWEEP_FUSED, SWEEP_SINGLE = 0, 1
 
 # Every entry point include/fsp.h declares.
 EXPORTS = [
@@ -45,7 +46,7 @@ class Options(C.Structure):
         ("gauge", C.c_int32), ("vcycle_storage", C.c_int32), ("replace_every", C.c_int32),
         ("refuse_degraded", C.c_int32), ("device", C.c_int32), ("coarse_operator", C.c_int32),
         ("graph", C.c_int32), ("learned_budget", C.c_int32), ("learned_smooth", C.c_int32),
-        ("learned_vcycle", C.c_int32),
+        ("learned_vcycle", C.c_int32), ("fine_sweeps", C.c_int32),
     ]
 
 

@@ -10,6 +10,7 @@
 // that produced its reduction, so the host only polls a small control block
 // between batches of iterations.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <atomic>
@@ -58,6 +59,7 @@ struct fsp_solver {
   // tail for the coarse levels from tail_first on
   bool quad = false;
   QuadGrid qfine{}, qrestr{};
+  SweepGrid sweep{};  // fused red-black sweeps (nq == 0: rows too wide, single passes)
   std::vector<QuadGrid> cq, cqr;  // compact levels: quads of the level, and of its restriction
   int tail_first = 0;
   size_t tail_lds = 0;
@@ -137,52 +139,54 @@ unsigned grid_for(size_t n) {
   return (unsigned)b;
 }
 
-// Launch on the solver's stream; when the kernel is the one being timed,
-// bracket the launch with a pair of events.
-// Event pair around the launch of the kernel being timed.
 // The loop-active flag a timed launch starts with (stream order: the value
-// the kernel itself reads), recorded before its start event.
+// the kernel itself reads), recorded before the launch.
 __global__ void k_prof_flag(const Ctl* ctl, int32_t* out) {
   if (threadIdx.x == 0) *out = ctl->active;
 }
 
-inline bool prof_begin(fsp_solver* S, const char* name, hipStream_t stream) {
+// When `name` is the kernel being timed (fsp_profile_begin), records its
+// loop-active flag and hands out the event pair of its launch: the launch
+// then goes through hipExtLaunchKernelGGL, whose events take the dispatch's
+// own start and end timestamps (no marker packets around it).
+inline bool prof_begin(fsp_solver* S, const char* name, hipStream_t stream, hipEvent_t* e0, hipEvent_t* e1) {
   const bool on = S->prof_on && S->prof_name == name && S->prof_used + 2 <= S->prof_events.size();
   if (on) {
     k_prof_flag<<<1, 64, 0, stream>>>(S->ctl, S->prof_active + S->prof_used / 2);
-    (void)hipEventRecord(S->prof_events[S->prof_used], stream);
+    *e0 = S->prof_events[S->prof_used];
+    *e1 = S->prof_events[S->prof_used + 1];
+    S->prof_used += 2;
   }
   return on;
 }
-inline void prof_end(fsp_solver* S, bool on, hipStream_t stream) {
-  if (!on) return;
-  (void)hipEventRecord(S->prof_events[S->prof_used + 1], stream);
-  S->prof_used += 2;
-}
 
-// Launch with BLOCK threads (FSP_LAUNCH_NB: with `block`); `name` is what
-// fsp_profile_begin matches (the kernel without its storage argument).
-#define FSP_LAUNCH_N(S, name, kern, grid, stream, ...)            \
-  do {                                                          \
-    const bool prof_ = prof_begin(S, name, stream);             \
-    kern<<<grid, dim3(BLOCK), 0, stream>>>(__VA_ARGS__);        \
-    prof_end(S, prof_, stream);                                 \
+// Launch with BLOCK threads (FSP_LAUNCH_NB: with `block`, FSP_LAUNCH_NS: with
+// `lds` bytes of dynamic LDS); `name` is what fsp_profile_begin matches (the
+// kernel without its storage argument).
+#define FSP_LAUNCH_NBS(S, name, kern, grid, block, lds, stream, ...)                            \
+  do {                                                                                         \
+    hipEvent_t e0_ = nullptr, e1_ = nullptr;                                                   \
+    if (prof_begin(S, name, stream, &e0_, &e1_))                                               \
+      hipExtLaunchKernelGGL(kern, dim3(grid), dim3(block), (uint32_t)(lds), stream, e0_, e1_, 0, \
+                            __VA_ARGS__);                                                      \
+    else                                                                                       \
+      kern<<<grid, dim3(block), lds, stream>>>(__VA_ARGS__);                                   \
   } while (0)
-#define FSP_LAUNCH_NB(S, name, kern, grid, block, stream, ...)    \
-  do {                                                          \
-    const bool prof_ = prof_begin(S, name, stream);             \
-    kern<<<grid, dim3(block), 0, stream>>>(__VA_ARGS__);        \
-    prof_end(S, prof_, stream);                                 \
-  } while (0)
+#define FSP_LAUNCH_N(S, name, kern, grid, stream, ...) \
+  FSP_LAUNCH_NBS(S, name, kern, grid, BLOCK, 0, stream, __VA_ARGS__)
+#define FSP_LAUNCH_NB(S, name, kern, grid, block, stream, ...) \
+  FSP_LAUNCH_NBS(S, name, kern, grid, block, 0, stream, __VA_ARGS__)
+#define FSP_LAUNCH_NS(S, name, kern, grid, lds, stream, ...) \
+  FSP_LAUNCH_NBS(S, name, kern, grid, BLOCK, lds, stream, __VA_ARGS__)
 #define FSP_LAUNCH(S, kern, grid, stream, ...) FSP_LAUNCH_N(S, #kern, kern, grid, stream, __VA_ARGS__)
 
 // Kernels templated on the V-cycle storage ST of the enclosing function.
-#define K_QGS(M) k_q_gs<M, ST>
-#define K_QSPMV k_q_spmv_dir<ST>
-#define K_CGS k_cgs<ST>
-#define K_CRESTRICT k_crestrict<ST>
-#define K_CPROLONG k_cprolong<ST>
-#define K_GCGS(M) k_gc_gs<M, ST>
+#define K_QGS(M) (k_q_gs<M, ST>)
+#define K_QSPMV (k_q_spmv_dir<ST>)
+#define K_CGS (k_cgs<ST>)
+#define K_CRESTRICT (k_crestrict<ST>)
+#define K_CPROLONG (k_cprolong<ST>)
+#define K_GCGS(M) (k_gc_gs<M, ST>)
 template <int ST> using TS = typename Store<ST>::T;
 
 // Calls f with the storage type of `storage` (FSP_STORE_*) as a constant.
@@ -249,6 +253,22 @@ bool gc_tail_ok(const fsp_options& o) {
 // one post-sweep).
 bool quad_vcycle_ok(bool quad, const fsp_options& o) {
   return quad && o.smoother == FSP_SMOOTH_RBGS && o.pre_sweeps >= 1 && o.post_sweeps >= 1;
+}
+
+// Tile grid of the fused sweeps: SW_TY rows per tile, chunks of kz planes.
+// kz = 0: planes per chunk so that the grid has about 512 workgroups (two
+// per CU), at least 4 (a chunk reads two planes more than it writes).
+SweepGrid make_sweep(int nq, int ny, int nz, int kz) {
+  SweepGrid W{};
+  W.nq = nq;
+  W.tiles = (ny + SW_TY - 1) / SW_TY;
+#ifdef FSP_SWEEP_KZ
+  if (kz <= 0) kz = FSP_SWEEP_KZ;
+#endif
+  if (kz <= 0) kz = std::max(4, std::min(nz, (int)(((size_t)W.tiles * (size_t)nz) / 512)));
+  W.kz = std::min(kz, nz);
+  W.chunks = (nz + W.kz - 1) / W.kz;
+  return W;
 }
 
 struct Launch {
@@ -579,6 +599,7 @@ struct Launch {
     TS<ST>* z = reinterpret_cast<TS<ST>*>(S->z);
     TS<ST>* zt = reinterpret_cast<TS<ST>*>(S->zt);
     const TS<ST>* none = nullptr;
+    if (S->sweep.nq > 0) return vcycle_fused<ST>(dot_op, from_guess);
     auto pass = [&](TS<ST>* x, int colour, int op) {
       FSP_LAUNCH_N(S, "k_q_gs<0>", K_QGS(0), g, st, G, Q, (const TS<ST>*)x, x, S->r, colour, L0, none, op, S->ctl,
                    S->partials);
@@ -615,6 +636,51 @@ struct Launch {
     }
     for (int pz = done; pz < passes; ++pz) pass(out, (pz % 2 == 0) ? 1 : 0, pz == passes - 1 ? dot_op : -1);
     return out;
+  }
+
+  // The same V-cycle with each red-black sweep (both colours) in one
+  // launch (k_q_sweep): pre-smoothing red-black from the first pair of
+  // passes on, post-smoothing black-red with the prolongation fused into
+  // its first sweep. A sweep reads one buffer and writes the other, so the
+  // iterate alternates between z and zt. Bit-identical to vcycle_quad's
+  // single passes.
+  template <int ST>
+  TS<ST>* vcycle_fused(int dot_op, bool from_guess) {
+    const QuadGrid& Q = S->qfine;
+    const SweepGrid& W = S->sweep;
+    const dim3 g((unsigned)(Q.nbx * Q.nby * Q.planes));
+    const dim3 gs((unsigned)(W.tiles * W.chunks));
+    const size_t lds = sweep_lds_bytes(G.nx);
+    Level L0 = S->levels.empty() ? Level{} : S->levels[0];
+    TS<ST>* a = reinterpret_cast<TS<ST>*>(S->z);
+    TS<ST>* b = reinterpret_cast<TS<ST>*>(S->zt);
+    const TS<ST>* none = nullptr;
+    auto sweep = [&](int colour, const TS<ST>* xc, int op) {
+      if (xc)
+        FSP_LAUNCH_NS(S, "k_q_sweep<2>", (k_q_sweep<2, ST>), gs, lds, st, G, W, (const TS<ST>*)a, b, S->r, colour,
+                      L0, xc, op, S->ctl, S->partials);
+      else
+        FSP_LAUNCH_NS(S, "k_q_sweep<0>", (k_q_sweep<0, ST>), gs, lds, st, G, W, (const TS<ST>*)a, b, S->r, colour,
+                      L0, none, op, S->ctl, S->partials);
+      if (op >= 0) FSP_FINISH(S, st, op, gs, false, 1);
+      std::swap(a, b);
+    };
+    if (from_guess) sweep(0, none, -1);
+    else FSP_LAUNCH_N(S, "k_q_gs_first", k_q_gs_first<ST>, g, st, G, Q, a, S->r, S->ctl);
+    for (int s = 1; s < S->opt.pre_sweeps; ++s) sweep(0, none, -1);
+    const TS<ST>* x0 = none;
+    if (!S->levels.empty()) {
+      QuadGrid H = S->qrestr;
+      H.RY = (QBLOCK / 2) / H.QX;
+      H.nby = (H.rows + H.RY - 1) / H.RY;
+      const dim3 gr((unsigned)(H.nbx * H.nby * H.planes));
+      FSP_LAUNCH_N(S, "k_q_restrict", (k_q_restrict2<ST>), gr, st, G, H, (const TS<ST>*)a, (const float*)S->r, L0,
+                   S->ctl);
+      x0 = coarse_cycle<ST>();
+    }
+    for (int s = 0; s < S->opt.post_sweeps; ++s)
+      sweep(1, s == 0 ? x0 : none, s == S->opt.post_sweeps - 1 ? dot_op : -1);
+    return a;
   }
 
   void precondition(int op) {
@@ -1172,6 +1238,7 @@ void fsp_default_options(fsp_options* o) {
   o->learned_budget = 0;
   o->learned_smooth = 1;
   o->learned_vcycle = 0;
+  o->fine_sweeps = FSP_SWEEP_SINGLE;
 }
 
 fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** out) {
@@ -1186,6 +1253,7 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
   if (opt.preconditioner < 0 || opt.preconditioner > 2 || opt.smoother < 0 || opt.smoother > 1) return FSP_INVALID;
   if (opt.learned_budget < 0 || opt.learned_smooth < 0 || opt.learned_smooth > 64) return FSP_INVALID;
   if (opt.learned_vcycle < 0 || opt.learned_vcycle > 1) return FSP_INVALID;
+  if (opt.fine_sweeps < 0 || opt.fine_sweeps > 1) return FSP_INVALID;
   if (opt.preconditioner == FSP_PRECOND_LEARNED && opt.learned_vcycle &&
       (opt.vcycle_storage != FSP_STORE_F32 || opt.smoother != FSP_SMOOTH_RBGS || opt.pre_sweeps < 1 ||
        opt.post_sweeps < 1))
@@ -1266,6 +1334,8 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
     S->qrestr = make_q((int)nx / 4, (int)ny / 2, (int)nz / 2);
     const size_t blocks = (size_t)S->qfine.nbx * S->qfine.nby * S->qfine.planes;
     S->quad = blocks <= (size_t)MAX_PARTIALS && (S->n / 4 + QBLOCK - 1) / QBLOCK <= (size_t)MAX_PARTIALS;
+    if (S->quad && nx / 4 <= SW_MAXQ && opt.fine_sweeps == FSP_SWEEP_FUSED)
+      S->sweep = make_sweep((int)nx / 4, (int)ny, (int)nz, 0);
   }
   if (opt.preconditioner == FSP_PRECOND_LEARNED && opt.learned_vcycle && !quad_vcycle_ok(S->quad, opt))
     return fail(FSP_INVALID);

@@ -77,8 +77,17 @@ __device__ __forceinline__ float ghost_row_sum(const Fine& G, int i, int j, int 
   return acc;
 }
 
+// ghost_row_sum as a call, for kernels that inline many rows: the rare
+// free-surface rows then cost no registers in the common path.
+__device__ __attribute__((noinline)) float ghost_row_sum_call(const Fine& G, int i, int j, int k, uint8_t m, float pc,
+                                                              const float nb[6], float& prec) {
+  return ghost_row_sum(G, i, j, k, m, pc, nb, prec);
+}
+
 // Row of the fine operator from neighbour values already in registers, in the
-// reference's face order; prec as rhs_mobile3 computes it.
+// reference's face order; prec as rhs_mobile3 computes it (CALL: the ghost
+// rows through ghost_row_sum_call).
+template <bool CALL = false>
 __device__ __forceinline__ float row_local(const Fine& G, int i, int j, int k, uint8_t m, float pc, const float nb[6],
                                           float& prec) {
 #pragma clang fp contract(off)
@@ -91,7 +100,7 @@ __device__ __forceinline__ float row_local(const Fine& G, int i, int j, int k, u
       if (m & (1u << f)) acc += pc - nb[f];
     prec = G.prec_tab[__popc(m & 0x3Fu)];
   } else {
-    acc = ghost_row_sum(G, i, j, k, m, pc, nb, prec);
+    acc = CALL ? ghost_row_sum_call(G, i, j, k, m, pc, nb, prec) : ghost_row_sum(G, i, j, k, m, pc, nb, prec);
   }
   return acc * G.inv;
 }
@@ -261,6 +270,281 @@ __global__ void __launch_bounds__(QBLOCK, FSP_QGS_MIN_BLOCKS) k_q_gs_first(Fine 
     if (prec > 0.f) q_set(out, e, SV::rnd(pc + prec * (q_get(rq, e) - az)));
   }
   SV::st4(zout, c0, out);
+}
+
+// ---------------------------------------------------------------- fused red-black sweep
+// Both colour passes of one red-black sweep (colour c, then 1 - c) in one
+// launch. A workgroup owns a tile of SW_TY whole rows (the full x extent,
+// nx <= 4 * SW_MAXQ) over a chunk of planes and marches up the chunk with a
+// window of SW_SLOTS planes of z in LDS, each plane held with two extra rows
+// on either side of the tile:
+//   step s: pass c on plane s, over the tile rows and one row on either side
+//           (its neighbours are colour 1 - c values, untouched by this pass);
+//           barrier; pass 1 - c on plane s - 1 over the tile rows (its
+//           neighbours are the colour-c values of planes s - 2, s - 1, s, all
+//           updated by now), stored to zout; the next planes' loads, issued
+//           at the start of the step, land in the slot no pass reads.
+// A chunk starts one plane early and ends one plane late (the first pass of
+// the planes next to it), so chunks are independent; zout must not alias zin
+// (neighbouring tiles read zin's halo rows while this one writes).
+// Every cell's update uses the operands the single-colour passes use, in the
+// same order, so the sweep equals k_q_gs twice bit for bit.
+// MODE 2: every value read from zin is first prolonged from the first coarse
+// level (z + parent, rounded to storage), rows with prec == 0 become 0
+// (prolong_add3 then the dry reset), as k_q_gs<2> does before its pass.
+// dot_op >= 0: <r, z> of the result over every cell into that slot.
+// Traffic per cell: mask, z, r in, z out (13 B in f32 storage), against
+// twice that for two single passes.
+#ifndef FSP_SWEEP_TY
+#define FSP_SWEEP_TY 8
+#endif
+constexpr int SW_TY = FSP_SWEEP_TY;  // tile rows
+constexpr int SW_MAXQ = 64;  // widest row, in quads
+constexpr int SW_SLOTS = 5;  // planes of z in LDS
+constexpr int SW_ROWS = SW_TY + 4;
+constexpr int SW_ZITEMS = (SW_ROWS * SW_MAXQ + QBLOCK - 1) / QBLOCK;     // z loads per thread and plane
+constexpr int SW_AITEMS = ((SW_TY + 2) * SW_MAXQ + QBLOCK - 1) / QBLOCK;  // pass items per thread and plane
+
+struct SweepGrid {
+  int nq;      // quads per row (nx / 4, at most SW_MAXQ)
+  int tiles;   // tiles along y (ceil(ny / SW_TY))
+  int kz;      // planes per chunk
+  int chunks;  // chunks along z (ceil(nz / kz))
+};
+
+// Workgroup barrier ordering LDS accesses only: global loads in flight
+// (the next planes) stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// LDS bytes of one sweep workgroup.
+inline size_t sweep_lds_bytes(int nx) { return (size_t)SW_SLOTS * SW_ROWS * (size_t)nx * sizeof(float); }
+
+template <int MODE, int ST>
+__global__ void __launch_bounds__(QBLOCK) k_q_sweep(Fine G, SweepGrid W, const typename Store<ST>::T* zin,
+                                                    typename Store<ST>::T* zout, const float* r, int colour, Level L0,
+                                                    const typename Store<ST>::T* xc, int dot_op, Ctl* ctl,
+                                                    double* partials) {
+#pragma clang fp contract(off)
+  using SV = Store<ST>;
+  static_assert(MODE == 0 || MODE == 2, "in place, or prolongation then the sweep");
+  extern __shared__ float4 sw_lds4[];
+  __shared__ float s_prec[8];  // prec of a plain row by its face count (0: no face, or dry)
+  float* lds = reinterpret_cast<float*>(sw_lds4);
+  if (!ctl->active) return;
+  if (threadIdx.x < 8) s_prec[threadIdx.x] = threadIdx.x < 7 ? G.prec_tab[threadIdx.x] : 0.f;
+  const int nx = G.nx, ny = G.ny, nz = G.nz, nq = W.nq;
+  const int tid = (int)threadIdx.x;
+  const int tile = (int)(blockIdx.x % (unsigned)W.tiles), chunk = (int)(blockIdx.x / (unsigned)W.tiles);
+  const int y0 = tile * SW_TY, k0 = chunk * W.kz, k1 = min(k0 + W.kz, nz);
+  const size_t plane = (size_t)nx * ny;
+  const int slot_floats = SW_ROWS * nx;
+  auto slot = [&](int p) { return lds + (size_t)((unsigned)(p + 2 * SW_SLOTS) % SW_SLOTS) * slot_floats; };
+  // this thread's load items (row of the plane window, quad) and pass items
+  // (row of the tile with its ring of one, quad)
+  int zrow[SW_ZITEMS], zq[SW_ZITEMS], arow[SW_AITEMS], aq[SW_AITEMS];
+#pragma unroll
+  for (int m = 0; m < SW_ZITEMS; ++m) {
+    const int idx = tid + QBLOCK * m;
+    zrow[m] = idx < SW_ROWS * nq ? idx / nq : -1;
+    zq[m] = idx < SW_ROWS * nq ? idx - zrow[m] * nq : 0;
+  }
+#pragma unroll
+  for (int m = 0; m < SW_AITEMS; ++m) {
+    const int idx = tid + QBLOCK * m;
+    const int row = idx < (SW_TY + 2) * nq ? idx / nq : -1;
+    const int y = y0 - 1 + row;
+    arow[m] = (row >= 0 && y >= 0 && y < ny) ? row : -1;
+    aq[m] = row >= 0 ? idx - row * nq : 0;
+  }
+  // z of plane p, window rows, into registers (prolonged in MODE 2)
+  auto load_z = [&](int p, float4 v[SW_ZITEMS]) {
+#pragma unroll
+    for (int m = 0; m < SW_ZITEMS; ++m) {
+      v[m] = f4zero();
+      const int y = y0 - 2 + zrow[m];
+      if (zrow[m] >= 0 && y >= 0 && y < ny && p >= 0 && p < nz) {
+        const size_t c = ((size_t)p * ny + y) * nx + 4 * (size_t)zq[m];
+        v[m] = SV::ld4(zin, c);
+        if (MODE == 2) {
+          const float2 a = SV::ld2(xc, lidx(L0, 2 * zq[m], y >> 1, p >> 1));
+          v[m].x = SV::rnd(v[m].x + a.x); v[m].y = SV::rnd(v[m].y + a.x);
+          v[m].z = SV::rnd(v[m].z + a.y); v[m].w = SV::rnd(v[m].w + a.y);
+        }
+      }
+    }
+  };
+  auto store_z = [&](int p, const float4 v[SW_ZITEMS]) {
+    float* s = slot(p);
+#pragma unroll
+    for (int m = 0; m < SW_ZITEMS; ++m)
+      if (zrow[m] >= 0) *reinterpret_cast<float4*>(s + zrow[m] * nx + 4 * zq[m]) = v[m];
+  };
+  // r and row codes of plane p for the pass items
+  auto load_rm = [&](int p, float4 rv[SW_AITEMS], uint32_t mv[SW_AITEMS]) {
+#pragma unroll
+    for (int m = 0; m < SW_AITEMS; ++m) {
+      rv[m] = f4zero();
+      mv[m] = 0;
+      if (arow[m] >= 0 && p >= 0 && p < nz) {
+        const size_t c = ((size_t)p * ny + (y0 - 1 + arow[m])) * nx + 4 * (size_t)aq[m];
+        rv[m] = ld4(r, c);
+        mv[m] = *reinterpret_cast<const uint32_t*>(G.mask + c);
+      }
+    }
+  };
+  // One colour on plane p for the quad at window row `row` (tile row + 1),
+  // neighbours from the window. The colour's two cells of the quad (x-cells
+  // par and par + 2) are computed without branches; plain rows only: the
+  // quad comes back with them updated, and `ghosts` gets a bit per cell of
+  // the colour on a free-surface row (about one row in a hundred), which
+  // relax_ghost updates afterwards. Per cell the arithmetic is row_local's:
+  // coupled faces summed in face order, prec of the row's face count.
+  auto relax = [&](int p, int row, int q, int col, const float4 rq, uint32_t mq, uint32_t& ghosts) -> float4 {
+    const int y = y0 - 1 + row;
+    const float* sc = slot(p) + (row + 1) * nx + 4 * q;
+    const float4 zc = *reinterpret_cast<const float4*>(sc);
+    const float4 ym = *reinterpret_cast<const float4*>(sc - nx);
+    const float4 yp = *reinterpret_cast<const float4*>(sc + nx);
+    const float4 zm = *reinterpret_cast<const float4*>(slot(p - 1) + (row + 1) * nx + 4 * q);
+    const float4 zp = *reinterpret_cast<const float4*>(slot(p + 1) + (row + 1) * nx + 4 * q);
+    const float left = q > 0 ? sc[-1] : 0.f, right = q + 1 < nq ? sc[4] : 0.f;
+    const bool odd = ((col + y + p) & 1) != 0;  // the colour's cells are x-cells 1 and 3, else 0 and 2
+    auto cell = [&](float pc, float xl, float xr, float vym, float vyp, float vzm, float vzp, float rv, uint32_t mc,
+                    bool& ghost) -> float {
+      ghost = (mc & M_GHOST) != 0;
+      const float nb[6] = {xl, xr, vym, vyp, vzm, vzp};
+      float acc = 0.f;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) acc = (mc & (1u << f)) ? acc + (pc - nb[f]) : acc;
+      const float prec = s_prec[(mc & M_WET) ? __popc(mc & 0x3Fu) : 0];
+      float v = prec > 0.f ? SV::rnd(pc + prec * (rv - acc * G.inv)) : (MODE == 2 ? 0.f : pc);
+      return ghost ? pc : v;
+    };
+    // selections of values (not of lvalues, which would put the operands in memory)
+    auto sel = [odd](float o, float e) -> float { return odd ? o : e; };
+    bool ga, gb;
+    const float a = cell(sel(zc.y, zc.x), sel(zc.x, left), sel(zc.z, zc.y), sel(ym.y, ym.x), sel(yp.y, yp.x),
+                         sel(zm.y, zm.x), sel(zp.y, zp.x), sel(rq.y, rq.x), (mq >> (odd ? 8 : 0)) & 0xFFu, ga);
+    const float b = cell(sel(zc.w, zc.z), sel(zc.z, zc.y), sel(right, zc.w), sel(ym.w, ym.z), sel(yp.w, yp.z),
+                         sel(zm.w, zm.z), sel(zp.w, zp.z), sel(rq.w, rq.z), (mq >> (odd ? 24 : 16)) & 0xFFu, gb);
+    ghosts = (ga ? (odd ? 2u : 1u) : 0u) | (gb ? (odd ? 8u : 4u) : 0u);
+    return make_float4(sel(zc.x, a), sel(a, zc.y), sel(zc.z, b), sel(b, zc.w));
+  };
+  // The free-surface cells `ghosts` of that quad (row_local's ghost form),
+  // r and the row codes read again (they are in cache); returns the new
+  // values in the quad's positions.
+  auto relax_ghost = [&](int p, int row, int q, uint32_t ghosts) -> float4 {
+    const int y = y0 - 1 + row;
+    const float* sc = slot(p) + (row + 1) * nx + 4 * q;
+    const size_t c0 = ((size_t)p * ny + y) * nx + 4 * (size_t)q;
+    const float4 rq = ld4(r, c0);
+    const uint32_t mq = *reinterpret_cast<const uint32_t*>(G.mask + c0);
+    float4 out = f4zero();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (!(ghosts & (1u << e))) continue;
+      const uint8_t mc = (uint8_t)(mq >> (8 * e));
+      const float pc = sc[e];
+      const float nb[6] = {q > 0 || e > 0 ? sc[e - 1] : 0.f, q + 1 < nq || e < 3 ? sc[e + 1] : 0.f, sc[e - nx],
+                           sc[e + nx], slot(p - 1)[(row + 1) * nx + 4 * q + e],
+                           slot(p + 1)[(row + 1) * nx + 4 * q + e]};
+      float prec;
+      const float az = row_local(G, 4 * q + e, y, p, mc, pc, nb, prec);
+      q_set(out, e, prec > 0.f ? SV::rnd(pc + prec * (q_get(rq, e) - az)) : (MODE == 2 ? 0.f : pc));
+    }
+    return out;
+  };
+  float4 zpf[SW_ZITEMS];
+  float4 r_prev[SW_AITEMS], r_cur[SW_AITEMS], r_next[SW_AITEMS];
+  uint32_t m_prev[SW_AITEMS], m_cur[SW_AITEMS], m_next[SW_AITEMS];
+  // the window's first planes and the r / row codes of the first pass
+  for (int p = k0 - 2; p <= k0; ++p) {
+    if (p < 0) continue;
+    load_z(p, zpf);
+    store_z(p, zpf);
+  }
+  load_rm(k0 - 1, r_cur, m_cur);
+#pragma unroll
+  for (int m = 0; m < SW_AITEMS; ++m) { r_prev[m] = f4zero(); m_prev[m] = 0; }
+  lds_barrier();
+  double s = 0.0, unused = 0.0;
+  const int c1 = colour, c2 = 1 - colour;
+  for (int sp = k0 - 1; sp <= k1; ++sp) {
+    // loads for the next steps first: z of plane sp + 2, r / row codes of sp + 1
+    const bool want_z = sp + 2 < nz && sp + 2 <= k1 + 1;
+    if (want_z) load_z(sp + 2, zpf);
+    load_rm(sp + 1 <= k1 ? sp + 1 : -1, r_next, m_next);
+    // pass c on plane sp (tile rows and the ring), back into the window
+    if (sp >= 0 && sp < nz) {
+      uint32_t pending = 0;  // 4 bits per item: free-surface cells left
+#pragma unroll
+      for (int m = 0; m < SW_AITEMS; ++m) {
+        if (arow[m] < 0) continue;
+        uint32_t gh;
+        const float4 o = relax(sp, arow[m], aq[m], c1, r_cur[m], m_cur[m], gh);
+        *reinterpret_cast<float4*>(slot(sp) + (arow[m] + 1) * nx + 4 * aq[m]) = o;
+        pending |= gh << (4 * m);
+      }
+#pragma unroll 1
+      for (int m = 0; m < SW_AITEMS; ++m) {
+        const uint32_t gh = (pending >> (4 * m)) & 15u;
+        if (!gh) continue;
+        const int idx = tid + QBLOCK * m, row = idx / nq, q = idx - row * nq;
+        const float4 o = relax_ghost(sp, row, q, gh);
+        float* dst = slot(sp) + (row + 1) * nx + 4 * q;
+        for (int e = 0; e < 4; ++e)
+          if (gh & (1u << e)) dst[e] = q_get(o, e);
+      }
+    }
+    lds_barrier();
+    // pass 1 - c on plane sp - 1 (tile rows), to zout
+    const int pb = sp - 1;
+    if (pb >= k0 && pb < k1) {
+      uint32_t pending = 0;
+#pragma unroll
+      for (int m = 0; m < SW_AITEMS; ++m) {
+        if (arow[m] < 1 || arow[m] > SW_TY) continue;
+        uint32_t gh;
+        const float4 o = relax(pb, arow[m], aq[m], c2, r_prev[m], m_prev[m], gh);
+        const size_t c = ((size_t)pb * ny + (y0 - 1 + arow[m])) * nx + 4 * (size_t)aq[m];
+        SV::st4(zout, c, o);
+        pending |= gh << (4 * m);
+        if (dot_op >= 0) {
+          const float4 rq = r_prev[m];
+          // free-surface cells are added with their new values below
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (!(gh & (1u << e))) s += (double)q_get(rq, e) * q_get(o, e);
+        }
+      }
+#pragma unroll 1
+      for (int m = 0; m < SW_AITEMS; ++m) {
+        const uint32_t gh = (pending >> (4 * m)) & 15u;
+        if (!gh) continue;
+        const int idx = tid + QBLOCK * m, row = idx / nq, q = idx - row * nq;
+        const float4 o = relax_ghost(pb, row, q, gh);
+        const size_t c = ((size_t)pb * ny + (y0 - 1 + row)) * nx + 4 * (size_t)q;
+        for (int e = 0; e < 4; ++e) {
+          if (!(gh & (1u << e))) continue;
+          SV::st(zout, c + e, q_get(o, e));
+          if (dot_op >= 0) s += (double)r[c + e] * q_get(o, e);
+        }
+      }
+    }
+    if (want_z) store_z(sp + 2, zpf);
+    lds_barrier();
+#pragma unroll
+    for (int m = 0; m < SW_AITEMS; ++m) {
+      r_prev[m] = r_cur[m]; m_prev[m] = m_cur[m];
+      r_cur[m] = r_next[m]; m_cur[m] = m_next[m];
+    }
+  }
+  if (dot_op >= 0) reduce_finish<false>(s, unused, dot_op, ctl, partials);
+  (void)plane;
 }
 
 // Row codes on quads (k_mask with four x-cells per thread: one decode, and

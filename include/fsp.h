@@ -54,6 +54,11 @@ enum { FSP_STORE_F32 = 0, FSP_STORE_F16 = 1, FSP_STORE_BF16 = 2 };
  * Galerkin product P^T A P / 16 with piecewise-constant P (equal to the
  * re-discretisation in the interior, exact at the surface and at solids). */
 enum { FSP_COARSE_REDISCRETISED = 0, FSP_COARSE_GALERKIN = 1 };
+/* Fine-level red-black smoothing: one launch per colour (the default), or
+ * both colours of a sweep in one launch over a window of planes in LDS where
+ * a grid row fits a workgroup (nx <= 256). The V-cycles are identical bit
+ * for bit. */
+enum { FSP_SWEEP_FUSED = 0, FSP_SWEEP_SINGLE = 1 };
 
 /* Solver settings. fsp_default_options gives the reference's multigrid
  * preconditioner (delta3d_multigrid.rs:28-34: damped Jacobi ω=6/7, 2 pre,
@@ -75,6 +80,7 @@ typedef struct {
   int32_t learned_vcycle;  /* FSP_PRECOND_LEARNED: 1 = the network's output is the initial guess of one
                               V-cycle (needs f32 storage and red-black smoothing) in place of the
                               learned_smooth sweeps; default 0 */
+  int32_t fine_sweeps;     /* FSP_SWEEP_*: how the fine level's red-black sweeps are launched */
 } fsp_options;
 
 /* Surface state (Volume3 fields rho, g_eff, rest, homogeneous_ghost;
