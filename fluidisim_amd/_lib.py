@@ -30,7 +30,8 @@ EXPORTS = [
     "fsp_extrapolate_device", "fsp_transport_device", "fsp_surface_scalars",
     # include/fsp_learned.h
     "fsp_net_param_count", "fsp_net_create", "fsp_net_destroy", "fsp_net_set_params", "fsp_net_get_params",
-    "fsp_net_reserve", "fsp_net_features_device", "fsp_net_forward_device", "fsp_net_loss_grad_device",
+    "fsp_net_reserve", "fsp_net_features_device", "fsp_net_forward_device", "fsp_net_forward_static_device",
+    "fsp_net_loss_grad_device",
     "fsp_net_train_step_device", "fsp_attach_net", "fsp_net_mfma_probe",
 ]
 
@@ -163,6 +164,8 @@ def lib():
     L.fsp_net_features_device.restype = i32
     L.fsp_net_forward_device.argtypes = [vp, i32, i32, i32, vp, vp]
     L.fsp_net_forward_device.restype = i32
+    L.fsp_net_forward_static_device.argtypes = [vp, i32, i32, i32, vp, vp]
+    L.fsp_net_forward_static_device.restype = i32
     L.fsp_net_loss_grad_device.argtypes = [vp, i32, i32, i32, vp, vp, vp, vp, vp]
     L.fsp_net_loss_grad_device.restype = i32
     L.fsp_net_train_step_device.argtypes = [vp, i32, i32, i32, vp, vp, vp, f32, vp, vp]

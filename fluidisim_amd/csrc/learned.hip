@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -668,26 +669,48 @@ int lvl_n(const fsp_net* N, int level, int axis) {
   return d >> level;
 }
 
-void forward(fsp_net* N) {
+// The buffers of one forward pass: input features, activations, pooled
+// levels and the output.
+struct Acts {
+  int nx, ny, nz;
+  const _Float16* x0;   // [cells][4]
+  _Float16* act[NL];    // layer outputs (the head writes y)
+  _Float16 *p1, *p2;    // pooled level-1 and level-2 inputs
+  float* y;             // [cells]
+};
+
+Acts net_acts(fsp_net* N) {
+  Acts A{};
+  A.nx = N->nx; A.ny = N->ny; A.nz = N->nz;
+  A.x0 = N->x0;
+  for (int l = 0; l < NL; ++l) A.act[l] = N->act[l];
+  A.p1 = N->p1; A.p2 = N->p2; A.y = N->y;
+  return A;
+}
+
+void forward(fsp_net* N, const Acts& A) {
   hipStream_t st = N->cur;
   const int32_t* gt = N->gate;
-  const int X0 = N->nx, Y0 = N->ny, Z0 = N->nz;
+  const int X0 = A.nx, Y0 = A.ny, Z0 = A.nz;
   const int X1 = X0 / 2, Y1 = Y0 / 2, Z1 = Z0 / 2, X2 = X0 / 4, Y2 = Y0 / 4, Z2 = Z0 / 4;
+  const size_t V1 = (size_t)X1 * Y1 * Z1, V2 = (size_t)X2 * Y2 * Z2;
   auto B = [&](int l) { return N->theta + N->poff[l] + (size_t)LAYERS[l].cout * 27 * LAYERS[l].cin; };
-  conv<4, 8, 0, 0>(st, X0, Y0, Z0, N->x0, N->wt[0], B(0), nullptr, N->act[0], nullptr, gt);
-  conv<8, 8, 0, 0>(st, X0, Y0, Z0, N->act[0], N->wt[1], B(1), nullptr, N->act[1], nullptr, gt);
-  k_pool<<<blocks_for(N->V[1] * 8), 256, 0, st>>>(N->act[1], N->p1, X1, Y1, Z1, 8, gt);
-  conv<8, 16, 0, 0>(st, X1, Y1, Z1, N->p1, N->wt[2], B(2), nullptr, N->act[2], nullptr, gt);
-  conv<16, 16, 0, 0>(st, X1, Y1, Z1, N->act[2], N->wt[3], B(3), nullptr, N->act[3], nullptr, gt);
-  k_pool<<<blocks_for(N->V[2] * 16), 256, 0, st>>>(N->act[3], N->p2, X2, Y2, Z2, 16, gt);
-  conv<16, 32, 0, 0>(st, X2, Y2, Z2, N->p2, N->wt[4], B(4), nullptr, N->act[4], nullptr, gt);
-  conv<32, 32, 0, 0>(st, X2, Y2, Z2, N->act[4], N->wt[5], B(5), nullptr, N->act[5], nullptr, gt);
-  conv<32, 16, 1, 3>(st, X1, Y1, Z1, N->act[5], N->wt[6], B(6), N->act[3], N->act[6], nullptr, gt);
-  conv<16, 16, 0, 0>(st, X1, Y1, Z1, N->act[6], N->wt[7], B(7), nullptr, N->act[7], nullptr, gt);
-  conv<16, 8, 1, 3>(st, X0, Y0, Z0, N->act[7], N->wt[8], B(8), N->act[1], N->act[8], nullptr, gt);
-  conv<8, 8, 0, 0>(st, X0, Y0, Z0, N->act[8], N->wt[9], B(9), nullptr, N->act[9], nullptr, gt);
-  conv<8, 1, 0, 1>(st, X0, Y0, Z0, N->act[9], N->wt[10], B(10), nullptr, nullptr, N->y, gt);
+  conv<4, 8, 0, 0>(st, X0, Y0, Z0, A.x0, N->wt[0], B(0), nullptr, A.act[0], nullptr, gt);
+  conv<8, 8, 0, 0>(st, X0, Y0, Z0, A.act[0], N->wt[1], B(1), nullptr, A.act[1], nullptr, gt);
+  k_pool<<<blocks_for(V1 * 8), 256, 0, st>>>(A.act[1], A.p1, X1, Y1, Z1, 8, gt);
+  conv<8, 16, 0, 0>(st, X1, Y1, Z1, A.p1, N->wt[2], B(2), nullptr, A.act[2], nullptr, gt);
+  conv<16, 16, 0, 0>(st, X1, Y1, Z1, A.act[2], N->wt[3], B(3), nullptr, A.act[3], nullptr, gt);
+  k_pool<<<blocks_for(V2 * 16), 256, 0, st>>>(A.act[3], A.p2, X2, Y2, Z2, 16, gt);
+  conv<16, 32, 0, 0>(st, X2, Y2, Z2, A.p2, N->wt[4], B(4), nullptr, A.act[4], nullptr, gt);
+  conv<32, 32, 0, 0>(st, X2, Y2, Z2, A.act[4], N->wt[5], B(5), nullptr, A.act[5], nullptr, gt);
+  conv<32, 16, 1, 3>(st, X1, Y1, Z1, A.act[5], N->wt[6], B(6), A.act[3], A.act[6], nullptr, gt);
+  conv<16, 16, 0, 0>(st, X1, Y1, Z1, A.act[6], N->wt[7], B(7), nullptr, A.act[7], nullptr, gt);
+  conv<16, 8, 1, 3>(st, X0, Y0, Z0, A.act[7], N->wt[8], B(8), A.act[1], A.act[8], nullptr, gt);
+  conv<8, 8, 0, 0>(st, X0, Y0, Z0, A.act[8], N->wt[9], B(9), nullptr, A.act[9], nullptr, gt);
+  conv<8, 1, 0, 1>(st, X0, Y0, Z0, A.act[9], N->wt[10], B(10), nullptr, nullptr, A.y, gt);
 }
+
+void forward(fsp_net* N) { forward(N, net_acts(N)); }
 
 void relu_back(fsp_net* N, const _Float16* h, _Float16* g, size_t n) {
   k_relu_back<<<blocks_for(n), 256, 0, N->cur>>>(h, g, n);
@@ -839,6 +862,55 @@ fsp_status loss_and_grad(fsp_net* N, const float* target, const uint8_t* wet, do
   return FSP_OK;
 }
 
+// ---------------------------------------------------------------- forward pass that reserves nothing
+// Activations of fsp_net_forward_static_device: a file-scope device array
+// of STATIC_BYTES, one copy per GPU, held until the process ends and shared
+// by every such call on that GPU (calls that overlap on one GPU race). The
+// largest grid it holds is STATIC_MAX_CELLS cells (84.5 bytes of
+// activations per cell).
+constexpr size_t STATIC_BYTES = (size_t)24 << 20;
+constexpr size_t STATIC_MAX_CELLS = 262144;
+constexpr int MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[STATIC_BYTES];
+
+// The device address of `scratch` on each GPU (host code never passes the
+// array's name to a kernel: on the host it is a stand-in, not a device
+// address), taken once per device with that device current.
+std::mutex scratch_mu;
+void* scratch_ptr[MAX_DEVICES] = {};
+
+fsp_status scratch_address(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return FSP_DEVICE;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_ptr[device]) return FSP_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    std::fprintf(stderr, "fsp: hipGetSymbolAddress(scratch) on device %d failed: %s\n", device, hipGetErrorString(e));
+    return FSP_DEVICE;
+  }
+  scratch_ptr[device] = p;
+  return FSP_OK;
+}
+
+// Activation buffers of an nx x ny x nz forward pass carved from `base`
+// (256-byte aligned, in layer order); returns the bytes used.
+size_t carve_acts(unsigned char* base, int nx, int ny, int nz, Acts* A) {
+  const size_t V[3] = {(size_t)nx * ny * nz, (size_t)nx * ny * nz / 8, (size_t)nx * ny * nz / 64};
+  size_t off = 0;
+  auto take = [&](size_t halves) {
+    _Float16* q = base ? reinterpret_cast<_Float16*>(base + off) : nullptr;
+    off += (halves * sizeof(_Float16) + 255) & ~(size_t)255;
+    return q;
+  };
+  A->nx = nx; A->ny = ny; A->nz = nz;
+  for (int l = 0; l < NL; ++l)
+    A->act[l] = l < NL - 1 ? take(V[LAYERS[l].level] * LAYERS[l].cout) : nullptr;
+  A->p1 = take(V[1] * 8);
+  A->p2 = take(V[2] * 16);
+  return off;
+}
+
 }  // namespace
 
 extern "C" {
@@ -855,6 +927,9 @@ fsp_status fsp_net_create(int32_t device, fsp_net** out) {
   N->device = device;
   auto fail = [&](fsp_status s) { fsp_net_destroy(N); return s; };
   if (hipSetDevice(device) != hipSuccess) return fail(FSP_DEVICE);
+  // the static activation array's address on this device, for the forward
+  // pass that reserves nothing
+  if (const fsp_status ss = scratch_address(device)) return fail(ss);
   if (hipStreamCreateWithFlags(&N->stream, hipStreamNonBlocking) != hipSuccess) return fail(FSP_DEVICE);
   N->cur = N->stream;
   N->P = param_count();
@@ -931,6 +1006,35 @@ fsp_status fsp_net_forward_device(fsp_net* N, int32_t nx, int32_t ny, int32_t nz
   NET_HIP(hipMemcpyAsync(N->x0, d_x, N->V[0] * 4 * sizeof(uint16_t), hipMemcpyDeviceToDevice, N->stream));
   forward(N);
   NET_HIP(hipMemcpyAsync(d_y, N->y, N->V[0] * sizeof(float), hipMemcpyDeviceToDevice, N->stream));
+  NET_HIP(hipStreamSynchronize(N->stream));
+  return FSP_OK;
+}
+
+fsp_status fsp_net_forward_static_device(fsp_net* N, int32_t nx, int32_t ny, int32_t nz, const uint16_t* d_x,
+                                         float* d_y) {
+  if (!N || !d_x || !d_y) return FSP_INVALID;
+  if (!fsp::net_shape_ok(nx, ny, nz)) return FSP_SHAPE;
+  Acts A{};
+  const size_t cells = (size_t)nx * ny * nz;
+  if (cells > STATIC_MAX_CELLS || carve_acts(nullptr, nx, ny, nz, &A) > STATIC_BYTES) {
+    std::fprintf(stderr, "fsp: fsp_net_forward_static_device: %d x %d x %d = %zu cells exceed the static "
+                 "activation array (%zu cells at most)\n", nx, ny, nz, cells, STATIC_MAX_CELLS);
+    return FSP_SHAPE;
+  }
+  NET_HIP(hipSetDevice(N->device));
+  unsigned char* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    base = static_cast<unsigned char*>(scratch_ptr[N->device]);
+  }
+  if (!base) return FSP_DEVICE;
+  carve_acts(base, nx, ny, nz, &A);
+  A.x0 = reinterpret_cast<const _Float16*>(d_x);
+  A.y = d_y;
+  N->cur = N->stream;
+  N->gate = nullptr;
+  forward(N, A);
+  NET_HIP(hipGetLastError());
   NET_HIP(hipStreamSynchronize(N->stream));
   return FSP_OK;
 }

@@ -94,6 +94,12 @@ class Network:
         _raise(_lib.lib().fsp_net_forward_device(self._h, nx, ny, nz, _ptr(x), _ptr(y)), "fsp_net_forward_device")
         return y
 
+    def forward_static(self, x, shape, y):
+        """y = network(x) through fsp_net_forward_static_device (no device
+        memory reserved; y is the caller's). Returns the status."""
+        nx, ny, nz = shape
+        return _lib.lib().fsp_net_forward_static_device(self._h, nx, ny, nz, _ptr(x), _ptr(y))
+
     def loss_grad(self, x, target, wet, shape):
         torch = _torch()
         nx, ny, nz = shape

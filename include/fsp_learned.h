@@ -55,6 +55,17 @@ fsp_status fsp_net_features_device(fsp_net* net, fsp_solver* solver, const float
 /* y = network(x), x binary16 [cells][4], y f32 [cells]. */
 fsp_status fsp_net_forward_device(fsp_net* net, int32_t nx, int32_t ny, int32_t nz, const uint16_t* d_x, float* d_y);
 
+/* No counterpart in the reference: fsp_net_forward_device for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops
+ * over patches of changing shapes). It reserves nothing: the activations live
+ * in a static device array of 24 MiB per GPU, which the library holds until
+ * the process ends and every call on that GPU shares (calls that overlap on
+ * one GPU race; each call completes before it returns). Grids of at most
+ * 262 144 cells (e.g. 64 x 64 x 64); larger ones return FSP_SHAPE. y is
+ * fsp_net_forward_device's y bit for bit. */
+fsp_status fsp_net_forward_static_device(fsp_net* net, int32_t nx, int32_t ny, int32_t nz, const uint16_t* d_x,
+                                         float* d_y);
+
 /* Loss = mean over wet cells of (network(x) - target)^2 and its gradient
  * w.r.t. the parameters (host f32, fsp_net_param_count values). */
 fsp_status fsp_net_loss_grad_device(fsp_net* net, int32_t nx, int32_t ny, int32_t nz, const uint16_t* d_x,
