@@ -30,8 +30,8 @@ HBM_PEAK_GBS = 8000.0
 # coarse value is 1/8 of a fine one).
 def bytes_per_cell(kernel, zb):
     table = {
-        "k_q_sweep<0>": 1 + zb + 4 + zb,           # red-black sweep, both colours: mask, z, r -> z'
-        "k_q_sweep<2>": 1 + zb + zb / 8 + 4 + zb,  # prolongation + sweep: mask, z, coarse x, r -> z'
+        "k_q_brick<0>": 1 + zb + 4 + zb,           # red-black sweep, both colours: mask, z, r -> z'
+        "k_q_brick<2>": 1 + zb + zb / 8 + 4 + zb,  # prolongation + sweep: mask, z, coarse x, r -> z'
         "k_q_gs<0>": 1 + zb + 4 + zb,              # red-black pass: mask, z, r -> z (whole quads)
         "k_q_gs_first": 1 + 4 + 4 + zb,            # first two passes from zero: mask, r, row_prec -> z
         "k_q_gs<2>": 1 + zb + zb / 8 + 4 + zb,     # prolongation + pass: mask, z, coarse x, r -> z'
@@ -51,14 +51,14 @@ def parse():
     ap.add_argument("--size", default="256x128x256", help="X x Y x Z, Y vertical")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--kernel", default=None,
-                    help="kernel timed live for the roofline (default: the fine red-black pass, or the fused sweep)")
+                    help="kernel timed live for the roofline (default: the fused fine sweep, or the single pass)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-runs", type=int, default=3,
                     help="complete oracle projections timed for the CPU baseline (median)")
     ap.add_argument("--coarse-sweeps", type=int, default=2,
                     help="red-black sweeps on the coarsest level (2 keeps the 9 iterations of 8 on this tank)")
-    ap.add_argument("--fused-sweeps", action="store_true",
-                    help="both red-black colours of a fine sweep in one launch (else one launch per colour)")
+    ap.add_argument("--single-sweeps", action="store_true",
+                    help="one launch per red-black colour on the fine level (else both colours in one launch)")
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
                     help="V-cycle vector storage (arithmetic stays f32)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "fluidisim_amd", "learned_weights.npz"),
@@ -281,7 +281,7 @@ def surface_step_line(args, torch, W):
 def main():
     args = parse()
     if args.kernel is None:
-        args.kernel = "k_q_sweep<0>" if args.fused_sweeps else "k_q_gs<0>"
+        args.kernel = "k_q_gs<0>" if args.single_sweeps else "k_q_brick<0>"
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
               f"described, not built.", flush=True)
@@ -299,7 +299,7 @@ def main():
     wl.options["coarse_sweeps"] = args.coarse_sweeps
     log(f"[bench] workload {wl.name}: {wl.domain.cells} cells, generated in {time.perf_counter() - t0:.1f} s")
     storage = {"f32": _lib.STORE_F32, "f16": _lib.STORE_F16, "bf16": _lib.STORE_BF16}[args.storage]
-    sweeps = _lib.SWEEP_FUSED if args.fused_sweeps else _lib.SWEEP_SINGLE
+    sweeps = _lib.SWEEP_SINGLE if args.single_sweeps else _lib.SWEEP_FUSED
     vol = W.build_volume(wl, vcycle_storage=storage, fine_sweeps=sweeps)
     log(f"[bench] solver ready, {vol.multigrid_levels()} coarse levels")
 

@@ -59,7 +59,7 @@ struct fsp_solver {
   // tail for the coarse levels from tail_first on
   bool quad = false;
   QuadGrid qfine{}, qrestr{};
-  SweepGrid sweep{};  // fused red-black sweeps (nq == 0: rows too wide, single passes)
+  BrickGrid brick{};  // fused red-black sweeps on bricks (bx == 0: off)
   std::vector<QuadGrid> cq, cqr;  // compact levels: quads of the level, and of its restriction
   int tail_first = 0;
   size_t tail_lds = 0;
@@ -253,22 +253,6 @@ bool gc_tail_ok(const fsp_options& o) {
 // one post-sweep).
 bool quad_vcycle_ok(bool quad, const fsp_options& o) {
   return quad && o.smoother == FSP_SMOOTH_RBGS && o.pre_sweeps >= 1 && o.post_sweeps >= 1;
-}
-
-// Tile grid of the fused sweeps: SW_TY rows per tile, chunks of kz planes.
-// kz = 0: planes per chunk so that the grid has about 512 workgroups (two
-// per CU), at least 4 (a chunk reads two planes more than it writes).
-SweepGrid make_sweep(int nq, int ny, int nz, int kz) {
-  SweepGrid W{};
-  W.nq = nq;
-  W.tiles = (ny + SW_TY - 1) / SW_TY;
-#ifdef FSP_SWEEP_KZ
-  if (kz <= 0) kz = FSP_SWEEP_KZ;
-#endif
-  if (kz <= 0) kz = std::max(4, std::min(nz, (int)(((size_t)W.tiles * (size_t)nz) / 512)));
-  W.kz = std::min(kz, nz);
-  W.chunks = (nz + W.kz - 1) / W.kz;
-  return W;
 }
 
 struct Launch {
@@ -599,7 +583,7 @@ struct Launch {
     TS<ST>* z = reinterpret_cast<TS<ST>*>(S->z);
     TS<ST>* zt = reinterpret_cast<TS<ST>*>(S->zt);
     const TS<ST>* none = nullptr;
-    if (S->sweep.nq > 0) return vcycle_fused<ST>(dot_op, from_guess);
+    if (S->brick.bx > 0) return vcycle_fused<ST>(dot_op, from_guess);
     auto pass = [&](TS<ST>* x, int colour, int op) {
       FSP_LAUNCH_N(S, "k_q_gs<0>", K_QGS(0), g, st, G, Q, (const TS<ST>*)x, x, S->r, colour, L0, none, op, S->ctl,
                    S->partials);
@@ -639,7 +623,7 @@ struct Launch {
   }
 
   // The same V-cycle with each red-black sweep (both colours) in one
-  // launch (k_q_sweep): pre-smoothing red-black from the first pair of
+  // launch (k_q_brick): pre-smoothing red-black from the first pair of
   // passes on, post-smoothing black-red with the prolongation fused into
   // its first sweep. A sweep reads one buffer and writes the other, so the
   // iterate alternates between z and zt. Bit-identical to vcycle_quad's
@@ -647,22 +631,20 @@ struct Launch {
   template <int ST>
   TS<ST>* vcycle_fused(int dot_op, bool from_guess) {
     const QuadGrid& Q = S->qfine;
-    const SweepGrid& W = S->sweep;
     const dim3 g((unsigned)(Q.nbx * Q.nby * Q.planes));
-    const dim3 gs((unsigned)(W.tiles * W.chunks));
-    const size_t lds = sweep_lds_bytes(G.nx);
+    const dim3 gb((unsigned)(S->brick.bx * S->brick.by * S->brick.bz));
     Level L0 = S->levels.empty() ? Level{} : S->levels[0];
     TS<ST>* a = reinterpret_cast<TS<ST>*>(S->z);
     TS<ST>* b = reinterpret_cast<TS<ST>*>(S->zt);
     const TS<ST>* none = nullptr;
     auto sweep = [&](int colour, const TS<ST>* xc, int op) {
       if (xc)
-        FSP_LAUNCH_NS(S, "k_q_sweep<2>", (k_q_sweep<2, ST>), gs, lds, st, G, W, (const TS<ST>*)a, b, S->r, colour,
-                      L0, xc, op, S->ctl, S->partials);
+        FSP_LAUNCH_NBS(S, "k_q_brick<2>", (k_q_brick<2, ST>), gb, BR_THREADS, 0, st, G, S->brick, (const TS<ST>*)a, b,
+                       S->r, colour, L0, xc, op, S->ctl, S->partials);
       else
-        FSP_LAUNCH_NS(S, "k_q_sweep<0>", (k_q_sweep<0, ST>), gs, lds, st, G, W, (const TS<ST>*)a, b, S->r, colour,
-                      L0, none, op, S->ctl, S->partials);
-      if (op >= 0) FSP_FINISH(S, st, op, gs, false, 1);
+        FSP_LAUNCH_NBS(S, "k_q_brick<0>", (k_q_brick<0, ST>), gb, BR_THREADS, 0, st, G, S->brick, (const TS<ST>*)a, b,
+                       S->r, colour, L0, none, op, S->ctl, S->partials);
+      if (op >= 0) FSP_FINISH(S, st, op, gb, false, 1);
       std::swap(a, b);
     };
     if (from_guess) sweep(0, none, -1);
@@ -1238,7 +1220,7 @@ void fsp_default_options(fsp_options* o) {
   o->learned_budget = 0;
   o->learned_smooth = 1;
   o->learned_vcycle = 0;
-  o->fine_sweeps = FSP_SWEEP_SINGLE;
+  o->fine_sweeps = FSP_SWEEP_FUSED;
 }
 
 fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** out) {
@@ -1334,8 +1316,12 @@ fsp_status fsp_create(const fsp_domain* d, const fsp_options* o, fsp_solver** ou
     S->qrestr = make_q((int)nx / 4, (int)ny / 2, (int)nz / 2);
     const size_t blocks = (size_t)S->qfine.nbx * S->qfine.nby * S->qfine.planes;
     S->quad = blocks <= (size_t)MAX_PARTIALS && (S->n / 4 + QBLOCK - 1) / QBLOCK <= (size_t)MAX_PARTIALS;
-    if (S->quad && nx / 4 <= SW_MAXQ && opt.fine_sweeps == FSP_SWEEP_FUSED)
-      S->sweep = make_sweep((int)nx / 4, (int)ny, (int)nz, 0);
+    if (S->quad && opt.fine_sweeps == FSP_SWEEP_FUSED) {
+      S->brick.bx = ((int)nx / 4 + BR_QX - 1) / BR_QX;
+      S->brick.by = ((int)ny + BR_Y - 1) / BR_Y;
+      S->brick.bz = ((int)nz + BR_Z - 1) / BR_Z;
+      if ((size_t)S->brick.bx * S->brick.by * S->brick.bz > (size_t)MAX_PARTIALS) S->brick = BrickGrid{};
+    }
   }
   if (opt.preconditioner == FSP_PRECOND_LEARNED && opt.learned_vcycle && !quad_vcycle_ok(S->quad, opt))
     return fail(FSP_INVALID);

@@ -272,279 +272,242 @@ __global__ void __launch_bounds__(QBLOCK, FSP_QGS_MIN_BLOCKS) k_q_gs_first(Fine 
   SV::st4(zout, c0, out);
 }
 
-// ---------------------------------------------------------------- fused red-black sweep
-// Both colour passes of one red-black sweep (colour c, then 1 - c) in one
-// launch. A workgroup owns a tile of SW_TY whole rows (the full x extent,
-// nx <= 4 * SW_MAXQ) over a chunk of planes and marches up the chunk with a
-// window of SW_SLOTS planes of z in LDS, each plane held with two extra rows
-// on either side of the tile:
-//   step s: pass c on plane s, over the tile rows and one row on either side
-//           (its neighbours are colour 1 - c values, untouched by this pass);
-//           barrier; pass 1 - c on plane s - 1 over the tile rows (its
-//           neighbours are the colour-c values of planes s - 2, s - 1, s, all
-//           updated by now), stored to zout; the next planes' loads, issued
-//           at the start of the step, land in the slot no pass reads.
-// A chunk starts one plane early and ends one plane late (the first pass of
-// the planes next to it), so chunks are independent; zout must not alias zin
-// (neighbouring tiles read zin's halo rows while this one writes).
-// Every cell's update uses the operands the single-colour passes use, in the
-// same order, so the sweep equals k_q_gs twice bit for bit.
+// ---------------------------------------------------------------- fused red-black sweep on bricks
+// Both colour passes of a red-black sweep in one launch, one brick per
+// workgroup: z of the brick (BR_QX quads x BR_Y rows x BR_Z planes) and a
+// halo of one quad in x and two cells in y and z is loaded into LDS at once;
+// pass c runs over the brick and a ring of one cell (its neighbours are
+// colour 1 - c values, untouched by it) back into LDS; after a barrier pass
+// 1 - c runs over the brick (its neighbours are the colour-c values, all
+// updated) and is stored to zout. Bricks are independent (zout must not
+// alias zin), so many of them per CU overlap their loads with each other's
+// arithmetic. Every cell's update uses the operands the single-colour passes
+// use, in the same order: the sweep equals k_q_gs twice bit for bit.
 // MODE 2: every value read from zin is first prolonged from the first coarse
 // level (z + parent, rounded to storage), rows with prec == 0 become 0
 // (prolong_add3 then the dry reset), as k_q_gs<2> does before its pass.
 // dot_op >= 0: <r, z> of the result over every cell into that slot.
 // Traffic per cell: mask, z, r in, z out (13 B in f32 storage), against
 // twice that for two single passes.
-#ifndef FSP_SWEEP_TY
-#define FSP_SWEEP_TY 8
+#ifndef FSP_BRICK_XCD
+#define FSP_BRICK_XCD 1
 #endif
-constexpr int SW_TY = FSP_SWEEP_TY;  // tile rows
-constexpr int SW_MAXQ = 64;  // widest row, in quads
-constexpr int SW_SLOTS = 5;  // planes of z in LDS
-constexpr int SW_ROWS = SW_TY + 4;
-constexpr int SW_ZITEMS = (SW_ROWS * SW_MAXQ + QBLOCK - 1) / QBLOCK;     // z loads per thread and plane
-constexpr int SW_AITEMS = ((SW_TY + 2) * SW_MAXQ + QBLOCK - 1) / QBLOCK;  // pass items per thread and plane
+#ifndef FSP_BRICK_QX
+#define FSP_BRICK_QX 8
+#endif
+#ifndef FSP_BRICK_Y
+#define FSP_BRICK_Y 8
+#endif
+#ifndef FSP_BRICK_Z
+#define FSP_BRICK_Z 8
+#endif
+constexpr int BR_QX = FSP_BRICK_QX, BR_Y = FSP_BRICK_Y, BR_Z = FSP_BRICK_Z;
+constexpr int BR_THREADS = 256;
+constexpr int BR_LQ = BR_QX + 2;                   // quads per LDS row
+constexpr int BR_LX = 4 * BR_LQ;                   // floats per LDS row
+constexpr int BR_LY = BR_Y + 4, BR_LZ = BR_Z + 4;  // LDS rows, planes
+constexpr int BR_PAD = 4;                          // floats before and after (halo cells' unused reads)
+constexpr int BR_FLOATS = BR_LX * BR_LY * BR_LZ + 2 * BR_PAD;
+constexpr int BR_NLOAD = BR_LQ * BR_LY * BR_LZ;
+constexpr int BR_LOADS = (BR_NLOAD + BR_THREADS - 1) / BR_THREADS;
+constexpr int BR_NA = BR_LQ * (BR_Y + 2) * (BR_Z + 2);  // first-pass quads
+constexpr int BR_AITEMS = (BR_NA + BR_THREADS - 1) / BR_THREADS;
+constexpr int BR_NB = BR_QX * BR_Y * BR_Z;  // second-pass quads
+constexpr int BR_BITEMS = (BR_NB + BR_THREADS - 1) / BR_THREADS;
 
-struct SweepGrid {
-  int nq;      // quads per row (nx / 4, at most SW_MAXQ)
-  int tiles;   // tiles along y (ceil(ny / SW_TY))
-  int kz;      // planes per chunk
-  int chunks;  // chunks along z (ceil(nz / kz))
+struct BrickGrid {
+  int bx, by, bz;  // bricks along x, y, z
 };
 
-// Workgroup barrier ordering LDS accesses only: global loads in flight
-// (the next planes) stay in flight across it.
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-// LDS bytes of one sweep workgroup.
-inline size_t sweep_lds_bytes(int nx) { return (size_t)SW_SLOTS * SW_ROWS * (size_t)nx * sizeof(float); }
-
 template <int MODE, int ST>
-__global__ void __launch_bounds__(QBLOCK) k_q_sweep(Fine G, SweepGrid W, const typename Store<ST>::T* zin,
-                                                    typename Store<ST>::T* zout, const float* r, int colour, Level L0,
-                                                    const typename Store<ST>::T* xc, int dot_op, Ctl* ctl,
-                                                    double* partials) {
+__global__ void __launch_bounds__(BR_THREADS) k_q_brick(Fine G, BrickGrid B, const typename Store<ST>::T* zin,
+                                                        typename Store<ST>::T* zout, const float* r, int colour,
+                                                        Level L0, const typename Store<ST>::T* xc, int dot_op,
+                                                        Ctl* ctl, double* partials) {
 #pragma clang fp contract(off)
   using SV = Store<ST>;
   static_assert(MODE == 0 || MODE == 2, "in place, or prolongation then the sweep");
-  extern __shared__ float4 sw_lds4[];
-  __shared__ float s_prec[8];  // prec of a plain row by its face count (0: no face, or dry)
-  float* lds = reinterpret_cast<float*>(sw_lds4);
+  __shared__ __attribute__((aligned(16))) float zs_all[BR_FLOATS];
+  __shared__ float s_prec[8];
+  __shared__ double s_red[BR_THREADS / 64];
+  float* zs = zs_all + BR_PAD;
   if (!ctl->active) return;
-  if (threadIdx.x < 8) s_prec[threadIdx.x] = threadIdx.x < 7 ? G.prec_tab[threadIdx.x] : 0.f;
-  const int nx = G.nx, ny = G.ny, nz = G.nz, nq = W.nq;
+  const int nx = G.nx, ny = G.ny, nz = G.nz;
   const int tid = (int)threadIdx.x;
-  const int tile = (int)(blockIdx.x % (unsigned)W.tiles), chunk = (int)(blockIdx.x / (unsigned)W.tiles);
-  const int y0 = tile * SW_TY, k0 = chunk * W.kz, k1 = min(k0 + W.kz, nz);
-  const size_t plane = (size_t)nx * ny;
-  const int slot_floats = SW_ROWS * nx;
-  auto slot = [&](int p) { return lds + (size_t)((unsigned)(p + 2 * SW_SLOTS) % SW_SLOTS) * slot_floats; };
-  // this thread's load items (row of the plane window, quad) and pass items
-  // (row of the tile with its ring of one, quad)
-  int zrow[SW_ZITEMS], zq[SW_ZITEMS], arow[SW_AITEMS], aq[SW_AITEMS];
+  // Bricks in x, then z, then y order, each XCD taking a contiguous range of
+  // them (dispatch deals workgroups to the 8 XCDs in turn): bricks that share
+  // halo planes and rows run on one XCD close in time, so their halos come
+  // from that XCD's L2.
+  const unsigned G8 = gridDim.x / 8u;
+  const unsigned bl = (FSP_BRICK_XCD && gridDim.x % 8u == 0u) ? (blockIdx.x % 8u) * G8 + blockIdx.x / 8u : blockIdx.x;
+  const int bxi = (int)(bl % (unsigned)B.bx), t = (int)(bl / (unsigned)B.bx);
+  const int bzi = t % B.bz, byi = t / B.bz;
+  const int qx0 = bxi * BR_QX - 1, y0 = byi * BR_Y - 2, z0 = bzi * BR_Z - 2;  // LDS origin (quad, row, plane)
+  const int nq = nx / 4;
+  if (tid < 8) s_prec[tid] = tid < 7 ? G.prec_tab[tid] : 0.f;
+  // every load of the workgroup in flight first: z of the brick and its halo
+  // (prolonged in MODE 2), then r and the row codes of both passes' quads
+  float4 rqa[BR_AITEMS], rqb[BR_BITEMS];
+  uint32_t mqa[BR_AITEMS], mqb[BR_BITEMS];
+  bool ona[BR_AITEMS], onb[BR_BITEMS];
+  {
+    float4 v[BR_LOADS];
 #pragma unroll
-  for (int m = 0; m < SW_ZITEMS; ++m) {
-    const int idx = tid + QBLOCK * m;
-    zrow[m] = idx < SW_ROWS * nq ? idx / nq : -1;
-    zq[m] = idx < SW_ROWS * nq ? idx - zrow[m] * nq : 0;
-  }
-#pragma unroll
-  for (int m = 0; m < SW_AITEMS; ++m) {
-    const int idx = tid + QBLOCK * m;
-    const int row = idx < (SW_TY + 2) * nq ? idx / nq : -1;
-    const int y = y0 - 1 + row;
-    arow[m] = (row >= 0 && y >= 0 && y < ny) ? row : -1;
-    aq[m] = row >= 0 ? idx - row * nq : 0;
-  }
-  // z of plane p, window rows, into registers (prolonged in MODE 2)
-  auto load_z = [&](int p, float4 v[SW_ZITEMS]) {
-#pragma unroll
-    for (int m = 0; m < SW_ZITEMS; ++m) {
+    for (int m = 0; m < BR_LOADS; ++m) {
+      const int idx = tid + BR_THREADS * m;
+      const int lq = idx % BR_LQ, rest = idx / BR_LQ, ly = rest % BR_LY, lz = rest / BR_LY;
+      const int q = qx0 + lq, y = y0 + ly, k = z0 + lz;
       v[m] = f4zero();
-      const int y = y0 - 2 + zrow[m];
-      if (zrow[m] >= 0 && y >= 0 && y < ny && p >= 0 && p < nz) {
-        const size_t c = ((size_t)p * ny + y) * nx + 4 * (size_t)zq[m];
+      if (idx < BR_NLOAD && q >= 0 && q < nq && y >= 0 && y < ny && k >= 0 && k < nz) {
+        const size_t c = ((size_t)k * ny + y) * nx + 4 * (size_t)q;
         v[m] = SV::ld4(zin, c);
         if (MODE == 2) {
-          const float2 a = SV::ld2(xc, lidx(L0, 2 * zq[m], y >> 1, p >> 1));
+          const float2 a = SV::ld2(xc, lidx(L0, 2 * q, y >> 1, k >> 1));
           v[m].x = SV::rnd(v[m].x + a.x); v[m].y = SV::rnd(v[m].y + a.x);
           v[m].z = SV::rnd(v[m].z + a.y); v[m].w = SV::rnd(v[m].w + a.y);
         }
       }
     }
-  };
-  auto store_z = [&](int p, const float4 v[SW_ZITEMS]) {
-    float* s = slot(p);
 #pragma unroll
-    for (int m = 0; m < SW_ZITEMS; ++m)
-      if (zrow[m] >= 0) *reinterpret_cast<float4*>(s + zrow[m] * nx + 4 * zq[m]) = v[m];
-  };
-  // r and row codes of plane p for the pass items
-  auto load_rm = [&](int p, float4 rv[SW_AITEMS], uint32_t mv[SW_AITEMS]) {
-#pragma unroll
-    for (int m = 0; m < SW_AITEMS; ++m) {
-      rv[m] = f4zero();
-      mv[m] = 0;
-      if (arow[m] >= 0 && p >= 0 && p < nz) {
-        const size_t c = ((size_t)p * ny + (y0 - 1 + arow[m])) * nx + 4 * (size_t)aq[m];
-        rv[m] = ld4(r, c);
-        mv[m] = *reinterpret_cast<const uint32_t*>(G.mask + c);
+    for (int m = 0; m < BR_AITEMS; ++m) {
+      const int idx = tid + BR_THREADS * m;
+      const int lq = idx % BR_LQ, rest = idx / BR_LQ, ly = 1 + rest % (BR_Y + 2), lz = 1 + rest / (BR_Y + 2);
+      const int q = qx0 + lq, y = y0 + ly, k = z0 + lz;
+      ona[m] = idx < BR_NA && q >= 0 && q < nq && y >= 0 && y < ny && k >= 0 && k < nz;
+      rqa[m] = f4zero();
+      mqa[m] = 0;
+      if (ona[m]) {
+        const size_t c = ((size_t)k * ny + y) * nx + 4 * (size_t)q;
+        rqa[m] = ld4(r, c);
+        mqa[m] = *reinterpret_cast<const uint32_t*>(G.mask + c);
       }
     }
-  };
-  // One colour on plane p for the quad at window row `row` (tile row + 1),
-  // neighbours from the window. The colour's two cells of the quad (x-cells
-  // par and par + 2) are computed without branches; plain rows only: the
-  // quad comes back with them updated, and `ghosts` gets a bit per cell of
-  // the colour on a free-surface row (about one row in a hundred), which
-  // relax_ghost updates afterwards. Per cell the arithmetic is row_local's:
-  // coupled faces summed in face order, prec of the row's face count.
-  auto relax = [&](int p, int row, int q, int col, const float4 rq, uint32_t mq, uint32_t& ghosts) -> float4 {
-    const int y = y0 - 1 + row;
-    const float* sc = slot(p) + (row + 1) * nx + 4 * q;
+#pragma unroll
+    for (int m = 0; m < BR_BITEMS; ++m) {
+      const int idx = tid + BR_THREADS * m;
+      const int lq = 1 + idx % BR_QX, rest = idx / BR_QX, ly = 2 + rest % BR_Y, lz = 2 + rest / BR_Y;
+      const int q = qx0 + lq, y = y0 + ly, k = z0 + lz;
+      onb[m] = idx < BR_NB && q < nq && y < ny && k < nz;
+      rqb[m] = f4zero();
+      mqb[m] = 0;
+      if (onb[m]) {
+        const size_t c = ((size_t)k * ny + y) * nx + 4 * (size_t)q;
+        rqb[m] = ld4(r, c);
+        mqb[m] = *reinterpret_cast<const uint32_t*>(G.mask + c);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < BR_LOADS; ++m) {
+      const int idx = tid + BR_THREADS * m;
+      if (idx < BR_NLOAD) *reinterpret_cast<float4*>(zs + 4 * idx) = v[m];
+    }
+  }
+  // One colour on the quad at LDS (lq, ly, lz): the colour's two cells
+  // (x-cells par and par + 2) without branches on plain rows; free-surface
+  // cells are left as they are and flagged in `late` (bits e) for
+  // relax_late. Returns the quad with the colour's plain cells updated.
+  auto relax = [&](int lq, int ly, int lz, int col, const float4 rq, uint32_t mq, uint32_t& late) -> float4 {
+    const int q = qx0 + lq, y = y0 + ly, k = z0 + lz;
+    const float* sc = zs + ((lz * BR_LY + ly) * BR_LX + 4 * lq);
     const float4 zc = *reinterpret_cast<const float4*>(sc);
-    const float4 ym = *reinterpret_cast<const float4*>(sc - nx);
-    const float4 yp = *reinterpret_cast<const float4*>(sc + nx);
-    const float4 zm = *reinterpret_cast<const float4*>(slot(p - 1) + (row + 1) * nx + 4 * q);
-    const float4 zp = *reinterpret_cast<const float4*>(slot(p + 1) + (row + 1) * nx + 4 * q);
-    const float left = q > 0 ? sc[-1] : 0.f, right = q + 1 < nq ? sc[4] : 0.f;
-    const bool odd = ((col + y + p) & 1) != 0;  // the colour's cells are x-cells 1 and 3, else 0 and 2
-    auto cell = [&](float pc, float xl, float xr, float vym, float vyp, float vzm, float vzp, float rv, uint32_t mc,
-                    bool& ghost) -> float {
-      ghost = (mc & M_GHOST) != 0;
+    const float4 ym = *reinterpret_cast<const float4*>(sc - BR_LX);
+    const float4 yp = *reinterpret_cast<const float4*>(sc + BR_LX);
+    const float4 zm = *reinterpret_cast<const float4*>(sc - BR_LX * BR_LY);
+    const float4 zp = *reinterpret_cast<const float4*>(sc + BR_LX * BR_LY);
+    // the x neighbours' quads as whole 16-byte reads (scalar reads 16 bytes
+    // apart would conflict in the LDS banks)
+    const float4 xl4 = *reinterpret_cast<const float4*>(sc - 4);
+    const float4 xr4 = *reinterpret_cast<const float4*>(sc + 4);
+    const float left = q > 0 ? xl4.w : 0.f, right = q + 1 < nq ? xr4.x : 0.f;
+    const bool odd = ((col + y + k) & 1) != 0;
+    auto sel = [odd](float o, float e) -> float { return odd ? o : e; };
+    auto cell = [&](float pc, float xl, float xr, float vym, float vyp, float vzm, float vzp, float rc,
+                    uint32_t mc) -> float {
       const float nb[6] = {xl, xr, vym, vyp, vzm, vzp};
       float acc = 0.f;
 #pragma unroll
       for (int f = 0; f < 6; ++f) acc = (mc & (1u << f)) ? acc + (pc - nb[f]) : acc;
       const float prec = s_prec[(mc & M_WET) ? __popc(mc & 0x3Fu) : 0];
-      float v = prec > 0.f ? SV::rnd(pc + prec * (rv - acc * G.inv)) : (MODE == 2 ? 0.f : pc);
-      return ghost ? pc : v;
+      const float v = prec > 0.f ? SV::rnd(pc + prec * (rc - acc * G.inv)) : (MODE == 2 ? 0.f : pc);
+      return (mc & M_GHOST) ? pc : v;
     };
-    // selections of values (not of lvalues, which would put the operands in memory)
-    auto sel = [odd](float o, float e) -> float { return odd ? o : e; };
-    bool ga, gb;
+    const uint32_t ma = (mq >> (odd ? 8 : 0)) & 0xFFu, mb = (mq >> (odd ? 24 : 16)) & 0xFFu;
     const float a = cell(sel(zc.y, zc.x), sel(zc.x, left), sel(zc.z, zc.y), sel(ym.y, ym.x), sel(yp.y, yp.x),
-                         sel(zm.y, zm.x), sel(zp.y, zp.x), sel(rq.y, rq.x), (mq >> (odd ? 8 : 0)) & 0xFFu, ga);
+                         sel(zm.y, zm.x), sel(zp.y, zp.x), sel(rq.y, rq.x), ma);
     const float b = cell(sel(zc.w, zc.z), sel(zc.z, zc.y), sel(right, zc.w), sel(ym.w, ym.z), sel(yp.w, yp.z),
-                         sel(zm.w, zm.z), sel(zp.w, zp.z), sel(rq.w, rq.z), (mq >> (odd ? 24 : 16)) & 0xFFu, gb);
-    ghosts = (ga ? (odd ? 2u : 1u) : 0u) | (gb ? (odd ? 8u : 4u) : 0u);
+                         sel(zm.w, zm.z), sel(zp.w, zp.z), sel(rq.w, rq.z), mb);
+    late = ((ma & M_GHOST) ? (odd ? 2u : 1u) : 0u) | ((mb & M_GHOST) ? (odd ? 8u : 4u) : 0u);
     return make_float4(sel(zc.x, a), sel(a, zc.y), sel(zc.z, b), sel(b, zc.w));
   };
-  // The free-surface cells `ghosts` of that quad (row_local's ghost form),
-  // r and the row codes read again (they are in cache); returns the new
-  // values in the quad's positions.
-  auto relax_ghost = [&](int p, int row, int q, uint32_t ghosts) -> float4 {
-    const int y = y0 - 1 + row;
-    const float* sc = slot(p) + (row + 1) * nx + 4 * q;
-    const size_t c0 = ((size_t)p * ny + y) * nx + 4 * (size_t)q;
-    const float4 rq = ld4(r, c0);
-    const uint32_t mq = *reinterpret_cast<const uint32_t*>(G.mask + c0);
-    float4 out = f4zero();
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      if (!(ghosts & (1u << e))) continue;
-      const uint8_t mc = (uint8_t)(mq >> (8 * e));
-      const float pc = sc[e];
-      const float nb[6] = {q > 0 || e > 0 ? sc[e - 1] : 0.f, q + 1 < nq || e < 3 ? sc[e + 1] : 0.f, sc[e - nx],
-                           sc[e + nx], slot(p - 1)[(row + 1) * nx + 4 * q + e],
-                           slot(p + 1)[(row + 1) * nx + 4 * q + e]};
-      float prec;
-      const float az = row_local(G, 4 * q + e, y, p, mc, pc, nb, prec);
-      q_set(out, e, prec > 0.f ? SV::rnd(pc + prec * (q_get(rq, e) - az)) : (MODE == 2 ? 0.f : pc));
-    }
-    return out;
+  // A free-surface cell e of that quad: row_local's ghost form (per-column
+  // table, else the geometry), neighbours from LDS.
+  auto relax_late = [&](int lq, int ly, int lz, int e) -> float {
+    const int q = qx0 + lq, y = y0 + ly, k = z0 + lz;
+    const size_t c = ((size_t)k * ny + y) * nx + 4 * (size_t)q + e;
+    const float* sc = zs + ((lz * BR_LY + ly) * BR_LX + 4 * lq + e);
+    const float nb[6] = {(q > 0 || e > 0) ? sc[-1] : 0.f, (q + 1 < nq || e < 3) ? sc[1] : 0.f, sc[-BR_LX],
+                         sc[BR_LX], sc[-BR_LX * BR_LY], sc[BR_LX * BR_LY]};
+    const float pc = sc[0];
+    float prec;
+    const float az = row_local(G, 4 * q + e, y, k, G.mask[c], pc, nb, prec);
+    return prec > 0.f ? SV::rnd(pc + prec * (r[c] - az)) : (MODE == 2 ? 0.f : pc);
   };
-  float4 zpf[SW_ZITEMS];
-  float4 r_prev[SW_AITEMS], r_cur[SW_AITEMS], r_next[SW_AITEMS];
-  uint32_t m_prev[SW_AITEMS], m_cur[SW_AITEMS], m_next[SW_AITEMS];
-  // the window's first planes and the r / row codes of the first pass
-  for (int p = k0 - 2; p <= k0; ++p) {
-    if (p < 0) continue;
-    load_z(p, zpf);
-    store_z(p, zpf);
-  }
-  load_rm(k0 - 1, r_cur, m_cur);
+  __syncthreads();
+  // pass c over the brick and its ring (LDS rows 1..BR_Y+2, planes 1..BR_Z+2, every LDS quad)
+  {
+    uint32_t late = 0;
 #pragma unroll
-  for (int m = 0; m < SW_AITEMS; ++m) { r_prev[m] = f4zero(); m_prev[m] = 0; }
-  lds_barrier();
-  double s = 0.0, unused = 0.0;
-  const int c1 = colour, c2 = 1 - colour;
-  for (int sp = k0 - 1; sp <= k1; ++sp) {
-    // loads for the next steps first: z of plane sp + 2, r / row codes of sp + 1
-    const bool want_z = sp + 2 < nz && sp + 2 <= k1 + 1;
-    if (want_z) load_z(sp + 2, zpf);
-    load_rm(sp + 1 <= k1 ? sp + 1 : -1, r_next, m_next);
-    // pass c on plane sp (tile rows and the ring), back into the window
-    if (sp >= 0 && sp < nz) {
-      uint32_t pending = 0;  // 4 bits per item: free-surface cells left
-#pragma unroll
-      for (int m = 0; m < SW_AITEMS; ++m) {
-        if (arow[m] < 0) continue;
-        uint32_t gh;
-        const float4 o = relax(sp, arow[m], aq[m], c1, r_cur[m], m_cur[m], gh);
-        *reinterpret_cast<float4*>(slot(sp) + (arow[m] + 1) * nx + 4 * aq[m]) = o;
-        pending |= gh << (4 * m);
-      }
-#pragma unroll 1
-      for (int m = 0; m < SW_AITEMS; ++m) {
-        const uint32_t gh = (pending >> (4 * m)) & 15u;
-        if (!gh) continue;
-        const int idx = tid + QBLOCK * m, row = idx / nq, q = idx - row * nq;
-        const float4 o = relax_ghost(sp, row, q, gh);
-        float* dst = slot(sp) + (row + 1) * nx + 4 * q;
-        for (int e = 0; e < 4; ++e)
-          if (gh & (1u << e)) dst[e] = q_get(o, e);
-      }
+    for (int m = 0; m < BR_AITEMS; ++m) {
+      if (!ona[m]) continue;
+      const int idx = tid + BR_THREADS * m;
+      const int lq = idx % BR_LQ, rest = idx / BR_LQ, ly = 1 + rest % (BR_Y + 2), lz = 1 + rest / (BR_Y + 2);
+      uint32_t lt;
+      const float4 o = relax(lq, ly, lz, colour, rqa[m], mqa[m], lt);
+      *reinterpret_cast<float4*>(zs + ((lz * BR_LY + ly) * BR_LX + 4 * lq)) = o;
+      late |= lt << (4 * m);
     }
-    lds_barrier();
-    // pass 1 - c on plane sp - 1 (tile rows), to zout
-    const int pb = sp - 1;
-    if (pb >= k0 && pb < k1) {
-      uint32_t pending = 0;
-#pragma unroll
-      for (int m = 0; m < SW_AITEMS; ++m) {
-        if (arow[m] < 1 || arow[m] > SW_TY) continue;
-        uint32_t gh;
-        const float4 o = relax(pb, arow[m], aq[m], c2, r_prev[m], m_prev[m], gh);
-        const size_t c = ((size_t)pb * ny + (y0 - 1 + arow[m])) * nx + 4 * (size_t)aq[m];
-        SV::st4(zout, c, o);
-        pending |= gh << (4 * m);
-        if (dot_op >= 0) {
-          const float4 rq = r_prev[m];
-          // free-surface cells are added with their new values below
-#pragma unroll
-          for (int e = 0; e < 4; ++e)
-            if (!(gh & (1u << e))) s += (double)q_get(rq, e) * q_get(o, e);
-        }
-      }
-#pragma unroll 1
-      for (int m = 0; m < SW_AITEMS; ++m) {
-        const uint32_t gh = (pending >> (4 * m)) & 15u;
-        if (!gh) continue;
-        const int idx = tid + QBLOCK * m, row = idx / nq, q = idx - row * nq;
-        const float4 o = relax_ghost(pb, row, q, gh);
-        const size_t c = ((size_t)pb * ny + (y0 - 1 + row)) * nx + 4 * (size_t)q;
-        for (int e = 0; e < 4; ++e) {
-          if (!(gh & (1u << e))) continue;
-          SV::st(zout, c + e, q_get(o, e));
-          if (dot_op >= 0) s += (double)r[c + e] * q_get(o, e);
-        }
-      }
-    }
-    if (want_z) store_z(sp + 2, zpf);
-    lds_barrier();
-#pragma unroll
-    for (int m = 0; m < SW_AITEMS; ++m) {
-      r_prev[m] = r_cur[m]; m_prev[m] = m_cur[m];
-      r_cur[m] = r_next[m]; m_cur[m] = m_next[m];
+    for (; late; late &= late - 1) {
+      const int bit = __ffs(late) - 1, m = bit >> 2, e = bit & 3;
+      const int idx = tid + BR_THREADS * m;
+      const int lq = idx % BR_LQ, rest = idx / BR_LQ, ly = 1 + rest % (BR_Y + 2), lz = 1 + rest / (BR_Y + 2);
+      zs[(lz * BR_LY + ly) * BR_LX + 4 * lq + e] = relax_late(lq, ly, lz, e);
     }
   }
-  if (dot_op >= 0) reduce_finish<false>(s, unused, dot_op, ctl, partials);
-  (void)plane;
+  __syncthreads();
+  // pass 1 - c over the brick (LDS quads 1..BR_QX, rows 2..BR_Y+1, planes 2..BR_Z+1), to zout
+  double s = 0.0;
+  {
+#pragma unroll
+    for (int m = 0; m < BR_BITEMS; ++m) {
+      if (!onb[m]) continue;
+      const int idx = tid + BR_THREADS * m;
+      const int lq = 1 + idx % BR_QX, rest = idx / BR_QX, ly = 2 + rest % BR_Y, lz = 2 + rest / BR_Y;
+      const int q = qx0 + lq, y = y0 + ly, k = z0 + lz;
+      uint32_t lt;
+      float4 o = relax(lq, ly, lz, 1 - colour, rqb[m], mqb[m], lt);
+      for (; lt; lt &= lt - 1) {
+        const int e = __ffs(lt) - 1;
+        q_set(o, e, relax_late(lq, ly, lz, e));
+      }
+      const size_t c = ((size_t)k * ny + y) * nx + 4 * (size_t)q;
+      SV::st4(zout, c, o);
+      if (dot_op >= 0)
+        s += (double)rqb[m].x * o.x + (double)rqb[m].y * o.y + (double)rqb[m].z * o.z + (double)rqb[m].w * o.w;
+    }
+  }
+  if (dot_op >= 0) {
+    const int lane = tid & 63, wid = tid >> 6;
+    s = wave_sum(s);
+    if (lane == 0) s_red[wid] = s;
+    __syncthreads();
+    if (tid == 0) {
+      double tt = s_red[0];
+      for (int w = 1; w < BR_THREADS / 64; ++w) tt += s_red[w];
+      double* part = partials + (size_t)dot_op * MAX_PARTIALS * 2;
+      part[2 * blockIdx.x] = tt;
+      part[2 * blockIdx.x + 1] = 0.0;
+    }
+  }
 }
 
 // Row codes on quads (k_mask with four x-cells per thread: one decode, and

@@ -54,10 +54,9 @@ enum { FSP_STORE_F32 = 0, FSP_STORE_F16 = 1, FSP_STORE_BF16 = 2 };
  * Galerkin product P^T A P / 16 with piecewise-constant P (equal to the
  * re-discretisation in the interior, exact at the surface and at solids). */
 enum { FSP_COARSE_REDISCRETISED = 0, FSP_COARSE_GALERKIN = 1 };
-/* Fine-level red-black smoothing: one launch per colour (the default), or
- * both colours of a sweep in one launch over a window of planes in LDS where
- * a grid row fits a workgroup (nx <= 256). The V-cycles are identical bit
- * for bit. */
+/* Fine-level red-black smoothing: both colours of a sweep in one launch, a
+ * brick of z with its halo in LDS per workgroup (the default), or one launch
+ * per colour. The V-cycles are identical bit for bit. */
 enum { FSP_SWEEP_FUSED = 0, FSP_SWEEP_SINGLE = 1 };
 
 /* Solver settings. fsp_default_options gives the reference's multigrid

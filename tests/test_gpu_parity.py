@@ -449,14 +449,14 @@ def test_autotuned_tiles_keep_the_projection():
 
 
 @pytest.mark.parametrize("shape,sweeps", [
-    ((256, 38, 200), (2, 2)),   # rows of 64 quads, 25 tiles, 10 chunks of 4 planes, the last one of 2
-    ((96, 40, 76), (3, 2)),     # rows of 24 quads, 10 tiles, the last one of 4 rows
-    ((32, 32, 32), (1, 3)),     # rows of 8 quads, 4 tiles, 8 chunks
+    ((256, 38, 200), (2, 2)),   # 8 x 25 x 5 bricks, the last one in z of 6 planes
+    ((96, 40, 76), (3, 2)),     # 3 x 10 x 5 bricks, the last one in y of 4 rows
+    ((32, 32, 32), (1, 3)),     # one brick across x
 ])
 @pytest.mark.parametrize("storage", [_lib.STORE_F32, _lib.STORE_BF16])
 def test_fused_sweeps_equal_single_passes_bit_for_bit(shape, sweeps, storage):
-    """Both colours of a red-black sweep in one launch (the fine level's
-    window of planes in LDS) give the V-cycle of one launch per colour bit
+    """Both colours of a red-black sweep in one launch (a brick of z with its
+    halo in LDS per workgroup) give the V-cycle of one launch per colour bit
     for bit, and the same projection."""
     wl = W.tank(*shape, seed=17)
     wl.options["pre_sweeps"], wl.options["post_sweeps"] = sweeps
