@@ -45,6 +45,10 @@ struct fsp_solver {
   int cut = 0;
   fsp_surface surf{};
   uint8_t *solid = nullptr, *ou = nullptr, *ov = nullptr, *ow = nullptr, *mask = nullptr;
+  // fractional apertures (code 2 in ou/ov/ow), reserved by the first
+  // fsp_set_geometry that has one
+  float *au = nullptr, *av = nullptr, *aw = nullptr;
+  int frac = 0;
   float *eta = nullptr, *eta_rt = nullptr;
   float* col_floor = nullptr;  // per column: top of its highest solid cell (0 without one)
   uint16_t* gk0 = nullptr;  // per column: lowest ghost row
@@ -100,6 +104,8 @@ struct fsp_solver {
     G.rho = surf.rho; G.g = surf.g_eff; G.rest = surf.rest;
     G.homogeneous = surf.homogeneous_ghost;
     G.solid = solid; G.ou = ou; G.ov = ov; G.ow = ow;
+    G.frac = frac;
+    G.au = frac ? au : nullptr; G.av = frac ? av : nullptr; G.aw = frac ? aw : nullptr;
     G.eta = eta; G.eta_rt = eta_rt;
     G.mask = mask;
     G.gk0 = gk0;
@@ -1398,12 +1404,27 @@ fsp_status fsp_set_geometry(fsp_solver* S, const fsp_surface* surf, const float*
   if (!std::isfinite(surf->rho) || !(surf->rho > 0.f) || !std::isfinite(surf->g_eff) || !std::isfinite(surf->rest))
     return FSP_NOT_FINITE;
   if (!all_finite(eta, S->ncols) || (eta_roundoff && !all_finite(eta_roundoff, S->ncols))) return FSP_NOT_FINITE;
+  // apertures in [0, 1]; any other than 0 or 1 makes the geometry fractional
   const float* opens[3] = {open_u, open_v, open_w};
   const size_t lens[3] = {S->nu, S->nv, S->nw};
+  int frac = 0;
   for (int a = 0; a < 3; ++a)
     if (opens[a])
-      for (size_t q = 0; q < lens[a]; ++q)
-        if (!(opens[a][q] == 0.f || opens[a][q] == 1.f)) return FSP_DOMAIN;  // fractional apertures: not supported
+      for (size_t q = 0; q < lens[a]; ++q) {
+        const float x = opens[a][q];
+        if (!std::isfinite(x)) return FSP_NOT_FINITE;
+        if (!(x >= 0.f && x <= 1.f)) return FSP_DOMAIN;
+        if (x != 0.f && x != 1.f) frac = 1;
+      }
+  // the Galerkin coarse operator is built for unit couplings (its compact
+  // levels hold them in binary16); fractional apertures take the
+  // re-discretised one, the reference's
+  if (frac && S->opt.coarse_operator == FSP_COARSE_GALERKIN && S->opt.preconditioner != FSP_PRECOND_JACOBI) {
+    std::fprintf(stderr, "fsp: fractional apertures need coarse_operator = FSP_COARSE_REDISCRETISED\n");
+    return FSP_DOMAIN;
+  }
+  // face code: 0 closed, 1 open, 2 fractional (the value in au/av/aw)
+  auto code = [](float x) -> uint8_t { return x == 0.f ? 0 : (x == 1.f ? 1 : 2); };
   const size_t nx = S->dom.nx, ny = S->dom.ny, nz = S->dom.nz;
   const int cut = (solid || open_u || open_v || open_w || surf->closed_lid) ? 1 : 0;
   std::vector<uint8_t> hs(S->n, 0), hu(S->nu, 1), hv(S->nv, 1), hw(S->nw, 1);
@@ -1417,14 +1438,14 @@ fsp_status fsp_set_geometry(fsp_solver* S, const fsp_surface* surf, const float*
         for (size_t i = 0; i <= nx; ++i) {
           const size_t f = (k * ny + j) * (nx + 1) + i;
           uint8_t a = (i == 0 || i == nx || Sd(i - 1, j, k) || Sd(i, j, k)) ? 0 : 1;
-          hu[f] = open_u ? (uint8_t)(open_u[f] == 1.f) : a;
+          hu[f] = open_u ? code(open_u[f]) : a;
         }
     for (size_t k = 0; k < nz; ++k)
       for (size_t j = 0; j <= ny; ++j)
         for (size_t i = 0; i < nx; ++i) {
           const size_t f = (k * (ny + 1) + j) * nx + i;
           uint8_t a = (j == 0 || j == ny || Sd(i, j - 1, k) || Sd(i, j, k)) ? 0 : 1;
-          hv[f] = open_v ? (uint8_t)(open_v[f] == 1.f) : a;
+          hv[f] = open_v ? code(open_v[f]) : a;
         }
     for (size_t k = 0; k <= nz; ++k)
       for (size_t j = 0; j < ny; ++j)
@@ -1434,7 +1455,7 @@ fsp_status fsp_set_geometry(fsp_solver* S, const fsp_surface* surf, const float*
           if (k == 0) a = 0;
           else if (k == nz) a = (Sd(i, j, nz - 1) || surf->closed_lid) ? 0 : 1;
           else a = (Sd(i, j, k - 1) || Sd(i, j, k)) ? 0 : 1;
-          uint8_t val = open_w ? (uint8_t)(open_w[f] == 1.f) : a;
+          uint8_t val = open_w ? code(open_w[f]) : a;
           if (k == nz && surf->closed_lid) val = 0;
           hw[f] = val;
         }
@@ -1447,6 +1468,20 @@ fsp_status fsp_set_geometry(fsp_solver* S, const fsp_surface* surf, const float*
     for (size_t c = 0; c < S->ncols; ++c)
       if (hs[k * S->ncols + c]) floors[c] = (float)(k + 1) * S->dom.dx;
   FSP_HIP(hipSetDevice(S->opt.device));
+  if (frac) {
+    // the fractional values, reserved here at configuration (set_geometry
+    // precedes every projection) the first time a geometry has them
+    if (!S->au) {
+      fsp_status as;
+      if ((as = dalloc(S, &S->au, S->nu)) || (as = dalloc(S, &S->av, S->nv)) || (as = dalloc(S, &S->aw, S->nw)))
+        return as;
+    }
+    float* dst[3] = {S->au, S->av, S->aw};
+    for (int a = 0; a < 3; ++a) {
+      if (opens[a]) FSP_HIP(hipMemcpyAsync(dst[a], opens[a], lens[a] * sizeof(float), hipMemcpyHostToDevice, S->stream));
+      else FSP_HIP(hipMemsetAsync(dst[a], 0, lens[a] * sizeof(float), S->stream));
+    }
+  }
   FSP_HIP(hipMemcpyAsync(S->col_floor, floors.data(), S->ncols * sizeof(float), hipMemcpyHostToDevice, S->stream));
   FSP_HIP(hipMemcpyAsync(S->solid, hs.data(), S->n, hipMemcpyHostToDevice, S->stream));
   FSP_HIP(hipMemcpyAsync(S->ou, hu.data(), S->nu, hipMemcpyHostToDevice, S->stream));
@@ -1461,7 +1496,17 @@ fsp_status fsp_set_geometry(fsp_solver* S, const fsp_surface* surf, const float*
   FSP_HIP(hipStreamSynchronize(S->stream));
   S->surf = *surf;
   S->cut = cut;
+  const bool recapture = S->frac != frac;
+  S->frac = frac;
   S->geom = true;
+  // the iteration graphs hold the fine geometry's kernel arguments: capture
+  // them again when fractional apertures come or go
+  if (recapture) {
+    for (int gi = 0; gi < 2; ++gi) drop_graph(S, gi);
+    fsp_status gst = prepare_graph(S, 0);
+    if (gst == FSP_OK && S->net && S->opt.preconditioner == FSP_PRECOND_LEARNED) gst = prepare_graph(S, 1);
+    if (gst) return gst;
+  }
   return FSP_OK;
 }
 
@@ -1679,6 +1724,7 @@ fsp_status fsp_step_surface_mobile_device(fsp_solver* S, float* d_u, float* d_v,
                                           fsp_balance* balance) {
   if (!S || !d_u || !d_v || !d_w || !d_p) return FSP_INVALID;
   if (!S->geom) return FSP_DOMAIN;
+  if (S->frac) return FSP_DOMAIN;  // the surface step is built for binary apertures
   if (duration_us == 0 || duration_us > (1ull << 53)) return FSP_NOT_FINITE;
   const double dt = (double)duration_us * 1e-6;
   const float dx = S->dom.dx, rho = S->surf.rho, g = S->surf.g_eff;
@@ -1750,6 +1796,7 @@ fsp_status fsp_advect_device(fsp_solver* S, const float* d_u, const float* d_v, 
                              float* d_us, float* d_vs, float* d_ws) {
   if (!S || !d_u || !d_v || !d_w || !d_us || !d_vs || !d_ws) return FSP_INVALID;
   if (!S->geom) return FSP_DOMAIN;
+  if (S->frac) return FSP_DOMAIN;  // the surface step is built for binary apertures
   if (!std::isfinite(dt)) return FSP_NOT_FINITE;
   FSP_HIP(hipSetDevice(S->opt.device));
   fsp_status st = advect(S, d_u, d_v, d_w, dt, d_us, d_vs, d_ws);
@@ -1761,6 +1808,7 @@ fsp_status fsp_advect_device(fsp_solver* S, const float* d_u, const float* d_v, 
 fsp_status fsp_extrapolate_device(fsp_solver* S, float* d_u, float* d_v, float* d_w) {
   if (!S || !d_u || !d_v || !d_w) return FSP_INVALID;
   if (!S->geom) return FSP_DOMAIN;
+  if (S->frac) return FSP_DOMAIN;  // the surface step is built for binary apertures
   FSP_HIP(hipSetDevice(S->opt.device));
   fsp_status st = extrapolate(S, d_u, d_v, d_w);
   if (st) return st;
@@ -1772,6 +1820,7 @@ fsp_status fsp_transport_device(fsp_solver* S, const float* d_u, const float* d_
                                 float* d_flux_y) {
   if (!S || !d_u || !d_v) return FSP_INVALID;
   if (!S->geom) return FSP_DOMAIN;
+  if (S->frac) return FSP_DOMAIN;  // the surface step is built for binary apertures
   if (!std::isfinite(factor)) return FSP_NOT_FINITE;
   FSP_HIP(hipSetDevice(S->opt.device));
   fsp_status st = step_reserve(S);

@@ -79,6 +79,13 @@ struct Fine {
   const uint16_t* gk0;   // columns: lowest ghost row (0xFFFF: none)
   const float* gtab;     // columns x G_SLOTS x 8: 1/theta of the ghost faces
   const float* precv;    // cells: row_prec of every row (k_prec)
+  // Fractional apertures (the reference's cut geometry): the u8 face arrays
+  // hold 0 closed, 1 open, 2 fractional, and a code-2 face's aperture is in
+  // au/av/aw (null and frac == 0 when every aperture is 0 or 1).
+  const float* au;
+  const float* av;
+  const float* aw;
+  int frac;
   float prec_tab[7];     // 1/(n * inv) for a plain row with n faces
 };
 

@@ -244,30 +244,37 @@ __device__ __forceinline__ void ghost_side(const Fine& G, int i, int j, int k, i
   val = G.rho * G.g * (zc - G.rest);
 }
 
-// The reference's mobile_row (delta3d_mobile.rs:152-188) for binary
-// apertures: type 1 = wet neighbour, 2 = ghost face, 0 = nothing.
+// Aperture of a face from its code in the u8 face arrays (0 closed, 1 open,
+// 2 the fraction held in au/av/aw); axis 0 u, 1 v, 2 w.
+__device__ __forceinline__ float aperture(const Fine& G, int axis, size_t f, uint8_t code) {
+  if (code != 2) return (float)code;
+  return axis == 0 ? G.au[f] : axis == 1 ? G.av[f] : G.aw[f];
+}
+
+// The reference's mobile_row (delta3d_mobile.rs:152-188): type 1 = wet
+// neighbour, 2 = ghost face, 0 = nothing; a = the face's aperture.
 struct RowG {
   uint8_t type[6];
   float it[6];
   float val[6];
+  float a[6];
 };
 
 __device__ void full_row(const Fine& G, int i, int j, int k, RowG& R) {
 #pragma unroll
-  for (int f = 0; f < 6; ++f) { R.type[f] = 0; R.it[f] = 0.f; R.val[f] = 0.f; }
+  for (int f = 0; f < 6; ++f) { R.type[f] = 0; R.it[f] = 0.f; R.val[f] = 0.f; R.a[f] = 0.f; }
   const bool has[4] = {i > 0, i + 1 < G.nx, j > 0, j + 1 < G.ny};
   const int xs[4] = {i - 1, i + 1, i, i};
   const int ys[4] = {j, j, j - 1, j + 1};
 #pragma unroll
   for (int f = 0; f < 4; ++f) {
     if (!has[f]) continue;
-    uint8_t a;
-    if (f == 0) a = G.ou[fuidx(G, i, j, k)];
-    else if (f == 1) a = G.ou[fuidx(G, i + 1, j, k)];
-    else if (f == 2) a = G.ov[fvidx(G, i, j, k)];
-    else a = G.ov[fvidx(G, i, j + 1, k)];
+    const size_t face = f == 0 ? fuidx(G, i, j, k) : f == 1 ? fuidx(G, i + 1, j, k)
+                      : f == 2 ? fvidx(G, i, j, k) : fvidx(G, i, j + 1, k);
+    const uint8_t a = f < 2 ? G.ou[face] : G.ov[face];
     const int x = xs[f], y = ys[f];
     if (a == 0 || solid_at(G, x, y, k)) continue;
+    R.a[f] = aperture(G, f < 2 ? 0 : 1, face, a);
     if (wet_at(G, x, y, k)) {
       R.type[f] = 1;
     } else {
@@ -275,8 +282,13 @@ __device__ void full_row(const Fine& G, int i, int j, int k, RowG& R) {
       ghost_side(G, i, j, k, x, y, R.it[f], R.val[f]);
     }
   }
-  if (k > 0 && G.ow[fwidx(G, i, j, k)] && !solid_at(G, i, j, k - 1)) R.type[4] = 1;
-  if (G.ow[fwidx(G, i, j, k + 1)]) {
+  const size_t fd = fwidx(G, i, j, k), fu = fwidx(G, i, j, k + 1);
+  if (k > 0 && G.ow[fd] && !solid_at(G, i, j, k - 1)) {
+    R.type[4] = 1;
+    R.a[4] = aperture(G, 2, fd, G.ow[fd]);
+  }
+  if (G.ow[fu]) {
+    R.a[5] = aperture(G, 2, fu, G.ow[fu]);
     if (k + 1 < G.nz && wet_at(G, i, j, k + 1)) {
       R.type[5] = 1;
     } else {
@@ -304,17 +316,24 @@ __device__ __forceinline__ bool ghost_its(const Fine& G, int i, int j, int k, fl
   return true;
 }
 
-// Per-face coefficients of a row with ghost faces: type 1 neighbour, 2 ghost.
-__device__ __forceinline__ void ghost_row(const Fine& G, int i, int j, int k, uint8_t m, uint8_t type[6], float it[6]) {
-  if (ghost_its(G, i, j, k, it)) {
+// Per-face coefficients of a row with ghost faces or fractional apertures:
+// type 1 neighbour, 2 ghost, with the face's aperture a (1 with binary
+// apertures). With fractional apertures every such row comes from the
+// geometry (the ghost table holds binary rows only).
+__device__ __forceinline__ void ghost_row(const Fine& G, int i, int j, int k, uint8_t m, uint8_t type[6], float it[6],
+                                          float a[6]) {
+  if (!G.frac && ghost_its(G, i, j, k, it)) {
 #pragma unroll
-    for (int f = 0; f < 6; ++f) type[f] = (m & (1u << f)) ? 1 : (it[f] > 0.f ? 2 : 0);
+    for (int f = 0; f < 6; ++f) {
+      type[f] = (m & (1u << f)) ? 1 : (it[f] > 0.f ? 2 : 0);
+      a[f] = 1.0f;
+    }
     return;
   }
   RowG R;
   full_row(G, i, j, k, R);
 #pragma unroll
-  for (int f = 0; f < 6; ++f) { type[f] = R.type[f]; it[f] = R.it[f]; }
+  for (int f = 0; f < 6; ++f) { type[f] = R.type[f]; it[f] = R.it[f]; a[f] = R.a[f]; }
 }
 
 // (A x)_c in accumulator precision A from storage T, in the reference's face
@@ -339,14 +358,14 @@ __device__ __forceinline__ A apply_row_t(const Fine& G, int i, int j, int k, siz
     if (prec_out) *prec_out = G.prec_tab[__popc(m & 0x3Fu)];
   } else {
     uint8_t type[6];
-    float it[6];
-    ghost_row(G, i, j, k, m, type, it);
+    float it[6], a[6];
+    ghost_row(G, i, j, k, m, type, it, a);
     const size_t nb[6] = {c - 1, c + 1, c - G.nx, c + G.nx, c - plane, c + plane};
     float diag = 0.f;
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-      if (type[f] == 1) { acc += pc - (A)x[nb[f]]; diag += 1.0f; }
-      else if (type[f] == 2) { acc += pc * (A)it[f]; diag += it[f]; }
+      if (type[f] == 1) { acc += (A)a[f] * (pc - (A)x[nb[f]]); diag += a[f]; }
+      else if (type[f] == 2) { acc += (A)a[f] * pc * (A)it[f]; diag += a[f] * it[f]; }
     }
     if (prec_out) *prec_out = diag > 0.f ? 1.0f / (diag * G.inv) : 0.f;
   }
@@ -370,12 +389,12 @@ __device__ __forceinline__ float row_prec(const Fine& G, int i, int j, int k, ui
   if (!(m & M_WET)) return 0.f;
   if (!(m & M_GHOST)) return G.prec_tab[__popc(m & 0x3Fu)];
   uint8_t type[6];
-  float it[6];
-  ghost_row(G, i, j, k, m, type, it);
+  float it[6], a[6];
+  ghost_row(G, i, j, k, m, type, it, a);
   float diag = 0.f;
   for (int f = 0; f < 6; ++f) {
-    if (type[f] == 1) diag += 1.0f;
-    else if (type[f] == 2) diag += it[f];
+    if (type[f] == 1) diag += a[f];
+    else if (type[f] == 2) diag += a[f] * it[f];
   }
   return diag > 0.f ? 1.0f / (diag * G.inv) : 0.f;
 }
@@ -403,9 +422,9 @@ __device__ __forceinline__ float divergence_at(const Fine& G, const float* u, co
   const size_t c0 = fwidx(G, i, j, k), c1 = fwidx(G, i, j, k + 1);
   float fl = u[a0], fr = u[a1], ff = v[b0], fk = v[b1], fb = w[c0], ft = w[c1];
   if (G.cut) {
-    fl = (float)G.ou[a0] * fl; fr = (float)G.ou[a1] * fr;
-    ff = (float)G.ov[b0] * ff; fk = (float)G.ov[b1] * fk;
-    fb = (float)G.ow[c0] * fb; ft = (float)G.ow[c1] * ft;
+    fl = aperture(G, 0, a0, G.ou[a0]) * fl; fr = aperture(G, 0, a1, G.ou[a1]) * fr;
+    ff = aperture(G, 1, b0, G.ov[b0]) * ff; fk = aperture(G, 1, b1, G.ov[b1]) * fk;
+    fb = aperture(G, 2, c0, G.ow[c0]) * fb; ft = aperture(G, 2, c1, G.ow[c1]) * ft;
   }
   return ((fr - fl + ft - fb) + (fk - ff)) / G.dx;
 }
@@ -427,18 +446,27 @@ __device__ __forceinline__ uint8_t row_code(const Fine& G, int i, int j, int k, 
   uint8_t m = 0;
   if (!s_c && zc < h) {
     m = M_WET;
+    bool fractional = false;  // a face of the row with a fractional aperture
     const bool has[4] = {i > 0, i + 1 < G.nx, j > 0, j + 1 < G.ny};
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
       if (!has[f] || ap[f] == 0 || sn[f]) continue;
+      fractional |= ap[f] == 2;
       if (zc < hn[f]) m |= (uint8_t)(1u << f);  // wet neighbour (not solid, below its surface)
       else { m |= M_GHOST; ghosts += 1.0; }
     }
-    if (k > 0 && ap[4] && !sn[4]) m |= M_ZM;
+    if (k > 0 && ap[4] && !sn[4]) {
+      m |= M_ZM;
+      fractional |= ap[4] == 2;
+    }
     if (ap[5]) {
+      fractional |= ap[5] == 2;
       if (k + 1 < G.nz && !sn[5] && zc_of(G, k + 1) < h) m |= M_ZP;
       else { m |= M_GHOST; ghosts += 1.0; }
     }
+    // rows with a fractional aperture take the general row (full_row) as
+    // free-surface rows do
+    if (fractional) m |= M_GHOST;
     wet += 1.0;
   }
   return m;
@@ -533,10 +561,10 @@ __global__ void __launch_bounds__(BLOCK) k_assemble(Fine G, const float* us, con
         RowG R;
         full_row(G, i, j, k, R);
         for (int f = 0; f < 6; ++f) {
-          if (R.type[f] == 1) diag += 1.0f;
+          if (R.type[f] == 1) diag += R.a[f];
           else if (R.type[f] == 2) {
-            diag += R.it[f];
-            bb += R.val[f] * R.it[f] * G.inv;
+            diag += R.a[f] * R.it[f];
+            bb += R.a[f] * R.val[f] * R.it[f] * G.inv;
           }
         }
       }
@@ -651,8 +679,8 @@ __global__ void __launch_bounds__(BLOCK) k_spmv_dir(Fine G, const float* z, floa
         full_row(G, i, j, k, R);
         const size_t nb[6] = {c - 1, c + 1, c - G.nx, c + G.nx, c - plane, c + plane};
         for (int f = 0; f < 6; ++f) {
-          if (R.type[f] == 1) acc += dc - dv(nb[f]);
-          else if (R.type[f] == 2) acc += dc * R.it[f];
+          if (R.type[f] == 1) acc += R.a[f] * (dc - dv(nb[f]));
+          else if (R.type[f] == 2) acc += R.a[f] * dc * R.it[f];
         }
       }
       qc = acc * G.inv;
@@ -877,9 +905,19 @@ __global__ void __launch_bounds__(BLOCK) k_cfaces_fine(Fine G, Level L) {
   const size_t total = (cx + 1) * cy * cz + cx * (cy + 1) * cz + cx * cy * (cz + 1);
   FSP_FOR(f, total) {
     coarse_faces(
-        L, f, [&](size_t i, size_t j, size_t k) { return (float)G.ou[fuidx(G, (int)i, (int)j, (int)k)]; },
-        [&](size_t i, size_t j, size_t k) { return (float)G.ov[fvidx(G, (int)i, (int)j, (int)k)]; },
-        [&](size_t i, size_t j, size_t k) { return (float)G.ow[fwidx(G, (int)i, (int)j, (int)k)]; });
+        L, f,
+        [&](size_t i, size_t j, size_t k) {
+          const size_t q = fuidx(G, (int)i, (int)j, (int)k);
+          return aperture(G, 0, q, G.ou[q]);
+        },
+        [&](size_t i, size_t j, size_t k) {
+          const size_t q = fvidx(G, (int)i, (int)j, (int)k);
+          return aperture(G, 1, q, G.ov[q]);
+        },
+        [&](size_t i, size_t j, size_t k) {
+          const size_t q = fwidx(G, (int)i, (int)j, (int)k);
+          return aperture(G, 2, q, G.ow[q]);
+        });
   }
 }
 // Coarse face f of L from the faces of the finer level F.
@@ -927,8 +965,8 @@ __device__ __forceinline__ void cdd_fine_cell(const Fine& G, const Level& L, siz
     float g = 0.f;
     if ((m & M_WET) && (m & M_GHOST)) {
       uint8_t type[6];
-      float it[6];
-      ghost_row(G, fi, fj, fk, m, type, it);
+      float it[6], a[6];
+      ghost_row(G, fi, fj, fk, m, type, it, a);
       for (int f = 0; f < 6; ++f)
         if (type[f] == 2) g += it[f];
     }
@@ -1176,7 +1214,15 @@ __global__ void __launch_bounds__(BLOCK) k_divmetric(Fine G, const float* u, con
     decode(G, c, i, j, k);
     const double d = (double)fabsf(divergence_at(G, u, v, w, i, j, k, c));
     all = fmax(all, d);
-    if (!(m & M_GHOST)) plain = fmax(plain, d);
+    // plain: no ghost face (a row flagged for a fractional aperture alone is plain)
+    bool plain_row = !(m & M_GHOST);
+    if (!plain_row && G.frac) {
+      RowG R;
+      full_row(G, i, j, k, R);
+      plain_row = true;
+      for (int f = 0; f < 6; ++f) plain_row = plain_row && R.type[f] != 2;
+    }
+    if (plain_row) plain = fmax(plain, d);
   }
   reduce_finish<true>(all, plain, OP_DIVMAX, ctl, partials);
 }
@@ -1216,11 +1262,11 @@ __global__ void __launch_bounds__(BLOCK) k_backward(Fine G, const double* p, con
         if (m & (1u << f)) acc += pc + fabs(pn[f]);
     } else {
       uint8_t type[6];
-      float it[6];
-      ghost_row(G, i, j, k, m, type, it);
+      float it[6], a[6];
+      ghost_row(G, i, j, k, m, type, it, a);
       for (int f = 0; f < 6; ++f) {
-        if (type[f] == 1) acc += pc + fabs(pn[f]);
-        else if (type[f] == 2) acc += pc * (double)it[f];
+        if (type[f] == 1) acc += (double)a[f] * (pc + fabs(pn[f]));
+        else if (type[f] == 2) acc += (double)a[f] * pc * (double)it[f];
       }
     }
     const double scale = fabs((double)bc) + acc * (double)G.inv;

@@ -65,13 +65,13 @@ __device__ __forceinline__ float ghost_row_sum(const Fine& G, int i, int j, int 
                                             float& prec) {
 #pragma clang fp contract(off)
   uint8_t type[6];
-  float it[6];
-  ghost_row(G, i, j, k, m, type, it);
+  float it[6], a[6];
+  ghost_row(G, i, j, k, m, type, it, a);
   float acc = 0.f, diag = 0.f;
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
-    if (type[f] == 1) { acc += pc - nb[f]; diag += 1.0f; }
-    else if (type[f] == 2) { acc += pc * it[f]; diag += it[f]; }
+    if (type[f] == 1) { acc += a[f] * (pc - nb[f]); diag += a[f]; }
+    else if (type[f] == 2) { acc += a[f] * pc * it[f]; diag += a[f] * it[f]; }
   }
   prec = diag > 0.f ? 1.0f / (diag * G.inv) : 0.f;
   return acc;
@@ -593,12 +593,12 @@ __global__ void __launch_bounds__(QBLOCK) k_residual_q(Fine G, QuadGrid Q, const
             if (m & (1u << f)) acc += pe - nb[f];
         } else {
           uint8_t type[6];
-          float it[6];
-          ghost_row(G, 4 * qx + e, y, k, m, type, it);
+          float it[6], a[6];
+          ghost_row(G, 4 * qx + e, y, k, m, type, it, a);
 #pragma unroll
           for (int f = 0; f < 6; ++f) {
-            if (type[f] == 1) acc += pe - nb[f];
-            else if (type[f] == 2) acc += pe * (double)it[f];
+            if (type[f] == 1) acc += (double)a[f] * (pe - nb[f]);
+            else if (type[f] == 2) acc += (double)a[f] * pe * (double)it[f];
           }
         }
         acc = acc * (double)G.inv;
@@ -652,9 +652,9 @@ __global__ void __launch_bounds__(QBLOCK) k_assemble_q(Fine G, QuadGrid Q, const
         } else {
           float fl = u5[e], fr = u5[e + 1], ff = q_get(v0, e), fk = q_get(v1, e), fb = q_get(w0, e), ft = q_get(w1, e);
           if (G.cut) {
-            fl = (float)G.ou[fu + e] * fl; fr = (float)G.ou[fu + e + 1] * fr;
-            ff = (float)G.ov[fv0 + e] * ff; fk = (float)G.ov[fv1 + e] * fk;
-            fb = (float)G.ow[fw0 + e] * fb; ft = (float)G.ow[fw1 + e] * ft;
+            fl = aperture(G, 0, fu + e, G.ou[fu + e]) * fl; fr = aperture(G, 0, fu + e + 1, G.ou[fu + e + 1]) * fr;
+            ff = aperture(G, 1, fv0 + e, G.ov[fv0 + e]) * ff; fk = aperture(G, 1, fv1 + e, G.ov[fv1 + e]) * fk;
+            fb = aperture(G, 2, fw0 + e, G.ow[fw0 + e]) * fb; ft = aperture(G, 2, fw1 + e, G.ow[fw1 + e]) * ft;
           }
           div = ((fr - fl + ft - fb) + (fk - ff)) / G.dx;
         }
@@ -666,10 +666,10 @@ __global__ void __launch_bounds__(QBLOCK) k_assemble_q(Fine G, QuadGrid Q, const
           RowG R;
           full_row(G, i0 + e, j, k, R);
           for (int f = 0; f < 6; ++f) {
-            if (R.type[f] == 1) diag += 1.0f;
+            if (R.type[f] == 1) diag += R.a[f];
             else if (R.type[f] == 2) {
-              diag += R.it[f];
-              bb += R.val[f] * R.it[f] * G.inv;
+              diag += R.a[f] * R.it[f];
+              bb += R.a[f] * R.val[f] * R.it[f] * G.inv;
             }
           }
         }
@@ -830,15 +830,23 @@ __global__ void __launch_bounds__(QBLOCK) k_divmetric_q(Fine G, QuadGrid Q, cons
         if (!(G.cut && G.solid[c0 + e])) {
           float fl = u5[e], fr = u5[e + 1], ff = q_get(v0, e), fk = q_get(v1, e), fb = q_get(w0, e), ft = q_get(w1, e);
           if (G.cut) {
-            fl = (float)G.ou[fu + e] * fl; fr = (float)G.ou[fu + e + 1] * fr;
-            ff = (float)G.ov[fv0 + e] * ff; fk = (float)G.ov[fv1 + e] * fk;
-            fb = (float)G.ow[fw0 + e] * fb; ft = (float)G.ow[fw1 + e] * ft;
+            fl = aperture(G, 0, fu + e, G.ou[fu + e]) * fl; fr = aperture(G, 0, fu + e + 1, G.ou[fu + e + 1]) * fr;
+            ff = aperture(G, 1, fv0 + e, G.ov[fv0 + e]) * ff; fk = aperture(G, 1, fv1 + e, G.ov[fv1 + e]) * fk;
+            fb = aperture(G, 2, fw0 + e, G.ow[fw0 + e]) * fb; ft = aperture(G, 2, fw1 + e, G.ow[fw1 + e]) * ft;
           }
           div = ((fr - fl + ft - fb) + (fk - ff)) / G.dx;
         }
         const double d = (double)fabsf(div);
         all = fmax(all, d);
-        if (!(m & M_GHOST)) plain = fmax(plain, d);
+        // plain: no ghost face (a row flagged for a fractional aperture alone is plain)
+        bool plain_row = !(m & M_GHOST);
+        if (!plain_row && G.frac) {
+          RowG R;
+          full_row(G, i0 + e, j, k, R);
+          plain_row = true;
+          for (int f = 0; f < 6; ++f) plain_row = plain_row && R.type[f] != 2;
+        }
+        if (plain_row) plain = fmax(plain, d);
       }
     }
   }
@@ -882,11 +890,11 @@ __global__ void __launch_bounds__(QBLOCK) k_backward_q(Fine G, QuadGrid Q, const
           if (m & (1u << f)) acc += pe + fabs(pn[f]);
       } else {
         uint8_t type[6];
-        float it[6];
-        ghost_row(G, 4 * qx + e, y, k, m, type, it);
+        float it[6], a[6];
+        ghost_row(G, 4 * qx + e, y, k, m, type, it, a);
         for (int f = 0; f < 6; ++f) {
-          if (type[f] == 1) acc += pe + fabs(pn[f]);
-          else if (type[f] == 2) acc += pe * (double)it[f];
+          if (type[f] == 1) acc += (double)a[f] * (pe + fabs(pn[f]));
+          else if (type[f] == 2) acc += (double)a[f] * pe * (double)it[f];
         }
       }
       const double scale = fabs((double)q_get(bq, e)) + acc * (double)G.inv;

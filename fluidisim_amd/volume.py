@@ -182,7 +182,7 @@ class Volume3:
         self._dirty = True
 
     def set_solid(self, solid=None, open_u=None, open_v=None, open_w=None):
-        """Solid cells (the reference's cut geometry with frac == 0) and optional apertures in {0, 1}."""
+        """Solid cells (the reference's cut geometry with frac == 0) and optional face apertures in [0, 1]."""
         self._solid = None if solid is None else np.ascontiguousarray(solid, np.uint8)
         self._open = tuple(None if a is None else np.ascontiguousarray(a, np.float32) for a in (open_u, open_v, open_w))
         self._dirty = True

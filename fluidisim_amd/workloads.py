@@ -44,6 +44,10 @@ class Workload:
     rest: float = 0.0
     options: dict = field(default_factory=dict)
     seed: int = 0
+    # face apertures in [0, 1] (the reference's cut geometry), or None: derived
+    open_u: np.ndarray | None = None
+    open_v: np.ndarray | None = None
+    open_w: np.ndarray | None = None
 
     def describe(self):
         d = self.domain
@@ -78,7 +82,7 @@ def build_volume(wl: Workload, **option_overrides):
     opts.update(option_overrides)
     vol = Volume3.configure(wl.domain, rho=wl.rho, g_eff=wl.g_eff, **opts)
     vol.set_surface_mode(wl.homogeneous_ghost, wl.closed_lid)
-    vol.set_solid(wl.solid)
+    vol.set_solid(wl.solid, wl.open_u, wl.open_v, wl.open_w)
     vol.set_free_surface(wl.eta, wl.rest)
     vol.set_velocity(wl.us, wl.vs, wl.ws)
     return vol

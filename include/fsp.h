@@ -119,11 +119,16 @@ void fsp_destroy(fsp_solver* s);
 int32_t fsp_levels(const fsp_solver* s);
 
 /* Replaces set_free_surface and the cut geometry (delta3d_mobile.rs:9-25,
- * delta3d.rs:300-319). Host pointers. eta: nx*ny column heights (required);
- * eta_roundoff: nx*ny or NULL; solid: cells, nonzero = solid, or NULL;
- * open_u/v/w: face apertures or NULL (derived: a face touching a solid cell or
- * the box walls is closed, the lid open). Only apertures 0 and 1 are
- * supported (else FSP_DOMAIN). */
+ * delta3d.rs:300-319, delta3d_cut.rs:172-258). Host pointers. eta: nx*ny
+ * column heights (required); eta_roundoff: nx*ny or NULL; solid: cells,
+ * nonzero = solid (the cut's zero volume fraction), or NULL; open_u/v/w: face
+ * apertures in [0, 1] (the cut's, weighting the operator and the divergence
+ * as apply_cut/divergence_cut and mobile_row do, delta3d.rs:769-855,
+ * delta3d_mobile.rs:152-188), or NULL (derived: a face touching a solid cell
+ * or the box walls is closed, the lid open). Fractional apertures need the
+ * re-discretised coarse operator (FSP_COARSE_GALERKIN returns FSP_DOMAIN) and
+ * are not taken by the surface-step entry points (FSP_DOMAIN); this is a
+ * configuration call and may reserve their device copy. */
 fsp_status fsp_set_geometry(fsp_solver* s, const fsp_surface* surface,
                             const float* eta, size_t n_eta,
                             const float* eta_roundoff,

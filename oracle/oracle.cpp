@@ -1,6 +1,7 @@
 // CPU ORACLE — test infrastructure only (see oracle.hpp). C entry points for
 // ctypes: tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
 #include "oracle.hpp"
+#include "cut.hpp"
 
 #include <algorithm>
 
@@ -134,6 +135,21 @@ Options no_multigrid() {
 }  // namespace
 
 extern "C" {
+
+// The cut of a bottom height field (delta3d_cut.rs:172-258): frac (cells),
+// open_u, open_v, open_w (faces) and floor (columns) from the nx*ny bottom
+// heights at the column centres.
+int orc_cut(int nx, int ny, int nz, float dx, const float* bottom, float* frac, float* open_u, float* open_v,
+            float* open_w, float* floor) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || !(dx > 0.f) || !bottom) return ERR_DOMAIN;
+  const cut::Cut c = cut::cut(nx, ny, nz, dx, bottom);
+  std::copy(c.frac.begin(), c.frac.end(), frac);
+  std::copy(c.open_u.begin(), c.open_u.end(), open_u);
+  std::copy(c.open_v.begin(), c.open_v.end(), open_v);
+  std::copy(c.open_w.begin(), c.open_w.end(), open_w);
+  std::copy(c.floor.begin(), c.floor.end(), floor);
+  return OK;
+}
 
 // Surface steps (delta3d_mobile.rs:819-927), `steps` of them in a row; u, v,
 // w, p, eta, eta_roundoff in and out (g's eta pointers are ignored). Returns

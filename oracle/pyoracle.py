@@ -65,6 +65,7 @@ def lib():
         vp, f32, u32, i32, sz = C.c_void_p, C.c_float, C.c_uint32, C.c_int, C.c_size_t
         L.orc_project.argtypes = [i32, vp, vp, vp, vp, vp, vp, f32, f32, u32, vp, vp, vp, vp, vp, vp]
         L.orc_apply.argtypes = [vp, vp, vp]
+        L.orc_cut.argtypes = [i32, i32, i32, f32, vp, vp, vp, vp, vp, vp]
         L.orc_round_storage.argtypes = [i32, vp, C.c_size_t]
         L.orc_round_storage.restype = None
         L.orc_apply64.argtypes = [vp, vp, vp]
@@ -320,10 +321,58 @@ def _sinf(x):
     return _libm.sinf(x)
 
 
-def s385_problem(nx: int) -> tuple[Problem, float, float]:
+def cut_geometry(nx: int, ny: int, nz: int, dx: float, bottom) -> dict:
+    """The reference's cut of a bottom height field given at the column
+    centres (delta3d_cut.rs:172-258): volume fractions, the three families
+    of face apertures and the column floors, f32."""
+    b = np.ascontiguousarray(bottom, np.float32)
+    assert b.size == nx * ny
+    out = {"frac": np.zeros(nx * ny * nz, np.float32), "open_u": np.zeros((nx + 1) * ny * nz, np.float32),
+           "open_v": np.zeros(nx * (ny + 1) * nz, np.float32), "open_w": np.zeros(nx * ny * (nz + 1), np.float32),
+           "floor": np.zeros(nx * ny, np.float32)}
+    st = lib().orc_cut(nx, ny, nz, float(np.float32(dx)), b.ctypes.data, *(out[k].ctypes.data for k in
+                                                                             ("frac", "open_u", "open_v", "open_w",
+                                                                              "floor")))
+    assert st == 0
+    return out
+
+
+_expf = None
+
+
+def _c_expf(x: float) -> np.float32:
+    """The C library's single-precision exp, which Rust's f32::exp calls."""
+    global _expf
+    if _expf is None:
+        m = C.CDLL("libm.so.6")
+        m.expf.restype, m.expf.argtypes = C.c_float, [C.c_float]
+        _expf = m.expf
+    return np.float32(_expf(float(x)))
+
+
+def s385_bump(nx: int, dx) -> np.ndarray:
+    """The bump of S324 off centre in y (examples/delta3d_multigrille.rs:43-47):
+    0.4 + 0.6 exp(-(((x - 3)/1.2)^2 + ((y - 1.3)/0.9)^2)) at the column
+    centres, f32 (exp the C library's expf)."""
+    ny = nx // 2
+    f32 = np.float32
+    dx = f32(dx)
+    out = np.empty(nx * ny, f32)
+    for j in range(ny):
+        y = (f32(j) + f32(0.5)) * dx
+        for i in range(nx):
+            x = (f32(i) + f32(0.5)) * dx
+            a, b = (x - f32(3.0)) / f32(1.2), (y - f32(1.3)) / f32(0.9)
+            out[j * nx + i] = f32(0.4) + f32(0.6) * _c_expf(-(a * a + b * b))
+    return out
+
+
+def s385_problem(nx: int, bump: bool = False) -> tuple[Problem, float, float]:
     """The reference's S385 bench case (examples/delta3d_multigrille.rs:28-36,
-    58-79): an 8 x 4 x 6 m tank, flat bottom, surface 4 m plus a 1 cm sine
-    along x, rho = 1025, g = 9.81, dt = 2 ms. Returns (problem, scale, k1) of
+    58-79): an 8 x 4 x 6 m tank, surface 4 m plus a 1 cm sine along x,
+    rho = 1025, g = 9.81, dt = 2 ms; the bottom flat, or (bump) the S324 bump
+    cut into the grid (configure_with_bottom: solid where the volume fraction
+    is 0, the cut apertures on every face). Returns (problem, scale, k1) of
     the first mobile step from rest (delta3d_mobile.rs:836-837)."""
     ny, nz = nx // 2, 3 * nx // 4
     f32 = np.float32
@@ -336,6 +385,10 @@ def s385_problem(nx: int) -> tuple[Problem, float, float]:
     row = (f32(4.0) + f32(0.01) * s).astype(f32)
     eta = np.tile(row, ny).astype(f32)
     prob = Problem(nx, ny, nz, float(dx), eta, rho=1025.0, g_eff=9.81, rest=4.0, homogeneous_ghost=False)
+    if bump:
+        g = cut_geometry(nx, ny, nz, float(dx), s385_bump(nx, dx))
+        prob.solid = (g["frac"] == 0).astype(np.uint8)
+        prob.open_u, prob.open_v, prob.open_w = g["open_u"], g["open_v"], g["open_w"]
     dt = 2000 * 1e-6
     scale = float(np.float32(-1025.0 / dt))
     k1 = float(np.float32(dt / 1025.0))

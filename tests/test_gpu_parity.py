@@ -33,21 +33,46 @@ def noise(n, seed):
     return out
 
 
-def s385_workload(nx):
-    prob, scale, k1 = po.s385_problem(nx)
+def s385_workload(nx, bump=False):
+    """The reference's S385 case; bump: the S324 bump cut into the grid (the
+    oracle's restatement of the reference's cut gives the solids and the
+    fractional apertures the library receives)."""
+    prob, scale, k1 = po.s385_problem(nx, bump=bump)
     d = Domain3(prob.nx, prob.ny, prob.nz, prob.dx)
     nu, nv, nw = d.faces
     z = lambda n: np.zeros(n, np.float32)
-    return W.Workload(f"s385_{nx}", d, prob.eta, None, z(nu), z(nv), z(nw), homogeneous_ghost=False,
-                      closed_lid=False, scale=scale, k1=k1, rho=1025.0, g_eff=9.81, rest=4.0,
+    return W.Workload(f"s385{'_bump' if bump else ''}_{nx}", d, prob.eta, prob.solid, z(nu), z(nv), z(nw),
+                      homogeneous_ghost=False, closed_lid=False, scale=scale, k1=k1, rho=1025.0, g_eff=9.81,
+                      rest=4.0,
                       options=dict(preconditioner=_lib.PRECOND_MULTIGRID, smoother=_lib.SMOOTH_JACOBI,
-                                   max_levels=-1, pre_sweeps=2, post_sweeps=2, coarse_sweeps=8))
+                                   max_levels=-1, pre_sweeps=2, post_sweeps=2, coarse_sweeps=8),
+                      open_u=prob.open_u, open_v=prob.open_v, open_w=prob.open_w)
+
+
+def cut_workload(nx, depth, seed):
+    """The S385 tank with the S324 bump cut into it (apertures from the
+    oracle's restatement of the reference's cut) under a surface `depth` m
+    high plus a small sine, random predicted faces: rows with fractional
+    apertures, free-surface rows and both at once where the bump nears the
+    surface."""
+    wl = s385_workload(nx, bump=True)
+    d = wl.domain
+    f32 = np.float32
+    i = np.arange(d.nx, dtype=f32)
+    row = (f32(depth) + f32(0.01) * np.sin(f32(2.0 * np.pi) * (i + f32(0.5)) * f32(d.dx) / f32(8.0))).astype(f32)
+    wl.eta = np.tile(row, d.ny).astype(f32)
+    wl.rest = float(depth)
+    rng = np.random.default_rng(seed)
+    wl.us, wl.vs, wl.ws = (rng.uniform(-1e-2, 1e-2, n).astype(np.float32) for n in d.faces)
+    wl.name = f"bump_shallow_{nx}"
+    return wl
 
 
 def problem(wl):
     d = wl.domain
     return po.Problem(d.nx, d.ny, d.nz, d.dx, wl.eta, rho=wl.rho, g_eff=wl.g_eff, rest=wl.rest,
-                      homogeneous_ghost=wl.homogeneous_ghost, closed_lid=wl.closed_lid, solid=wl.solid)
+                      homogeneous_ghost=wl.homogeneous_ghost, closed_lid=wl.closed_lid, solid=wl.solid,
+                      open_u=wl.open_u, open_v=wl.open_v, open_w=wl.open_w)
 
 
 def oracle_options(wl, multigrid=1):
@@ -73,6 +98,10 @@ WORKLOADS = {
     "tank34_generic": lambda: W.tank(34, 32, 32, seed=4),
     # two compact coarse levels above the one-workgroup tail
     "tank128x64x96": lambda: W.tank(128, 64, 96, seed=6),
+    # fractional apertures: the S324 bump cut into the S385 tank, and the same
+    # bump under a shallow surface it pierces
+    "s385_bump32": lambda: s385_workload(32, bump=True),
+    "bump_shallow32": lambda: cut_workload(32, depth=0.9, seed=8),
 }
 
 
@@ -477,3 +506,97 @@ def test_fused_sweeps_equal_single_passes_bit_for_bit(shape, sweeps, storage):
     assert np.linalg.norm(ps[0] - ps[1]) <= 1e-6 * np.linalg.norm(ps[1])
     for v in vols:
         v.close()
+
+
+# ---------------------------------------------------------------- fractional apertures (cut geometry)
+@pytest.mark.parametrize("name", ["s385_bump32", "bump_shallow32"])
+def test_cut_divergence_is_the_reference_bit_for_bit(name):
+    """Aperture-weighted divergence (divergence_cut, delta3d.rs:806-823)."""
+    wl = WORKLOADS[name]()
+    vol = W.build_volume(wl)
+    rng = np.random.default_rng(3)
+    faces = [rng.uniform(-1, 1, n).astype(np.float32) for n in wl.domain.faces]
+    gpu = host(vol.divergence(*(torch.from_numpy(f).cuda() for f in faces)))
+    ref = po.divergence(problem(wl), *faces)
+    assert np.array_equal(bits(gpu), bits(ref))
+
+
+@pytest.mark.parametrize("name", ["s385_bump32", "bump_shallow32"])
+@pytest.mark.parametrize("smoother", [_lib.SMOOTH_JACOBI, _lib.SMOOTH_RBGS])
+def test_cut_vcycle_matches_the_oracle(name, smoother):
+    """The V-cycle over a cut geometry (fractional apertures on the fine
+    level, coarse apertures their quarter sums, delta3d_multigrid.rs:141-206)
+    is the oracle's bit for bit, with both smoothers."""
+    wl = WORKLOADS[name]()
+    wl.options["smoother"] = smoother
+    wl.options["coarse_operator"] = _lib.COARSE_REDISCRETISED
+    vol = W.build_volume(wl)
+    prob = problem(wl)
+    _, prec = po.assemble(prob, np.zeros(wl.domain.cells, np.float32), 1.0)
+    r = noise(wl.domain.cells, 21)
+    r[prec == 0] = 0.0
+    z = host(vol.precondition(torch.from_numpy(r).cuda()))
+    z_ref = po.vcycle(prob, r, oracle_options(wl)).astype(np.float32)
+    assert np.abs(z_ref).max() > 0
+    assert np.array_equal(bits(z), bits(z_ref)), f"max diff {np.abs(z - z_ref).max()}"
+
+
+def test_s385_bump_first_step_matches_the_reference():
+    """The reference's S385 bench over the S324 bump (MULTIGRILLE-3D-S385.md:59-60):
+    9 iterations with the multigrid V-cycle and 117 with Jacobi at nx = 32;
+    the oracle reproduces both exactly, the card within the f64 reductions."""
+    wl = s385_workload(32, bump=True)
+    vol, rep, (oref, p_ref, *_rest) = run_both(wl, precision=32)
+    print(f"[bump MG] card {rep.iterations} it, oracle {oref['iterations']}, divergence {rep.divergence_plain:.2e}")
+    assert oref["iterations"] == 9
+    assert abs(rep.iterations - 9) <= 1
+    assert not rep.degraded and rep.residual <= 1e-6 and rep.divergence_plain <= 1e-5
+    assert_pressure_parity(wl, host(vol.pressure()), p_ref)
+    vol.close()
+    wl.options["preconditioner"] = _lib.PRECOND_JACOBI
+    vol = W.build_volume(wl)
+    rep = vol.project_mobile3(wl.scale, wl.k1, 20000, warm=False)
+    oref, p_ref, *_ = po.project(problem(wl), wl.us, wl.vs, wl.ws, wl.scale, wl.k1, 20000,
+                                 po.options(multigrid=0), precision=32)
+    print(f"[bump Jacobi] card {rep.iterations} it, oracle {oref['iterations']}")
+    assert oref["iterations"] == 117
+    assert abs(rep.iterations - 117) <= 3
+    assert not rep.degraded
+    assert_pressure_parity(wl, host(vol.pressure()), p_ref)
+    vol.close()
+
+
+@pytest.mark.parametrize("smoother", [_lib.SMOOTH_JACOBI, _lib.SMOOTH_RBGS])
+def test_cut_projection_matches_the_f64_oracle(smoother):
+    """A moving field over the bump under a shallow surface: the projection
+    against the f64 oracle (pressure 1e-4 rel. L2, divergence 1e-5), and
+    the report's divergence metrics equal to the oracle's metric of the
+    returned faces."""
+    wl = cut_workload(32, depth=0.9, seed=8)
+    wl.options["smoother"] = smoother
+    vol, rep, (oref, p32, u, v, w, p64) = run_both(wl, precision=64)
+    print(f"[cut] {rep.iterations} it (oracle {oref['iterations']}), residual {rep.residual:.2e}, "
+          f"divergence {rep.divergence_plain:.2e}")
+    assert not rep.degraded and rep.residual <= 1e-6 and rep.divergence_plain <= 1e-5
+    assert_pressure_parity(wl, host(vol.pressure()), p64)
+    a, pl = po.divergence_metric(problem(wl), host(vol.velocity_u()), host(vol.velocity_v()), host(vol.velocity_w()))
+    assert rep.divergence == a and rep.divergence_plain == pl
+    vol.close()
+
+
+def test_fractional_apertures_are_refused_where_unsupported():
+    """The Galerkin coarse operator (unit couplings) and the surface step
+    (binary apertures) refuse a fractional geometry with FSP_DOMAIN."""
+    wl = s385_workload(32, bump=True)
+    wl.options.update(smoother=_lib.SMOOTH_RBGS, coarse_operator=_lib.COARSE_GALERKIN)
+    vol = W.build_volume(wl)
+    with pytest.raises(ProjectionError) as e:
+        vol.project_mobile3(wl.scale, wl.k1, 100, warm=False)
+    assert e.value.error == Error.Domain
+    vol.close()
+    wl.options["coarse_operator"] = _lib.COARSE_REDISCRETISED
+    vol = W.build_volume(wl)
+    with pytest.raises(ProjectionError) as e:
+        vol.step_surface_mobile(2000, 100)
+    assert e.value.error == Error.Domain
+    vol.close()

@@ -44,6 +44,72 @@ def test_s385_first_step_iterations_are_the_published_ones(nx, mg_its, jacobi_it
         assert rep["iterations"] == jacobi_its and not rep["degraded"]
 
 
+@pytest.mark.parametrize("nx,mg_its,jacobi_its", [(32, 9, 117), (64, 10, 219), (128, 11, 444)])
+def test_s385_bump_first_step_iterations_are_the_published_ones(nx, mg_its, jacobi_its):
+    """MULTIGRILLE-3D-S385.md:59-60: the same first step over the S324 bump
+    (0.4 + 0.6 exp(-(((x-3)/1.2)^2 + ((y-1.3)/0.9)^2)) m, examples/delta3d_multigrille.rs:43-47)
+    cut into the grid (configure_with_bottom -> cut, delta3d_cut.rs:172-258):
+    9/10/11 iterations with multigrid, 117/219/444 with Jacobi. Pins the
+    restated cut geometry and the aperture-weighted operator together."""
+    prob, scale, k1 = po.s385_problem(nx, bump=True)
+    assert prob.solid.any() and ((prob.open_u > 0) & (prob.open_u < 1)).any()
+    us, vs, ws = zeros_faces(prob)
+    rep, *_ = po.project(prob, us, vs, ws, scale, k1, 20000, po.options(multigrid=1))
+    assert rep["iterations"] == mg_its and not rep["degraded"]
+    rep, *_ = po.project(prob, us, vs, ws, scale, k1, 20000, po.options(multigrid=0))
+    assert rep["iterations"] == jacobi_its and not rep["degraded"]
+
+
+def test_cut_of_a_flat_bottom_is_the_closed_form():
+    """A flat bottom at h (delta3d_cut.rs:172-258 with the 2D formulas of
+    delta_projection.rs:795-827): frac and the side apertures are
+    clamp((top - h)/dx) per layer, a horizontal face is open iff h < z, the
+    walls and the domain's bottom are closed, and cells with no open face
+    are solid."""
+    nx, ny, nz, dx = 6, 5, 7, 0.5
+    h = np.float32(1.3)
+    g = po.cut_geometry(nx, ny, nz, dx, np.full(nx * ny, h, np.float32))
+    f32 = np.float32
+    for k in range(nz):
+        top = f32((k + 1) * dx)
+        want = np.clip((top - h) / f32(dx), f32(0), f32(1))
+        assert np.all(g["frac"].reshape(nz, ny, nx)[k] == want)
+        ou = g["open_u"].reshape(nz, ny, nx + 1)[k]
+        assert np.all(ou[:, 1:nx] == want) and np.all(ou[:, [0, nx]] == 0)
+        ov = g["open_v"].reshape(nz, ny + 1, nx)[k]
+        assert np.all(ov[1:ny, :] == want) and np.all(ov[[0, ny], :] == 0)
+    ow = g["open_w"].reshape(nz + 1, ny, nx)
+    assert np.all(ow[0] == 0)
+    for k in range(1, nz + 1):
+        assert np.all(ow[k] == (1.0 if h < f32(k * dx) else 0.0))
+    assert np.all(g["floor"] == h)
+
+
+@pytest.mark.parametrize("slope_x,slope_y", [(0.37, 0.0), (0.0, 0.41), (0.23, -0.31)])
+def test_cut_volume_is_the_exact_integral(slope_x, slope_y):
+    """The fluid volume of each inner column, sum_k frac * dx, is the domain
+    height minus the mean bottom over the footprint: for the corner model of
+    a linear bottom that mean is the mean of the four corners (the four
+    triangles around the centre integrate exactly), through the 2D formula
+    (no y dependence) and through the triangles (with one)."""
+    nx, ny, nz, dx = 8, 7, 9, 0.25
+    f32 = np.float32
+    x = (np.arange(nx) + 0.5) * dx
+    y = (np.arange(ny) + 0.5) * dx
+    bottom = (0.6 + slope_x * x[None, :] + slope_y * y[:, None]).astype(f32).ravel()
+    g = po.cut_geometry(nx, ny, nz, dx, bottom)
+    frac = g["frac"].reshape(nz, ny, nx).astype(np.float64)
+    col = frac.sum(axis=0) * dx
+    b2 = bottom.reshape(ny, nx).astype(np.float64)
+    for j in range(1, ny - 1):
+        for i in range(1, nx - 1):
+            # corners: mean of the four surrounding column values (interior)
+            cs = [0.25 * (b2[jj, ii] + b2[jj, ii - 1] + b2[jj - 1, ii] + b2[jj - 1, ii - 1])
+                  for jj, ii in ((j, i), (j, i + 1), (j + 1, i + 1), (j + 1, i))]
+            want = nz * dx - np.mean(cs)
+            assert abs(col[j, i] - want) <= 1e-5, (i, j, col[j, i], want)
+
+
 def test_s385_plain_divergence_within_published_bound():
     """MULTIGRILLE-3D-S385.md:76: plain-row divergence <= 5.9e-6 with multigrid."""
     for nx in (32, 64):
