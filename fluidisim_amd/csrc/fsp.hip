@@ -49,6 +49,12 @@ struct fsp_solver {
   // fsp_set_geometry that has one
   float *au = nullptr, *av = nullptr, *aw = nullptr;
   int frac = 0;
+  // level-set geometry (fsp_set_level_set): water labels and phi, reserved
+  // by its first call
+  uint8_t* water = nullptr;
+  float* phi = nullptr;
+  float theta_min = 0.f;
+  int levelset = 0;
   float *eta = nullptr, *eta_rt = nullptr;
   float* col_floor = nullptr;  // per column: top of its highest solid cell (0 without one)
   uint16_t* gk0 = nullptr;  // per column: lowest ghost row
@@ -106,6 +112,9 @@ struct fsp_solver {
     G.solid = solid; G.ou = ou; G.ov = ov; G.ow = ow;
     G.frac = frac;
     G.au = frac ? au : nullptr; G.av = frac ? av : nullptr; G.aw = frac ? aw : nullptr;
+    G.water = levelset ? water : nullptr;
+    G.phi = levelset ? phi : nullptr;
+    G.theta_min = theta_min;
     G.eta = eta; G.eta_rt = eta_rt;
     G.mask = mask;
     G.gk0 = gk0;
@@ -269,6 +278,13 @@ struct Launch {
 
   void mask() {
     const dim3 g(grid_for(G.n));
+    if (S->levelset) {
+      // labels and level set: no column structure, no ghost-row table
+      FSP_LAUNCH(S, k_mask_ls, g, st, G, S->mask, S->ctl, S->partials);
+      FSP_FINISH(S, st, OP_COUNT, g, false, 0);
+      if (S->precv) FSP_LAUNCH(S, k_prec, g, st, G, S->precv);
+      return;
+    }
     if (S->quad) {
       const dim3 gq((unsigned)(S->qfine.nbx * S->qfine.nby * S->qfine.planes));
       FSP_LAUNCH(S, k_mask_q, gq, st, G, S->qfine, S->mask, S->ctl, S->partials);
@@ -1496,11 +1512,74 @@ fsp_status fsp_set_geometry(fsp_solver* S, const fsp_surface* surf, const float*
   FSP_HIP(hipStreamSynchronize(S->stream));
   S->surf = *surf;
   S->cut = cut;
-  const bool recapture = S->frac != frac;
+  const bool recapture = S->frac != frac || S->levelset;
   S->frac = frac;
+  S->levelset = 0;
   S->geom = true;
   // the iteration graphs hold the fine geometry's kernel arguments: capture
   // them again when fractional apertures come or go
+  if (recapture) {
+    for (int gi = 0; gi < 2; ++gi) drop_graph(S, gi);
+    fsp_status gst = prepare_graph(S, 0);
+    if (gst == FSP_OK && S->net && S->opt.preconditioner == FSP_PRECOND_LEARNED) gst = prepare_graph(S, 1);
+    if (gst) return gst;
+  }
+  return FSP_OK;
+}
+
+fsp_status fsp_set_level_set(fsp_solver* S, const uint8_t* labels, size_t n_labels, const float* phi, size_t n_phi,
+                             float theta_min) {
+  if (!S || !labels || !phi) return FSP_INVALID;
+  if (n_labels != S->n || n_phi != S->n) return FSP_SHAPE;
+  if (!std::isfinite(theta_min)) return FSP_NOT_FINITE;
+  if (!(theta_min > 0.f && theta_min <= 1.f)) return FSP_DOMAIN;
+  if (!all_finite(phi, S->n)) return FSP_NOT_FINITE;
+  for (size_t q = 0; q < S->n; ++q)
+    if (labels[q] > FSP_LABEL_SOLID) return FSP_DOMAIN;
+  const size_t nx = S->dom.nx, ny = S->dom.ny, nz = S->dom.nz;
+  std::vector<uint8_t> hs(S->n), hwat(S->n), hu(S->nu), hv(S->nv), hw(S->nw);
+  for (size_t q = 0; q < S->n; ++q) {
+    hs[q] = labels[q] == FSP_LABEL_SOLID;
+    hwat[q] = labels[q] == FSP_LABEL_WATER;
+  }
+  // the domain's boundary is a wall (apic3d.rs:786-806); every inner face
+  // open: a face next to a solid carries the body's velocity into the
+  // divergence and is skipped by the operator and the correction
+  for (size_t k = 0; k < nz; ++k)
+    for (size_t j = 0; j < ny; ++j)
+      for (size_t i = 0; i <= nx; ++i) hu[(k * ny + j) * (nx + 1) + i] = (i == 0 || i == nx) ? 0 : 1;
+  for (size_t k = 0; k < nz; ++k)
+    for (size_t j = 0; j <= ny; ++j)
+      for (size_t i = 0; i < nx; ++i) hv[(k * (ny + 1) + j) * nx + i] = (j == 0 || j == ny) ? 0 : 1;
+  for (size_t k = 0; k <= nz; ++k)
+    for (size_t j = 0; j < ny; ++j)
+      for (size_t i = 0; i < nx; ++i) hw[(k * ny + j) * nx + i] = (k == 0 || k == nz) ? 0 : 1;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  if (!S->water) {
+    fsp_status as;
+    if ((as = dalloc(S, &S->water, S->n)) || (as = dalloc(S, &S->phi, S->n))) return as;
+  }
+  FSP_HIP(hipMemcpyAsync(S->solid, hs.data(), S->n, hipMemcpyHostToDevice, S->stream));
+  FSP_HIP(hipMemcpyAsync(S->water, hwat.data(), S->n, hipMemcpyHostToDevice, S->stream));
+  FSP_HIP(hipMemcpyAsync(S->phi, phi, S->n * sizeof(float), hipMemcpyHostToDevice, S->stream));
+  FSP_HIP(hipMemcpyAsync(S->ou, hu.data(), S->nu, hipMemcpyHostToDevice, S->stream));
+  FSP_HIP(hipMemcpyAsync(S->ov, hv.data(), S->nv, hipMemcpyHostToDevice, S->stream));
+  FSP_HIP(hipMemcpyAsync(S->ow, hw.data(), S->nw, hipMemcpyHostToDevice, S->stream));
+  FSP_HIP(hipMemsetAsync(S->eta, 0, S->ncols * sizeof(float), S->stream));
+  FSP_HIP(hipMemsetAsync(S->eta_rt, 0, S->ncols * sizeof(float), S->stream));
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  fsp_surface surf{};
+  surf.rho = S->surf.rho > 0.f ? S->surf.rho : 1.f;
+  surf.homogeneous_ghost = 1;  // p = 0 on the zero level
+  surf.closed_lid = 1;
+  S->surf = surf;
+  S->cut = 1;
+  const bool recapture = !S->levelset || S->frac || S->theta_min != theta_min;
+  S->frac = 0;
+  S->levelset = 1;
+  S->theta_min = theta_min;
+  S->geom = true;
+  // the iteration graphs hold the fine geometry's kernel arguments
   if (recapture) {
     for (int gi = 0; gi < 2; ++gi) drop_graph(S, gi);
     fsp_status gst = prepare_graph(S, 0);
@@ -1724,7 +1803,7 @@ fsp_status fsp_step_surface_mobile_device(fsp_solver* S, float* d_u, float* d_v,
                                           fsp_balance* balance) {
   if (!S || !d_u || !d_v || !d_w || !d_p) return FSP_INVALID;
   if (!S->geom) return FSP_DOMAIN;
-  if (S->frac) return FSP_DOMAIN;  // the surface step is built for binary apertures
+  if (S->frac || S->levelset) return FSP_DOMAIN;  // the surface step is built for column geometries
   if (duration_us == 0 || duration_us > (1ull << 53)) return FSP_NOT_FINITE;
   const double dt = (double)duration_us * 1e-6;
   const float dx = S->dom.dx, rho = S->surf.rho, g = S->surf.g_eff;
@@ -1796,7 +1875,7 @@ fsp_status fsp_advect_device(fsp_solver* S, const float* d_u, const float* d_v, 
                              float* d_us, float* d_vs, float* d_ws) {
   if (!S || !d_u || !d_v || !d_w || !d_us || !d_vs || !d_ws) return FSP_INVALID;
   if (!S->geom) return FSP_DOMAIN;
-  if (S->frac) return FSP_DOMAIN;  // the surface step is built for binary apertures
+  if (S->frac || S->levelset) return FSP_DOMAIN;  // the surface step is built for column geometries
   if (!std::isfinite(dt)) return FSP_NOT_FINITE;
   FSP_HIP(hipSetDevice(S->opt.device));
   fsp_status st = advect(S, d_u, d_v, d_w, dt, d_us, d_vs, d_ws);
@@ -1808,7 +1887,7 @@ fsp_status fsp_advect_device(fsp_solver* S, const float* d_u, const float* d_v, 
 fsp_status fsp_extrapolate_device(fsp_solver* S, float* d_u, float* d_v, float* d_w) {
   if (!S || !d_u || !d_v || !d_w) return FSP_INVALID;
   if (!S->geom) return FSP_DOMAIN;
-  if (S->frac) return FSP_DOMAIN;  // the surface step is built for binary apertures
+  if (S->frac || S->levelset) return FSP_DOMAIN;  // the surface step is built for column geometries
   FSP_HIP(hipSetDevice(S->opt.device));
   fsp_status st = extrapolate(S, d_u, d_v, d_w);
   if (st) return st;
@@ -1820,7 +1899,7 @@ fsp_status fsp_transport_device(fsp_solver* S, const float* d_u, const float* d_
                                 float* d_flux_y) {
   if (!S || !d_u || !d_v) return FSP_INVALID;
   if (!S->geom) return FSP_DOMAIN;
-  if (S->frac) return FSP_DOMAIN;  // the surface step is built for binary apertures
+  if (S->frac || S->levelset) return FSP_DOMAIN;  // the surface step is built for column geometries
   if (!std::isfinite(factor)) return FSP_NOT_FINITE;
   FSP_HIP(hipSetDevice(S->opt.device));
   fsp_status st = step_reserve(S);

@@ -86,6 +86,14 @@ struct Fine {
   const float* av;
   const float* aw;
   int frac;
+  // Level-set geometry (the APIC-3D projection): cells labelled water, air
+  // or solid, wetness from the labels and the ghost fraction of an air face
+  // theta = max(phi_c / (phi_c - phi_a), theta_min) from the level set phi at
+  // the cell centres (p = 0 on the zero level). phi == null: the mobile
+  // geometry (wetness and ghost fractions from the column heights).
+  const uint8_t* water;  // cells: 1 water
+  const float* phi;      // cells
+  float theta_min;
   float prec_tab[7];     // 1/(n * inv) for a plain row with n faces
 };
 

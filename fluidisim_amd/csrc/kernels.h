@@ -218,14 +218,26 @@ __device__ __forceinline__ float zc_of(const Fine& G, int k) {
 #pragma clang fp contract(off)
   return ((float)k + 0.5f) * G.dx;
 }
-// delta3d_mobile.rs:97-103
+// delta3d_mobile.rs:97-103; with a level set, the water label
+// (apic3d.rs:688-692).
 __device__ __forceinline__ bool wet_at(const Fine& G, int i, int j, int k) {
+  if (G.phi) return G.water[cidx(G, i, j, k)] != 0;
   if (solid_at(G, i, j, k)) return false;
   return zc_of(G, k) < G.eta[(size_t)j * G.nx + i];
+}
+// The ghost fraction between water cell (i, j, k) and air cell (x, y, z)
+// from the level set (apic3d.rs:902-907): 1/theta, value 0.
+__device__ __forceinline__ void ghost_phi(const Fine& G, int i, int j, int k, int x, int y, int z, float& it,
+                                          float& val) {
+#pragma clang fp contract(off)
+  const float fc = G.phi[cidx(G, i, j, k)], fa = G.phi[cidx(G, x, y, z)];
+  it = 1.0f / fmaxf(fc / (fc - fa), G.theta_min);
+  val = 0.f;
 }
 // delta3d_mobile.rs:126-134
 __device__ __forceinline__ void ghost_up(const Fine& G, int i, int j, int k, float& it, float& val) {
 #pragma clang fp contract(off)
+  if (G.phi) { ghost_phi(G, i, j, k, i, j, k + 1, it, val); return; }
   const size_t col = (size_t)j * G.nx + i;
   const float h = G.eta[col];
   const float theta = fmaxf((h - zc_of(G, k)) / G.dx, 1e-3f);
@@ -236,6 +248,7 @@ __device__ __forceinline__ void ghost_up(const Fine& G, int i, int j, int k, flo
 // delta3d_mobile.rs:136-147
 __device__ __forceinline__ void ghost_side(const Fine& G, int i, int j, int k, int x, int y, float& it, float& val) {
 #pragma clang fp contract(off)
+  if (G.phi) { ghost_phi(G, i, j, k, x, y, k, it, val); return; }
   const float zc = zc_of(G, k);
   const float h = G.eta[(size_t)j * G.nx + i];
   const float theta = fmaxf((h - zc) / (h - G.eta[(size_t)y * G.nx + x]), 1e-3f);
@@ -284,10 +297,16 @@ __device__ void full_row(const Fine& G, int i, int j, int k, RowG& R) {
   }
   const size_t fd = fwidx(G, i, j, k), fu = fwidx(G, i, j, k + 1);
   if (k > 0 && G.ow[fd] && !solid_at(G, i, j, k - 1)) {
-    R.type[4] = 1;
     R.a[4] = aperture(G, 2, fd, G.ow[fd]);
+    if (!G.phi || wet_at(G, i, j, k - 1)) {
+      R.type[4] = 1;
+    } else {
+      // a level set can put air below water
+      R.type[4] = 2;
+      ghost_phi(G, i, j, k, i, j, k - 1, R.it[4], R.val[4]);
+    }
   }
-  if (G.ow[fu]) {
+  if (G.ow[fu] && !(G.phi && k + 1 < G.nz && solid_at(G, i, j, k + 1))) {
     R.a[5] = aperture(G, 2, fu, G.ow[fu]);
     if (k + 1 < G.nz && wet_at(G, i, j, k + 1)) {
       R.type[5] = 1;
@@ -322,7 +341,7 @@ __device__ __forceinline__ bool ghost_its(const Fine& G, int i, int j, int k, fl
 // geometry (the ghost table holds binary rows only).
 __device__ __forceinline__ void ghost_row(const Fine& G, int i, int j, int k, uint8_t m, uint8_t type[6], float it[6],
                                           float a[6]) {
-  if (!G.frac && ghost_its(G, i, j, k, it)) {
+  if (!G.frac && !G.phi && ghost_its(G, i, j, k, it)) {
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
       type[f] = (m & (1u << f)) ? 1 : (it[f] > 0.f ? 2 : 0);
@@ -495,6 +514,36 @@ __global__ void __launch_bounds__(BLOCK) k_mask(Fine G, uint8_t* mask, Ctl* ctl,
     const uint8_t ap[6] = {a0, a1, a2, a3, a4, a5}, sn[6] = {s0, s1, s2, s3, s4, s5};
     const float hn[4] = {h0, h1, h2, h3};
     mask[c] = row_code(G, i, j, k, s_c, sn, ap, h, hn, wet, ghosts);
+  }
+  reduce_finish<false>(wet, ghosts, OP_COUNT, ctl, partials);
+}
+
+// Row codes of a level-set geometry (the APIC-3D projection's labels,
+// apic3d.rs:912-935): bit f for a water neighbour across an open face f,
+// M_GHOST for an air neighbour (either side, below or above); solid
+// neighbours and closed faces add nothing. Counts wet rows and ghost faces.
+__global__ void __launch_bounds__(BLOCK) k_mask_ls(Fine G, uint8_t* mask, Ctl* ctl, double* partials) {
+  double wet = 0.0, ghosts = 0.0;
+  const int nx = G.nx, ny = G.ny, nz = G.nz;
+  FSP_FOR(c, G.n) {
+    int i, j, k;
+    decode(G, c, i, j, k);
+    uint8_t m = 0;
+    if (G.water[c]) {
+      m = M_WET;
+      const bool has[6] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny, k > 0, k + 1 < nz};
+      const size_t nb[6] = {c - 1, c + 1, c - nx, c + nx, c - (size_t)nx * ny, c + (size_t)nx * ny};
+      const uint8_t ap[6] = {G.ou[fuidx(G, i, j, k)], G.ou[fuidx(G, i + 1, j, k)], G.ov[fvidx(G, i, j, k)],
+                             G.ov[fvidx(G, i, j + 1, k)], G.ow[fwidx(G, i, j, k)], G.ow[fwidx(G, i, j, k + 1)]};
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        if (!has[f] || ap[f] == 0 || G.solid[nb[f]]) continue;
+        if (G.water[nb[f]]) m |= (uint8_t)(1u << f);
+        else { m |= M_GHOST; ghosts += 1.0; }
+      }
+      wet += 1.0;
+    }
+    mask[c] = m;
   }
   reduce_finish<false>(wet, ghosts, OP_COUNT, ctl, partials);
 }
@@ -1177,7 +1226,18 @@ __global__ void __launch_bounds__(BLOCK) k_correct(Fine G, const float* us, cons
     const uint8_t open = G.ow[g], mb = G.mask[below], ma = G.mask[above];
     const double pb = p[below], pa = p[above];
     float val = ws[g];
-    if (k >= 1 && open && (mb & M_WET)) {
+    if (G.phi) {
+      // level set (apic3d.rs:1016-1048): a face next to a solid is not
+      // corrected, and air may lie below water
+      if (k >= 1 && k < nz && open && !G.solid[below] && !G.solid[above]) {
+        const double kk = (double)k1, dx = (double)G.dx;
+        const bool wb = (mb & M_WET) != 0, wa = (ma & M_WET) != 0;
+        float a, b;
+        if (wb && wa) val = (float)((double)val - kk * (pa - pb) / dx);
+        else if (wb) { ghost_up(G, i, j, k - 1, a, b); val = (float)((double)val - kk * ((double)b - pb) * (double)a / dx); }
+        else if (wa) { ghost_phi(G, i, j, k, i, j, k - 1, a, b); val = (float)((double)val - kk * (pa - (double)b) * (double)a / dx); }
+      }
+    } else if (k >= 1 && open && (mb & M_WET)) {
       const double kk = (double)k1, dx = (double)G.dx;
       if (k < nz && (ma & M_WET)) {
         val = (float)((double)val - kk * (pa - pb) / dx);

@@ -768,7 +768,20 @@ __global__ void __launch_bounds__(QBLOCK) k_correct_q(Fine G, QuadGrid Q, const 
       mx = fmax(mx, (double)fmaxf(fmaxf(fabsf(vt.x), fabsf(vt.y)), fmaxf(fabsf(vt.z), fabsf(vt.w))));
     }
     // z- faces, and the closing +z faces
-    auto zface = [&](float val, uint8_t open, int kf, uint8_t mb, double pb, uint8_t ma, double pa, int i) {
+    auto zface = [&](float val, uint8_t open, int kf, uint8_t mb, double pb, uint8_t ma, double pa, int i, bool sb,
+                     bool sa) {
+      if (G.phi) {
+        // level set (apic3d.rs:1016-1048): a face next to a solid is not
+        // corrected, and air may lie below water
+        if (kf >= 1 && kf < nz && open && !sb && !sa) {
+          const bool wb = (mb & M_WET) != 0, wa = (ma & M_WET) != 0;
+          float a, b;
+          if (wb && wa) val = (float)((double)val - kk * (pa - pb) / dx);
+          else if (wb) { ghost_up(G, i, j, kf - 1, a, b); val = (float)((double)val - kk * ((double)b - pb) * (double)a / dx); }
+          else if (wa) { ghost_phi(G, i, j, kf, i, j, kf - 1, a, b); val = (float)((double)val - kk * (pa - (double)b) * (double)a / dx); }
+        }
+        return val;
+      }
       if (kf >= 1 && open && (mb & M_WET)) {
         if (kf < nz && (ma & M_WET)) {
           val = (float)((double)val - kk * (pa - pb) / dx);
@@ -783,7 +796,8 @@ __global__ void __launch_bounds__(QBLOCK) k_correct_q(Fine G, QuadGrid Q, const 
     float4 wo;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const float val = zface(q_get(wq, e), byte(awq, e), k, byte(mz, e), pz[e], byte(mq, e), pq[e], i0 + e);
+      const float val = zface(q_get(wq, e), byte(awq, e), k, byte(mz, e), pz[e], byte(mq, e), pq[e], i0 + e,
+                              byte(sz, e) != 0, byte(sq, e) != 0);
       q_set(wo, e, val);
       mx = fmax(mx, (double)fabsf(val));
     }
@@ -793,7 +807,8 @@ __global__ void __launch_bounds__(QBLOCK) k_correct_q(Fine G, QuadGrid Q, const 
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         // face nz: below and (clamped) above are the top cell itself
-        const float val = zface(q_get(wt, e), byte(awt, e), nz, byte(mq, e), pq[e], byte(mq, e), pq[e], i0 + e);
+        const float val = zface(q_get(wt, e), byte(awt, e), nz, byte(mq, e), pq[e], byte(mq, e), pq[e], i0 + e,
+                                byte(sq, e) != 0, byte(sq, e) != 0);
         q_set(wt2, e, val);
         mx = fmax(mx, (double)fabsf(val));
       }

@@ -187,6 +187,17 @@ class Volume3:
         self._open = tuple(None if a is None else np.ascontiguousarray(a, np.float32) for a in (open_u, open_v, open_w))
         self._dirty = True
 
+    def set_level_set(self, labels, phi, theta_min: float = 0.01):
+        """The APIC-3D projection's geometry (apic3d.rs:940-1051): cell labels
+        (LABEL_AIR / LABEL_WATER / LABEL_SOLID) and the level set phi at the
+        cell centres; replaces the column geometry until set_solid or
+        set_free_surface is called again."""
+        labels = np.ascontiguousarray(labels, np.uint8)
+        phi = np.ascontiguousarray(phi, np.float32)
+        _raise(_lib.lib().fsp_set_level_set(self._h, _ptr(labels), labels.size, _ptr(phi), phi.size,
+                                            float(theta_min)), "set_level_set")
+        self._dirty = False
+
     def set_surface_mode(self, homogeneous_ghost: bool, closed_lid: bool = False):
         """homogeneous_ghost: p = 0 at the surface (delta3d_coupling.rs:226-249); closed_lid: Neumann top."""
         self.homogeneous_ghost = bool(homogeneous_ghost)

@@ -137,6 +137,25 @@ fsp_status fsp_set_geometry(fsp_solver* s, const fsp_surface* surface,
                             const float* open_v, size_t n_open_v,
                             const float* open_w, size_t n_open_w);
 
+/* Cell labels of a level-set geometry (apic3d.rs:33-36). */
+enum { FSP_LABEL_AIR = 0, FSP_LABEL_WATER = 1, FSP_LABEL_SOLID = 2 };
+
+/* The geometry of the APIC-3D projection (apic3d.rs:940-1051), replacing its
+ * labels and particle level set: labels (cells, FSP_LABEL_*) and phi (cells,
+ * the level set at the cell centres, negative in water). Water rows couple
+ * to water neighbours; an air neighbour is a ghost face with the pressure 0
+ * on the zero level, theta = max(phi_c / (phi_c - phi_a), theta_min) (0.01
+ * in the reference, :724); solid cells are walls that still carry their
+ * faces' velocities into the divergence; the domain's boundary is a wall.
+ * fsp_project then solves the reference's system (its operator times 1/dx^2,
+ * its right-hand side times 1/dx^2: the same pressure) with the configured
+ * preconditioner, scale = -rho/dt and k1 = dt/rho. A configuration call: the
+ * first one reserves the labels' and phi's device copies; fsp_set_geometry
+ * returns to the column geometry. The surface-step entry points refuse a
+ * level-set geometry (FSP_DOMAIN). */
+fsp_status fsp_set_level_set(fsp_solver* s, const uint8_t* labels, size_t n_labels, const float* phi, size_t n_phi,
+                             float theta_min);
+
 /* Replaces Volume3::project_mobile3 (delta3d_mobile.rs:465-609). Host
  * pointers. us/vs/ws: predicted faces; p: in = warm start (p_warm != 0) or
  * ignored, out = pressure; u/v/w out: corrected faces. scale = -rho/dt,

@@ -2,6 +2,7 @@
 // ctypes: tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
 #include "oracle.hpp"
 #include "cut.hpp"
+#include "apic.hpp"
 
 #include <algorithm>
 
@@ -135,6 +136,46 @@ Options no_multigrid() {
 }  // namespace
 
 extern "C" {
+
+// The APIC-3D pressure projection (apic3d.rs:940-1051) in f32 (precision 32)
+// or f64 (64): u, v, w in (predicted, walls and body imposed) and out
+// (corrected); p (cells, f64) out; iterations, relative residual and the
+// divergence metric (apic3d.rs:1053-1080) out.
+int orc_apic_project(int precision, int nx, int ny, int nz, float dx, float rho, float dt, const uint8_t* label,
+                     const float* phi, float* u, float* v, float* w, double* p_out, uint32_t* iterations,
+                     double* residual, double* divergence) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || !(dx > 0.f) || !label || !phi) return ERR_DOMAIN;
+  auto run = [&](auto proj) {
+    proj.nx = nx; proj.ny = ny; proj.nz = nz; proj.dx = dx; proj.label = label; proj.phi = phi;
+    const apic::Result r = proj.project(rho, dt, u, v, w);
+    for (size_t c = 0; c < proj.p.size(); ++c) p_out[c] = (double)proj.p[c];
+    *iterations = r.iterations;
+    *residual = r.residual;
+    *divergence = r.divergence;
+  };
+  if (precision == 64) run(apic::Projection<double>{});
+  else run(apic::Projection<float>{});
+  return OK;
+}
+
+// y = A x of the APIC operator (apic3d.rs:912-935), f64.
+int orc_apic_apply(int nx, int ny, int nz, const uint8_t* label, const float* phi, const double* x, double* y) {
+  apic::Projection<double> proj{};
+  proj.nx = nx; proj.ny = ny; proj.nz = nz; proj.dx = 1.0; proj.label = label; proj.phi = phi;
+  const size_t n = (size_t)nx * ny * nz;
+  std::vector<double> xv(x, x + n), yv(n, 0.0);
+  proj.apply(xv, yv);
+  std::copy(yv.begin(), yv.end(), y);
+  return OK;
+}
+
+// The APIC divergence metric (apic3d.rs:1053-1080) of given faces.
+double orc_apic_divergence_metric(int nx, int ny, int nz, const uint8_t* label, const float* u, const float* v,
+                                  const float* w) {
+  apic::Projection<float> proj{};
+  proj.nx = nx; proj.ny = ny; proj.nz = nz; proj.dx = 1.f; proj.label = label; proj.phi = nullptr;
+  return proj.divergence_metric(u, v, w);
+}
 
 // The cut of a bottom height field (delta3d_cut.rs:172-258): frac (cells),
 // open_u, open_v, open_w (faces) and floor (columns) from the nx*ny bottom

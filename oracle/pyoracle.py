@@ -66,6 +66,10 @@ def lib():
         L.orc_project.argtypes = [i32, vp, vp, vp, vp, vp, vp, f32, f32, u32, vp, vp, vp, vp, vp, vp]
         L.orc_apply.argtypes = [vp, vp, vp]
         L.orc_cut.argtypes = [i32, i32, i32, f32, vp, vp, vp, vp, vp, vp]
+        L.orc_apic_project.argtypes = [i32, i32, i32, i32, f32, f32, f32, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_apic_apply.argtypes = [i32, i32, i32, vp, vp, vp, vp]
+        L.orc_apic_divergence_metric.argtypes = [i32, i32, i32, vp, vp, vp, vp]
+        L.orc_apic_divergence_metric.restype = C.c_double
         L.orc_round_storage.argtypes = [i32, vp, C.c_size_t]
         L.orc_round_storage.restype = None
         L.orc_apply64.argtypes = [vp, vp, vp]
@@ -455,3 +459,39 @@ def mobile_in_bounds(prob: Problem):
 
 def perturbation_volume(prob: Problem):
     return lib().orc_perturbation_volume(C.byref(prob.geometry()))
+
+
+# ---------------------------------------------------------------- the APIC-3D projection
+APIC_AIR, APIC_WATER, APIC_SOLID = 0, 1, 2
+
+
+def apic_project(nx, ny, nz, dx, rho, dt, label, phi, u, v, w, precision=32):
+    """The APIC-3D pressure projection (apic3d.rs:940-1051): returns (p f64,
+    u, v, w corrected, iterations, relative residual, divergence metric)."""
+    label = np.ascontiguousarray(label, np.uint8)
+    phi = np.ascontiguousarray(phi, np.float32)
+    u, v, w = (np.array(a, np.float32, copy=True) for a in (u, v, w))
+    p = np.zeros(nx * ny * nz, np.float64)
+    it, res, div = C.c_uint32(), C.c_double(), C.c_double()
+    st = lib().orc_apic_project(precision, nx, ny, nz, float(np.float32(dx)), float(np.float32(rho)),
+                                float(np.float32(dt)), label.ctypes.data, phi.ctypes.data, u.ctypes.data,
+                                v.ctypes.data, w.ctypes.data, p.ctypes.data, C.byref(it), C.byref(res), C.byref(div))
+    assert st == 0
+    return p, u, v, w, it.value, res.value, div.value
+
+
+def apic_apply(nx, ny, nz, label, phi, x):
+    """y = A x of the APIC operator (apic3d.rs:912-935), f64."""
+    label = np.ascontiguousarray(label, np.uint8)
+    phi = np.ascontiguousarray(phi, np.float32)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros_like(x)
+    assert lib().orc_apic_apply(nx, ny, nz, label.ctypes.data, phi.ctypes.data, x.ctypes.data, y.ctypes.data) == 0
+    return y
+
+
+def apic_divergence_metric(nx, ny, nz, label, u, v, w):
+    """The APIC divergence metric (apic3d.rs:1053-1080) of the faces u, v, w."""
+    label = np.ascontiguousarray(label, np.uint8)
+    u, v, w = (np.ascontiguousarray(a, np.float32) for a in (u, v, w))
+    return lib().orc_apic_divergence_metric(nx, ny, nz, label.ctypes.data, u.ctypes.data, v.ctypes.data, w.ctypes.data)
