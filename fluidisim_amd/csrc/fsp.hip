@@ -55,6 +55,10 @@ struct fsp_solver {
   float* phi = nullptr;
   float theta_min = 0.f;
   int levelset = 0;
+  // coupled surface (fsp_set_coupling): the total surface per column and the
+  // background's ghost terms, reserved by its first call
+  float *total = nullptr, *bg_up = nullptr, *bg_x = nullptr, *bg_y = nullptr;
+  int coupled = 0;
   float *eta = nullptr, *eta_rt = nullptr;
   float* col_floor = nullptr;  // per column: top of its highest solid cell (0 without one)
   uint16_t* gk0 = nullptr;  // per column: lowest ghost row
@@ -115,7 +119,11 @@ struct fsp_solver {
     G.water = levelset ? water : nullptr;
     G.phi = levelset ? phi : nullptr;
     G.theta_min = theta_min;
-    G.eta = eta; G.eta_rt = eta_rt;
+    G.eta = coupled ? total : eta; G.eta_rt = eta_rt;
+    G.eta_val = eta;
+    G.bg_up = coupled ? bg_up : nullptr;
+    G.bg_x = coupled ? bg_x : nullptr;
+    G.bg_y = coupled ? bg_y : nullptr;
     G.mask = mask;
     G.gk0 = gk0;
     G.gtab = gtab;
@@ -964,6 +972,15 @@ fsp_status prepare_graph(fsp_solver* S, int phase) {
   return st;
 }
 
+// The iteration graphs hold the fine geometry's kernel arguments: after a
+// change of geometry mode, capture them again.
+fsp_status regraph(fsp_solver* S) {
+  for (int gi = 0; gi < 2; ++gi) drop_graph(S, gi);
+  fsp_status gst = prepare_graph(S, 0);
+  if (gst == FSP_OK && S->net && S->opt.preconditioner == FSP_PRECOND_LEARNED) gst = prepare_graph(S, 1);
+  return gst;
+}
+
 // Geometry rebuilt from the surface at the start of every projection
 // (delta3d_mobile.rs:479-484).
 void build_geometry(fsp_solver* S, bool hierarchy) {
@@ -1512,18 +1529,15 @@ fsp_status fsp_set_geometry(fsp_solver* S, const fsp_surface* surf, const float*
   FSP_HIP(hipStreamSynchronize(S->stream));
   S->surf = *surf;
   S->cut = cut;
-  const bool recapture = S->frac != frac || S->levelset;
+  // a coupled surface stays on over a new column geometry (the coupled
+  // step sets its surface every step) but not over fractional apertures,
+  // which the coupled step refuses (delta3d_coupling.rs:729)
+  const bool recapture = S->frac != frac || S->levelset || (S->coupled && frac);
   S->frac = frac;
   S->levelset = 0;
+  if (frac) S->coupled = 0;
   S->geom = true;
-  // the iteration graphs hold the fine geometry's kernel arguments: capture
-  // them again when fractional apertures come or go
-  if (recapture) {
-    for (int gi = 0; gi < 2; ++gi) drop_graph(S, gi);
-    fsp_status gst = prepare_graph(S, 0);
-    if (gst == FSP_OK && S->net && S->opt.preconditioner == FSP_PRECOND_LEARNED) gst = prepare_graph(S, 1);
-    if (gst) return gst;
-  }
+  if (recapture) return regraph(S);
   return FSP_OK;
 }
 
@@ -1574,18 +1588,97 @@ fsp_status fsp_set_level_set(fsp_solver* S, const uint8_t* labels, size_t n_labe
   surf.closed_lid = 1;
   S->surf = surf;
   S->cut = 1;
-  const bool recapture = !S->levelset || S->frac || S->theta_min != theta_min;
+  const bool recapture = !S->levelset || S->frac || S->coupled || S->theta_min != theta_min;
   S->frac = 0;
   S->levelset = 1;
+  S->coupled = 0;
   S->theta_min = theta_min;
   S->geom = true;
-  // the iteration graphs hold the fine geometry's kernel arguments
-  if (recapture) {
-    for (int gi = 0; gi < 2; ++gi) drop_graph(S, gi);
-    fsp_status gst = prepare_graph(S, 0);
-    if (gst == FSP_OK && S->net && S->opt.preconditioner == FSP_PRECOND_LEARNED) gst = prepare_graph(S, 1);
-    if (gst) return gst;
+  if (recapture) return regraph(S);
+  return FSP_OK;
+}
+
+namespace {
+// fsp_set_coupling and fsp_set_coupling_device: copies of kind `kind`.
+fsp_status set_coupling(fsp_solver* S, const float* total, size_t n_total, const float* bg_up, size_t n_up,
+                        const float* bg_x, size_t n_x, const float* bg_y, size_t n_y, hipMemcpyKind kind) {
+  if (!S) return FSP_INVALID;
+  if (!S->geom || S->levelset || S->frac) return FSP_DOMAIN;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  if (!total) {
+    const bool was = S->coupled != 0;
+    S->coupled = 0;
+    return was ? regraph(S) : FSP_OK;
   }
+  if (n_total != S->ncols || (bg_up && n_up != S->ncols) || (bg_x && n_x != S->nu) || (bg_y && n_y != S->nv))
+    return FSP_SHAPE;
+  if (kind == hipMemcpyHostToDevice &&
+      (!all_finite(total, S->ncols) || (bg_up && !all_finite(bg_up, S->ncols)) ||
+       (bg_x && !all_finite(bg_x, S->nu)) || (bg_y && !all_finite(bg_y, S->nv))))
+    return FSP_NOT_FINITE;
+  if (!S->total) {
+    fsp_status as;
+    if ((as = dalloc(S, &S->total, S->ncols)) || (as = dalloc(S, &S->bg_up, S->ncols)) ||
+        (as = dalloc(S, &S->bg_x, S->nu)) || (as = dalloc(S, &S->bg_y, S->nv)))
+      return as;
+  }
+  const float* src[4] = {total, bg_up, bg_x, bg_y};
+  float* dst[4] = {S->total, S->bg_up, S->bg_x, S->bg_y};
+  const size_t len[4] = {S->ncols, S->ncols, S->nu, S->nv};
+  for (int a = 0; a < 4; ++a) {
+    if (src[a]) FSP_HIP(hipMemcpyAsync(dst[a], src[a], len[a] * sizeof(float), kind, S->stream));
+    else FSP_HIP(hipMemsetAsync(dst[a], 0, len[a] * sizeof(float), S->stream));
+  }
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  const bool recapture = !S->coupled;
+  S->coupled = 1;
+  if (recapture) return regraph(S);
+  return FSP_OK;
+}
+}  // namespace
+
+fsp_status fsp_set_coupling(fsp_solver* S, const float* total, size_t n_total, const float* bg_up, size_t n_up,
+                            const float* bg_x, size_t n_x, const float* bg_y, size_t n_y) {
+  return set_coupling(S, total, n_total, bg_up, n_up, bg_x, n_x, bg_y, n_y, hipMemcpyHostToDevice);
+}
+
+fsp_status fsp_set_coupling_device(fsp_solver* S, const float* d_total, const float* d_bg_up, const float* d_bg_x,
+                                   const float* d_bg_y) {
+  if (!S) return FSP_INVALID;
+  return set_coupling(S, d_total, S->ncols, d_bg_up, S->ncols, d_bg_x, S->nu, d_bg_y, S->nv,
+                      hipMemcpyDeviceToDevice);
+}
+
+fsp_status fsp_refine_device(fsp_solver* S, float* d_p, float* d_u, float* d_v, float* d_w, float scale, float k1,
+                             double rel_tol, uint32_t max_iters, fsp_report* report) {
+  if (!S || !d_p || !d_u || !d_v || !d_w) return FSP_INVALID;
+  if (!std::isfinite(scale) || !std::isfinite(k1) || !(rel_tol > 0.0)) return FSP_NOT_FINITE;
+  if (!S->geom) return FSP_DOMAIN;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  // base pressure and the corrected faces as the new predicted faces
+  // (delta3d_coupling.rs:233-237), in the solver's staging buffers
+  FSP_HIP(hipMemcpyAsync(S->p, d_p, S->n * sizeof(float), hipMemcpyDeviceToDevice, S->stream));
+  FSP_HIP(hipMemcpyAsync(S->us, d_u, S->nu * sizeof(float), hipMemcpyDeviceToDevice, S->stream));
+  FSP_HIP(hipMemcpyAsync(S->vs, d_v, S->nv * sizeof(float), hipMemcpyDeviceToDevice, S->stream));
+  FSP_HIP(hipMemcpyAsync(S->ws, d_w, S->nw * sizeof(float), hipMemcpyDeviceToDevice, S->stream));
+  // homogeneous ghosts from p = 0 (:238-240); the iteration graphs read no
+  // ghost value, so they stay as captured
+  const int saved = S->surf.homogeneous_ghost;
+  S->surf.homogeneous_ghost = 1;
+  fsp_report R{};
+  const fsp_status st = run(S, S->us, S->vs, S->ws, d_p, 0, d_u, d_v, d_w, scale, k1, rel_tol, max_iters, &R);
+  S->surf.homogeneous_ghost = saved;
+  if (st != FSP_OK) {
+    // the base pressure back in d_p
+    FSP_HIP(hipMemcpyAsync(d_p, S->p, S->n * sizeof(float), hipMemcpyDeviceToDevice, S->stream));
+    FSP_HIP(hipStreamSynchronize(S->stream));
+    return st;
+  }
+  hipLaunchKernelGGL(k_add_base, dim3(grid_for(S->n)), dim3(BLOCK), 0, S->stream, d_p, (const float*)S->p, S->n);
+  FSP_HIP(hipGetLastError());
+  FSP_HIP(hipStreamSynchronize(S->stream));
+  R.refinements = 1;
+  if (report) *report = R;
   return FSP_OK;
 }
 
@@ -1803,7 +1896,7 @@ fsp_status fsp_step_surface_mobile_device(fsp_solver* S, float* d_u, float* d_v,
                                           fsp_balance* balance) {
   if (!S || !d_u || !d_v || !d_w || !d_p) return FSP_INVALID;
   if (!S->geom) return FSP_DOMAIN;
-  if (S->frac || S->levelset) return FSP_DOMAIN;  // the surface step is built for column geometries
+  if (S->frac || S->levelset || S->coupled) return FSP_DOMAIN;  // built for uncoupled column geometries
   if (duration_us == 0 || duration_us > (1ull << 53)) return FSP_NOT_FINITE;
   const double dt = (double)duration_us * 1e-6;
   const float dx = S->dom.dx, rho = S->surf.rho, g = S->surf.g_eff;
@@ -1875,7 +1968,7 @@ fsp_status fsp_advect_device(fsp_solver* S, const float* d_u, const float* d_v, 
                              float* d_us, float* d_vs, float* d_ws) {
   if (!S || !d_u || !d_v || !d_w || !d_us || !d_vs || !d_ws) return FSP_INVALID;
   if (!S->geom) return FSP_DOMAIN;
-  if (S->frac || S->levelset) return FSP_DOMAIN;  // the surface step is built for column geometries
+  if (S->frac || S->levelset || S->coupled) return FSP_DOMAIN;  // built for uncoupled column geometries
   if (!std::isfinite(dt)) return FSP_NOT_FINITE;
   FSP_HIP(hipSetDevice(S->opt.device));
   fsp_status st = advect(S, d_u, d_v, d_w, dt, d_us, d_vs, d_ws);
@@ -1887,7 +1980,7 @@ fsp_status fsp_advect_device(fsp_solver* S, const float* d_u, const float* d_v, 
 fsp_status fsp_extrapolate_device(fsp_solver* S, float* d_u, float* d_v, float* d_w) {
   if (!S || !d_u || !d_v || !d_w) return FSP_INVALID;
   if (!S->geom) return FSP_DOMAIN;
-  if (S->frac || S->levelset) return FSP_DOMAIN;  // the surface step is built for column geometries
+  if (S->frac || S->levelset || S->coupled) return FSP_DOMAIN;  // built for uncoupled column geometries
   FSP_HIP(hipSetDevice(S->opt.device));
   fsp_status st = extrapolate(S, d_u, d_v, d_w);
   if (st) return st;
@@ -1899,7 +1992,7 @@ fsp_status fsp_transport_device(fsp_solver* S, const float* d_u, const float* d_
                                 float* d_flux_y) {
   if (!S || !d_u || !d_v) return FSP_INVALID;
   if (!S->geom) return FSP_DOMAIN;
-  if (S->frac || S->levelset) return FSP_DOMAIN;  // the surface step is built for column geometries
+  if (S->frac || S->levelset || S->coupled) return FSP_DOMAIN;  // built for uncoupled column geometries
   if (!std::isfinite(factor)) return FSP_NOT_FINITE;
   FSP_HIP(hipSetDevice(S->opt.device));
   fsp_status st = step_reserve(S);

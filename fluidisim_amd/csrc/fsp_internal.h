@@ -94,6 +94,16 @@ struct Fine {
   const uint8_t* water;  // cells: 1 water
   const float* phi;      // cells
   float theta_min;
+  // Coupled surface (the perturbation step over a background wave,
+  // delta3d_coupling.rs:97-189, delta3d_mobile.rs:92-147): eta above holds
+  // the total surface, whose heights set wetness and theta; eta_val is the
+  // perturbation's own surface (the up ghost's value) and bg_up (columns),
+  // bg_x (u faces), bg_y (v faces) the background's terms added to the ghost
+  // values. bg_up == null: not coupled, eta_val == eta.
+  const float* eta_val;
+  const float* bg_up;
+  const float* bg_x;
+  const float* bg_y;
   float prec_tab[7];     // 1/(n * inv) for a plain row with n faces
 };
 

@@ -234,7 +234,8 @@ __device__ __forceinline__ void ghost_phi(const Fine& G, int i, int j, int k, in
   it = 1.0f / fmaxf(fc / (fc - fa), G.theta_min);
   val = 0.f;
 }
-// delta3d_mobile.rs:126-134
+// delta3d_mobile.rs:126-134; coupled, theta from the total surface and the
+// background's column term added to the value (:130-132)
 __device__ __forceinline__ void ghost_up(const Fine& G, int i, int j, int k, float& it, float& val) {
 #pragma clang fp contract(off)
   if (G.phi) { ghost_phi(G, i, j, k, i, j, k + 1, it, val); return; }
@@ -243,9 +244,12 @@ __device__ __forceinline__ void ghost_up(const Fine& G, int i, int j, int k, flo
   const float theta = fmaxf((h - zc_of(G, k)) / G.dx, 1e-3f);
   it = 1.0f / theta;
   if (G.homogeneous) { val = 0.f; return; }
-  val = G.rho * G.g * ((h - G.rest) - G.eta_rt[col]);
+  if (!G.bg_up) { val = G.rho * G.g * ((h - G.rest) - G.eta_rt[col]); return; }
+  val = G.rho * G.g * ((G.eta_val[col] - G.rest) - G.eta_rt[col]);
+  val += G.bg_up[col];
 }
-// delta3d_mobile.rs:136-147
+// delta3d_mobile.rs:136-147: wet cell (i, j, k), dry neighbour (x, y, k);
+// coupled, the background's term of the face between them (:142-145)
 __device__ __forceinline__ void ghost_side(const Fine& G, int i, int j, int k, int x, int y, float& it, float& val) {
 #pragma clang fp contract(off)
   if (G.phi) { ghost_phi(G, i, j, k, x, y, k, it, val); return; }
@@ -255,6 +259,8 @@ __device__ __forceinline__ void ghost_side(const Fine& G, int i, int j, int k, i
   it = 1.0f / theta;
   if (G.homogeneous) { val = 0.f; return; }
   val = G.rho * G.g * (zc - G.rest);
+  if (G.bg_up)
+    val += x != i ? G.bg_x[fuidx(G, i > x ? i : x, j, k)] : G.bg_y[fvidx(G, i, j > y ? j : y, k)];
 }
 
 // Aperture of a face from its code in the u8 face arrays (0 closed, 1 open,
@@ -665,6 +671,11 @@ __global__ void __launch_bounds__(BLOCK) k_store_p(size_t n, const double* p, fl
 
 __global__ void __launch_bounds__(BLOCK) k_fill(float* x, size_t n, float value) {
   FSP_FOR(c, n) x[c] = value;
+}
+// p += base in f32 (refine_mobile3's sum, delta3d_coupling.rs:244-246)
+__global__ void __launch_bounds__(BLOCK) k_add_base(float* p, const float* base, size_t n) {
+#pragma clang fp contract(off)
+  FSP_FOR(c, n) p[c] = p[c] + base[c];
 }
 
 // r = b - A p, |r|^2 (delta3d_mobile.rs:497-503, :547-553), with the

@@ -198,6 +198,23 @@ class Volume3:
                                             float(theta_min)), "set_level_set")
         self._dirty = False
 
+    def set_coupling(self, total=None, bg_up=None, bg_x=None, bg_y=None):
+        """The coupled surface of the perturbation step over a background wave
+        (delta3d_coupling.rs:97-189 as the projection reads it,
+        delta3d_mobile.rs:92-147): total heights per column (wetness and
+        ghost fractions), and the background's terms added to the up ghosts
+        (per column) and to the side ghosts (per u face, per v face); None
+        terms are zeros. total=None switches the coupling off."""
+        self._upload_geometry()
+        if total is None:
+            _raise(_lib.lib().fsp_set_coupling(self._h, None, 0, None, 0, None, 0, None, 0), "set_coupling")
+            return
+        arr = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+        total, bg_up, bg_x, bg_y = arr(total), arr(bg_up), arr(bg_x), arr(bg_y)
+        sz = lambda a: 0 if a is None else a.size
+        _raise(_lib.lib().fsp_set_coupling(self._h, _ptr(total), total.size, _ptr(bg_up), sz(bg_up), _ptr(bg_x),
+                                           sz(bg_x), _ptr(bg_y), sz(bg_y)), "set_coupling")
+
     def set_surface_mode(self, homogeneous_ghost: bool, closed_lid: bool = False):
         """homogeneous_ghost: p = 0 at the surface (delta3d_coupling.rs:226-249); closed_lid: Neumann top."""
         self.homogeneous_ghost = bool(homogeneous_ghost)
@@ -263,6 +280,33 @@ class Volume3:
                                            max_iters, C.byref(rep))
         _raise(st, "project_mobile3")
         return Report.from_c(rep)
+
+    def refine_mobile3(self, scale: float, k1: float, max_iters: int, rel_tol: float = 1e-6) -> Report:
+        """delta3d_coupling.rs:226-249: the corrected u, v, w become the
+        predicted faces, the pressure is solved again from 0 with homogeneous
+        ghosts and added to the current p; refinements = 1."""
+        self._upload_geometry()
+        self._sync()
+        rep = _lib.Report()
+        st = _lib.lib().fsp_refine_device(self._h, _ptr(self.p), _ptr(self.u), _ptr(self.v), _ptr(self.w), scale, k1,
+                                          rel_tol, max_iters, C.byref(rep))
+        _raise(st, "refine_mobile3")
+        return Report.from_c(rep)
+
+    def project_perturbation(self, scale: float, k1: float, max_iters: int, rel_tol: float = 1e-6,
+                             warm: bool = True) -> Report:
+        """The projection path of step_perturbation_mobile
+        (delta3d_coupling.rs:835-843): project; after a degraded result that
+        stopped on the floor, refine, with the first solve's iterations added;
+        a result still degraded raises Error.Convergence."""
+        report = self.project_mobile3(scale, k1, max_iters, rel_tol, warm)
+        if report.degraded and report.floor:
+            first = report.iterations
+            report = self.refine_mobile3(scale, k1, max_iters, rel_tol)
+            report.iterations = min(report.iterations + first, 0xFFFFFFFF)
+        if report.degraded:
+            raise ProjectionError(Error.Convergence, "step_perturbation_mobile")
+        return report
 
     # ------------------------------------------------------------ the surface step
     def step_surface_mobile(self, duration_us: int, max_iters: int) -> Report:

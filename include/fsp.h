@@ -156,6 +156,41 @@ enum { FSP_LABEL_AIR = 0, FSP_LABEL_WATER = 1, FSP_LABEL_SOLID = 2 };
 fsp_status fsp_set_level_set(fsp_solver* s, const uint8_t* labels, size_t n_labels, const float* phi, size_t n_phi,
                              float theta_min);
 
+/* The coupled surface of the perturbation step over a background wave
+ * (step_perturbation_mobile, delta3d_coupling.rs:720-864), as its projection
+ * reads it (delta3d_mobile.rs:92-147 with surface_coupled; the terms
+ * prepare_background3 computes, delta3d_coupling.rs:97-189). Host pointers.
+ * total: nx*ny heights of the total surface (eta + the background's
+ * elevation), which set wetness and the ghost fractions in place of eta;
+ * bg_up: nx*ny terms added to the up ghost's value; bg_x: (nx+1)*ny*nz terms
+ * added to the value of the side ghost across that u face; bg_y: nx*(ny+1)*nz
+ * for v faces; a NULL term array is zeros. The up ghost's value keeps
+ * rho*g*((eta - rest) - eta_roundoff) from fsp_set_geometry's eta. total ==
+ * NULL switches the coupling off. Needs a column geometry without
+ * fractional apertures (FSP_DOMAIN otherwise, as the coupled step refuses
+ * a cut, :729); stays on over later fsp_set_geometry calls of that kind;
+ * fsp_set_level_set and fractional apertures switch it off. The surface-step
+ * entry points refuse a coupled surface (FSP_DOMAIN). A configuration call:
+ * the first one reserves the device copies. */
+fsp_status fsp_set_coupling(fsp_solver* s, const float* total, size_t n_total, const float* bg_up, size_t n_up,
+                            const float* bg_x, size_t n_x, const float* bg_y, size_t n_y);
+
+/* The same from device pointers (copied on the solver's stream; sizes as
+ * above, finiteness not checked). */
+fsp_status fsp_set_coupling_device(fsp_solver* s, const float* d_total, const float* d_bg_up, const float* d_bg_x,
+                                   const float* d_bg_y);
+
+/* Replaces Volume3::refine_mobile3 (delta3d_coupling.rs:226-249), the
+ * coupled step's second solve after a degraded floor stop (:836-840): the
+ * corrected faces d_u/d_v/d_w become the predicted faces, the pressure is
+ * solved from 0 with homogeneous ghosts (value 0), the faces are corrected
+ * again and the result is added to the base pressure d_p in f32. Device
+ * pointers, all in/out. report->refinements = 1 and report->iterations the
+ * second solve's (the step adds the first's). On a non-OK status d_p holds
+ * the base pressure. */
+fsp_status fsp_refine_device(fsp_solver* s, float* d_p, float* d_u, float* d_v, float* d_w, float scale, float k1,
+                             double rel_tol, uint32_t max_iters, fsp_report* report);
+
 /* Replaces Volume3::project_mobile3 (delta3d_mobile.rs:465-609). Host
  * pointers. us/vs/ws: predicted faces; p: in = warm start (p_warm != 0) or
  * ignored, out = pressure; u/v/w out: corrected faces. scale = -rho/dt,

@@ -16,13 +16,40 @@ Status setup(Volume<T>& vol, const Geometry* g, const Options* o) {
   return vol.configure(*g, opt);
 }
 
+// The coupled surface's arrays (total heights and background ghost terms
+// per column, u face and v face); a null term array is zeros.
+struct Coupling {
+  const float* total;
+  const float* bg_up;
+  const float* bg_x;
+  const float* bg_y;
+};
 template <class T>
-int project(const Geometry* g, const Options* o, const float* us, const float* vs, const float* ws, const float* p_warm,
-            float scale, float k1, uint32_t max_iters, float* p_out, float* u_out, float* v_out, float* w_out,
-            double* p_out64, Report* rep) {
+void couple(Volume<T>& vol, const Coupling* cpl) {
+  const size_t cols = vol.nx * vol.ny;
+  auto take = [](std::vector<T>& dst, const float* src, size_t n) {
+    dst.assign(n, T(0));
+    if (src) for (size_t q = 0; q < n; ++q) dst[q] = (T)src[q];
+  };
+  vol.coupled = true;
+  take(vol.surface_total, cpl->total, cols);
+  take(vol.ghost_bg_up, cpl->bg_up, cols);
+  take(vol.ghost_bg_x, cpl->bg_x, vol.nu());
+  take(vol.ghost_bg_y, cpl->bg_y, vol.nv());
+}
+// refine: 0 the projection alone; 1 step_perturbation_mobile's rule, a
+// refinement after a degraded floor stop with the first solve's iterations
+// added (delta3d_coupling.rs:835-840); 2 a refinement always. refine_iters
+// bounds the refinement (the reference's max_iters when 0).
+template <class T>
+int project(const Geometry* g, const Options* o, const Coupling* cpl, const float* us, const float* vs,
+            const float* ws, const float* p_warm, float scale, float k1, uint32_t max_iters, int refine,
+            uint32_t refine_iters, float* p_out,
+            float* u_out, float* v_out, float* w_out, double* p_out64, Report* rep) {
   Volume<T> vol;
   Status st = setup(vol, g, o);
   if (st) return st;
+  if (cpl && cpl->total) couple(vol, cpl);
   for (size_t q = 0; q < vol.nu(); ++q) vol.us[q] = (T)us[q];
   for (size_t q = 0; q < vol.nv(); ++q) vol.vs[q] = (T)vs[q];
   for (size_t q = 0; q < vol.nw(); ++q) vol.ws[q] = (T)ws[q];
@@ -30,6 +57,12 @@ int project(const Geometry* g, const Options* o, const float* us, const float* v
   Report r{};
   st = vol.project_mobile3((T)scale, (T)k1, max_iters, r);
   if (st) return st;
+  if (refine == 2 || (refine == 1 && r.degraded && r.floor)) {
+    const uint32_t first = r.iterations;
+    st = vol.refine_mobile3((T)scale, (T)k1, refine_iters ? refine_iters : max_iters, r);
+    if (st) return st;
+    if (refine == 1) r.iterations = first + r.iterations < first ? 0xffffffffu : first + r.iterations;
+  }
   if (rep) *rep = r;
   if (p_out) for (size_t q = 0; q < vol.cells(); ++q) p_out[q] = (float)vol.p[q];
   if (p_out64) for (size_t q = 0; q < vol.cells(); ++q) p_out64[q] = (double)vol.p[q];
@@ -277,8 +310,24 @@ int orc_project(int precision, const Geometry* g, const Options* o, const float*
                 const float* p_warm, float scale, float k1, uint32_t max_iters, float* p_out, float* u_out,
                 float* v_out, float* w_out, double* p_out64, Report* rep) {
   if (precision == 64)
-    return project<double>(g, o, us, vs, ws, p_warm, scale, k1, max_iters, p_out, u_out, v_out, w_out, p_out64, rep);
-  return project<float>(g, o, us, vs, ws, p_warm, scale, k1, max_iters, p_out, u_out, v_out, w_out, p_out64, rep);
+    return project<double>(g, o, nullptr, us, vs, ws, p_warm, scale, k1, max_iters, 0, 0, p_out, u_out, v_out,
+                           w_out, p_out64, rep);
+  return project<float>(g, o, nullptr, us, vs, ws, p_warm, scale, k1, max_iters, 0, 0, p_out, u_out, v_out, w_out,
+                        p_out64, rep);
+}
+
+// The projection path of step_perturbation_mobile (delta3d_coupling.rs:
+// 720-864): the projection over a coupled surface, with refine_mobile3 as
+// `refine` says.
+int orc_project_coupled(int precision, const Geometry* g, const Options* o, const Coupling* cpl, const float* us,
+                        const float* vs, const float* ws, const float* p_warm, float scale, float k1,
+                        uint32_t max_iters, int refine, uint32_t refine_iters, float* p_out, float* u_out,
+                        float* v_out, float* w_out, double* p_out64, Report* rep) {
+  if (precision == 64)
+    return project<double>(g, o, cpl, us, vs, ws, p_warm, scale, k1, max_iters, refine, refine_iters, p_out, u_out,
+                           v_out, w_out, p_out64, rep);
+  return project<float>(g, o, cpl, us, vs, ws, p_warm, scale, k1, max_iters, refine, refine_iters, p_out, u_out,
+                        v_out, w_out, p_out64, rep);
 }
 
 // Operator action (delta3d_mobile.rs:936-949).
@@ -368,13 +417,23 @@ int orc_divergence_metric(const Geometry* g, const float* u, const float* v, con
 
 // Gradient subtract (delta3d_mobile.rs:284-343) followed by the acceptance
 // metric (delta3d_mobile.rs:398-433).
+int orc_correct_coupled(const Geometry* g, const Coupling* cpl, const float* us, const float* vs, const float* ws,
+                        const float* p, float k1, float* u, float* v, float* w, double* div_all, double* div_plain);
+
 int orc_correct(const Geometry* g, const float* us, const float* vs, const float* ws, const float* p, float k1,
                 float* u, float* v, float* w, double* div_all, double* div_plain) {
+  return orc_correct_coupled(g, nullptr, us, vs, ws, p, k1, u, v, w, div_all, div_plain);
+}
+
+// The same over a coupled surface (cpl null: uncoupled).
+int orc_correct_coupled(const Geometry* g, const Coupling* cpl, const float* us, const float* vs, const float* ws,
+                        const float* p, float k1, float* u, float* v, float* w, double* div_all, double* div_plain) {
   Volume<float> vol;
   Options o = default_options();
   o.multigrid = 0;
   Status st = vol.configure(*g, o);
   if (st) return st;
+  if (cpl && cpl->total) couple(vol, cpl);
   std::copy(us, us + vol.nu(), vol.us.begin());
   std::copy(vs, vs + vol.nv(), vol.vs.begin());
   std::copy(ws, ws + vol.nw(), vol.ws.begin());

@@ -206,6 +206,11 @@ template <class T> class Volume {
   std::vector<T> open_u, open_v, open_w;
   std::vector<T> us, vs, ws, u, v, w;
   std::vector<T> p, rhs, res, dir, tmp, prec;
+  // coupled surface (delta3d.rs:109-114, set by prepare_background3,
+  // delta3d_coupling.rs:97-189): heights of the total surface and the
+  // background's ghost terms
+  bool coupled = false;
+  std::vector<T> surface_total, ghost_bg_up, ghost_bg_x, ghost_bg_y;
   // multigrid (delta3d_multigrid.rs:65-69)
   bool mg_on = false;
   std::vector<Level3<T>> levels;
@@ -282,7 +287,9 @@ template <class T> class Volume {
   }
 
   // ---------------------------------------------------------------- classification
-  T height3(size_t i, size_t j) const { return eta[col(i, j)]; }                         // delta3d_mobile.rs:92-95
+  T height3(size_t i, size_t j) const {                                                 // delta3d_mobile.rs:92-95
+    return coupled ? surface_total[col(i, j)] : eta[col(i, j)];
+  }
   bool solid3(size_t cc) const { return solid[cc] != 0; }                               // delta3d_mobile.rs:122-124
   bool wet3(size_t i, size_t j, size_t k) const {                                       // delta3d_mobile.rs:97-103
     if (solid3(c(i, j, k))) return false;
@@ -302,6 +309,7 @@ template <class T> class Volume {
     it = T(1) / theta;
     if (homogeneous) { value = T(0); return; }
     value = rho * g * ((eta[cc] - rest) - eta_roundoff[cc]);
+    if (coupled) value += ghost_bg_up[cc];
   }
   // delta3d_mobile.rs:136-147
   void ghost_side3(size_t i, size_t j, size_t k, size_t x, size_t y, T& it, T& value) const {
@@ -311,6 +319,7 @@ template <class T> class Volume {
     it = T(1) / theta;
     if (homogeneous) { value = T(0); return; }
     value = rho * g * (zc - rest);
+    if (coupled) value += i != x ? ghost_bg_x[fu(std::max(i, x), j, k)] : ghost_bg_y[fv(i, std::max(j, y), k)];
   }
 
   struct Row { int64_t n[6]; T a[6], it[6], val[6]; };
@@ -1040,6 +1049,22 @@ template <class T> class Volume {
     rep.floor = floor_stop ? 1 : 0;
     rep.backward_error = (double)backward_error_mobile3();
     rep.divergence_plain = d_plain;
+    return OK;
+  }
+
+  // delta3d_coupling.rs:226-249: the corrected faces as the predicted ones,
+  // a solve from p = 0 with homogeneous ghosts, added to the base pressure.
+  Status refine_mobile3(T scale, T k1, uint32_t max_iters, Report& rep) {
+    const std::vector<T> base = p;
+    us = u; vs = v; ws = w;
+    std::fill(p.begin(), p.end(), T(0));
+    const bool saved = homogeneous;
+    homogeneous = true;
+    const Status st = project_mobile3(scale, k1, max_iters, rep);
+    homogeneous = saved;
+    if (st) return st;
+    for (size_t q = 0; q < p.size(); ++q) p[q] += base[q];
+    rep.refinements = 1;
     return OK;
   }
 

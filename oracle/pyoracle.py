@@ -65,6 +65,8 @@ def lib():
         vp, f32, u32, i32, sz = C.c_void_p, C.c_float, C.c_uint32, C.c_int, C.c_size_t
         L.orc_project.argtypes = [i32, vp, vp, vp, vp, vp, vp, f32, f32, u32, vp, vp, vp, vp, vp, vp]
         L.orc_apply.argtypes = [vp, vp, vp]
+        L.orc_project_coupled.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, f32, f32, u32, i32, u32, vp, vp, vp, vp, vp,
+                                          vp]
         L.orc_cut.argtypes = [i32, i32, i32, f32, vp, vp, vp, vp, vp, vp]
         L.orc_apic_project.argtypes = [i32, i32, i32, i32, f32, f32, f32, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_apic_apply.argtypes = [i32, i32, i32, vp, vp, vp, vp]
@@ -76,6 +78,7 @@ def lib():
         L.orc_assemble.argtypes = [vp, vp, f32, vp, vp]
         L.orc_divergence.argtypes = [vp, vp, vp, vp, vp]
         L.orc_correct.argtypes = [vp, vp, vp, vp, vp, f32, vp, vp, vp, vp, vp]
+        L.orc_correct_coupled.argtypes = [vp, vp, vp, vp, vp, vp, f32, vp, vp, vp, vp, vp]
         L.orc_divergence_metric.argtypes = [vp, vp, vp, vp, vp, vp]
         L.orc_backward_error.argtypes = [i32, vp, vp, vp, vp, vp]
         L.orc_coarse_apply.argtypes = [i32, vp, vp, i32, vp, vp]
@@ -185,6 +188,43 @@ def project(prob: Problem, us, vs, ws, scale, k1, max_iters=20000, opts=None, p_
     return out + (p64,) if precision == 64 else out
 
 
+class Coupling(C.Structure):
+    _fields_ = [("total", C.c_void_p), ("bg_up", C.c_void_p), ("bg_x", C.c_void_p), ("bg_y", C.c_void_p)]
+
+
+REFINE_NONE, REFINE_STEP, REFINE_ALWAYS = 0, 1, 2
+
+
+def project_coupled(prob: Problem, us, vs, ws, scale, k1, total, bg_up=None, bg_x=None, bg_y=None, refine=REFINE_NONE,
+                    max_iters=20000, opts=None, p_warm=None, precision=32, refine_iters=0):
+    """The projection over a coupled surface (total heights and background
+    ghost terms; None terms are zeros), with refine_mobile3 as `refine`
+    says (bounded by refine_iters, else max_iters); returns (report dict, p,
+    u, v, w[, p64])."""
+    g = prob.geometry()
+    o = opts if opts is not None else options()
+    nu, nv, nw = prob.face_counts
+    arr = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+    keep = [arr(total), arr(bg_up), arr(bg_x), arr(bg_y)]
+    cpl = Coupling(*[_ptr(a) for a in keep])
+    us, vs, ws = arr(us), arr(vs), arr(ws)
+    pw = arr(p_warm)
+    p = np.zeros(prob.cells, np.float32)
+    u = np.zeros(nu, np.float32)
+    v = np.zeros(nv, np.float32)
+    w = np.zeros(nw, np.float32)
+    p64 = np.zeros(prob.cells, np.float64) if precision == 64 else None
+    rep = Report()
+    st = lib().orc_project_coupled(precision, C.byref(g), C.byref(o), C.byref(cpl), _ptr(us), _ptr(vs), _ptr(ws),
+                                   _ptr(pw), scale, k1, max_iters, refine, refine_iters, _ptr(p), _ptr(u), _ptr(v),
+                                   _ptr(w),
+                                   _ptr(p64), C.byref(rep))
+    if st != 0:
+        raise RuntimeError(f"oracle project_coupled failed: status {st}")
+    out = (rep.as_dict(), p, u, v, w)
+    return out + (p64,) if precision == 64 else out
+
+
 def apply(prob: Problem, x):
     g = prob.geometry()
     x = np.ascontiguousarray(x, np.float32)
@@ -218,14 +258,18 @@ def divergence(prob: Problem, u, v, w):
     return out
 
 
-def correct(prob: Problem, us, vs, ws, p, k1):
+def correct(prob: Problem, us, vs, ws, p, k1, coupling=None):
+    """correct_mobile3 of p (delta3d_mobile.rs:284-343) and the acceptance
+    metric of the result; coupling: (total, bg_up, bg_x, bg_y) or None."""
     g = prob.geometry()
     us, vs, ws, p = (np.ascontiguousarray(a, np.float32) for a in (us, vs, ws, p))
     nu, nv, nw = prob.face_counts
     u, v, w = np.zeros(nu, np.float32), np.zeros(nv, np.float32), np.zeros(nw, np.float32)
     a, b = C.c_double(), C.c_double()
-    assert lib().orc_correct(C.byref(g), _ptr(us), _ptr(vs), _ptr(ws), _ptr(p), k1, _ptr(u), _ptr(v), _ptr(w),
-                             C.byref(a), C.byref(b)) == 0
+    keep = [None if x is None else np.ascontiguousarray(x, np.float32) for x in (coupling or ())]
+    cpl = Coupling(*[_ptr(x) for x in keep]) if coupling is not None else None
+    assert lib().orc_correct_coupled(C.byref(g), None if cpl is None else C.byref(cpl), _ptr(us), _ptr(vs), _ptr(ws),
+                                     _ptr(p), k1, _ptr(u), _ptr(v), _ptr(w), C.byref(a), C.byref(b)) == 0
     return u, v, w, a.value, b.value
 
 
