@@ -26,6 +26,7 @@ LABEL_AIR, LABEL_WATER, LABEL_SOLID = 0, 1, 2
 EXPORTS = [
     "fsp_abi_version", "fsp_status_string", "fsp_default_options", "fsp_create", "fsp_destroy", "fsp_levels",
     "fsp_set_geometry", "fsp_set_level_set", "fsp_set_coupling", "fsp_set_coupling_device", "fsp_refine_device",
+    "fsp_enable_graded", "fsp_pressure_unknowns_per_column",
     "fsp_project", "fsp_project_device", "fsp_apply_operator_device", "fsp_assemble_device",
     "fsp_precondition_device", "fsp_divergence_device", "fsp_profile_begin", "fsp_profile_end", "fsp_generate",
     "fsp_project_batch", "fsp_autotune", "fsp_step_surface_mobile_device", "fsp_get_surface", "fsp_advect_device",
@@ -125,6 +126,10 @@ def lib():
     L.fsp_set_coupling_device.restype = i32
     L.fsp_refine_device.argtypes = [vp, vp, vp, vp, vp, f32, f32, f64, u32, vp]
     L.fsp_refine_device.restype = i32
+    L.fsp_enable_graded.argtypes = [vp, vp, sz]
+    L.fsp_enable_graded.restype = i32
+    L.fsp_pressure_unknowns_per_column.argtypes = [vp]
+    L.fsp_pressure_unknowns_per_column.restype = i32
     L.fsp_project.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp, sz, i32, vp, sz, vp, sz, vp, sz, f32, f32, f64, u32, vp]
     L.fsp_project.restype = i32
     L.fsp_project_device.argtypes = [vp, vp, vp, vp, vp, i32, vp, vp, vp, f32, f32, f64, u32, vp]

@@ -31,6 +31,7 @@
 #include "kernels_fast.h"
 #include "kernels_coarse.h"
 #include "kernels_step.h"
+#include "kernels_graded.h"
 #include "learned_internal.h"
 
 using namespace fsp;
@@ -59,6 +60,15 @@ struct fsp_solver {
   // background's ghost terms, reserved by its first call
   float *total = nullptr, *bg_up = nullptr, *bg_x = nullptr, *bg_y = nullptr;
   int coupled = 0;
+  // graded column (fsp_enable_graded): node layers, per-layer segment and
+  // weights and per-node weights on the device, the reduced vectors (columns
+  // x nodes each), partials and folded scalars; reserved by its call
+  std::vector<int> gnodes;  // empty: the fine scheme
+  int *g_node = nullptr, *g_seg = nullptr;
+  float *g_wa = nullptr, *g_wb = nullptr, *g_weight = nullptr;
+  float *gx = nullptr, *gr = nullptr, *gd = nullptr, *gq = nullptr, *gb = nullptr, *gm = nullptr, *gz = nullptr;
+  size_t gcap = 0;
+  double *gpart = nullptr, *gout = nullptr, *hgout = nullptr;
   float *eta = nullptr, *eta_rt = nullptr;
   float* col_floor = nullptr;  // per column: top of its highest solid cell (0 without one)
   uint16_t* gk0 = nullptr;  // per column: lowest ghost row
@@ -103,6 +113,19 @@ struct fsp_solver {
   std::vector<hipEvent_t> prof_events;
   size_t prof_used = 0;
   int32_t* prof_active = nullptr;  // per timed launch: the loop-active flag it started with
+
+  GradedCol graded_col() const {
+    GradedCol D;
+    D.cols = (int)ncols;
+    D.nn = (int)gnodes.size();
+    D.nz = dom.nz;
+    D.top = gnodes.empty() ? 0 : gnodes.back();
+    D.node = g_node;
+    D.seg = g_seg;
+    D.wa = g_wa;
+    D.wb = g_wb;
+    return D;
+  }
 
   Fine fine() const {
     Fine G;
@@ -814,6 +837,7 @@ void cleanup(fsp_solver* S) {
   if (S->hctl) (void)hipHostFree(S->hctl);
   if (S->hhash) (void)hipHostFree(S->hhash);
   if (S->hstep) (void)hipHostFree(S->hstep);
+  if (S->hgout) (void)hipHostFree(S->hgout);
   if (S->ev0) (void)hipEventDestroy(S->ev0);
   if (S->ev1) (void)hipEventDestroy(S->ev1);
   if (S->stream) (void)hipStreamDestroy(S->stream);
@@ -990,10 +1014,192 @@ void build_geometry(fsp_solver* S, bool hierarchy) {
   if (hierarchy && S->opt.preconditioner != FSP_PRECOND_JACOBI) K.hierarchy();
 }
 
+// project_mobile3_graded (delta3d_graded.rs:310-447) on device buffers:
+// conjugate gradients on P^T A P x = P^T b preconditioned by the condensed
+// diagonal, f64 dot products folded on the device and read by the host each
+// iteration, the true residual and the restricted divergence metric deciding
+// each pass, the fine pressure P x corrected into u, v, w.
+fsp_status run_graded(fsp_solver* S, const float* us, const float* vs, const float* ws, float* p, int warm, float* u,
+                      float* v, float* w, float scale, float k1, double rel_tol, uint32_t max_iters,
+                      fsp_report* rep) {
+  // a cut or level-set geometry is refused (delta3d_graded.rs:70-72)
+  if (S->cut || S->levelset) return FSP_DOMAIN;
+  const GradedCol D = S->graded_col();
+  const size_t cols = S->ncols, n = cols * (size_t)D.nn, N = S->n;
+  const int nz = S->dom.nz;
+  int j0 = D.nn - 1;
+  while (j0 > 0 && S->gnodes[j0 - 1] + 1 == S->gnodes[j0]) --j0;
+  const int k0 = S->gnodes[j0];
+  Launch K(S);
+  const Fine& G = K.G;
+  hipStream_t st = K.st;
+  FSP_HIP(hipEventRecord(S->ev0, st));
+  build_geometry(S, false);
+  // the guard (delta3d_graded.rs:316-322): the lowest cubic node's layer and
+  // the one above it wet in every column
+  if (k0 + 1 >= nz) return FSP_DOMAIN;
+  {
+    std::vector<uint8_t> rows(2 * cols);
+    FSP_HIP(hipMemcpyAsync(rows.data(), S->mask + (size_t)k0 * cols, 2 * cols, hipMemcpyDeviceToHost, st));
+    FSP_HIP(hipStreamSynchronize(st));
+    for (uint8_t m : rows)
+      if (!(m & M_WET)) return FSP_DOMAIN;
+  }
+  // right-hand side and prec on the fine grid (rhs_mobile3), restricted; the
+  // condensed diagonal from prec
+  if (S->quad && aligned16(vs) && aligned16(ws)) {
+    const dim3 gq((unsigned)(S->qfine.nbx * S->qfine.nby * S->qfine.planes));
+    FSP_LAUNCH(S, k_assemble_q, gq, st, G, S->qfine, us, vs, ws, (const float*)nullptr, scale, S->b, S->d0, S->ctl,
+               S->partials);
+    FSP_FINISH(S, st, OP_B2, gq, false, 0);
+  } else {
+    FSP_LAUNCH(S, k_assemble, dim3(grid_for(N)), st, G, us, vs, ws, (const float*)nullptr, scale, S->b, S->d0,
+               S->ctl, S->partials);
+    FSP_FINISH(S, st, OP_B2, dim3(grid_for(N)), false, 0);
+  }
+  fsp_status sts = sync_ctl(S);
+  if (sts) return sts;
+  if (S->hctl->nonfinite) return FSP_NOT_FINITE;
+  const dim3 gn(grid_for(n)), gN(grid_for(N));
+  FSP_LAUNCH(S, k_gr_restrict<false>, gn, st, D, (const float*)S->b, S->gb);
+  FSP_LAUNCH(S, k_gr_restrict<true>, gn, st, D, (const float*)S->d0, S->gm);
+  // warm start by injection; a dry node starts from 0 (:378-383)
+  if (warm) FSP_LAUNCH(S, k_gr_inject, gn, st, D, (const float*)p, (const float*)S->gm, S->gx);
+  else FSP_LAUNCH(S, k_fill, gn, st, S->gx, n, 0.f);
+  auto dots = [&](const float* a, const float* b, const float* c, const float* e, double& s1,
+                  double& s2) -> fsp_status {
+    hipLaunchKernelGGL(k_gr_dot, gn, dim3(BLOCK), 0, st, n, a, b, c, e, S->gpart);
+    hipLaunchKernelGGL(k_gr_fold<false>, dim3(1), dim3(BLOCK), 0, st, (int)gn.x, (const double*)S->gpart, S->gout);
+    FSP_HIP(hipMemcpyAsync(S->hgout, S->gout, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    FSP_HIP(hipStreamSynchronize(st));
+    s1 = S->hgout[0];
+    s2 = S->hgout[1];
+    return FSP_OK;
+  };
+  // out = P^T A P x through the fine grid (:147-162)
+  auto gapply = [&](const float* x, float* out) {
+    FSP_LAUNCH(S, k_gr_expand, gN, st, D, x, S->d1);
+    FSP_LAUNCH(S, k_apply, gN, st, G, (const float*)S->d1, S->q);
+    FSP_LAUNCH(S, k_gr_restrict<false>, gn, st, D, (const float*)S->q, out);
+  };
+  double b2, unused;
+  if ((sts = dots(S->gb, S->gb, nullptr, nullptr, b2, unused))) return sts;
+  if (!std::isfinite(b2)) return FSP_NOT_FINITE;
+  const double tol = rel_tol * rel_tol;
+  double target = tol * b2, rr = 0.0, rz = 0.0;
+  uint32_t it = 0;
+  bool floor_stop = false, have_ck = false;
+  uint64_t ck = 0;
+  uint32_t power = 1, since = 0;
+  double d_all = 0.0, d_plain = 0.0;
+  std::vector<float> hx;
+  for (;;) {
+    // the true residual, then PCG to the target
+    gapply(S->gx, S->gq);
+    FSP_LAUNCH(S, k_gr_restart, gn, st, n, (const float*)S->gb, (const float*)S->gq, (const float*)S->gm, S->gr,
+               S->gz, S->gd);
+    if ((sts = dots(S->gr, S->gr, S->gr, S->gz, rr, rz))) return sts;
+    while (b2 > 0.0 && rr > target && it < max_iters) {
+      gapply(S->gd, S->gq);
+      double dq;
+      if ((sts = dots(S->gd, S->gq, nullptr, nullptr, dq, unused))) return sts;
+      if (!(dq > 0.0)) break;
+      const float alpha = (float)(rz / dq);
+      FSP_LAUNCH(S, k_gr_update, gn, st, n, alpha, (const float*)S->gd, (const float*)S->gq, (const float*)S->gm,
+                 S->gx, S->gr, S->gz);
+      double zn, rn;
+      if ((sts = dots(S->gr, S->gz, S->gr, S->gr, zn, rn))) return sts;
+      const float beta = (float)(zn / rz);
+      FSP_LAUNCH(S, k_gr_direction, gn, st, n, beta, (const float*)S->gz, S->gd);
+      rz = zn;
+      rr = rn;
+      ++it;
+    }
+    gapply(S->gx, S->gq);
+    FSP_LAUNCH(S, k_gr_restart, gn, st, n, (const float*)S->gb, (const float*)S->gq, (const float*)S->gm, S->gr,
+               (float*)nullptr, (float*)nullptr);
+    FSP_LAUNCH(S, k_gr_expand, gN, st, D, (const float*)S->gx, p);
+    if ((sts = dots(S->gr, S->gr, nullptr, nullptr, rr, unused))) return sts;
+    if (!std::isfinite(rr)) return FSP_NOT_FINITE;
+    // correction from the fine pressure, then the restricted metric (:451-486)
+    FSP_LAUNCH(S, k_gr_widen, gN, st, N, (const float*)p, S->p64);
+    launch_correct(S, us, vs, ws, S->p64, k1, u, v, w);
+    FSP_LAUNCH(S, k_divergence, gN, st, G, (const float*)u, (const float*)v, (const float*)w, S->d1);
+    FSP_LAUNCH(S, k_gr_restrict<false>, gn, st, D, (const float*)S->d1, S->gz);
+    hipLaunchKernelGGL(k_gr_metric, gn, dim3(BLOCK), 0, st, D, (const float*)S->gz, (const float*)S->g_weight,
+                       (const uint8_t*)S->mask, j0, S->gpart);
+    hipLaunchKernelGGL(k_gr_fold<true>, dim3(1), dim3(BLOCK), 0, st, (int)gn.x, (const double*)S->gpart, S->gout);
+    const dim3 gf(grid_for(S->nu + S->nv + S->nw));
+    hipLaunchKernelGGL(k_gr_absmax, gf, dim3(BLOCK), 0, st, (const float*)u, S->nu, (const float*)v, S->nv,
+                       (const float*)w, S->nw, S->gpart);
+    hipLaunchKernelGGL(k_gr_fold<true>, dim3(1), dim3(BLOCK), 0, st, (int)gf.x, (const double*)S->gpart,
+                       S->gout + 2);
+    FSP_HIP(hipMemcpyAsync(S->hgout, S->gout, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+    FSP_HIP(hipStreamSynchronize(st));
+    {
+      const float mx = (float)S->hgout[2];
+      const float ratio = mx > 0.f ? S->dom.dx / std::max(mx, 1e-4f) : 0.f;
+      d_all = (double)((float)S->hgout[0] * ratio);
+      d_plain = (double)((float)S->hgout[1] * ratio);
+    }
+    const bool converged = rr <= tol * b2;
+    if (b2 == 0.0 || (converged && d_plain <= 1e-5) || it >= max_iters) break;
+    // a state that comes back bit for bit is the floor (:430-438)
+    hx.resize(n);
+    FSP_HIP(hipMemcpy(hx.data(), S->gx, n * sizeof(float), hipMemcpyDeviceToHost));
+    uint64_t state = 0xcbf29ce484222325ull;
+    for (float xv : hx) {
+      uint32_t bits;
+      std::memcpy(&bits, &xv, 4);
+      for (int b = 0; b < 4; ++b) {
+        state ^= (bits >> (8 * b)) & 0xFFu;
+        state *= 0x100000001b3ull;
+      }
+    }
+    if (have_ck && ck == state) {
+      floor_stop = true;
+      break;
+    }
+    ++since;
+    if (since == power) {
+      have_ck = true;
+      ck = state;
+      power = power > 0x7fffffffu ? 0xffffffffu : power * 2;
+      since = 0;
+    }
+    if (converged) {
+      const double ratio = 1e-5 / std::max(d_plain, std::numeric_limits<double>::min());
+      target = rr * ratio * ratio;
+    } else {
+      target = tol * b2;
+    }
+  }
+  const bool accepted = b2 == 0.0 || ((rr <= tol * b2 || floor_stop) && d_plain <= 1e-5);
+  FSP_HIP(hipEventRecord(S->ev1, st));
+  FSP_HIP(hipEventSynchronize(S->ev1));
+  float ms = 0.f;
+  FSP_HIP(hipEventElapsedTime(&ms, S->ev0, S->ev1));
+  fsp_report R{};
+  R.refinements = 0;
+  R.iterations = it;
+  R.degraded = accepted ? 0 : 1;
+  R.residual = b2 > 0.0 ? std::sqrt(rr / b2) : 0.0;
+  R.divergence = d_all;
+  R.floor = floor_stop ? 1 : 0;
+  R.backward_error = 0.0;
+  R.divergence_plain = d_plain;
+  R.ms_device = ms;
+  if (rep) *rep = R;
+  if (R.degraded && S->opt.refuse_degraded) return FSP_CONVERGENCE;
+  return FSP_OK;
+}
+
 // The projection on device buffers (project_mobile3, delta3d_mobile.rs:465-609).
 fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws, float* p, int warm, float* u,
                float* v, float* w, float scale, float k1, double rel_tol, uint32_t max_iters, fsp_report* rep) {
   if (!S->geom) return FSP_DOMAIN;
+  // the graded column takes the projection (delta3d_mobile.rs:473-475)
+  if (!S->gnodes.empty()) return run_graded(S, us, vs, ws, p, warm, u, v, w, scale, k1, rel_tol, max_iters, rep);
   S->net_phase = 0;
   if (S->opt.preconditioner == FSP_PRECOND_LEARNED) {
     if (!S->net) return FSP_INVALID;
@@ -1647,6 +1853,67 @@ fsp_status fsp_set_coupling_device(fsp_solver* S, const float* d_total, const fl
   if (!S) return FSP_INVALID;
   return set_coupling(S, d_total, S->ncols, d_bg_up, S->ncols, d_bg_x, S->nu, d_bg_y, S->nv,
                       hipMemcpyDeviceToDevice);
+}
+
+fsp_status fsp_enable_graded(fsp_solver* S, const int32_t* nodes, size_t count) {
+  if (!S) return FSP_INVALID;
+  if (!nodes || count == 0) {
+    S->gnodes.clear();
+    return FSP_OK;
+  }
+  const int nz = S->dom.nz;
+  // the nodes start at layer 0, end at the top layer and increase strictly
+  // (delta3d_graded.rs:66-68)
+  if (nodes[0] != 0 || nodes[count - 1] != nz - 1) return FSP_SHAPE;
+  for (size_t q = 1; q < count; ++q)
+    if (nodes[q] <= nodes[q - 1]) return FSP_SHAPE;
+  // no cut geometry (:70-72)
+  if (S->geom && (S->cut || S->levelset)) return FSP_DOMAIN;
+  FSP_HIP(hipSetDevice(S->opt.device));
+  const size_t len = S->ncols * count;
+  if (len > S->gcap) {
+    float** vecs[7] = {&S->gx, &S->gr, &S->gd, &S->gq, &S->gb, &S->gm, &S->gz};
+    for (float** v : vecs)
+      if (fsp_status as = dalloc(S, v, len)) return as;
+    S->gcap = len;
+  }
+  if (!S->g_node) {
+    fsp_status as;
+    if ((as = dalloc(S, &S->g_node, (size_t)nz)) || (as = dalloc(S, &S->g_seg, (size_t)nz)) ||
+        (as = dalloc(S, &S->g_wa, (size_t)nz)) || (as = dalloc(S, &S->g_wb, (size_t)nz)) ||
+        (as = dalloc(S, &S->g_weight, (size_t)nz)) || (as = dalloc(S, &S->gpart, (size_t)2 * MAX_PARTIALS)) ||
+        (as = dalloc(S, &S->gout, (size_t)4)))
+      return as;
+    FSP_HIP(hipHostMalloc((void**)&S->hgout, 4 * sizeof(double), hipHostMallocDefault));
+  }
+  // per layer its segment and (1 - t, t), t = (k - a) / (b - a) in f32
+  // (:48-53); per node its weight sum_k P_kj in the reference's order (:81-91)
+  std::vector<int> seg((size_t)nz, 0);
+  std::vector<float> wa((size_t)nz, 0.f), wb((size_t)nz, 0.f), weight(count, 0.f);
+  for (size_t sg = 0; sg + 1 < count; ++sg) {
+    const int a = nodes[sg], b = nodes[sg + 1];
+    for (int k = a; k < b; ++k) {
+      const float t = (float)(k - a) / (float)(b - a);
+      wa[k] = 1.0f - t;
+      wb[k] = t;
+      seg[k] = (int)sg;
+      weight[sg] += wa[k];
+      weight[sg + 1] += wb[k];
+    }
+  }
+  weight[count - 1] += 1.0f;
+  FSP_HIP(hipMemcpy(S->g_node, nodes, count * sizeof(int32_t), hipMemcpyHostToDevice));
+  FSP_HIP(hipMemcpy(S->g_seg, seg.data(), (size_t)nz * sizeof(int), hipMemcpyHostToDevice));
+  FSP_HIP(hipMemcpy(S->g_wa, wa.data(), (size_t)nz * sizeof(float), hipMemcpyHostToDevice));
+  FSP_HIP(hipMemcpy(S->g_wb, wb.data(), (size_t)nz * sizeof(float), hipMemcpyHostToDevice));
+  FSP_HIP(hipMemcpy(S->g_weight, weight.data(), count * sizeof(float), hipMemcpyHostToDevice));
+  S->gnodes.assign(nodes, nodes + count);
+  return FSP_OK;
+}
+
+int32_t fsp_pressure_unknowns_per_column(const fsp_solver* S) {
+  if (!S) return -1;
+  return S->gnodes.empty() ? S->dom.nz : (int32_t)S->gnodes.size();
 }
 
 fsp_status fsp_refine_device(fsp_solver* S, float* d_p, float* d_u, float* d_v, float* d_w, float scale, float k1,

@@ -215,6 +215,22 @@ class Volume3:
         _raise(_lib.lib().fsp_set_coupling(self._h, _ptr(total), total.size, _ptr(bg_up), sz(bg_up), _ptr(bg_x),
                                            sz(bg_x), _ptr(bg_y), sz(bg_y)), "set_coupling")
 
+    def enable_graded(self, nodes=None):
+        """Volume3::enable_graded (delta3d_graded.rs:64-100): the pressure at
+        the node layers only (from 0 to nz - 1, increasing), linear between
+        them; None returns to the fine scheme."""
+        self._upload_geometry()
+        if nodes is None:
+            _raise(_lib.lib().fsp_enable_graded(self._h, None, 0), "enable_graded")
+            return
+        arr = np.ascontiguousarray(nodes, np.int32)
+        self._graded_nodes = arr
+        _raise(_lib.lib().fsp_enable_graded(self._h, _ptr(arr), arr.size), "enable_graded")
+
+    def pressure_unknowns_per_column(self) -> int:
+        """delta3d_graded.rs:102-105."""
+        return _lib.lib().fsp_pressure_unknowns_per_column(self._h)
+
     def set_surface_mode(self, homogeneous_ghost: bool, closed_lid: bool = False):
         """homogeneous_ghost: p = 0 at the surface (delta3d_coupling.rs:226-249); closed_lid: Neumann top."""
         self.homogeneous_ghost = bool(homogeneous_ghost)

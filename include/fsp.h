@@ -180,6 +180,25 @@ fsp_status fsp_set_coupling(fsp_solver* s, const float* total, size_t n_total, c
 fsp_status fsp_set_coupling_device(fsp_solver* s, const float* d_total, const float* d_bg_up, const float* d_bg_x,
                                    const float* d_bg_y);
 
+/* Replaces Volume3::enable_graded (delta3d_graded.rs:64-100): the graded
+ * column. Below the cubic top layers the pressure lives only at the node
+ * layers `nodes` (fine layers from 0 to nz - 1, strictly increasing:
+ * FSP_SHAPE otherwise) and is linear between two nodes; fsp_project and
+ * fsp_project_device then solve P^T A P x = P^T b with the condensed
+ * diagonal as preconditioner and correct u, v, w from P x, as
+ * project_mobile3_graded does (:310-447), reporting the restricted divergence
+ * metric (:451-486) and backward_error 0. Refused on a cut or level-set
+ * geometry (FSP_DOMAIN, here and at the projection); the projection refuses
+ * a surface that leaves the lowest cubic node's layer or the one above it
+ * dry in some column (FSP_DOMAIN, nothing written). nodes == NULL or
+ * count == 0 returns to the fine scheme. A configuration call: it reserves
+ * seven vectors of nx*ny*count floats the first time (or when count grows). */
+fsp_status fsp_enable_graded(fsp_solver* s, const int32_t* nodes, size_t count);
+
+/* Volume3::pressure_unknowns_per_column (delta3d_graded.rs:102-105): the
+ * graded column's nodes, or nz. */
+int32_t fsp_pressure_unknowns_per_column(const fsp_solver* s);
+
 /* Replaces Volume3::refine_mobile3 (delta3d_coupling.rs:226-249), the
  * coupled step's second solve after a degraded floor stop (:836-840): the
  * corrected faces d_u/d_v/d_w become the predicted faces, the pressure is

@@ -41,15 +41,31 @@ void couple(Volume<T>& vol, const Coupling* cpl) {
 // refinement after a degraded floor stop with the first solve's iterations
 // added (delta3d_coupling.rs:835-840); 2 a refinement always. refine_iters
 // bounds the refinement (the reference's max_iters when 0).
+// The graded column's nodes (fine layers, bottom up), or none.
+struct Nodes {
+  const int32_t* layers;
+  size_t count;
+};
 template <class T>
-int project(const Geometry* g, const Options* o, const Coupling* cpl, const float* us, const float* vs,
-            const float* ws, const float* p_warm, float scale, float k1, uint32_t max_iters, int refine,
-            uint32_t refine_iters, float* p_out,
+Status grade(Volume<T>& vol, const Nodes* nodes) {
+  if (!nodes || !nodes->layers) return OK;
+  std::vector<size_t> k(nodes->count);
+  for (size_t q = 0; q < nodes->count; ++q) {
+    if (nodes->layers[q] < 0) return ERR_SHAPE;
+    k[q] = (size_t)nodes->layers[q];
+  }
+  return vol.enable_graded(k.data(), k.size());
+}
+template <class T>
+int project(const Geometry* g, const Options* o, const Coupling* cpl, const Nodes* nodes, const float* us,
+            const float* vs, const float* ws, const float* p_warm, float scale, float k1, uint32_t max_iters,
+            int refine, uint32_t refine_iters, float* p_out,
             float* u_out, float* v_out, float* w_out, double* p_out64, Report* rep) {
   Volume<T> vol;
   Status st = setup(vol, g, o);
   if (st) return st;
   if (cpl && cpl->total) couple(vol, cpl);
+  if ((st = grade(vol, nodes))) return st;
   for (size_t q = 0; q < vol.nu(); ++q) vol.us[q] = (T)us[q];
   for (size_t q = 0; q < vol.nv(); ++q) vol.vs[q] = (T)vs[q];
   for (size_t q = 0; q < vol.nw(); ++q) vol.ws[q] = (T)ws[q];
@@ -310,10 +326,10 @@ int orc_project(int precision, const Geometry* g, const Options* o, const float*
                 const float* p_warm, float scale, float k1, uint32_t max_iters, float* p_out, float* u_out,
                 float* v_out, float* w_out, double* p_out64, Report* rep) {
   if (precision == 64)
-    return project<double>(g, o, nullptr, us, vs, ws, p_warm, scale, k1, max_iters, 0, 0, p_out, u_out, v_out,
-                           w_out, p_out64, rep);
-  return project<float>(g, o, nullptr, us, vs, ws, p_warm, scale, k1, max_iters, 0, 0, p_out, u_out, v_out, w_out,
-                        p_out64, rep);
+    return project<double>(g, o, nullptr, nullptr, us, vs, ws, p_warm, scale, k1, max_iters, 0, 0, p_out, u_out,
+                           v_out, w_out, p_out64, rep);
+  return project<float>(g, o, nullptr, nullptr, us, vs, ws, p_warm, scale, k1, max_iters, 0, 0, p_out, u_out, v_out,
+                        w_out, p_out64, rep);
 }
 
 // The projection path of step_perturbation_mobile (delta3d_coupling.rs:
@@ -324,10 +340,72 @@ int orc_project_coupled(int precision, const Geometry* g, const Options* o, cons
                         uint32_t max_iters, int refine, uint32_t refine_iters, float* p_out, float* u_out,
                         float* v_out, float* w_out, double* p_out64, Report* rep) {
   if (precision == 64)
-    return project<double>(g, o, cpl, us, vs, ws, p_warm, scale, k1, max_iters, refine, refine_iters, p_out, u_out,
-                           v_out, w_out, p_out64, rep);
-  return project<float>(g, o, cpl, us, vs, ws, p_warm, scale, k1, max_iters, refine, refine_iters, p_out, u_out,
-                        v_out, w_out, p_out64, rep);
+    return project<double>(g, o, cpl, nullptr, us, vs, ws, p_warm, scale, k1, max_iters, refine, refine_iters, p_out,
+                           u_out, v_out, w_out, p_out64, rep);
+  return project<float>(g, o, cpl, nullptr, us, vs, ws, p_warm, scale, k1, max_iters, refine, refine_iters, p_out,
+                        u_out, v_out, w_out, p_out64, rep);
+}
+
+// project_mobile3 over the graded column (delta3d_graded.rs:310-447), with an
+// optional coupled surface.
+int orc_project_graded(int precision, const Geometry* g, const Options* o, const Coupling* cpl, const Nodes* nodes,
+                       const float* us, const float* vs, const float* ws, const float* p_warm, float scale, float k1,
+                       uint32_t max_iters, float* p_out, float* u_out, float* v_out, float* w_out, double* p_out64,
+                       Report* rep) {
+  if (!nodes || !nodes->layers) return ERR_SHAPE;
+  if (precision == 64)
+    return project<double>(g, o, cpl, nodes, us, vs, ws, p_warm, scale, k1, max_iters, 0, 0, p_out, u_out, v_out,
+                           w_out, p_out64, rep);
+  return project<float>(g, o, cpl, nodes, us, vs, ws, p_warm, scale, k1, max_iters, 0, 0, p_out, u_out, v_out, w_out,
+                        p_out64, rep);
+}
+
+// The graded column's operators (delta3d_graded.rs:109-162) in precision T:
+// op 0 fine = P x, 1 x = P^T fine, 2 x = P^T A P x, 3 the nodes' weights,
+// 4 the graded right-hand side P^T b of the predicted faces `in` (u, v, w
+// one after the other) and `scale`.
+int orc_graded_op(int precision, const Geometry* g, const Nodes* nodes, int op, const double* in, double* out,
+                  double scale) {
+  auto run = [&](auto tag) -> int {
+    using T = decltype(tag);
+    Volume<T> vol;
+    Options opt = default_options();
+    opt.multigrid = 0;
+    Status st = vol.configure(*g, opt);
+    if (st) return st;
+    if (!nodes || !nodes->layers) return ERR_SHAPE;
+    if ((st = grade(vol, nodes))) return st;
+    const size_t n = vol.graded_len(), cells = vol.cells();
+    if (op == 0) {
+      std::vector<T> x(in, in + n), fine(cells);
+      vol.graded_expand(x, fine);
+      std::copy(fine.begin(), fine.end(), out);
+    } else if (op == 1) {
+      std::vector<T> fine(in, in + cells), x(n);
+      vol.graded_restrict(fine, x);
+      std::copy(x.begin(), x.end(), out);
+    } else if (op == 2) {
+      std::vector<T> x(in, in + n), y(n);
+      vol.graded_apply(x, y);
+      std::copy(y.begin(), y.end(), out);
+    } else if (op == 3) {
+      std::copy(vol.gweight.begin(), vol.gweight.end(), out);
+    } else if (op == 4) {
+      const size_t nu = vol.nu(), nv = vol.nv(), nw = vol.nw();
+      vol.us.assign(in, in + nu);
+      vol.vs.assign(in + nu, in + nu + nv);
+      vol.ws.assign(in + nu + nv, in + nu + nv + nw);
+      vol.divergence(vol.us, vol.vs, vol.ws, vol.rhs);
+      vol.rhs_mobile3((T)scale);
+      std::vector<T> x(n);
+      vol.graded_restrict(vol.rhs, x);
+      std::copy(x.begin(), x.end(), out);
+    } else {
+      return ERR_DOMAIN;
+    }
+    return OK;
+  };
+  return precision == 64 ? run(double()) : run(float());
 }
 
 // Operator action (delta3d_mobile.rs:936-949).

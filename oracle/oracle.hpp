@@ -28,6 +28,7 @@
 // sweep count, pure-Neumann gauge) are marked EXTENSION; with their default
 // settings the oracle is the reference algorithm.
 #pragma once
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -920,6 +921,7 @@ template <class T> class Volume {
   // delta3d_mobile.rs:465-609. On entry us/vs/ws hold the predicted faces and
   // p the warm start.
   Status project_mobile3(T scale, T k1, uint32_t max_iters, Report& rep) {
+    if (!gnodes.empty()) return project_mobile3_graded(scale, k1, max_iters, rep);  // delta3d_mobile.rs:473-475
     divergence(us, vs, ws, rhs);
     rhs_mobile3(scale);
     if (mg_on) prepare_multigrid3();
@@ -1065,6 +1067,205 @@ template <class T> class Volume {
     if (st) return st;
     for (size_t q = 0; q < p.size(); ++q) p[q] += base[q];
     rep.refinements = 1;
+    return OK;
+  }
+
+  // ---------------------------------------------------------------- the graded column
+  // delta3d_graded.rs: below the cubic top layers the pressure lives only at
+  // node layers (the same for every column) and is linear between two nodes;
+  // the projection solves P^T A P x = P^T b (SPD), preconditioned by the
+  // condensed diagonal sum_k P_kj^2 A_kk, and the velocities stay fine.
+  std::vector<size_t> gnodes;  // empty: the fine scheme
+  std::vector<T> gweight;      // sum_k P_kj per node
+
+  // delta3d_graded.rs:48-53: (1 - t, t), t = (k - a) / (b - a)
+  static void gweights(size_t k, size_t a, size_t b, T& wa, T& wb) {
+    const T t = (T)(k - a) / (T)(b - a);
+    wa = T(1) - t;
+    wb = t;
+  }
+  // enable_graded (delta3d_graded.rs:64-100)
+  Status enable_graded(const size_t* nodes, size_t count) {
+    if (count == 0 || nodes[0] != 0 || nodes[count - 1] != nz - 1) return ERR_SHAPE;
+    for (size_t q = 1; q < count; ++q)
+      if (nodes[q] <= nodes[q - 1]) return ERR_SHAPE;
+    if (cut) return ERR_DOMAIN;
+    gnodes.assign(nodes, nodes + count);
+    gweight.assign(count, T(0));
+    for (size_t s = 0; s + 1 < count; ++s)
+      for (size_t k = gnodes[s]; k < gnodes[s + 1]; ++k) {
+        T wa, wb;
+        gweights(k, gnodes[s], gnodes[s + 1], wa, wb);
+        gweight[s] += wa;
+        gweight[s + 1] += wb;
+      }
+    gweight[count - 1] += T(1);
+    return OK;
+  }
+  size_t graded_len() const { return nx * ny * gnodes.size(); }
+  // fine = P x (delta3d_graded.rs:109-126)
+  void graded_expand(const std::vector<T>& x, std::vector<T>& fine) const {
+    const size_t cols = nx * ny, last = gnodes.size() - 1;
+    for (size_t s = 0; s < last; ++s)
+      for (size_t k = gnodes[s]; k < gnodes[s + 1]; ++k) {
+        T wa, wb;
+        gweights(k, gnodes[s], gnodes[s + 1], wa, wb);
+        for (size_t cc = 0; cc < cols; ++cc) fine[k * cols + cc] = wa * x[s * cols + cc] + wb * x[(s + 1) * cols + cc];
+      }
+    for (size_t cc = 0; cc < cols; ++cc) fine[gnodes[last] * cols + cc] = x[last * cols + cc];
+  }
+  // out = P^T fine, the reference's summation order (delta3d_graded.rs:128-145);
+  // square: out = sum_k (w w) fine_k, the condensed diagonal's weights (:363-377)
+  void graded_restrict(const std::vector<T>& fine, std::vector<T>& out, bool square = false) const {
+    const size_t cols = nx * ny, last = gnodes.size() - 1;
+    std::fill(out.begin(), out.end(), T(0));
+    for (size_t s = 0; s < last; ++s)
+      for (size_t k = gnodes[s]; k < gnodes[s + 1]; ++k) {
+        T wa, wb;
+        gweights(k, gnodes[s], gnodes[s + 1], wa, wb);
+        if (square) { wa = wa * wa; wb = wb * wb; }
+        for (size_t cc = 0; cc < cols; ++cc) {
+          out[s * cols + cc] += wa * fine[k * cols + cc];
+          out[(s + 1) * cols + cc] += wb * fine[k * cols + cc];
+        }
+      }
+    for (size_t cc = 0; cc < cols; ++cc) out[last * cols + cc] += fine[gnodes[last] * cols + cc];
+  }
+  // out = P^T A P x through the fine grid (:147-162), the mobile operator
+  void graded_apply(const std::vector<T>& x, std::vector<T>& out) {
+    graded_expand(x, dir);
+    apply_mobile3(dir, tmp);
+    graded_restrict(tmp, out);
+  }
+  static double dot_f64(const std::vector<T>& a, const std::vector<T>& b) {
+    double s = 0.0;
+    for (size_t q = 0; q < a.size(); ++q) s += (double)a[q] * (double)b[q];
+    return s;
+  }
+  // the lowest cubic node: from it the nodes are consecutive to the top (:300-307)
+  size_t cubic_node() const {
+    size_t j = gnodes.size() - 1;
+    while (j > 0 && gnodes[j - 1] + 1 == gnodes[j]) --j;
+    return j;
+  }
+  // graded_divergence_mobile (:451-486): P^T div u over each node's weight,
+  // on wet node cells; plain rows: graded nodes and the lowest cubic node, and
+  // above it the rows without a ghost face
+  void graded_divergence_mobile(size_t j0, std::vector<T>& restricted, double& all_out, double& plain_out) {
+    const size_t cols = nx * ny;
+    divergence(u, v, w, tmp);
+    graded_restrict(tmp, restricted);
+    T all = T(0), plain = T(0);
+    for (size_t s = 0; s < gnodes.size(); ++s) {
+      const size_t k = gnodes[s];
+      for (size_t cc = 0; cc < cols; ++cc) {
+        const size_t i = cc % nx, j = cc / nx;
+        if (!wet3(i, j, k)) continue;
+        const T d = std::fabs(restricted[s * cols + cc]) / gweight[s];
+        all = tmax(all, d);
+        bool ghost = false;
+        if (s > j0) {
+          Row row = mobile_row(i, j, k);
+          for (int f = 0; f < 6; ++f) if (row.n[f] < 0 && row.a[f] > T(0)) ghost = true;
+        }
+        if (!ghost) plain = tmax(plain, d);
+      }
+    }
+    T mx = T(0);
+    for (T x : u) mx = tmax(mx, std::fabs(x));
+    for (T x : v) mx = tmax(mx, std::fabs(x));
+    for (T x : w) mx = tmax(mx, std::fabs(x));
+    const T ratio = mx > T(0) ? dx / tmax(mx, (T)1e-4) : T(0);
+    all_out = (double)(all * ratio);
+    plain_out = (double)(plain * ratio);
+  }
+  // project_mobile3_graded (delta3d_graded.rs:310-447); p holds the warm start
+  Status project_mobile3_graded(T scale, T k1, uint32_t max_iters, Report& rep) {
+    const size_t cols = nx * ny, n = graded_len(), j0 = cubic_node(), k0 = gnodes[j0];
+    bool inside = k0 + 1 < nz;
+    for (size_t j = 0; j < ny && inside; ++j)
+      for (size_t i = 0; i < nx && inside; ++i)
+        if (!wet3(i, j, k0) || !wet3(i, j, k0 + 1)) inside = false;
+    if (!inside) return ERR_DOMAIN;
+    divergence(us, vs, ws, rhs);
+    rhs_mobile3(scale);
+    std::vector<T> gx(n), gr(n), gd(n), gq(n), gb(n), gm(n), gz(n);
+    graded_restrict(rhs, gb);
+    // condensed diagonal: A_kk = 1 / prec_k on a wet cell, 0 on a dry one
+    for (size_t cc = 0; cc < cells(); ++cc) tmp[cc] = prec[cc] > T(0) ? T(1) / prec[cc] : T(0);
+    graded_restrict(tmp, gm, true);
+    // warm start by injection; a dry node starts from 0
+    for (size_t s = 0; s < gnodes.size(); ++s)
+      for (size_t cc = 0; cc < cols; ++cc)
+        gx[s * cols + cc] = gm[s * cols + cc] > T(0) ? p[gnodes[s] * cols + cc] : T(0);
+    const double b2 = dot_f64(gb, gb);
+    const double tol = opt.rel_tol == 1e-6 ? 1e-12 : opt.rel_tol * opt.rel_tol;
+    double target = tol * b2, rr = 0.0;
+    uint32_t it = 0;
+    bool floor_stop = false, have_ck = false;
+    uint64_t ck = 0;
+    uint32_t power = 1, since = 0;
+    double d_all = 0.0, d_plain = 0.0;
+    for (;;) {
+      // the true residual, then PCG to the target
+      graded_apply(gx, gq);
+      for (size_t q = 0; q < n; ++q) gr[q] = gb[q] - gq[q];
+      rr = dot_f64(gr, gr);
+      for (size_t q = 0; q < n; ++q) {
+        gz[q] = gm[q] > T(0) ? gr[q] / gm[q] : T(0);
+        gd[q] = gz[q];
+      }
+      double rz = dot_f64(gr, gz);
+      while (b2 > 0.0 && rr > target && it < max_iters) {
+        graded_apply(gd, gq);
+        const double dq = dot_f64(gd, gq);
+        if (!(dq > 0.0)) break;
+        const T alpha = (T)(rz / dq);
+        for (size_t q = 0; q < n; ++q) {
+          gx[q] += alpha * gd[q];
+          gr[q] -= alpha * gq[q];
+          gz[q] = gm[q] > T(0) ? gr[q] / gm[q] : T(0);
+        }
+        const double zn = dot_f64(gr, gz);
+        const T beta = (T)(zn / rz);
+        for (size_t q = 0; q < n; ++q) gd[q] = gz[q] + beta * gd[q];
+        rz = zn;
+        rr = dot_f64(gr, gr);
+        ++it;
+      }
+      graded_apply(gx, gq);
+      for (size_t q = 0; q < n; ++q) gr[q] = gb[q] - gq[q];
+      graded_expand(gx, p);
+      rr = dot_f64(gr, gr);
+      correct_mobile3(k1);
+      graded_divergence_mobile(j0, gz, d_all, d_plain);
+      const bool converged = rr <= tol * b2;
+      if (b2 == 0.0 || (converged && d_plain <= 1e-5) || it >= max_iters) break;
+      const uint64_t state = fingerprint(gx);
+      if (have_ck && ck == state) { floor_stop = true; break; }
+      ++since;
+      if (since == power) {
+        have_ck = true;
+        ck = state;
+        power = power > 0x7fffffffu ? 0xffffffffu : power * 2;
+        since = 0;
+      }
+      if (converged) {
+        const double ratio = 1e-5 / std::max(d_plain, std::numeric_limits<double>::min());
+        target = rr * ratio * ratio;
+      } else {
+        target = tol * b2;
+      }
+    }
+    const bool accepted = b2 == 0.0 || ((rr <= tol * b2 || floor_stop) && d_plain <= 1e-5);
+    rep.refinements = 0;
+    rep.iterations = it;
+    rep.degraded = accepted ? 0 : 1;
+    rep.residual = b2 > 0.0 ? std::sqrt(rr / b2) : 0.0;
+    rep.divergence = d_all;
+    rep.floor = floor_stop ? 1 : 0;
+    rep.backward_error = 0.0;
+    rep.divergence_plain = d_plain;
     return OK;
   }
 

@@ -65,6 +65,8 @@ def lib():
         vp, f32, u32, i32, sz = C.c_void_p, C.c_float, C.c_uint32, C.c_int, C.c_size_t
         L.orc_project.argtypes = [i32, vp, vp, vp, vp, vp, vp, f32, f32, u32, vp, vp, vp, vp, vp, vp]
         L.orc_apply.argtypes = [vp, vp, vp]
+        L.orc_project_graded.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, f32, f32, u32, vp, vp, vp, vp, vp, vp]
+        L.orc_graded_op.argtypes = [i32, vp, vp, i32, vp, vp, C.c_double]
         L.orc_project_coupled.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, f32, f32, u32, i32, u32, vp, vp, vp, vp, vp,
                                           vp]
         L.orc_cut.argtypes = [i32, i32, i32, f32, vp, vp, vp, vp, vp, vp]
@@ -223,6 +225,61 @@ def project_coupled(prob: Problem, us, vs, ws, scale, k1, total, bg_up=None, bg_
         raise RuntimeError(f"oracle project_coupled failed: status {st}")
     out = (rep.as_dict(), p, u, v, w)
     return out + (p64,) if precision == 64 else out
+
+
+class Nodes(C.Structure):
+    _fields_ = [("layers", C.c_void_p), ("count", C.c_size_t)]
+
+
+def _nodes(nodes):
+    arr = np.ascontiguousarray(nodes, np.int32)
+    return Nodes(arr.ctypes.data, arr.size), arr
+
+
+def project_graded(prob: Problem, us, vs, ws, scale, k1, nodes, coupling=None, max_iters=20000, opts=None,
+                   p_warm=None, precision=32):
+    """project_mobile3 over the graded column on `nodes` (fine layers, from 0
+    to nz - 1, increasing; delta3d_graded.rs:310-447), coupling (total, bg_up,
+    bg_x, bg_y) or None; returns (report dict, p, u, v, w[, p64])."""
+    g = prob.geometry()
+    o = opts if opts is not None else options()
+    nu, nv, nw = prob.face_counts
+    arr = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+    keep = [arr(x) for x in (coupling or ())]
+    cpl = Coupling(*[_ptr(x) for x in keep]) if coupling is not None else None
+    nd, nd_keep = _nodes(nodes)
+    us, vs, ws, pw = arr(us), arr(vs), arr(ws), arr(p_warm)
+    p = np.zeros(prob.cells, np.float32)
+    u, v, w = np.zeros(nu, np.float32), np.zeros(nv, np.float32), np.zeros(nw, np.float32)
+    p64 = np.zeros(prob.cells, np.float64) if precision == 64 else None
+    rep = Report()
+    st = lib().orc_project_graded(precision, C.byref(g), C.byref(o), None if cpl is None else C.byref(cpl),
+                                  C.byref(nd), _ptr(us), _ptr(vs), _ptr(ws), _ptr(pw), scale, k1, max_iters, _ptr(p),
+                                  _ptr(u), _ptr(v), _ptr(w), _ptr(p64), C.byref(rep))
+    if st != 0:
+        raise RuntimeError(f"oracle project_graded failed: status {st}")
+    out = (rep.as_dict(), p, u, v, w)
+    return out + (p64,) if precision == 64 else out
+
+
+GRADED_EXPAND, GRADED_RESTRICT, GRADED_APPLY, GRADED_WEIGHTS, GRADED_RHS = 0, 1, 2, 3, 4
+
+
+def graded_op(prob: Problem, nodes, op, x=None, precision=64, scale=1.0):
+    """The graded column's P (op 0: graded -> cells), P^T (1: cells ->
+    graded), P^T A P (2), node weights (3) and right-hand side P^T b of the
+    predicted faces x = (u, v, w) concatenated (4), computed in f64 or f32;
+    a non-OK status raises RuntimeError."""
+    g = prob.geometry()
+    nd, nd_keep = _nodes(nodes)
+    n = prob.nx * prob.ny * len(nd_keep)
+    size = {0: prob.cells, 1: n, 2: n, 3: len(nd_keep), 4: n}[op]
+    x = None if x is None else np.ascontiguousarray(x, np.float64)
+    out = np.zeros(size, np.float64)
+    st = lib().orc_graded_op(precision, C.byref(g), C.byref(nd), op, _ptr(x), _ptr(out), scale)
+    if st != 0:
+        raise RuntimeError(f"oracle graded_op failed: status {st}")
+    return out
 
 
 def apply(prob: Problem, x):
