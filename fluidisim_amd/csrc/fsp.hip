@@ -97,8 +97,11 @@ struct fsp_solver {
   std::vector<void*> allocs;
   // live timing of one kernel (fsp_profile_begin / fsp_profile_end)
   // one CG iteration captured as a graph, replayed per iteration
-  hipGraph_t iter_graph[2] = {};      // one CG iteration: [0] as configured, [1] learned phase
-  hipGraphExec_t iter_exec[2] = {};
+  // one CG iteration per (phase, kernel form): [2 * phase + table_only],
+  // phase 0 as configured, 1 the learned phase
+  hipGraph_t iter_graph[4] = {};
+  hipGraphExec_t iter_exec[4] = {};
+  int table_only = 0;                 // 1: the fine kernels' table-only forms (set per projection)
   fsp_net* net = nullptr;             // learned preconditioner (not owned)
   uint64_t net_gen = 0;               // the network's buffer generation iter_graph[1] captured
   int net_phase = 0;                  // 1 while the current pass uses the network
@@ -226,9 +229,16 @@ inline bool prof_begin(fsp_solver* S, const char* name, hipStream_t stream, hipE
   FSP_LAUNCH_NBS(S, name, kern, grid, BLOCK, lds, stream, __VA_ARGS__)
 #define FSP_LAUNCH(S, kern, grid, stream, ...) FSP_LAUNCH_N(S, #kern, kern, grid, stream, __VA_ARGS__)
 
+// A fine-level kernel in its table-only form (template argument GEN false:
+// every ghost row of the geometry lies in its column's table, known after
+// the projection's first sync) or its general form.
+#define FSP_LAUNCH_GEN(S, name, KT, KG, grid, block, stream, ...)                             \
+  do {                                                                                     \
+    if ((S)->table_only) FSP_LAUNCH_NBS(S, name, KT, grid, block, 0, stream, __VA_ARGS__); \
+    else FSP_LAUNCH_NBS(S, name, KG, grid, block, 0, stream, __VA_ARGS__);                 \
+  } while (0)
+
 // Kernels templated on the V-cycle storage ST of the enclosing function.
-#define K_QGS(M) (k_q_gs<M, ST>)
-#define K_QSPMV (k_q_spmv_dir<ST>)
 #define K_CGS (k_cgs<ST>)
 #define K_CRESTRICT (k_crestrict<ST>)
 #define K_CPROLONG (k_cprolong<ST>)
@@ -326,7 +336,7 @@ struct Launch {
     }
     // ghost-row table from the fresh row codes (k_mask's finish does not touch it)
     Fine Gm = G;
-    FSP_LAUNCH(S, k_ghost_table, dim3(grid_for(S->ncols)), st, Gm, S->gk0, S->gtab);
+    FSP_LAUNCH(S, k_ghost_table, dim3(grid_for(S->ncols)), st, Gm, S->gk0, S->gtab, S->ctl);
     if (S->precv) FSP_LAUNCH(S, k_prec, g, st, Gm, S->precv);
   }
 
@@ -638,15 +648,16 @@ struct Launch {
     const TS<ST>* none = nullptr;
     if (S->brick.bx > 0) return vcycle_fused<ST>(dot_op, from_guess);
     auto pass = [&](TS<ST>* x, int colour, int op) {
-      FSP_LAUNCH_N(S, "k_q_gs<0>", K_QGS(0), g, st, G, Q, (const TS<ST>*)x, x, S->r, colour, L0, none, op, S->ctl,
-                   S->partials);
+      FSP_LAUNCH_GEN(S, "k_q_gs<0>", (k_q_gs<0, ST, false>), (k_q_gs<0, ST, true>), g, BLOCK, st, G, Q,
+                     (const TS<ST>*)x, x, S->r, colour, L0, none, op, S->ctl, S->partials);
       if (op >= 0) FSP_FINISH(S, st, op, g, false, 1);
     };
     if (from_guess) {
       pass(z, 0, -1);
       pass(z, 1, -1);
     } else {
-      FSP_LAUNCH_N(S, "k_q_gs_first", k_q_gs_first<ST>, g, st, G, Q, z, S->r, S->ctl);
+      FSP_LAUNCH_GEN(S, "k_q_gs_first", (k_q_gs_first<ST, false>), (k_q_gs_first<ST, true>), g, BLOCK, st, G, Q, z,
+                     S->r, S->ctl);
     }
     for (int s = 1; s < S->opt.pre_sweeps; ++s) {
       pass(z, 0, -1);
@@ -661,12 +672,12 @@ struct Launch {
       H.RY = (QBLOCK / 2) / H.QX;
       H.nby = (H.rows + H.RY - 1) / H.RY;
       const dim3 gr((unsigned)(H.nbx * H.nby * H.planes));
-      FSP_LAUNCH_N(S, "k_q_restrict", (k_q_restrict2<ST>), gr, st, G, H, (const TS<ST>*)z, (const float*)S->r, L0,
-                   S->ctl);
+      FSP_LAUNCH_GEN(S, "k_q_restrict", (k_q_restrict2<ST, false>), (k_q_restrict2<ST, true>), gr, BLOCK, st, G, H,
+                     (const TS<ST>*)z, (const float*)S->r, L0, S->ctl);
       const TS<ST>* x0 = coarse_cycle<ST>();
       const int op = passes == 1 ? dot_op : -1;
-      FSP_LAUNCH_N(S, "k_q_gs<2>", K_QGS(2), g, st, G, Q, (const TS<ST>*)z, zt, S->r, 1, L0, x0, op, S->ctl,
-                   S->partials);
+      FSP_LAUNCH_GEN(S, "k_q_gs<2>", (k_q_gs<2, ST, false>), (k_q_gs<2, ST, true>), g, BLOCK, st, G, Q,
+                     (const TS<ST>*)z, zt, S->r, 1, L0, x0, op, S->ctl, S->partials);
       if (op >= 0) FSP_FINISH(S, st, op, g, false, 1);
       out = zt;
       done = 1;
@@ -692,16 +703,18 @@ struct Launch {
     const TS<ST>* none = nullptr;
     auto sweep = [&](int colour, const TS<ST>* xc, int op) {
       if (xc)
-        FSP_LAUNCH_NBS(S, "k_q_brick<2>", (k_q_brick<2, ST>), gb, BR_THREADS, 0, st, G, S->brick, (const TS<ST>*)a, b,
-                       S->r, colour, L0, xc, op, S->ctl, S->partials);
+        FSP_LAUNCH_GEN(S, "k_q_brick<2>", (k_q_brick<2, ST, false>), (k_q_brick<2, ST, true>), gb, BR_THREADS, st, G,
+                       S->brick, (const TS<ST>*)a, b, S->r, colour, L0, xc, op, S->ctl, S->partials);
       else
-        FSP_LAUNCH_NBS(S, "k_q_brick<0>", (k_q_brick<0, ST>), gb, BR_THREADS, 0, st, G, S->brick, (const TS<ST>*)a, b,
-                       S->r, colour, L0, none, op, S->ctl, S->partials);
+        FSP_LAUNCH_GEN(S, "k_q_brick<0>", (k_q_brick<0, ST, false>), (k_q_brick<0, ST, true>), gb, BR_THREADS, st, G,
+                       S->brick, (const TS<ST>*)a, b, S->r, colour, L0, none, op, S->ctl, S->partials);
       if (op >= 0) FSP_FINISH(S, st, op, gb, false, 1);
       std::swap(a, b);
     };
     if (from_guess) sweep(0, none, -1);
-    else FSP_LAUNCH_N(S, "k_q_gs_first", k_q_gs_first<ST>, g, st, G, Q, a, S->r, S->ctl);
+    else
+      FSP_LAUNCH_GEN(S, "k_q_gs_first", (k_q_gs_first<ST, false>), (k_q_gs_first<ST, true>), g, BLOCK, st, G, Q, a,
+                     S->r, S->ctl);
     for (int s = 1; s < S->opt.pre_sweeps; ++s) sweep(0, none, -1);
     const TS<ST>* x0 = none;
     if (!S->levels.empty()) {
@@ -709,8 +722,8 @@ struct Launch {
       H.RY = (QBLOCK / 2) / H.QX;
       H.nby = (H.rows + H.RY - 1) / H.RY;
       const dim3 gr((unsigned)(H.nbx * H.nby * H.planes));
-      FSP_LAUNCH_N(S, "k_q_restrict", (k_q_restrict2<ST>), gr, st, G, H, (const TS<ST>*)a, (const float*)S->r, L0,
-                   S->ctl);
+      FSP_LAUNCH_GEN(S, "k_q_restrict", (k_q_restrict2<ST, false>), (k_q_restrict2<ST, true>), gr, BLOCK, st, G, H,
+                     (const TS<ST>*)a, (const float*)S->r, L0, S->ctl);
       x0 = coarse_cycle<ST>();
     }
     for (int s = 0; s < S->opt.post_sweeps; ++s)
@@ -782,8 +795,8 @@ struct Launch {
                                                                                            : FSP_STORE_F32;
       with_storage(storage, [&](auto c) {
         constexpr int ST = decltype(c)::value;
-        FSP_LAUNCH_N(S, "k_q_spmv_dir", K_QSPMV, g, st, G, Q, (const TS<ST>*)S->zres, S->d0, S->d1, S->q, S->ctl,
-                     S->partials);
+        FSP_LAUNCH_GEN(S, "k_q_spmv_dir", (k_q_spmv_dir<ST, false>), (k_q_spmv_dir<ST, true>), g, BLOCK, st, G, Q,
+                       (const TS<ST>*)S->zres, S->d0, S->d1, S->q, S->ctl, S->partials);
         return 0;
       });
       FSP_FINISH(S, st, OP_DQ, g, false, 1);
@@ -822,7 +835,7 @@ fsp_status sync_ctl(fsp_solver* S) {
 }
 
 void cleanup(fsp_solver* S) {
-  for (int g = 0; g < 2; ++g) {
+  for (int g = 0; g < 4; ++g) {
     if (S->iter_exec[g]) (void)hipGraphExecDestroy(S->iter_exec[g]);
     if (S->iter_graph[g]) (void)hipGraphDestroy(S->iter_graph[g]);
     S->iter_exec[g] = nullptr;
@@ -934,7 +947,7 @@ fsp_status fingerprint(fsp_solver* S, const double* p, uint64_t& h) {
 // control block and every buffer is the solver's own, so the graph stays
 // valid for the solver's lifetime) and instantiates it once.
 fsp_status capture_iteration(fsp_solver* S) {
-  const int slot = S->net_phase;
+  const int slot = 2 * S->net_phase + S->table_only;
   if (S->iter_exec[slot]) return FSP_OK;
   const bool prof = S->prof_on;
   S->prof_on = false;
@@ -955,15 +968,19 @@ fsp_status capture_iteration(fsp_solver* S) {
   // device resources of the graph now, not at its first replay
   FSP_HIP(hipGraphUpload(S->iter_exec[slot], S->stream));
   FSP_HIP(hipStreamSynchronize(S->stream));
-  if (slot == 1) S->net_gen = net_generation(S->net);
+  if (S->net_phase == 1) S->net_gen = net_generation(S->net);
   return FSP_OK;
 }
 
-void drop_graph(fsp_solver* S, int slot) {
-  if (S->iter_exec[slot]) (void)hipGraphExecDestroy(S->iter_exec[slot]);
-  if (S->iter_graph[slot]) (void)hipGraphDestroy(S->iter_graph[slot]);
-  S->iter_exec[slot] = nullptr;
-  S->iter_graph[slot] = nullptr;
+// Drops both kernel forms' graphs of phase `phase`.
+void drop_graph(fsp_solver* S, int phase) {
+  for (int t = 0; t < 2; ++t) {
+    const int slot = 2 * phase + t;
+    if (S->iter_exec[slot]) (void)hipGraphExecDestroy(S->iter_exec[slot]);
+    if (S->iter_graph[slot]) (void)hipGraphDestroy(S->iter_graph[slot]);
+    S->iter_exec[slot] = nullptr;
+    S->iter_graph[slot] = nullptr;
+  }
 }
 
 // Captures and instantiates the iteration graph of phase `phase` (0: as
@@ -971,11 +988,12 @@ void drop_graph(fsp_solver* S, int slot) {
 // projection allocates nothing (the reference reserves every buffer before
 // seal(), delta3d.rs:225-241). A throwaway capture of the prime first sets
 // which buffer holds M^-1 r, as the prime of a projection does before its
-// first iteration.
+// first iteration. Both kernel forms (general and table-only) are captured;
+// each projection replays the one its geometry allows.
 fsp_status prepare_graph(fsp_solver* S, int phase) {
-  if (!S->opt.graph || S->iter_exec[phase]) return FSP_OK;
+  if (!S->opt.graph || (S->iter_exec[2 * phase] && S->iter_exec[2 * phase + 1])) return FSP_OK;
   if (phase == 1 && !S->net) return FSP_OK;
-  const int saved = S->net_phase;
+  const int saved = S->net_phase, saved_table = S->table_only;
   const bool prof = S->prof_on;
   S->net_phase = phase;
   S->prof_on = false;
@@ -990,7 +1008,12 @@ fsp_status prepare_graph(fsp_solver* S, int phase) {
     if (g) (void)hipGraphDestroy(g);
   }
   S->prof_on = prof;
-  const fsp_status st = e == hipSuccess ? capture_iteration(S) : FSP_DEVICE;
+  fsp_status st = e == hipSuccess ? FSP_OK : FSP_DEVICE;
+  for (int t = 0; t < 2 && st == FSP_OK; ++t) {
+    S->table_only = t;
+    st = capture_iteration(S);
+  }
+  S->table_only = saved_table;
   S->net_phase = saved;
   if (e != hipSuccess) std::fprintf(stderr, "fsp: prime capture failed: %s\n", hipGetErrorString(e));
   return st;
@@ -1200,6 +1223,12 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
   if (!S->geom) return FSP_DOMAIN;
   // the graded column takes the projection (delta3d_mobile.rs:473-475)
   if (!S->gnodes.empty()) return run_graded(S, us, vs, ws, p, warm, u, v, w, scale, k1, rel_tol, max_iters, rep);
+  // the table-only kernel forms hold for this projection only
+  struct TableGuard {
+    fsp_solver* S;
+    ~TableGuard() { S->table_only = 0; }
+  } table_guard{S};
+  S->table_only = 0;
   S->net_phase = 0;
   if (S->opt.preconditioner == FSP_PRECOND_LEARNED) {
     if (!S->net) return FSP_INVALID;
@@ -1207,7 +1236,7 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
     if (ns) return ns;
     // the network's buffers moved since the learned graph was captured (a
     // forward pass or another solver at another shape): capture it again
-    if (S->iter_exec[1] && S->net_gen != net_generation(S->net)) {
+    if (S->iter_exec[2] && S->net_gen != net_generation(S->net)) {
       drop_graph(S, 1);
       if (const fsp_status gs = prepare_graph(S, 1)) return gs;
     }
@@ -1249,6 +1278,9 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
   fsp_status st = sync_ctl(S);
   if (st) return st;
   if (S->hctl->nonfinite) return FSP_NOT_FINITE;
+  // every ghost row in its column's table (binary apertures, column
+  // geometry): the fine kernels' table-only forms
+  S->table_only = (!S->frac && !S->levelset && !S->hctl->table_miss) ? 1 : 0;
 
   const double b2 = S->hctl->b2;
   const double tol_b2 = rel_tol * rel_tol * b2;
@@ -1287,7 +1319,8 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
       for (;;) {
         if (S->opt.graph && !S->prof_on && (st = capture_iteration(S))) return st;
         for (int s = 0; s < batch; ++s) {
-          if (S->iter_exec[S->net_phase] && !S->prof_on) FSP_HIP(hipGraphLaunch(S->iter_exec[S->net_phase], S->stream));
+          if (hipGraphExec_t gx = S->iter_exec[2 * S->net_phase + S->table_only]; gx && !S->prof_on)
+            FSP_HIP(hipGraphLaunch(gx, S->stream));
           else K.iteration();
         }
         if ((st = sync_ctl(S))) return st;

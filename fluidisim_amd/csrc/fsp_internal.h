@@ -58,6 +58,7 @@ struct Ctl {
   int32_t stalled;       // a replaced true residual did not fall below the previous one: the pass ends
   double rr_replaced;    // |r|^2 of the last replaced true residual of the pass
   uint32_t wet_cells, ghost_faces;
+  int32_t table_miss;    // a ghost row lies past its column's table: the general fine kernels run
   uint32_t counters[NUM_SLOTS];
 };
 

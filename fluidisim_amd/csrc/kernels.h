@@ -345,6 +345,10 @@ __device__ __forceinline__ bool ghost_its(const Fine& G, int i, int j, int k, fl
 // type 1 neighbour, 2 ghost, with the face's aperture a (1 with binary
 // apertures). With fractional apertures every such row comes from the
 // geometry (the ghost table holds binary rows only).
+// GEN false: the table only (the host runs these forms when every ghost row
+// of a binary column geometry lies in its column's table; the geometry's
+// arithmetic, which sets a kernel's register count, is then not compiled).
+template <bool GEN = true>
 __device__ __forceinline__ void ghost_row(const Fine& G, int i, int j, int k, uint8_t m, uint8_t type[6], float it[6],
                                           float a[6]) {
   if (!G.frac && !G.phi && ghost_its(G, i, j, k, it)) {
@@ -353,6 +357,11 @@ __device__ __forceinline__ void ghost_row(const Fine& G, int i, int j, int k, ui
       type[f] = (m & (1u << f)) ? 1 : (it[f] > 0.f ? 2 : 0);
       a[f] = 1.0f;
     }
+    return;
+  }
+  if (!GEN) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) { type[f] = 0; it[f] = 0.f; a[f] = 0.f; }
     return;
   }
   RowG R;
@@ -558,7 +567,7 @@ __global__ void __launch_bounds__(BLOCK) k_mask_ls(Fine G, uint8_t* mask, Ctl* c
 // is not a ghost), for the G_SLOTS rows from the column's lowest ghost row:
 // the operator reads it instead of re-deriving the surface geometry. Rows
 // past the table fall back to full_row.
-__global__ void __launch_bounds__(BLOCK) k_ghost_table(Fine G, uint16_t* gk0, float* gtab) {
+__global__ void __launch_bounds__(BLOCK) k_ghost_table(Fine G, uint16_t* gk0, float* gtab, Ctl* ctl) {
   const size_t cols = (size_t)G.nx * G.ny;
   const size_t plane = cols;
   constexpr int SCAN = 16;  // rows per round of independent loads
@@ -592,6 +601,16 @@ __global__ void __launch_bounds__(BLOCK) k_ghost_table(Fine G, uint16_t* gk0, fl
       e[6] = 0.f;
       e[7] = 0.f;
     }
+    // a ghost row above the table: the fine kernels take their general form
+    bool miss = false;
+    for (int kb = k0 + G_SLOTS; kb < G.nz && !miss; kb += SCAN) {
+      uint8_t mm[SCAN];
+#pragma unroll
+      for (int u = 0; u < SCAN; ++u) mm[u] = kb + u < G.nz ? G.mask[(size_t)(kb + u) * plane + col] : 0;
+#pragma unroll
+      for (int u = 0; u < SCAN; ++u) miss = miss || (mm[u] & M_GHOST);
+    }
+    if (miss) ctl->table_miss = 1;
   }
 }
 

@@ -60,13 +60,14 @@ __device__ __forceinline__ float4 ld4(const float* p, size_t c) { return *reinte
 __device__ __forceinline__ void st4(float* p, size_t c, const float4& v) { *reinterpret_cast<float4*>(p + c) = v; }
 __device__ __forceinline__ float4 f4zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
 
-// The sum of a free-surface row (about one row in a hundred).
+// The sum of a free-surface row (about one row in a hundred); GEN as ghost_row.
+template <bool GEN = true>
 __device__ __forceinline__ float ghost_row_sum(const Fine& G, int i, int j, int k, uint8_t m, float pc, const float nb[6],
                                             float& prec) {
 #pragma clang fp contract(off)
   uint8_t type[6];
   float it[6], a[6];
-  ghost_row(G, i, j, k, m, type, it, a);
+  ghost_row<GEN>(G, i, j, k, m, type, it, a);
   float acc = 0.f, diag = 0.f;
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
@@ -86,8 +87,8 @@ __device__ __attribute__((noinline)) float ghost_row_sum_call(const Fine& G, int
 
 // Row of the fine operator from neighbour values already in registers, in the
 // reference's face order; prec as rhs_mobile3 computes it (CALL: the ghost
-// rows through ghost_row_sum_call).
-template <bool CALL = false>
+// rows through ghost_row_sum_call; GEN as ghost_row).
+template <bool CALL = false, bool GEN = true>
 __device__ __forceinline__ float row_local(const Fine& G, int i, int j, int k, uint8_t m, float pc, const float nb[6],
                                           float& prec) {
 #pragma clang fp contract(off)
@@ -100,7 +101,7 @@ __device__ __forceinline__ float row_local(const Fine& G, int i, int j, int k, u
       if (m & (1u << f)) acc += pc - nb[f];
     prec = G.prec_tab[__popc(m & 0x3Fu)];
   } else {
-    acc = CALL ? ghost_row_sum_call(G, i, j, k, m, pc, nb, prec) : ghost_row_sum(G, i, j, k, m, pc, nb, prec);
+    acc = CALL ? ghost_row_sum_call(G, i, j, k, m, pc, nb, prec) : ghost_row_sum<GEN>(G, i, j, k, m, pc, nb, prec);
   }
   return acc * G.inv;
 }
@@ -131,7 +132,7 @@ __device__ __forceinline__ void x_neighbours(const QuadGrid& Q, bool valid, int 
 // dot_op >= 0: also reduce <r, z> over every cell into that slot.
 // z, zout and the parents xc are held in storage ST (r is the f32 CG
 // residual); every value entering the arithmetic is a stored value.
-template <int MODE, int ST>
+template <int MODE, int ST, bool GEN = true>
 __global__ void __launch_bounds__(QBLOCK, FSP_QGS_MIN_BLOCKS) k_q_gs(Fine G, QuadGrid Q, const typename Store<ST>::T* z,
                                                  typename Store<ST>::T* zout, const float* r, int colour, Level L0,
                                                  const typename Store<ST>::T* xc, int dot_op, Ctl* ctl,
@@ -193,7 +194,7 @@ __global__ void __launch_bounds__(QBLOCK, FSP_QGS_MIN_BLOCKS) k_q_gs(Fine G, Qua
       const float xr = e == 3 ? right : q_get(zc, e + 1);
       const float nb[6] = {xl, xr, q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
       float prec;
-      const float az = row_local(G, 4 * qx + e, y, k, m, pc, nb, prec);
+      const float az = row_local<false, GEN>(G, 4 * qx + e, y, k, m, pc, nb, prec);
       if (prec > 0.f) q_set(out, e, SV::rnd(pc + prec * (q_get(rq, e) - az)));
     }
     SV::st4(zout, c0, out);
@@ -209,7 +210,7 @@ __global__ void __launch_bounds__(QBLOCK, FSP_QGS_MIN_BLOCKS) k_q_gs(Fine G, Qua
 // Every value is the one the two single passes store, so the result is
 // theirs bit for bit, for one pass's traffic plus the row_prec reads (a
 // variant deriving plain rows' factor from the row code ran slower).
-template <int ST>
+template <int ST, bool GEN = true>
 __global__ void __launch_bounds__(QBLOCK, FSP_QGS_MIN_BLOCKS) k_q_gs_first(Fine G, QuadGrid Q,
                                                                          typename Store<ST>::T* zout, const float* r,
                                                                          Ctl* ctl) {
@@ -266,7 +267,7 @@ __global__ void __launch_bounds__(QBLOCK, FSP_QGS_MIN_BLOCKS) k_q_gs_first(Fine 
     const float xr = e == 3 ? right : q_get(zc, e + 1);
     const float nb[6] = {xl, xr, q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
     float prec;
-    const float az = row_local(G, 4 * qx + e, y, k, m, pc, nb, prec);
+    const float az = row_local<false, GEN>(G, 4 * qx + e, y, k, m, pc, nb, prec);
     if (prec > 0.f) q_set(out, e, SV::rnd(pc + prec * (q_get(rq, e) - az)));
   }
   SV::st4(zout, c0, out);
@@ -319,8 +320,20 @@ struct BrickGrid {
   int bx, by, bz;  // bricks along x, y, z
 };
 
-template <int MODE, int ST>
-__global__ void __launch_bounds__(BR_THREADS) k_q_brick(Fine G, BrickGrid B, const typename Store<ST>::T* zin,
+// Register budget of the brick kernel: at least FSP_BRICK_WAVES wavefronts
+// per SIMD (0: the compiler's choice, 100 VGPRs and 4 wavefronts; 5: 96
+// VGPRs with two spilled on the free-surface path, measured faster).
+#ifndef FSP_BRICK_WAVES
+#define FSP_BRICK_WAVES 5
+#endif
+#if FSP_BRICK_WAVES > 0
+#define FSP_BRICK_BUDGET __attribute__((amdgpu_waves_per_eu(FSP_BRICK_WAVES)))
+#else
+#define FSP_BRICK_BUDGET
+#endif
+
+template <int MODE, int ST, bool GEN = true>
+__global__ void __launch_bounds__(BR_THREADS) FSP_BRICK_BUDGET k_q_brick(Fine G, BrickGrid B, const typename Store<ST>::T* zin,
                                                         typename Store<ST>::T* zout, const float* r, int colour,
                                                         Level L0, const typename Store<ST>::T* xc, int dot_op,
                                                         Ctl* ctl, double* partials) {
@@ -449,7 +462,7 @@ __global__ void __launch_bounds__(BR_THREADS) k_q_brick(Fine G, BrickGrid B, con
                          sc[BR_LX], sc[-BR_LX * BR_LY], sc[BR_LX * BR_LY]};
     const float pc = sc[0];
     float prec;
-    const float az = row_local(G, 4 * q + e, y, k, G.mask[c], pc, nb, prec);
+    const float az = row_local<false, GEN>(G, 4 * q + e, y, k, G.mask[c], pc, nb, prec);
     return prec > 0.f ? SV::rnd(pc + prec * (r[c] - az)) : (MODE == 2 ? 0.f : pc);
   };
   __syncthreads();
@@ -474,25 +487,45 @@ __global__ void __launch_bounds__(BR_THREADS) k_q_brick(Fine G, BrickGrid B, con
     }
   }
   __syncthreads();
-  // pass 1 - c over the brick (LDS quads 1..BR_QX, rows 2..BR_Y+1, planes 2..BR_Z+1), to zout
+  // pass 1 - c over the brick (LDS quads 1..BR_QX, rows 2..BR_Y+1, planes 2..BR_Z+1), to zout:
+  // the plain cells of every quad, then the free-surface cells (their
+  // neighbours are colour c, final in LDS, and their own value before the
+  // pass is still there), then the stores. Kept apart so that the rare
+  // free-surface path is not scheduled into the items' registers.
   double s = 0.0;
   {
+    float4 ob[BR_BITEMS];
+    uint32_t late = 0;
+#pragma unroll
+    for (int m = 0; m < BR_BITEMS; ++m) {
+      ob[m] = f4zero();
+      if (!onb[m]) continue;
+      const int idx = tid + BR_THREADS * m;
+      const int lq = 1 + idx % BR_QX, rest = idx / BR_QX, ly = 2 + rest % BR_Y, lz = 2 + rest / BR_Y;
+      uint32_t lt;
+      ob[m] = relax(lq, ly, lz, 1 - colour, rqb[m], mqb[m], lt);
+      late |= lt << (4 * m);
+    }
+    for (; late; late &= late - 1) {
+      const int bit = __ffs(late) - 1, m = bit >> 2, e = bit & 3;
+      const int idx = tid + BR_THREADS * m;
+      const int lq = 1 + idx % BR_QX, rest = idx / BR_QX, ly = 2 + rest % BR_Y, lz = 2 + rest / BR_Y;
+      const float x = relax_late(lq, ly, lz, e);
+#pragma unroll
+      for (int mm = 0; mm < BR_BITEMS; ++mm)
+        if (mm == m) q_set(ob[mm], e, x);
+    }
 #pragma unroll
     for (int m = 0; m < BR_BITEMS; ++m) {
       if (!onb[m]) continue;
       const int idx = tid + BR_THREADS * m;
       const int lq = 1 + idx % BR_QX, rest = idx / BR_QX, ly = 2 + rest % BR_Y, lz = 2 + rest / BR_Y;
       const int q = qx0 + lq, y = y0 + ly, k = z0 + lz;
-      uint32_t lt;
-      float4 o = relax(lq, ly, lz, 1 - colour, rqb[m], mqb[m], lt);
-      for (; lt; lt &= lt - 1) {
-        const int e = __ffs(lt) - 1;
-        q_set(o, e, relax_late(lq, ly, lz, e));
-      }
       const size_t c = ((size_t)k * ny + y) * nx + 4 * (size_t)q;
-      SV::st4(zout, c, o);
+      SV::st4(zout, c, ob[m]);
       if (dot_op >= 0)
-        s += (double)rqb[m].x * o.x + (double)rqb[m].y * o.y + (double)rqb[m].z * o.z + (double)rqb[m].w * o.w;
+        s += (double)rqb[m].x * ob[m].x + (double)rqb[m].y * ob[m].y + (double)rqb[m].z * ob[m].z +
+             (double)rqb[m].w * ob[m].w;
     }
   }
   if (dot_op >= 0) {
@@ -948,7 +981,7 @@ __global__ void __launch_bounds__(QBLOCK) k_start_cold(size_t nquads, const floa
 // and so the coarse r, is the one-thread sum bit for bit, with twice the
 // waves in flight to overlap the loads with the arithmetic (38 -> 28 us at
 // 256x256x128). H: the restriction's quad grid for 128-thread halves.
-template <int ST>
+template <int ST, bool GEN = true>
 __global__ void __launch_bounds__(QBLOCK) k_q_restrict2(Fine G, QuadGrid H, const typename Store<ST>::T* z,
                                                         const float* r, Level L0, Ctl* ctl) {
 #pragma clang fp contract(off)
@@ -990,7 +1023,7 @@ __global__ void __launch_bounds__(QBLOCK) k_q_restrict2(Fine G, QuadGrid H, cons
       const float xr = e == 3 ? right : q_get(zc, e + 1);
       const float nb[6] = {xl, xr, q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
       float prec;
-      const float az = valid ? row_local(G, 4 * qx + e, y, k, m, pc, nb, prec) : 0.f;
+      const float az = valid ? row_local<false, GEN>(G, 4 * qx + e, y, k, m, pc, nb, prec) : 0.f;
       t[dy][e] = q_get(rq, e) - az;
     }
   }
@@ -1028,7 +1061,7 @@ __global__ void __launch_bounds__(BLOCK) k_unpack(size_t n, const typename Store
 // ---------------------------------------------------------------- CG steps on quads
 // d = z (first) or z + beta d_old, q = A d, <d, q> (delta3d_mobile.rs:520-523, :460-465).
 // z is held in storage ST.
-template <int ST>
+template <int ST, bool GEN = true>
 __global__ void __launch_bounds__(QBLOCK) k_q_spmv_dir(Fine G, QuadGrid Q, const typename Store<ST>::T* z, float* d0,
                                                        float* d1, float* q, Ctl* ctl, double* partials) {
 #pragma clang fp contract(off)
@@ -1076,7 +1109,7 @@ __global__ void __launch_bounds__(QBLOCK) k_q_spmv_dir(Fine G, QuadGrid Q, const
       const float xr = e == 3 ? right : q_get(dc, e + 1);
       const float nb[6] = {xl, xr, q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
       float prec;
-      q_set(qo, e, row_local(G, 4 * qx + e, y, k, m, pc, nb, prec));
+      q_set(qo, e, row_local<false, GEN>(G, 4 * qx + e, y, k, m, pc, nb, prec));
     }
     st4(dnew, c0, dc);
     st4(q, c0, qo);
