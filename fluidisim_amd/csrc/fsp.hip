@@ -399,7 +399,8 @@ struct Launch {
     const int gated = replacement ? 1 : 0, fin_gate = replacement ? 3 : 0;
     if (S->quad) {
       const dim3 gq((unsigned)(S->qfine.nbx * S->qfine.nby * S->qfine.planes));
-      FSP_LAUNCH(S, k_residual_q, gq, st, G, S->qfine, p64, S->b, S->r, op, S->ctl, S->partials, gated);
+      FSP_LAUNCH_GEN(S, "k_residual_q", k_residual_q<false>, k_residual_q<true>, gq, QBLOCK, st, G, S->qfine, p64,
+                     S->b, S->r, op, S->ctl, S->partials, gated);
       FSP_FINISH(S, st, op, gq, false, fin_gate);
     } else {
       const dim3 g(grid_for(G.n));
@@ -916,7 +917,8 @@ void launch_backward(fsp_solver* S, const double* p) {
   Launch K(S);
   if (S->quad) {
     const dim3 gq((unsigned)(S->qfine.nbx * S->qfine.nby * S->qfine.planes));
-    FSP_LAUNCH(S, k_backward_q, gq, K.st, K.G, S->qfine, (const double*)p, S->b, S->r, S->ctl, S->partials);
+    FSP_LAUNCH_GEN(S, "k_backward_q", k_backward_q<false>, k_backward_q<true>, gq, QBLOCK, K.st, K.G, S->qfine,
+                   (const double*)p, S->b, S->r, S->ctl, S->partials);
     FSP_FINISH(S, K.st, OP_BACKWARD, gq, true, 0);
   } else {
     FSP_LAUNCH(S, k_backward, dim3(grid_for(K.G.n)), K.st, K.G, p, S->b, S->r, S->ctl, S->partials);

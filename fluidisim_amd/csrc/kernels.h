@@ -573,12 +573,26 @@ __global__ void __launch_bounds__(BLOCK) k_ghost_table(Fine G, uint16_t* gk0, fl
   constexpr int SCAN = 16;  // rows per round of independent loads
   FSP_FOR(col, cols) {
     const int i = (int)(col % (size_t)G.nx), j = (int)(col / (size_t)G.nx);
+    // Ghost rows lie between the lowest surface around the column and its
+    // own top wet row: an up ghost needs the row above dry (zc(k+1) >= h),
+    // a side ghost a dry neighbour at the same height (zc(k) >= h_n). Rows
+    // outside, less a margin of two rows for rounding, are not scanned.
+    const float h = G.eta[col];
+    float hn = h;
+    if (i > 0) hn = fminf(hn, G.eta[col - 1]);
+    if (i + 1 < G.nx) hn = fminf(hn, G.eta[col + 1]);
+    if (j > 0) hn = fminf(hn, G.eta[col - G.nx]);
+    if (j + 1 < G.ny) hn = fminf(hn, G.eta[col + G.nx]);
+    const float lo = hn / G.dx - 3.0f, hi = h / G.dx + 2.0f;
+    // (the top row stays in range: the top of the domain is a ghost face)
+    const int kstart = lo > 0.f ? (lo < (float)(G.nz - 1) ? (int)lo : G.nz - 1) : 0;
+    const int kend = hi > 0.f ? (hi < (float)(G.nz - 1) ? (int)hi + 1 : G.nz) : 0;  // exclusive
     // the lowest ghost row, SCAN rows per round
     int k0 = -1;
-    for (int kb = 0; kb < G.nz && k0 < 0; kb += SCAN) {
+    for (int kb = kstart; kb < kend && k0 < 0; kb += SCAN) {
       uint8_t mm[SCAN];
 #pragma unroll
-      for (int u = 0; u < SCAN; ++u) mm[u] = kb + u < G.nz ? G.mask[(size_t)(kb + u) * plane + col] : 0;
+      for (int u = 0; u < SCAN; ++u) mm[u] = kb + u < kend ? G.mask[(size_t)(kb + u) * plane + col] : 0;
 #pragma unroll
       for (int u = 0; u < SCAN; ++u)
         if (k0 < 0 && (mm[u] & M_GHOST)) k0 = kb + u;
@@ -603,10 +617,10 @@ __global__ void __launch_bounds__(BLOCK) k_ghost_table(Fine G, uint16_t* gk0, fl
     }
     // a ghost row above the table: the fine kernels take their general form
     bool miss = false;
-    for (int kb = k0 + G_SLOTS; kb < G.nz && !miss; kb += SCAN) {
+    for (int kb = k0 + G_SLOTS; kb < kend && !miss; kb += SCAN) {
       uint8_t mm[SCAN];
 #pragma unroll
-      for (int u = 0; u < SCAN; ++u) mm[u] = kb + u < G.nz ? G.mask[(size_t)(kb + u) * plane + col] : 0;
+      for (int u = 0; u < SCAN; ++u) mm[u] = kb + u < kend ? G.mask[(size_t)(kb + u) * plane + col] : 0;
 #pragma unroll
       for (int u = 0; u < SCAN; ++u) miss = miss || (mm[u] & M_GHOST);
     }

@@ -326,8 +326,11 @@ struct BrickGrid {
 #ifndef FSP_BRICK_WAVES
 #define FSP_BRICK_WAVES 5
 #endif
+#ifndef FSP_BRICK_WAVES_TABLE
+#define FSP_BRICK_WAVES_TABLE FSP_BRICK_WAVES
+#endif
 #if FSP_BRICK_WAVES > 0
-#define FSP_BRICK_BUDGET __attribute__((amdgpu_waves_per_eu(FSP_BRICK_WAVES)))
+#define FSP_BRICK_BUDGET __attribute__((amdgpu_waves_per_eu(GEN ? FSP_BRICK_WAVES : FSP_BRICK_WAVES_TABLE)))
 #else
 #define FSP_BRICK_BUDGET
 #endif
@@ -589,6 +592,7 @@ __global__ void __launch_bounds__(QBLOCK) k_mask_q(Fine G, QuadGrid Q, uint8_t* 
 // pressures of the quad and of its y and z neighbours as 16-byte loads with
 // indices clamped into the grid, the row's case decided after; apply_row_t's
 // arithmetic per cell).
+template <bool GEN = true>
 __global__ void __launch_bounds__(QBLOCK) k_residual_q(Fine G, QuadGrid Q, const double* p, const float* b, float* r,
                                                        int op, Ctl* ctl, double* partials, int gated) {
 #pragma clang fp contract(off)
@@ -627,7 +631,7 @@ __global__ void __launch_bounds__(QBLOCK) k_residual_q(Fine G, QuadGrid Q, const
         } else {
           uint8_t type[6];
           float it[6], a[6];
-          ghost_row(G, 4 * qx + e, y, k, m, type, it, a);
+          ghost_row<GEN>(G, 4 * qx + e, y, k, m, type, it, a);
 #pragma unroll
           for (int f = 0; f < 6; ++f) {
             if (type[f] == 1) acc += (double)a[f] * (pe - nb[f]);
@@ -904,6 +908,7 @@ __global__ void __launch_bounds__(QBLOCK) k_divmetric_q(Fine G, QuadGrid Q, cons
 // Oettli-Prager backward error on quads (k_backward with four x-cells per
 // thread, the pressures of the quad and of its y and z neighbours as 32-byte
 // loads with indices clamped into the grid; k_backward's arithmetic per cell).
+template <bool GEN = true>
 __global__ void __launch_bounds__(QBLOCK) k_backward_q(Fine G, QuadGrid Q, const double* p, const float* b,
                                                        const float* r, Ctl* ctl, double* partials) {
 #pragma clang fp contract(off)
@@ -939,7 +944,7 @@ __global__ void __launch_bounds__(QBLOCK) k_backward_q(Fine G, QuadGrid Q, const
       } else {
         uint8_t type[6];
         float it[6], a[6];
-        ghost_row(G, 4 * qx + e, y, k, m, type, it, a);
+        ghost_row<GEN>(G, 4 * qx + e, y, k, m, type, it, a);
         for (int f = 0; f < 6; ++f) {
           if (type[f] == 1) acc += (double)a[f] * (pe + fabs(pn[f]));
           else if (type[f] == 2) acc += (double)a[f] * pe * (double)it[f];

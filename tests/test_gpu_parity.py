@@ -600,3 +600,27 @@ def test_fractional_apertures_are_refused_where_unsupported():
         vol.step_surface_mobile(2000, 100)
     assert e.value.error == Error.Domain
     vol.close()
+
+
+def test_tall_ghost_columns_take_the_general_kernels():
+    """A dam-break step in the surface: the columns beside the step hold
+    side-ghost rows from the low surface to the high one, far more than a
+    column's ghost table holds, so the projection must run the fine
+    kernels' general forms (the rows past the table from the geometry). The
+    pressure against the f64 oracle, and the same projection on a smooth
+    surface (table-only forms) for contrast."""
+    for step in (True, False):
+        wl = W.tank(64, 48, 64, seed=8)  # X x Y x Z with Y up: nz = 48
+        d = wl.domain
+        if step:
+            i = np.arange(d.nx)
+            row = np.where(i < d.nx // 2, 0.8 * d.nz * d.dx, 0.3 * d.nz * d.dx).astype(np.float32)
+            wl.eta = np.tile(row, d.ny).astype(np.float32)
+            # ghost rows per column at the step: far past the table
+            assert (0.8 - 0.3) * d.nz > 16
+        vol, rep, (oref, _p32, _u, _v, _w, p64) = run_both(wl, precision=64)
+        print(f"[ghost columns step={step}] {rep.iterations} it (oracle {oref['iterations']}), "
+              f"residual {rep.residual:.2e}", flush=True)
+        assert not rep.degraded and rep.residual <= 1e-6
+        assert_pressure_parity(wl, host(vol.pressure()), p64)
+        vol.close()
