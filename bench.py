@@ -226,33 +226,46 @@ def learned_fcg(wl, args, torch, _lib, W, vcycle):
             "training": trained}
 
 
-def host_pointer_line(vol, wl, steps):
+def host_pointer_line(vol, wl, steps, torch):
     """The same projection through the host-pointer entry point (fsp_project):
-    the predicted faces copied in and p, u, v, w copied out over PCIe from
-    pageable host arrays; reported beside the device-resident value."""
+    the predicted faces copied in and p, u, v, w copied out over PCIe, from
+    pageable numpy arrays and from pinned host arrays (torch pin_memory, the
+    DMA rate); reported beside the device-resident value."""
     import ctypes as C
     from fluidisim_amd import _lib
     d = wl.domain
     nu, nv, nw = d.faces
-    us, vs, ws = (np.ascontiguousarray(a, np.float32) for a in (wl.us, wl.vs, wl.ws))
-    p = np.zeros(d.cells, np.float32)
-    u, v, w = np.zeros(nu, np.float32), np.zeros(nv, np.float32), np.zeros(nw, np.float32)
-    rep = _lib.Report()
-    ms, transfer = [], []
-    for q in range(steps + 1):
-        t0 = time.perf_counter()
-        st = _lib.lib().fsp_project(vol._h, us.ctypes.data, nu, vs.ctypes.data, nv, ws.ctypes.data, nw, p.ctypes.data,
-                                    d.cells, 0, u.ctypes.data, nu, v.ctypes.data, nv, w.ctypes.data, nw, wl.scale,
-                                    wl.k1, 1e-6, 2000, C.byref(rep))
-        if st != 0:
-            raise RuntimeError(f"fsp_project: status {st}")
-        if q:  # the first call warms the path
-            ms.append(1000.0 * (time.perf_counter() - t0))
-            transfer.append(rep.ms_transfer)
-    out = {"ms_per_projection": round(float(np.mean(ms)), 4), "ms_transfer": round(float(np.mean(transfer)), 4),
-           "bytes_moved": int(4 * (nu + nv + nw) * 2 + 4 * d.cells), "host_memory": "pageable numpy arrays"}
-    log(f"[bench] host pointers: {out['ms_per_projection']:.3f} ms/projection, of which {out['ms_transfer']:.3f} ms "
-        f"PCIe transfer")
+
+    def arrays(pinned):
+        def make(n, src=None):
+            a = (torch.empty(n, dtype=torch.float32, pin_memory=True).numpy() if pinned
+                 else np.empty(n, np.float32))
+            a[:] = 0.0 if src is None else src
+            return a
+        return (make(nu, wl.us), make(nv, wl.vs), make(nw, wl.ws), make(d.cells), make(nu), make(nv), make(nw))
+
+    def run(pinned):
+        us, vs, ws, p, u, v, w = arrays(pinned)
+        rep = _lib.Report()
+        ms, transfer = [], []
+        for q in range(steps + 1):
+            t0 = time.perf_counter()
+            st = _lib.lib().fsp_project(vol._h, us.ctypes.data, nu, vs.ctypes.data, nv, ws.ctypes.data, nw,
+                                        p.ctypes.data, d.cells, 0, u.ctypes.data, nu, v.ctypes.data, nv,
+                                        w.ctypes.data, nw, wl.scale, wl.k1, 1e-6, 2000, C.byref(rep))
+            if st != 0:
+                raise RuntimeError(f"fsp_project: status {st}")
+            if q:  # the first call warms the path
+                ms.append(1000.0 * (time.perf_counter() - t0))
+                transfer.append(rep.ms_transfer)
+        return {"ms_per_projection": round(float(np.mean(ms)), 4), "ms_transfer": round(float(np.mean(transfer)), 4)}
+
+    out = {"bytes_moved": int(4 * (nu + nv + nw) * 2 + 4 * d.cells),
+           "pageable": run(False), "pinned": run(True)}
+    for kind in ("pageable", "pinned"):
+        r = out[kind]
+        log(f"[bench] host pointers ({kind}): {r['ms_per_projection']:.3f} ms/projection, of which "
+            f"{r['ms_transfer']:.3f} ms PCIe transfer ({out['bytes_moved'] / r['ms_transfer'] / 1e6:.1f} GB/s)")
     return out
 
 
@@ -361,7 +374,7 @@ def main():
                     "avg_us": round(avg_ms * 1000.0, 2), "bytes_per_launch": int(bytes_launch)}
         log(f"[bench] {args.kernel}: {eff.size} launches, {avg_ms * 1000:.1f} us avg, {achieved:.0f} GB/s")
 
-    host = None if args.no_host else host_pointer_line(vol, wl, max(1, min(args.steps, 3)))
+    host = None if args.no_host else host_pointer_line(vol, wl, max(1, min(args.steps, 3)), torch)
 
     batched = batch_line(args, torch, W) if args.batch > 0 else None
 
