@@ -303,7 +303,10 @@ __global__ void __launch_bounds__(QBLOCK, FSP_QGS_MIN_BLOCKS) k_q_gs_first(Fine 
 #define FSP_BRICK_Z 8
 #endif
 constexpr int BR_QX = FSP_BRICK_QX, BR_Y = FSP_BRICK_Y, BR_Z = FSP_BRICK_Z;
-constexpr int BR_THREADS = 256;
+#ifndef FSP_BRICK_THREADS
+#define FSP_BRICK_THREADS 512
+#endif
+constexpr int BR_THREADS = FSP_BRICK_THREADS;
 constexpr int BR_LQ = BR_QX + 2;                   // quads per LDS row
 constexpr int BR_LX = 4 * BR_LQ;                   // floats per LDS row
 constexpr int BR_LY = BR_Y + 4, BR_LZ = BR_Z + 4;  // LDS rows, planes
@@ -320,23 +323,9 @@ struct BrickGrid {
   int bx, by, bz;  // bricks along x, y, z
 };
 
-// Register budget of the brick kernel: at least FSP_BRICK_WAVES wavefronts
-// per SIMD (0: the compiler's choice, 100 VGPRs and 4 wavefronts; 5: 96
-// VGPRs with two spilled on the free-surface path, measured faster).
-#ifndef FSP_BRICK_WAVES
-#define FSP_BRICK_WAVES 5
-#endif
-#ifndef FSP_BRICK_WAVES_TABLE
-#define FSP_BRICK_WAVES_TABLE FSP_BRICK_WAVES
-#endif
-#if FSP_BRICK_WAVES > 0
-#define FSP_BRICK_BUDGET __attribute__((amdgpu_waves_per_eu(GEN ? FSP_BRICK_WAVES : FSP_BRICK_WAVES_TABLE)))
-#else
-#define FSP_BRICK_BUDGET
-#endif
 
 template <int MODE, int ST, bool GEN = true>
-__global__ void __launch_bounds__(BR_THREADS) FSP_BRICK_BUDGET k_q_brick(Fine G, BrickGrid B, const typename Store<ST>::T* zin,
+__global__ void __launch_bounds__(BR_THREADS) k_q_brick(Fine G, BrickGrid B, const typename Store<ST>::T* zin,
                                                         typename Store<ST>::T* zout, const float* r, int colour,
                                                         Level L0, const typename Store<ST>::T* xc, int dot_op,
                                                         Ctl* ctl, double* partials) {
