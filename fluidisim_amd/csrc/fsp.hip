@@ -751,8 +751,8 @@ struct Launch {
         const float* none = nullptr;
         for (int s = 0; s < S->opt.learned_smooth; ++s)
           for (int c = 0; c < 2; ++c)
-            FSP_LAUNCH_N(S, "k_q_gs<0>", (k_q_gs<0, ST_F32>), g, st, G, Q, (const float*)S->z, S->z, S->r, c,
-                         none_l, none, -1, S->ctl, S->partials);
+            FSP_LAUNCH_GEN(S, "k_q_gs<0>", (k_q_gs<0, ST_F32, false>), (k_q_gs<0, ST_F32, true>), g, BLOCK, st,
+                           G, Q, (const float*)S->z, S->z, S->r, c, none_l, none, -1, S->ctl, S->partials);
       }
       FSP_LAUNCH(S, k_dot_rz_qz, dim3(grid_for(G.n)), st, G.n, (const float*)S->r, (const float*)zout,
                  (const float*)S->q, op, S->ctl, S->partials);
