@@ -317,7 +317,9 @@ struct Launch {
   Fine G;
   explicit Launch(fsp_solver* s) : S(s), st(s->stream), G(s->fine()) {}
 
-  void mask() {
+  // prec: also the per-row factors k_q_gs_first reads (k_prec), unless the
+  // caller's assembly writes them.
+  void mask(bool prec = true) {
     const dim3 g(grid_for(G.n));
     if (S->levelset) {
       // labels and level set: no column structure, no ghost-row table
@@ -337,7 +339,7 @@ struct Launch {
     // ghost-row table from the fresh row codes (k_mask's finish does not touch it)
     Fine Gm = G;
     FSP_LAUNCH(S, k_ghost_table, dim3(grid_for(S->ncols)), st, Gm, S->gk0, S->gtab, S->ctl);
-    if (S->precv) FSP_LAUNCH(S, k_prec, g, st, Gm, S->precv);
+    if (S->precv && prec) FSP_LAUNCH(S, k_prec, g, st, Gm, S->precv);
   }
 
   // A level whose V-cycle kernels read only couplings and Dirichlet parts:
@@ -407,6 +409,16 @@ struct Launch {
       FSP_LAUNCH(S, k_residual, g, st, G, p64, S->b, S->r, op, S->ctl, S->partials, gated);
       FSP_FINISH(S, st, op, g, false, fin_gate);
     }
+  }
+
+  // The true residual into OP_RR with the backward error of the same r into
+  // OP_BACKWARD, in one pass over p64 (quads only).
+  void residual_backward(const double* p64) {
+    const dim3 gq((unsigned)(S->qfine.nbx * S->qfine.nby * S->qfine.planes));
+    FSP_LAUNCH_GEN(S, "k_residual_q", (k_residual_q<false, true>), (k_residual_q<true, true>), gq, QBLOCK, st, G,
+                   S->qfine, p64, S->b, S->r, OP_RR, S->ctl, S->partials, 0);
+    FSP_FINISH(S, st, OP_RR, gq, false, 0);
+    FSP_FINISH(S, st, OP_BACKWARD, gq, true, 0);
   }
 
   void fill_gated(float* x, size_t n, float v) {
@@ -1032,10 +1044,10 @@ fsp_status regraph(fsp_solver* S) {
 
 // Geometry rebuilt from the surface at the start of every projection
 // (delta3d_mobile.rs:479-484).
-void build_geometry(fsp_solver* S, bool hierarchy) {
+void build_geometry(fsp_solver* S, bool hierarchy, bool prec = true) {
   Launch K(S);
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, K.st, S->ctl, S->opt.gauge);
-  K.mask();
+  K.mask(prec);
   if (hierarchy && S->opt.preconditioner != FSP_PRECOND_JACOBI) K.hierarchy();
 }
 
@@ -1248,15 +1260,19 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
   Launch K(S);
   const Fine& G = K.G;
   FSP_HIP(hipEventRecord(S->ev0, S->stream));
-  build_geometry(S, true);
+  // On a column geometry with binary apertures the assembly writes the
+  // per-row factors the first red-black pass reads (the diagonal it forms
+  // anyway, the arithmetic of k_prec's row_prec), in place of k_prec's pass.
+  float* const prec_out = (S->precv && !S->frac && !S->levelset) ? S->precv : nullptr;
+  build_geometry(S, true, prec_out == nullptr);
   if (S->quad && aligned16(vs) && aligned16(ws)) {
     const dim3 gq((unsigned)(S->qfine.nbx * S->qfine.nby * S->qfine.planes));
-    FSP_LAUNCH(S, k_assemble_q, gq, K.st, G, S->qfine, us, vs, ws, (const float*)nullptr, scale, S->b,
-               (float*)nullptr, S->ctl, S->partials);
+    FSP_LAUNCH(S, k_assemble_q, gq, K.st, G, S->qfine, us, vs, ws, (const float*)nullptr, scale, S->b, prec_out,
+               S->ctl, S->partials);
     FSP_FINISH(S, K.st, OP_B2, gq, false, 0);
   } else {
     FSP_LAUNCH(S, k_assemble, dim3(grid_for(G.n)), K.st, G, us, vs, ws, (const float*)nullptr,
-               scale, S->b, (float*)nullptr, S->ctl, S->partials);
+               scale, S->b, prec_out, S->ctl, S->partials);
     FSP_FINISH(S, K.st, OP_B2, dim3(grid_for(G.n)), false, 0);
   }
   if (S->opt.gauge) {
@@ -1357,15 +1373,21 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
       if (net_failed) K.set_pass(0, rr, tol_b2, physical_target, it, max_iters);  // clears the flag
     }
     // True residual, recomputed: it decides, never the recurrence (:546-553).
-    K.residual(p64, OP_RR);
     // The correction with its metric and the backward error ride on the same
     // sync as the true residual (read below if the pass ends in them; the
     // correction writes only u, v, w, which a later one overwrites). Not with
     // the gauge of a pure-Neumann problem, whose correction shifts p first.
+    // On quads the backward error comes out of the residual's own pass.
     const bool spec = !(S->opt.gauge && S->hctl->pure_neumann);
-    if (spec) {
+    if (spec && S->quad) {
+      K.residual_backward(p64);
       launch_correct(S, us, vs, ws, p64, k1, u, v, w);
-      launch_backward(S, p64);
+    } else {
+      K.residual(p64, OP_RR);
+      if (spec) {
+        launch_correct(S, us, vs, ws, p64, k1, u, v, w);
+        launch_backward(S, p64);
+      }
     }
     if ((st = sync_ctl(S))) return st;
     if (S->hctl->nonfinite) return FSP_NOT_FINITE;

@@ -581,12 +581,15 @@ __global__ void __launch_bounds__(QBLOCK) k_mask_q(Fine G, QuadGrid Q, uint8_t* 
 // pressures of the quad and of its y and z neighbours as 16-byte loads with
 // indices clamped into the grid, the row's case decided after; apply_row_t's
 // arithmetic per cell).
-template <bool GEN = true>
+// BACKWARD: the same pass also folds the backward error of the r it stores
+// into OP_BACKWARD, k_backward_q's arithmetic from the same pressures (|A||p|
+// beside A p), so that the wrap-up reads p once for both.
+template <bool GEN = true, bool BACKWARD = false>
 __global__ void __launch_bounds__(QBLOCK) k_residual_q(Fine G, QuadGrid Q, const double* p, const float* b, float* r,
                                                        int op, Ctl* ctl, double* partials, int gated) {
 #pragma clang fp contract(off)
   if (gated && !replacing(ctl)) return;
-  double s = 0.0, unused = 0.0;
+  double s = 0.0, unused = 0.0, worst = 0.0;
   int qx, y, k;
   const bool valid = quad_coords(Q, qx, y, k);
   if (valid) {
@@ -609,33 +612,47 @@ __global__ void __launch_bounds__(QBLOCK) k_residual_q(Fine G, QuadGrid Q, const
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const uint8_t m = (uint8_t)(mq >> (8 * e));
-      double acc = 0.0;
+      double acc = 0.0, aabs = 0.0;
       if (m & M_WET) {
-        const double pe = pc[e];
+        const double pe = pc[e], pa = fabs(pc[e]);
         const double nb[6] = {e > 0 ? pc[e - 1] : pxm, e < 3 ? pc[e + 1] : pxp, pym[e], pyp[e], pzm[e], pzp[e]};
         if (!(m & M_GHOST)) {
 #pragma unroll
           for (int f = 0; f < 6; ++f)
-            if (m & (1u << f)) acc += pe - nb[f];
+            if (m & (1u << f)) {
+              acc += pe - nb[f];
+              if (BACKWARD) aabs += pa + fabs(nb[f]);
+            }
         } else {
           uint8_t type[6];
           float it[6], a[6];
           ghost_row<GEN>(G, 4 * qx + e, y, k, m, type, it, a);
 #pragma unroll
           for (int f = 0; f < 6; ++f) {
-            if (type[f] == 1) acc += (double)a[f] * (pe - nb[f]);
-            else if (type[f] == 2) acc += (double)a[f] * pe * (double)it[f];
+            if (type[f] == 1) {
+              acc += (double)a[f] * (pe - nb[f]);
+              if (BACKWARD) aabs += (double)a[f] * (pa + fabs(nb[f]));
+            } else if (type[f] == 2) {
+              acc += (double)a[f] * pe * (double)it[f];
+              if (BACKWARD) aabs += (double)a[f] * pa * (double)it[f];
+            }
           }
         }
         acc = acc * (double)G.inv;
       }
       const double rc = (double)q_get(bq, e) - acc;
-      q_set(out, e, (float)rc);
+      const float rf = (float)rc;
+      q_set(out, e, rf);
       s += rc * rc;
+      if (BACKWARD && (m & M_WET)) {
+        const double scale = fabs((double)q_get(bq, e)) + aabs * (double)G.inv;
+        if (scale > 0.0) worst = fmax(worst, fabs((double)rf) / scale);
+      }
     }
     st4(r, c0, out);
   }
   if (op >= 0) reduce_finish<false>(s, unused, op, ctl, partials);
+  if (BACKWARD) reduce_finish<true>(worst, unused, OP_BACKWARD, ctl, partials);
 }
 
 // b = scale * div + ghost terms, prec (optional) and |b|^2 on quads
