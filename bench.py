@@ -68,6 +68,7 @@ def parse():
     ap.add_argument("--batch-concurrency", default="1,4", help="concurrency levels timed for the batched line")
     ap.add_argument("--no-step", action="store_true", help="skip the surface-step line")
     ap.add_argument("--no-host", action="store_true", help="skip the host-pointer (PCIe-inclusive) line")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the 16-bit V-cycle storage line")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the 256 MiB device copy (counter calibration) and exit")
     return ap.parse_args()
@@ -269,6 +270,42 @@ def host_pointer_line(vol, wl, steps, torch):
     return out
 
 
+def mixed_precision_line(wl, vol, args, torch, _lib, W):
+    """The 16-bit V-cycle configuration: bfloat16 storage of the V-cycle
+    vectors (f32 arithmetic, f32 CG vectors, f64 pressure, residual and
+    reductions) with the true residual replacing the recurrence every 50
+    iterations, on the headline workload, timed like the headline. Its
+    pressure is compared with the f32-storage projection's (relative L2); the
+    comparison with the f64 oracle at this size is a test
+    (test_c2_bf16_vcycle_with_residual_replacement_matches_the_f64_oracle).
+    Binary16 storage is not timed: this workload's V-cycle values leave its
+    range and the projection runs to its iteration limit, degraded."""
+    p_ref = vol.pressure().double()
+    out = {"replace_every": 50, "f16": "not timed: binary16 leaves its range on this workload (degraded)"}
+    for name, storage in (("bf16", _lib.STORE_BF16),):
+        v = W.build_volume(wl, vcycle_storage=storage, replace_every=50)
+
+        def step():
+            return v.project_mobile3(wl.scale, wl.k1, 2000, rel_tol=1e-6, warm=False)
+
+        for _ in range(max(1, args.warmup)):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reps = [step() for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+        r = reps[-1]
+        err = float(torch.linalg.norm(v.pressure().double() - p_ref) / torch.linalg.norm(p_ref))
+        log(f"[bench] {name} V-cycle storage, replacement every 50: {ms:.3f} ms/projection, {r.iterations} "
+            f"iterations, residual {r.residual:.2e}, pressure {err:.1e} from the f32-storage projection")
+        out[name] = {"ms_per_projection": round(ms, 4), "iterations": r.iterations, "residual": r.residual,
+                     "divergence_plain": r.divergence_plain, "degraded": r.degraded,
+                     "p_rel_l2_vs_f32_storage": err}
+        v.close()
+    return out
+
+
 def surface_step_line(args, torch, W):
     """The whole surface step (advect, project with a warm start,
     extrapolate, transport, balance and checks) on the sloshing tank of the
@@ -376,6 +413,8 @@ def main():
 
     host = None if args.no_host else host_pointer_line(vol, wl, max(1, min(args.steps, 3)), torch)
 
+    mixed = None if args.no_mixed else mixed_precision_line(wl, vol, args, torch, _lib, W)
+
     batched = batch_line(args, torch, W) if args.batch > 0 else None
 
     learned = learned_sweeps = None
@@ -403,6 +442,7 @@ def main():
                    "ms_device_last": round(last.ms_device, 4), "seed": args.seed},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
         "learned_fcg": learned,
         "learned_fcg_sweeps": learned_sweeps,
         "batched": batched,
