@@ -78,17 +78,9 @@ __device__ __forceinline__ float ghost_row_sum(const Fine& G, int i, int j, int 
   return acc;
 }
 
-// ghost_row_sum as a call, for kernels that inline many rows: the rare
-// free-surface rows then cost no registers in the common path.
-__device__ __attribute__((noinline)) float ghost_row_sum_call(const Fine& G, int i, int j, int k, uint8_t m, float pc,
-                                                              const float nb[6], float& prec) {
-  return ghost_row_sum(G, i, j, k, m, pc, nb, prec);
-}
-
 // Row of the fine operator from neighbour values already in registers, in the
-// reference's face order; prec as rhs_mobile3 computes it (CALL: the ghost
-// rows through ghost_row_sum_call; GEN as ghost_row).
-template <bool CALL = false, bool GEN = true>
+// reference's face order; prec as rhs_mobile3 computes it (GEN as ghost_row).
+template <bool GEN = true>
 __device__ __forceinline__ float row_local(const Fine& G, int i, int j, int k, uint8_t m, float pc, const float nb[6],
                                           float& prec) {
 #pragma clang fp contract(off)
@@ -101,7 +93,7 @@ __device__ __forceinline__ float row_local(const Fine& G, int i, int j, int k, u
       if (m & (1u << f)) acc += pc - nb[f];
     prec = G.prec_tab[__popc(m & 0x3Fu)];
   } else {
-    acc = CALL ? ghost_row_sum_call(G, i, j, k, m, pc, nb, prec) : ghost_row_sum<GEN>(G, i, j, k, m, pc, nb, prec);
+    acc = ghost_row_sum<GEN>(G, i, j, k, m, pc, nb, prec);
   }
   return acc * G.inv;
 }
@@ -194,7 +186,7 @@ __global__ void __launch_bounds__(QBLOCK, FSP_QGS_MIN_BLOCKS) k_q_gs(Fine G, Qua
       const float xr = e == 3 ? right : q_get(zc, e + 1);
       const float nb[6] = {xl, xr, q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
       float prec;
-      const float az = row_local<false, GEN>(G, 4 * qx + e, y, k, m, pc, nb, prec);
+      const float az = row_local<GEN>(G, 4 * qx + e, y, k, m, pc, nb, prec);
       if (prec > 0.f) q_set(out, e, SV::rnd(pc + prec * (q_get(rq, e) - az)));
     }
     SV::st4(zout, c0, out);
@@ -267,7 +259,7 @@ __global__ void __launch_bounds__(QBLOCK, FSP_QGS_MIN_BLOCKS) k_q_gs_first(Fine 
     const float xr = e == 3 ? right : q_get(zc, e + 1);
     const float nb[6] = {xl, xr, q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
     float prec;
-    const float az = row_local<false, GEN>(G, 4 * qx + e, y, k, m, pc, nb, prec);
+    const float az = row_local<GEN>(G, 4 * qx + e, y, k, m, pc, nb, prec);
     if (prec > 0.f) q_set(out, e, SV::rnd(pc + prec * (q_get(rq, e) - az)));
   }
   SV::st4(zout, c0, out);
@@ -454,7 +446,7 @@ __global__ void __launch_bounds__(BR_THREADS) k_q_brick(Fine G, BrickGrid B, con
                          sc[BR_LX], sc[-BR_LX * BR_LY], sc[BR_LX * BR_LY]};
     const float pc = sc[0];
     float prec;
-    const float az = row_local<false, GEN>(G, 4 * q + e, y, k, G.mask[c], pc, nb, prec);
+    const float az = row_local<GEN>(G, 4 * q + e, y, k, G.mask[c], pc, nb, prec);
     return prec > 0.f ? SV::rnd(pc + prec * (r[c] - az)) : (MODE == 2 ? 0.f : pc);
   };
   __syncthreads();
@@ -1034,7 +1026,7 @@ __global__ void __launch_bounds__(QBLOCK) k_q_restrict2(Fine G, QuadGrid H, cons
       const float xr = e == 3 ? right : q_get(zc, e + 1);
       const float nb[6] = {xl, xr, q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
       float prec;
-      const float az = valid ? row_local<false, GEN>(G, 4 * qx + e, y, k, m, pc, nb, prec) : 0.f;
+      const float az = valid ? row_local<GEN>(G, 4 * qx + e, y, k, m, pc, nb, prec) : 0.f;
       t[dy][e] = q_get(rq, e) - az;
     }
   }
@@ -1120,7 +1112,7 @@ __global__ void __launch_bounds__(QBLOCK) k_q_spmv_dir(Fine G, QuadGrid Q, const
       const float xr = e == 3 ? right : q_get(dc, e + 1);
       const float nb[6] = {xl, xr, q_get(ym, e), q_get(yp, e), q_get(zm, e), q_get(zp, e)};
       float prec;
-      q_set(qo, e, row_local<false, GEN>(G, 4 * qx + e, y, k, m, pc, nb, prec));
+      q_set(qo, e, row_local<GEN>(G, 4 * qx + e, y, k, m, pc, nb, prec));
     }
     st4(dnew, c0, dc);
     st4(q, c0, qo);

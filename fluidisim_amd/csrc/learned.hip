@@ -431,12 +431,6 @@ __global__ void k_relu_back(const _Float16* h, _Float16* g, size_t n) {
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
     if (!((float)h[q] > 0.f)) g[q] = (_Float16)0.f;
 }
-// g += a
-__global__ void k_add16(_Float16* g, const _Float16* a, size_t n) {
-  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
-    g[q] = (_Float16)((float)g[q] + (float)a[q]);
-}
-
 // Loss: mean of (y - t)^2 over wet cells; per-block f64 partials; the
 // output gradient scale * 2 (y - t) wet / n_wet as a 4-channel binary16
 // tensor (channels 1-3 zero) for the head's backward-data convolution.
