@@ -950,7 +950,7 @@ fsp_status fingerprint(fsp_solver* S, const double* p, uint64_t& h) {
   // one partial per block; S->hashes holds (n + 1023) / 1024 of them
   const size_t chunks = std::min((size_t)grid_for(S->n), (S->n + 1023) / 1024);
   FSP_LAUNCH(S, k_hash_chunks, dim3((unsigned)chunks), S->stream, p, S->n, S->hashes);
-  hipLaunchKernelGGL(k_hash_combine, dim3(1), dim3(64), 0, S->stream, S->hashes, chunks, S->hash_out);
+  hipLaunchKernelGGL(k_hash_combine, dim3(1), dim3(HASH_THREADS), 0, S->stream, S->hashes, chunks, S->hash_out);
   FSP_HIP(hipMemcpyAsync(S->hhash, S->hash_out, sizeof(unsigned long long), hipMemcpyDeviceToHost, S->stream));
   FSP_HIP(hipStreamSynchronize(S->stream));
   h = *S->hhash;

@@ -1404,11 +1404,21 @@ __global__ void __launch_bounds__(BLOCK) k_hash_chunks(const double* p, size_t n
   }
   if (threadIdx.x == 0) hashes[blockIdx.x] = part[0];
 }
-__global__ void k_hash_combine(const unsigned long long* hashes, size_t chunks, unsigned long long* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// The chunk hashes' wrapping sum (exact in any order) by one workgroup of
+// HASH_THREADS threads, then the final mix.
+constexpr int HASH_THREADS = 1024;
+__global__ void __launch_bounds__(HASH_THREADS) k_hash_combine(const unsigned long long* hashes, size_t chunks,
+                                                               unsigned long long* out) {
+  __shared__ unsigned long long part[HASH_THREADS];
   unsigned long long h = 0;
-  for (size_t q = 0; q < chunks; ++q) h += hashes[q];
-  *out = mix64(h);
+  for (size_t q = threadIdx.x; q < chunks; q += HASH_THREADS) h += hashes[q];
+  part[threadIdx.x] = h;
+  __syncthreads();
+  for (int w = HASH_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = mix64(part[0]);
 }
 
 // Host-side parameters of a CG pass, written on the stream.
