@@ -2260,9 +2260,11 @@ fsp_status fsp_step_surface_mobile_device(fsp_solver* S, float* d_u, float* d_v,
   const Fine G = S->fine();
   k_step_reset<<<1, 1, 0, S->stream>>>(S->step_sc);
   k_step_scalars<<<1, STEP_THREADS, 0, S->stream>>>(G, S->col_floor, S->flux_x, S->flux_y, S->step_sc);
-  const float* fields[] = {d_u, d_v, d_w, d_p, S->eta, S->eta_rt, S->us, S->vs, S->ws, S->b, S->r};
-  const size_t lens[] = {S->nu, S->nv, S->nw, S->n, S->ncols, S->ncols, S->nu, S->nv, S->nw, S->n, S->n};
-  for (int q = 0; q < 11; ++q) k_finite<<<grid_for(lens[q]), BLOCK, 0, S->stream>>>(fields[q], lens[q], S->step_sc);
+  const FiniteSet F{{d_u, d_v, d_w, d_p, S->eta, S->eta_rt, S->us, S->vs, S->ws, S->b, S->r},
+                    {S->nu, S->nv, S->nw, S->n, S->ncols, S->ncols, S->nu, S->nv, S->nw, S->n, S->n}};
+  size_t longest = 0;
+  for (size_t len : F.n) longest = std::max(longest, len);
+  k_finite<<<dim3(grid_for(longest), FINITE_FIELDS), BLOCK, 0, S->stream>>>(F, S->step_sc);
   FSP_HIP(hipGetLastError());
   FSP_HIP(hipMemcpyAsync(S->hstep, S->step_sc, sizeof(StepScalars), hipMemcpyDeviceToHost, S->stream));
   FSP_HIP(hipStreamSynchronize(S->stream));

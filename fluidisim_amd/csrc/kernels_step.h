@@ -282,11 +282,25 @@ __global__ void __launch_bounds__(STEP_THREADS) k_step_scalars(Fine G, const flo
   __syncthreads();
   double vol = 0.0;
   int ok = 1;
-  for (size_t c = threadIdx.x; c < cols; c += STEP_THREADS) {
-    vol += ((double)G.eta[c] - (double)G.eta_rt[c]) - (double)G.rest;
-    const float e = G.eta[c];
-    if (!(e >= floor[c] + 2.f * dx && e <= top)) ok = 0;
+  auto column = [&](float e, float rt, float fl) {
+    vol += ((double)e - (double)rt) - (double)G.rest;
+    if (!(e >= fl + 2.f * dx && e <= top)) ok = 0;
+  };
+  // eight columns' loads in flight per round, summed in the thread's order
+  constexpr int U = 8;
+  size_t c = threadIdx.x;
+  for (; c + (size_t)(U - 1) * STEP_THREADS < cols; c += (size_t)U * STEP_THREADS) {
+    float e[U], rt[U], fl[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      e[u] = G.eta[c + (size_t)u * STEP_THREADS];
+      rt[u] = G.eta_rt[c + (size_t)u * STEP_THREADS];
+      fl[u] = floor[c + (size_t)u * STEP_THREADS];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) column(e[u], rt[u], fl[u]);
   }
+  for (; c < cols; c += STEP_THREADS) column(G.eta[c], G.eta_rt[c], floor[c]);
   if (!ok) ok_all = 0;  // every writer stores the same value
   red[threadIdx.x] = vol;
   __syncthreads();
@@ -318,7 +332,16 @@ __global__ void __launch_bounds__(STEP_THREADS) k_step_scalars(Fine G, const flo
 
 // Clears s->finite to 0 when a value of a is not finite (every writer
 // stores the same value).
-__global__ void __launch_bounds__(256) k_finite(const float* a, size_t n, StepScalars* s) {
+// The fields a step writes, checked for non-finite values in one launch: the
+// grid's row y takes field y.
+constexpr int FINITE_FIELDS = 11;
+struct FiniteSet {
+  const float* a[FINITE_FIELDS];
+  size_t n[FINITE_FIELDS];
+};
+__global__ void __launch_bounds__(256) k_finite(FiniteSet F, StepScalars* s) {
+  const float* a = F.a[blockIdx.y];
+  const size_t n = F.n[blockIdx.y];
   bool bad = false;
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
     if (!isfinite(a[q])) bad = true;
