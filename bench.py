@@ -65,7 +65,7 @@ def parse():
                     help="trained network for the learned-FCG line (skipped when the file is absent)")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--batch", type=int, default=48, help="patches of the batched line (0: skip)")
-    ap.add_argument("--batch-concurrency", default="1,4", help="concurrency levels timed for the batched line")
+    ap.add_argument("--batch-concurrency", default="1,4,8", help="concurrency levels timed for the batched line")
     ap.add_argument("--no-step", action="store_true", help="skip the surface-step line")
     ap.add_argument("--no-host", action="store_true", help="skip the host-pointer (PCIe-inclusive) line")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 16-bit V-cycle storage line")
