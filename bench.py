@@ -416,6 +416,9 @@ def main():
     mixed = None if args.no_mixed else mixed_precision_line(wl, vol, args, torch, _lib, W)
 
     batched = batch_line(args, torch, W) if args.batch > 0 else None
+    if batched:
+        # the headline projection's rate, for comparison with the patches'
+        batched["single_domain_mcells_per_s"] = round(wl.domain.cells / (ms_per_step * 1e-3) / 1e6, 1)
 
     learned = learned_sweeps = None
     if not args.no_learned and os.path.exists(args.weights):
