@@ -870,11 +870,13 @@ void cleanup(fsp_solver* S) {
 }
 
 // The correction and its metric, launched (read_measure reads the metric
-// after the next sync of the control block).
+// after the next sync of the control block). gauge: the mean-free pressure
+// of a pure-Neumann problem first (the kernels return at once otherwise;
+// a caller that has read the problem's flag passes it and launches none).
 void launch_correct(fsp_solver* S, const float* us, const float* vs, const float* ws, double* p, float k1, float* u,
-                    float* v, float* w) {
+                    float* v, float* w, bool gauge) {
   Launch K(S);
-  if (S->opt.gauge) {
+  if (gauge) {
     FSP_LAUNCH(S, k_sum_active<double>, dim3(grid_for(K.G.n)), K.st, K.G, (const double*)p,
                        OP_MEAN_P, S->ctl, S->partials);
     FSP_FINISH(S, K.st, OP_MEAN_P, dim3(grid_for(K.G.n)), false, 2);
@@ -915,8 +917,8 @@ void read_measure(fsp_solver* S, double& all, double& plain) {
 // Correction and acceptance metric (delta3d_mobile.rs:560-561); returns
 // (divergence over all wet rows, over plain rows), normalised as the reference.
 fsp_status correct_and_measure(fsp_solver* S, const float* us, const float* vs, const float* ws, double* p, float k1,
-                               float* u, float* v, float* w, double& all, double& plain) {
-  launch_correct(S, us, vs, ws, p, k1, u, v, w);
+                               float* u, float* v, float* w, double& all, double& plain, bool gauge) {
+  launch_correct(S, us, vs, ws, p, k1, u, v, w, gauge);
   fsp_status st = sync_ctl(S);
   if (st) return st;
   read_measure(S, all, plain);
@@ -1160,7 +1162,7 @@ fsp_status run_graded(fsp_solver* S, const float* us, const float* vs, const flo
     if (!std::isfinite(rr)) return FSP_NOT_FINITE;
     // correction from the fine pressure, then the restricted metric (:451-486)
     FSP_LAUNCH(S, k_gr_widen, gN, st, N, (const float*)p, S->p64);
-    launch_correct(S, us, vs, ws, S->p64, k1, u, v, w);
+    launch_correct(S, us, vs, ws, S->p64, k1, u, v, w, S->opt.gauge != 0);
     FSP_LAUNCH(S, k_divergence, gN, st, G, (const float*)u, (const float*)v, (const float*)w, S->d1);
     FSP_LAUNCH(S, k_gr_restrict<false>, gn, st, D, (const float*)S->d1, S->gz);
     hipLaunchKernelGGL(k_gr_metric, gn, dim3(BLOCK), 0, st, D, (const float*)S->gz, (const float*)S->g_weight,
@@ -1299,6 +1301,8 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
   // every ghost row in its column's table (binary apertures, column
   // geometry): the fine kernels' table-only forms
   S->table_only = (!S->frac && !S->levelset && !S->hctl->table_miss) ? 1 : 0;
+  // the correction's gauge runs only for a pure-Neumann problem (known now)
+  const bool gauge_now = S->opt.gauge && S->hctl->pure_neumann;
 
   const double b2 = S->hctl->b2;
   const double tol_b2 = rel_tol * rel_tol * b2;
@@ -1381,11 +1385,11 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
     const bool spec = !(S->opt.gauge && S->hctl->pure_neumann);
     if (spec && S->quad) {
       K.residual_backward(p64);
-      launch_correct(S, us, vs, ws, p64, k1, u, v, w);
+      launch_correct(S, us, vs, ws, p64, k1, u, v, w, gauge_now);
     } else {
       K.residual(p64, OP_RR);
       if (spec) {
-        launch_correct(S, us, vs, ws, p64, k1, u, v, w);
+        launch_correct(S, us, vs, ws, p64, k1, u, v, w, gauge_now);
         launch_backward(S, p64);
       }
     }
@@ -1407,7 +1411,7 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
       if (exhausted) { actual_rr = actual; wrapped = spec; break; }
       double a, pl;
       if (spec) read_measure(S, a, pl);
-      else if ((st = correct_and_measure(S, us, vs, ws, p64, k1, u, v, w, a, pl))) return st;
+      else if ((st = correct_and_measure(S, us, vs, ws, p64, k1, u, v, w, a, pl, gauge_now))) return st;
       if (pl <= 1e-5) { actual_rr = actual; settled = true; set_all = a; set_plain = pl; wrapped = spec; break; }
       have_measured = true; m_all = a; m_plain = pl;
       const float ratio = (float)(1e-5 / pl);
@@ -1452,7 +1456,7 @@ fsp_status run(fsp_solver* S, const float* us, const float* vs, const float* ws,
   double d_all, d_plain;
   if (settled) { d_all = set_all; d_plain = set_plain; }
   else if (wrapped) read_measure(S, d_all, d_plain);
-  else if ((st = correct_and_measure(S, us, vs, ws, p64, k1, u, v, w, d_all, d_plain))) return st;
+  else if ((st = correct_and_measure(S, us, vs, ws, p64, k1, u, v, w, d_all, d_plain, gauge_now))) return st;
   double be;
   if (wrapped) be = S->hctl->red[OP_BACKWARD][0];
   else if ((st = backward_error(S, p64, be))) return st;

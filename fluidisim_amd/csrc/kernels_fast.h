@@ -647,6 +647,25 @@ __global__ void __launch_bounds__(QBLOCK) k_residual_q(Fine G, QuadGrid Q, const
   if (BACKWARD) reduce_finish<true>(worst, unused, OP_BACKWARD, ctl, partials);
 }
 
+// The aperture codes and solid flags of a quad's faces and cells for the
+// aperture-weighted divergence: the y and z faces and the solid flags as one
+// 4-byte load each (rows of nx bytes, nx % 4 == 0), the five x faces as bytes.
+struct ApQuad {
+  uint8_t u[5];
+  uint32_t v0, v1, w0, w1, solid;
+};
+__device__ __forceinline__ ApQuad ap_quad(const Fine& G, size_t c0, size_t fu, size_t fv0, size_t fv1, size_t fw0,
+                                          size_t fw1) {
+  auto u32 = [](const uint8_t* a, size_t c) { return *reinterpret_cast<const uint32_t*>(a + c); };
+  ApQuad A;
+#pragma unroll
+  for (int e = 0; e < 5; ++e) A.u[e] = G.ou[fu + e];
+  A.v0 = u32(G.ov, fv0); A.v1 = u32(G.ov, fv1); A.w0 = u32(G.ow, fw0); A.w1 = u32(G.ow, fw1);
+  A.solid = u32(G.solid, c0);
+  return A;
+}
+__device__ __forceinline__ uint8_t ap_byte(uint32_t word, int e) { return (uint8_t)(word >> (8 * e)); }
+
 // b = scale * div + ghost terms, prec (optional) and |b|^2 on quads
 // (k_assemble with four x-cells per thread: the faces of the quad as 16-byte
 // loads, its x faces as five scalars, or the given divergence; the row's
@@ -666,12 +685,14 @@ __global__ void __launch_bounds__(QBLOCK) k_assemble_q(Fine G, QuadGrid Q, const
     const size_t fw0 = fwidx(G, i0, j, k), fw1 = fwidx(G, i0, j, k + 1);
     float u5[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
     float4 v0 = f4zero(), v1 = f4zero(), w0 = f4zero(), w1 = f4zero(), dq = f4zero();
+    ApQuad A{};
     if (div_in) {
       dq = ld4(div_in, c0);
     } else {
 #pragma unroll
       for (int e = 0; e < 5; ++e) u5[e] = us[fu + e];
       v0 = ld4(vs, fv0); v1 = ld4(vs, fv1); w0 = ld4(ws, fw0); w1 = ld4(ws, fw1);
+      if (G.cut) A = ap_quad(G, c0, fu, fv0, fv1, fw0, fw1);
     }
     float4 out, pout = f4zero();
 #pragma unroll
@@ -682,14 +703,14 @@ __global__ void __launch_bounds__(QBLOCK) k_assemble_q(Fine G, QuadGrid Q, const
         float div;
         if (div_in) {
           div = q_get(dq, e);
-        } else if (G.cut && G.solid[c0 + e]) {
+        } else if (G.cut && ap_byte(A.solid, e)) {
           div = 0.f;
         } else {
           float fl = u5[e], fr = u5[e + 1], ff = q_get(v0, e), fk = q_get(v1, e), fb = q_get(w0, e), ft = q_get(w1, e);
           if (G.cut) {
-            fl = aperture(G, 0, fu + e, G.ou[fu + e]) * fl; fr = aperture(G, 0, fu + e + 1, G.ou[fu + e + 1]) * fr;
-            ff = aperture(G, 1, fv0 + e, G.ov[fv0 + e]) * ff; fk = aperture(G, 1, fv1 + e, G.ov[fv1 + e]) * fk;
-            fb = aperture(G, 2, fw0 + e, G.ow[fw0 + e]) * fb; ft = aperture(G, 2, fw1 + e, G.ow[fw1 + e]) * ft;
+            fl = aperture(G, 0, fu + e, A.u[e]) * fl; fr = aperture(G, 0, fu + e + 1, A.u[e + 1]) * fr;
+            ff = aperture(G, 1, fv0 + e, ap_byte(A.v0, e)) * ff; fk = aperture(G, 1, fv1 + e, ap_byte(A.v1, e)) * fk;
+            fb = aperture(G, 2, fw0 + e, ap_byte(A.w0, e)) * fb; ft = aperture(G, 2, fw1 + e, ap_byte(A.w1, e)) * ft;
           }
           div = ((fr - fl + ft - fb) + (fk - ff)) / G.dx;
         }
@@ -872,17 +893,18 @@ __global__ void __launch_bounds__(QBLOCK) k_divmetric_q(Fine G, QuadGrid Q, cons
 #pragma unroll
       for (int e = 0; e < 5; ++e) u5[e] = u[fu + e];
       const float4 v0 = ld4(v, fv0), v1 = ld4(v, fv1), w0 = ld4(w, fw0), w1 = ld4(w, fw1);
+      const ApQuad A = G.cut ? ap_quad(G, c0, fu, fv0, fv1, fw0, fw1) : ApQuad{};
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const uint8_t m = (uint8_t)(mq >> (8 * e));
         if (!(m & M_WET)) continue;
         float div = 0.f;
-        if (!(G.cut && G.solid[c0 + e])) {
+        if (!(G.cut && ap_byte(A.solid, e))) {
           float fl = u5[e], fr = u5[e + 1], ff = q_get(v0, e), fk = q_get(v1, e), fb = q_get(w0, e), ft = q_get(w1, e);
           if (G.cut) {
-            fl = aperture(G, 0, fu + e, G.ou[fu + e]) * fl; fr = aperture(G, 0, fu + e + 1, G.ou[fu + e + 1]) * fr;
-            ff = aperture(G, 1, fv0 + e, G.ov[fv0 + e]) * ff; fk = aperture(G, 1, fv1 + e, G.ov[fv1 + e]) * fk;
-            fb = aperture(G, 2, fw0 + e, G.ow[fw0 + e]) * fb; ft = aperture(G, 2, fw1 + e, G.ow[fw1 + e]) * ft;
+            fl = aperture(G, 0, fu + e, A.u[e]) * fl; fr = aperture(G, 0, fu + e + 1, A.u[e + 1]) * fr;
+            ff = aperture(G, 1, fv0 + e, ap_byte(A.v0, e)) * ff; fk = aperture(G, 1, fv1 + e, ap_byte(A.v1, e)) * fk;
+            fb = aperture(G, 2, fw0 + e, ap_byte(A.w0, e)) * fb; ft = aperture(G, 2, fw1 + e, ap_byte(A.w1, e)) * ft;
           }
           div = ((fr - fl + ft - fb) + (fk - ff)) / G.dx;
         }
