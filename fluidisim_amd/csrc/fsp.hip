@@ -2133,6 +2133,14 @@ fsp_status fsp_precondition_device(fsp_solver* S, const float* d_r, float* d_z) 
   if (!S->geom) return FSP_DOMAIN;
   FSP_HIP(hipSetDevice(S->opt.device));
   build_geometry(S, true);
+  // the fine kernels' form a projection on this geometry takes (the
+  // table-only one when every ghost row lies in its column's table)
+  if (const fsp_status st = sync_ctl(S)) return st;
+  struct TableGuard {
+    fsp_solver* S;
+    ~TableGuard() { S->table_only = 0; }
+  } table_guard{S};
+  S->table_only = (!S->frac && !S->levelset && !S->hctl->table_miss) ? 1 : 0;
   Launch K(S);
   FSP_HIP(hipMemcpyAsync(S->r, d_r, S->n * 4, hipMemcpyDeviceToDevice, S->stream));
   K.set_pass(1, 0.0, 0.0, 0.0, 0, 1);

@@ -2,6 +2,8 @@
 fsp_create (the reference's I-06, every buffer reserved before seal(),
 delta3d.rs:225-241), and a learned-phase graph stays valid when the shared
 network's buffers move to another shape."""
+import gc
+
 import numpy as np
 import pytest
 
@@ -33,11 +35,18 @@ def test_projection_and_surface_step_reserve_no_device_memory():
     rng = np.random.default_rng(4)
     vol.set_velocity(*(rng.uniform(-1e-2, 1e-2, n).astype(np.float32) for n in wl.domain.faces))
     vol._upload_geometry()
-    f0 = free_bytes()
-    rep = vol.project_mobile3(wl.scale, wl.k1, 2000, warm=False)
-    f1 = free_bytes()
-    vol.step_surface_mobile(1000, 4000)
-    f2 = free_bytes()
+    # the count is device-wide: solvers of earlier tests that the collector
+    # has yet to destroy must not give their memory back inside the window
+    gc.collect()
+    gc.disable()
+    try:
+        f0 = free_bytes()
+        rep = vol.project_mobile3(wl.scale, wl.k1, 2000, warm=False)
+        f1 = free_bytes()
+        vol.step_surface_mobile(1000, 4000)
+        f2 = free_bytes()
+    finally:
+        gc.enable()
     print(f"free bytes {f0} -> {f1} (projection, {rep.iterations} it) -> {f2} (surface step)", flush=True)
     assert rep.iterations > 0
     assert f0 == f1 == f2

@@ -602,6 +602,27 @@ def test_fractional_apertures_are_refused_where_unsupported():
     vol.close()
 
 
+def test_vcycle_on_tall_ghost_columns_matches_the_oracle():
+    """The dam-break geometry below (ghost rows past the column tables): the
+    V-cycle runs the fine kernels' general forms, and equals the oracle's bit
+    for bit, as the table-only forms do on the other V-cycle tests."""
+    wl = W.tank(64, 48, 64, seed=8)
+    d = wl.domain
+    i = np.arange(d.nx)
+    row = np.where(i < d.nx // 2, 0.8 * d.nz * d.dx, 0.3 * d.nz * d.dx).astype(np.float32)
+    wl.eta = np.tile(row, d.ny).astype(np.float32)
+    vol = W.build_volume(wl)
+    prob = problem(wl)
+    _, prec = po.assemble(prob, np.zeros(wl.domain.cells, np.float32), 1.0)
+    r = noise(wl.domain.cells, 14)
+    r[prec == 0] = 0.0
+    z = host(vol.precondition(torch.from_numpy(r).cuda()))
+    z_ref = po.vcycle(prob, r, oracle_options(wl)).astype(np.float32)
+    assert np.abs(z_ref).max() > 0
+    assert np.array_equal(bits(z), bits(z_ref)), f"max diff {np.abs(z - z_ref).max()}"
+    vol.close()
+
+
 def test_tall_ghost_columns_take_the_general_kernels():
     """A dam-break step in the surface: the columns beside the step hold
     side-ghost rows from the low surface to the high one, far more than a
