@@ -2,7 +2,7 @@
 //
 // A from-scratch C++ restatement of the reference's δ-3D pressure projection
 // (`Volume3::project_mobile3` with its multigrid preconditioner). It exists to
-// judge the HIP library: only tests/, __graft_entry__.smoke() and bench.py's
+// check the HIP library: only tests/, __graft_entry__.smoke() and bench.py's
 // cpu_baseline leg may load it, and only as the checker or as the timed CPU
 // baseline. The product path never calls it.
 //

@@ -1,10 +1,12 @@
-"""Every north-star configuration on the card at its stated size, against
+"""Every target configuration (closed box 64^3, tank 128^3, the 256x128x256
+tank with 16-bit V-cycle storage and residual replacement, 48 patches, the
+learned preconditioner) on the card at its stated size, against
 the f64 oracle (the reference's MGPCG restated in C++, run in double), plus
 the report's metrics against the oracle's on the same inputs and the
 driver's acceptance paths forced from the host (the divergence-tightening
 restart and the round-off floor stop of delta3d_mobile.rs:546-585).
 
-Parity bar (north star): pressure within 1e-4 relative L2 of the f64 oracle
+Parity bar: pressure within 1e-4 relative L2 of the f64 oracle
 on wet cells (mean-free on the pure-Neumann closed box), max |div u| (the
 reference's normalised plain-row metric) within 1e-5, residual 1e-6.
 """
