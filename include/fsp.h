@@ -267,7 +267,8 @@ fsp_status fsp_apply_operator_device(fsp_solver* s, const float* d_p, float* d_o
 fsp_status fsp_assemble_device(fsp_solver* s, const float* d_div, float scale, float* d_rhs, float* d_prec);
 
 /* One preconditioner application z = M^-1 r on the current geometry
- * (v_cycle3, delta3d_multigrid.rs:393-429). Device pointers. */
+ * (v_cycle3, delta3d_multigrid.rs:393-429), with the fine kernels a
+ * projection on this geometry runs. Device pointers. */
 fsp_status fsp_precondition_device(fsp_solver* s, const float* d_r, float* d_z);
 
 /* Divergence of faces (delta3d.rs:747-763, :806-823). Device pointers. */
