@@ -4,7 +4,12 @@ residual on the 256x128x256 free-surface tank (plain MGPCG), one MI355X.
 A step is one cold-start projection (p = 0) of the whole workload: geometry
 rebuild, divergence and right-hand side, the preconditioned CG to 1e-6,
 the gradient subtract and the acceptance metrics, with every input already
-resident in HBM. Prints one JSON line.
+resident in HBM. Prints one JSON line; beside the headline it carries the
+dominant kernel's roofline (live launch timing, HBM traffic from the
+committed counter table), the CPU oracle's time for the same projection,
+the bfloat16 V-cycle configuration with residual replacement, the learned
+preconditioner in its two modes, the 48-patch batch, the surface step of the
+sloshing tank and the host-pointer (PCIe-inclusive) projection.
 """
 from __future__ import annotations
 
