@@ -188,6 +188,17 @@ unsigned grid_for(size_t n) {
   return (unsigned)b;
 }
 
+// One item per thread for a grid-stride kernel that reduces nothing (no
+// per-block partial): every item's loads are in flight at once instead of
+// a thread walking several items one after the other (the advection's
+// gathers: 55-62 -> 47-52 us; plain streams ran slower so, 17.5 -> 20 us).
+unsigned grid_items(size_t n) {
+  size_t b = (n + BLOCK - 1) / BLOCK;
+  if (b > (size_t)(1u << 24)) b = 1u << 24;
+  if (b == 0) b = 1;
+  return (unsigned)b;
+}
+
 // The loop-active flag a timed launch starts with (stream order: the value
 // the kernel itself reads), recorded before the launch.
 __global__ void k_prof_flag(const Ctl* ctl, int32_t* out) {
@@ -2201,9 +2212,9 @@ fsp_status advect(fsp_solver* S, const float* u, const float* v, const float* w,
                   float* ws) {
   const Fine G = S->fine();
   const float h = 0.5f / G.dx;
-  FSP_LAUNCH(S, k_advect_h<0>, dim3(grid_for(S->nu)), S->stream, G, u, v, w, dt, h, us);
-  FSP_LAUNCH(S, k_advect_h<1>, dim3(grid_for(S->nv)), S->stream, G, u, v, w, dt, h, vs);
-  FSP_LAUNCH(S, k_advect_w, dim3(grid_for(S->nw)), S->stream, G, u, v, w, dt, h, ws);
+  FSP_LAUNCH(S, k_advect_h<0>, dim3(grid_items(S->nu)), S->stream, G, u, v, w, dt, h, us);
+  FSP_LAUNCH(S, k_advect_h<1>, dim3(grid_items(S->nv)), S->stream, G, u, v, w, dt, h, vs);
+  FSP_LAUNCH(S, k_advect_w, dim3(grid_items(S->nw)), S->stream, G, u, v, w, dt, h, ws);
   FSP_HIP(hipGetLastError());
   return FSP_OK;
 }
