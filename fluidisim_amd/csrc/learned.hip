@@ -381,20 +381,34 @@ __global__ void __launch_bounds__(256) k_bgrad(const _Float16* g, size_t cells, 
 }
 
 // ---------------------------------------------------------------- elementwise
-__global__ void k_pool(const _Float16* in, _Float16* out, int cnx, int cny, int cnz, int C, const int32_t* gate) {
+// Mean over each 2x2x2 block of a channels-last activation with C = 8 G channels.
+// A thread averages one group of eight channels (16 bytes) of one coarse cell, in
+// the order dz, dy, dx of the children. Block (64, 4): x runs over (coarse x,
+// group), y and blockIdx.y over coarse rows (y, z).
+template <int G>
+__global__ void k_pool(const _Float16* in, _Float16* out, int cnx, int cny, int cnz, const int32_t* gate) {
   if (gate && !*gate) return;
-  const size_t total = (size_t)cnx * cny * cnz * C;
-  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
-    const int c = (int)(q % C);
-    const size_t cell = q / C;
-    const int i = (int)(cell % cnx), j = (int)((cell / cnx) % cny), k = (int)(cell / ((size_t)cnx * cny));
-    const int fnx = 2 * cnx, fny = 2 * cny;
-    float s = 0.f;
+  constexpr int C = 8 * G;
+  const int t = blockIdx.x * 64 + threadIdx.x;
+  if (t >= cnx * G) return;
+  const int i = t / G, g = t % G;
+  const int fnx = 2 * cnx, fny = 2 * cny, rows = cny * cnz;
+  for (int row = blockIdx.y * 4 + threadIdx.y; row < rows; row += gridDim.y * 4) {
+    const int j = row % cny, k = row / cny;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int dz = 0; dz < 2; ++dz)
-      for (int dy = 0; dy < 2; ++dy)
-        for (int dx = 0; dx < 2; ++dx)
-          s += (float)in[(((size_t)(2 * k + dz) * fny + 2 * j + dy) * fnx + 2 * i + dx) * C + c];
-    out[q] = (_Float16)(s * 0.125f);
+      for (int dy = 0; dy < 2; ++dy) {
+        const size_t f = ((size_t)(2 * k + dz) * fny + 2 * j + dy) * fnx + 2 * i;
+        const h8 a = *reinterpret_cast<const h8*>(in + f * C + 8 * g);
+        const h8 b = *reinterpret_cast<const h8*>(in + (f + 1) * C + 8 * g);
+        for (int c = 0; c < 8; ++c) {
+          s[c] += (float)a[c];
+          s[c] += (float)b[c];
+        }
+      }
+    h8 o;
+    for (int c = 0; c < 8; ++c) o[c] = (_Float16)(s[c] * 0.125f);
+    *reinterpret_cast<h8*>(out + ((size_t)row * cnx + i) * C + 8 * g) = o;
   }
 }
 // g_fine = g_coarse[parent] / 8 (+ add[fine])
@@ -498,7 +512,10 @@ __global__ void k_adam(float* theta, float* m1, float* m2, const float* grad, si
 }
 
 // Network input of a residual: (r * sigma, wet, ghost row, solid) in binary16,
-// sigma = sqrt(n_wet / |r|^2) taken from the device (f64) or given.
+// sigma = sqrt(n_wet / |r|^2) taken from the device (f64) or given. With VEC a
+// thread converts four consecutive cells through 16-byte loads and stores (the
+// cell count is a multiple of four); without it, one cell per step.
+template <bool VEC>
 __global__ void k_features(const float* r, const uint8_t* mask, const uint8_t* solid, size_t cells,
                            const double* rr, const uint32_t* nwet, float sigma_given, _Float16* x, float* sigma_out,
                            const int32_t* gate) {
@@ -509,25 +526,52 @@ __global__ void k_features(const float* r, const uint8_t* mask, const uint8_t* s
     sigma = (float)s;
   }
   if (sigma_out && blockIdx.x == 0 && threadIdx.x == 0) *sigma_out = sigma;
-  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += (size_t)gridDim.x * blockDim.x) {
-    const uint8_t m = mask[c];
+  auto cell = [&](float rc, uint32_t m, uint32_t sd) {
     const bool wet = (m & 64u) != 0;
     h4 v;
-    v[0] = (_Float16)(wet ? r[c] * sigma : 0.f);
+    v[0] = (_Float16)(wet ? rc * sigma : 0.f);
     v[1] = (_Float16)(wet ? 1.f : 0.f);
     v[2] = (_Float16)((wet && (m & 128u)) ? 1.f : 0.f);
-    v[3] = (_Float16)((solid && solid[c]) ? 1.f : 0.f);
-    *reinterpret_cast<h4*>(x + 4 * c) = v;
+    v[3] = (_Float16)(sd ? 1.f : 0.f);
+    return v;
+  };
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  if (VEC) {
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < cells / 4; q += stride) {
+      const f4 rv = reinterpret_cast<const f4*>(r)[q];
+      const uint32_t m4 = reinterpret_cast<const uint32_t*>(mask)[q];
+      const uint32_t s4 = solid ? reinterpret_cast<const uint32_t*>(solid)[q] : 0u;
+      h4 v[4];
+      for (int e = 0; e < 4; ++e) v[e] = cell(rv[e], (m4 >> (8 * e)) & 255u, (s4 >> (8 * e)) & 255u);
+      h8* o = reinterpret_cast<h8*>(x + 16 * q);
+      o[0] = h8{v[0][0], v[0][1], v[0][2], v[0][3], v[1][0], v[1][1], v[1][2], v[1][3]};
+      o[1] = h8{v[2][0], v[2][1], v[2][2], v[2][3], v[3][0], v[3][1], v[3][2], v[3][3]};
+    }
+  } else {
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += stride)
+      *reinterpret_cast<h4*>(x + 4 * c) = cell(r[c], mask[c], solid ? solid[c] : 0u);
   }
 }
 
-// z = y / sigma on wet cells, 0 elsewhere.
+// z = y / sigma on wet cells, 0 elsewhere; four cells per thread with VEC.
+template <bool VEC>
 __global__ void k_unscale(const float* y, const uint8_t* mask, const float* sigma, size_t cells, float* z,
                           const int32_t* gate) {
   if (gate && !*gate) return;
   const float s = *sigma;
-  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += (size_t)gridDim.x * blockDim.x)
-    z[c] = (mask[c] & 64u) ? y[c] / s : 0.f;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  if (VEC) {
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < cells / 4; q += stride) {
+      const f4 yv = reinterpret_cast<const f4*>(y)[q];
+      const uint32_t m4 = reinterpret_cast<const uint32_t*>(mask)[q];
+      f4 zv;
+      for (int e = 0; e < 4; ++e) zv[e] = ((m4 >> (8 * e)) & 64u) ? yv[e] / s : 0.f;
+      reinterpret_cast<f4*>(z)[q] = zv;
+    }
+  } else {
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += stride)
+      z[c] = (mask[c] & 64u) ? y[c] / s : 0.f;
+  }
 }
 
 // One 16x16x16 product, for checking the MFMA operand layout.
@@ -558,6 +602,18 @@ inline unsigned blocks_for(size_t n, unsigned threads = 256, unsigned cap = 4096
   size_t b = (n + threads - 1) / threads;
   if (b < 1) b = 1;
   return (unsigned)(b < cap ? b : cap);
+}
+
+// One thread per item (up to 2^24 blocks): for elementwise kernels, whose time a
+// capped grid-stride loop lengthens by serialising each thread's steps.
+inline unsigned blocks_per_item(size_t n, unsigned threads = 256) { return blocks_for(n, threads, 1u << 24); }
+
+// Whether four cells per thread fit the pointers: a16/b16 (four floats or four f16
+// quadruples) on 16 bytes, a4/b4 (four byte masks) on 4 bytes, a cell count that
+// is a multiple of four.
+inline bool quad_cells(size_t cells, const void* a16, const void* b16, const void* a4, const void* b4) {
+  auto al = [](const void* p, uintptr_t a) { return p == nullptr || ((uintptr_t)p & (a - 1)) == 0; };
+  return cells % 4 == 0 && al(a16, 16) && al(b16, 16) && al(a4, 4) && al(b4, 4);
 }
 
 }  // namespace net
@@ -682,6 +738,12 @@ Acts net_acts(fsp_net* N) {
   return A;
 }
 
+// k_pool's grid over coarse rows of cnx cells and g channel groups.
+dim3 pool_grid(int cnx, int cny, int cnz, int g) {
+  const unsigned rows = (unsigned)cny * (unsigned)cnz;
+  return dim3(((unsigned)cnx * g + 63) / 64, std::min<unsigned>((rows + 3) / 4, 65535u));
+}
+
 void forward(fsp_net* N, const Acts& A) {
   hipStream_t st = N->cur;
   const int32_t* gt = N->gate;
@@ -691,10 +753,10 @@ void forward(fsp_net* N, const Acts& A) {
   auto B = [&](int l) { return N->theta + N->poff[l] + (size_t)LAYERS[l].cout * 27 * LAYERS[l].cin; };
   conv<4, 8, 0, 0>(st, X0, Y0, Z0, A.x0, N->wt[0], B(0), nullptr, A.act[0], nullptr, gt);
   conv<8, 8, 0, 0>(st, X0, Y0, Z0, A.act[0], N->wt[1], B(1), nullptr, A.act[1], nullptr, gt);
-  k_pool<<<blocks_for(V1 * 8), 256, 0, st>>>(A.act[1], A.p1, X1, Y1, Z1, 8, gt);
+  k_pool<1><<<pool_grid(X1, Y1, Z1, 1), dim3(64, 4), 0, st>>>(A.act[1], A.p1, X1, Y1, Z1, gt);
   conv<8, 16, 0, 0>(st, X1, Y1, Z1, A.p1, N->wt[2], B(2), nullptr, A.act[2], nullptr, gt);
   conv<16, 16, 0, 0>(st, X1, Y1, Z1, A.act[2], N->wt[3], B(3), nullptr, A.act[3], nullptr, gt);
-  k_pool<<<blocks_for(V2 * 16), 256, 0, st>>>(A.act[3], A.p2, X2, Y2, Z2, 16, gt);
+  k_pool<2><<<pool_grid(X2, Y2, Z2, 2), dim3(64, 4), 0, st>>>(A.act[3], A.p2, X2, Y2, Z2, gt);
   conv<16, 32, 0, 0>(st, X2, Y2, Z2, A.p2, N->wt[4], B(4), nullptr, A.act[4], nullptr, gt);
   conv<32, 32, 0, 0>(st, X2, Y2, Z2, A.act[4], N->wt[5], B(5), nullptr, A.act[5], nullptr, gt);
   conv<32, 16, 1, 3>(st, X1, Y1, Z1, A.act[5], N->wt[6], B(6), A.act[3], A.act[6], nullptr, gt);
@@ -796,20 +858,29 @@ uint64_t net_generation(const fsp_net* N) { return N ? N->generation : 0; }
 fsp_status net_features(fsp_net* N, hipStream_t stream, const float* r, const uint8_t* mask, const uint8_t* solid,
                         const double* rr, const uint32_t* nwet, float sigma, uint16_t* x_out) {
   const size_t V = N->V[0];
-  k_features<<<blocks_for(V), 256, 0, stream>>>(r, mask, solid, V, rr, nwet, sigma,
-                                               reinterpret_cast<_Float16*>(x_out), N->sigma, nullptr);
+  _Float16* x = reinterpret_cast<_Float16*>(x_out);
+  if (quad_cells(V, r, x, mask, solid))
+    k_features<true><<<blocks_per_item(V / 4), 256, 0, stream>>>(r, mask, solid, V, rr, nwet, sigma, x, N->sigma, nullptr);
+  else
+    k_features<false><<<blocks_for(V), 256, 0, stream>>>(r, mask, solid, V, rr, nwet, sigma, x, N->sigma, nullptr);
   return hipGetLastError() == hipSuccess ? FSP_OK : FSP_DEVICE;
 }
 
 fsp_status net_apply(fsp_net* N, hipStream_t stream, const float* r, const uint8_t* mask, const uint8_t* solid,
                      const double* rr, const uint32_t* nwet, const int32_t* gate, float* z) {
   const size_t V = N->V[0];
-  k_features<<<blocks_for(V), 256, 0, stream>>>(r, mask, solid, V, rr, nwet, 0.f, N->x0, N->sigma, gate);
+  if (quad_cells(V, r, N->x0, mask, solid))
+    k_features<true><<<blocks_per_item(V / 4), 256, 0, stream>>>(r, mask, solid, V, rr, nwet, 0.f, N->x0, N->sigma, gate);
+  else
+    k_features<false><<<blocks_for(V), 256, 0, stream>>>(r, mask, solid, V, rr, nwet, 0.f, N->x0, N->sigma, gate);
   N->cur = stream;
   N->gate = gate;
   forward(N);
   N->gate = nullptr;
-  k_unscale<<<blocks_for(V), 256, 0, stream>>>(N->y, mask, N->sigma, V, z, gate);
+  if (quad_cells(V, N->y, z, mask, nullptr))
+    k_unscale<true><<<blocks_per_item(V / 4), 256, 0, stream>>>(N->y, mask, N->sigma, V, z, gate);
+  else
+    k_unscale<false><<<blocks_for(V), 256, 0, stream>>>(N->y, mask, N->sigma, V, z, gate);
   return hipGetLastError() == hipSuccess ? FSP_OK : FSP_DEVICE;
 }
 

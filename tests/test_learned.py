@@ -189,6 +189,46 @@ def test_training_steps_reduce_the_loss(net):
 
 
 @pytest.mark.gpu
+def test_features_follow_the_cell_classes(net):
+    """The network input of a residual: (r sigma, wet, ghost row, solid) per
+    cell in binary16 with sigma = sqrt(n_wet / |r|^2), checked against the
+    oracle's classification; a residual whose address is not 16-byte aligned
+    (one cell at a time) gives the same bits as an aligned one (four cells
+    per thread)."""
+    t = _t()
+    wl = W.tank(64, 32, 64, seed=21)
+    d = wl.domain
+    prob = po.Problem(d.nx, d.ny, d.nz, d.dx, wl.eta, rho=wl.rho, g_eff=wl.g_eff, rest=wl.rest,
+                      homogeneous_ghost=wl.homogeneous_ghost, closed_lid=wl.closed_lid, solid=wl.solid)
+    kind, ghost = po.classify(prob)
+    wet = kind == 1
+    vol = W.build_volume(wl)
+    r = np.random.default_rng(22).standard_normal(d.cells).astype(np.float32) * wet
+    ra = t.tensor(r, device="cuda")
+    buf = t.zeros(d.cells + 1, dtype=t.float32, device="cuda")
+    buf[1:] = ra
+    assert buf[1:].data_ptr() % 16 != 0
+    xa, sa = net.features(vol, ra)
+    xb, sb = net.features(vol, buf[1:])
+    assert sa == sb and t.equal(xa, xb)
+    sigma = np.float32(np.sqrt(wet.sum() / np.sum(r.astype(np.float64) ** 2)))
+    assert abs(sa - sigma) <= 1e-6 * sigma
+    x = xa.view(t.float16).view(-1, 4).cpu().numpy()
+    assert np.array_equal(x[:, 1], wet.astype(np.float16))
+    # r sigma rounded once to binary16 (the card's mixed fused multiply-add
+    # rounds the exact product; the float64 product of two floats is exact)
+    want = np.where(wet, (r.astype(np.float64) * np.float64(sa)).astype(np.float16), np.float16(0))
+    bad = np.flatnonzero(x[:, 0] != want)
+    for c in bad[:8]:
+        print(f"cell {c}: r {r[c]!r} got {x[c, 0]!r} want {want[c]!r}")
+    assert bad.size == 0
+    assert np.array_equal(x[:, 2], ((ghost != 0) & wet).astype(np.float16))
+    solid = np.zeros(d.cells, np.uint8) if wl.solid is None else np.asarray(wl.solid).ravel()
+    assert np.array_equal(x[:, 3], (solid != 0).astype(np.float16))
+    vol.close()
+
+
+@pytest.mark.gpu
 def test_learned_fcg_with_an_untrained_network_still_converges(net):
     """A network that is no preconditioner: the MGPCG fallback brings the
     projection to 1e-6 with the reference's pressure."""
