@@ -282,6 +282,151 @@ __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv(ConvArgs 
   }
 }
 
+// ---------------------------------------------------------------- upsampling convolution
+// The 3x3x3 convolution of the nearest-neighbour upsampled coarse input as a
+// 2x2x2 convolution of the coarse input itself. Per axis, the three taps of
+// an output cell of parity p (x = 2c + p) fall on two coarse cells,
+// c - 1 + p + o for o = 0, 1: taps {-1} and {0, +1} when p = 0, {-1, 0} and
+// {+1} when p = 1. One set of summed weights per parity (k_pack_up) gives
+// K = 8 CIN instead of 27 CIN, and the LDS halo holds coarse cells.
+//
+// Output brick 32 x BY x 4 cells (BY = 8, or 4 for 32 input channels, whose
+// weights take twice the registers); wavefront w takes the cells of parity
+// (px, py, pz) = (w & 1, (w >> 1) & 1, w >> 2): a 16-lane row holds the
+// cells x = x0 + 2m + px of one row (y, z), so every lane of a product uses
+// the same weights, and the wavefront keeps its parity's weights in
+// registers. Its rows: y = y0 + 2a + py (a < BY / 2), z = z0 + 2b + pz (b < 2).
+// Epilogue: bias + skip + ReLU -> binary16.
+template <int CIN>
+struct UpBrick {
+  static constexpr int BX = 32, BY = CIN >= 32 ? 4 : 8, BZ = 4;
+  static constexpr int HX = BX / 2 + 2, HY = BY / 2 + 2, HZ = BZ / 2 + 2;
+  static constexpr int HCELLS = HX * HY * HZ;
+  static constexpr int AY = BY / 2;  // rows of one parity per plane
+  static constexpr int TPW = AY * (BZ / 2);
+};
+static_assert(CWAVES == 8, "one wavefront per parity of the upsampling convolution");
+
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv_up(ConvArgs a) {
+  using Bk = UpBrick<CIN>;
+  constexpr int NT = npad(COUT) / 16;
+  constexpr int NP = npad(COUT);
+  constexpr int K = 8 * CIN;
+  constexpr int CHUNKS = K / 32;
+  static_assert(CIN % 8 == 0 && K % 32 == 0 && COUT % 4 == 0, "channel groups");
+  __shared__ __attribute__((aligned(16))) _Float16 halo[Bk::HCELLS * CIN];
+  if (a.gate && !*a.gate) return;
+  const int x0 = blockIdx.x * Bk::BX, y0 = blockIdx.y * Bk::BY, z0 = blockIdx.z * Bk::BZ;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const int px = wave & 1, py = (wave >> 1) & 1, pz = wave >> 2;
+
+  // coarse halo (zero outside the coarse grid) -> LDS; origin (x0/2 - 1, y0/2 - 1, z0/2 - 1)
+  {
+    constexpr int VPC = CIN / 8;
+    constexpr int TOTAL = Bk::HCELLS * VPC;
+    constexpr int PER = (TOTAL + CTHREADS - 1) / CTHREADS;
+    const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 1, cz0 = z0 / 2 - 1;
+    uint4 buf[PER];
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      const int q = (int)threadIdx.x + r * CTHREADS;
+      const int cell = q / VPC, part = q - cell * VPC;
+      const int lx = cell % Bk::HX, t = cell / Bk::HX, ly = t % Bk::HY, lz = t / Bk::HY;
+      const int cx = cx0 + lx, cy = cy0 + ly, cz = cz0 + lz;
+      const bool in = q < TOTAL && cx >= 0 && cx < a.cnx && cy >= 0 && cy < a.cny && cz >= 0 && cz < a.cnz;
+      buf[r] = make_uint4(0u, 0u, 0u, 0u);
+      if (in) buf[r] = *reinterpret_cast<const uint4*>(a.in + (((size_t)cz * a.cny + cy) * a.cnx + cx) * CIN + part * 8);
+    }
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      const int q = (int)threadIdx.x + r * CTHREADS;
+      if (q < TOTAL) *reinterpret_cast<uint4*>(halo + (size_t)q * 8) = buf[r];
+    }
+  }
+  __syncthreads();
+
+  // this parity's weights: lane (m, g) holds row t*16 + m, k = 32 ch + 8 g .. +7
+  h8 w[CHUNKS][NT];
+#pragma unroll
+  for (int ch = 0; ch < CHUNKS; ++ch)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+      w[ch][t] = *reinterpret_cast<const h8*>(a.wt + (size_t)(wave * NP + t * 16 + m) * K + 32 * ch + 8 * g);
+  // skip values of the epilogue (8 channels of a cell per item, see below),
+  // loaded before the products
+  constexpr int CELLS = Bk::BX * Bk::BY * Bk::BZ;
+  constexpr int GROUPS = COUT / 8;  // 8-channel groups of a cell
+  constexpr int IPT = CELLS * GROUPS / CTHREADS;
+  static_assert(COUT % 8 == 0 && (CELLS * GROUPS) % CTHREADS == 0, "epilogue items");
+  h8 skv[IPT];
+  size_t ocell[IPT];
+  bool olive[IPT];
+#pragma unroll
+  for (int r = 0; r < IPT; ++r) {
+    const int q = (int)threadIdx.x + r * CTHREADS, cl = q / GROUPS, part = q - cl * GROUPS;
+    const int lx = cl % Bk::BX, t = cl / Bk::BX, ly = t % Bk::BY, lz = t / Bk::BY;
+    const int y = y0 + ly, z = z0 + lz;
+    olive[r] = y < a.ny && z < a.nz;
+    ocell[r] = (((size_t)z * a.ny + y) * a.nx + x0 + lx) * COUT + part * 8;
+    skv[r] = h8(0);
+    if (olive[r]) skv[r] = *reinterpret_cast<const h8*>(a.skip + ocell[r]);
+  }
+  f4 acc[Bk::TPW][NT];
+#pragma unroll
+  for (int i = 0; i < Bk::TPW; ++i)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[i][t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ch = 0; ch < CHUNKS; ++ch) {
+    // 8 consecutive channels of one coarse tap (ox, oy, oz)
+    const int kk = 32 * ch + 8 * g;
+    const int tap = kk / CIN, ci = kk - tap * CIN;
+    const int ox = tap & 1, oy = (tap >> 1) & 1, oz = tap >> 2;
+#pragma unroll
+    for (int i = 0; i < Bk::TPW; ++i) {
+      const int hx = m + px + ox, hy = i % Bk::AY + py + oy, hz = i / Bk::AY + pz + oz;
+      const h8 xv = *reinterpret_cast<const h8*>(halo + ((hz * Bk::HY + hy) * Bk::HX + hx) * CIN + ci);
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(w[ch][t], xv, acc[i][t], 0, 0, 0);
+    }
+  }
+
+  // epilogue through LDS: each wavefront stages acc + bias (f32) of its cells,
+  // then a thread finishes 8 channels of a cell per item (+ skip, ReLU), so the
+  // skip loads and the output stores are contiguous 16-byte accesses
+  __shared__ __attribute__((aligned(16))) float stage[CELLS * COUT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int co = t * 16 + 4 * g;
+    if (co >= COUT) continue;
+    const f4 bias = *reinterpret_cast<const f4*>(a.bias + co);
+#pragma unroll
+    for (int i = 0; i < Bk::TPW; ++i) {
+      const int lx = 2 * m + px, ly = 2 * (i % Bk::AY) + py, lz = 2 * (i / Bk::AY) + pz;
+      *reinterpret_cast<f4*>(stage + ((lz * Bk::BY + ly) * Bk::BX + lx) * COUT + co) = acc[i][t] + bias;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < IPT; ++r) {
+    if (!olive[r]) continue;
+    const int q = (int)threadIdx.x + r * CTHREADS;
+    const f4 lo = *reinterpret_cast<const f4*>(stage + q * 8);
+    const f4 hi = *reinterpret_cast<const f4*>(stage + q * 8 + 4);
+    h8 o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      float u = e < 4 ? lo[e] : hi[e - 4];
+      u += (float)skv[r][e];
+      o[e] = (_Float16)(u > 0.f ? u : 0.f);
+    }
+    *reinterpret_cast<h8*>(a.out + ocell[r]) = o;
+  }
+}
+
 // ---------------------------------------------------------------- weight gradient
 // Partial dW[co][k] = sum over the cells of one partition of g[cell][co] *
 // in[cell + tap][ci] (k = tap*CIN + ci), one wavefront per (partition, k
@@ -496,6 +641,31 @@ __global__ void k_pack(const float* theta, int cin, int cout, _Float16* wt, _Flo
   }
 }
 
+// Summed weights of an upsampling layer for k_conv_up: [8 parities][npad(cout)]
+// [8 coarse taps x cin], coarse tap (ox, oy, oz) of parity (px, py, pz) the
+// f32 sum of the layer's taps d (index d + 1 per axis) that fall on it:
+// {0} and {1, 2} for p = 0, {0, 1} and {2} for p = 1; rounded once to binary16.
+__global__ void k_pack_up(const float* theta, int cin, int cout, _Float16* wu) {
+  const int NP = npad(cout), K = 8 * cin, total = 8 * NP * K;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+    const int P = q / (NP * K), r = q - P * NP * K, co = r / K, k = r - co * K;
+    const int tap = k / cin, ci = k - tap * cin;
+    float v = 0.f;
+    if (co < cout) {
+      const int p[3] = {P & 1, (P >> 1) & 1, P >> 2}, o[3] = {tap & 1, (tap >> 1) & 1, tap >> 2};
+      int lo[3], hi[3];
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = p[d] == 0 ? (o[d] == 0 ? 0 : 1) : (o[d] == 0 ? 0 : 2);
+        hi[d] = p[d] == 0 ? (o[d] == 0 ? 0 : 2) : (o[d] == 0 ? 1 : 2);
+      }
+      for (int dz = lo[2]; dz <= hi[2]; ++dz)
+        for (int dy = lo[1]; dy <= hi[1]; ++dy)
+          for (int dx = lo[0]; dx <= hi[0]; ++dx) v += theta[((size_t)co * 27 + dz * 9 + dy * 3 + dx) * cin + ci];
+    }
+    wu[q] = (_Float16)v;
+  }
+}
+
 // Adam (Kingma & Ba 2015) on the f32 master parameters; skipped entirely
 // when a gradient was non-finite.
 __global__ void k_adam(float* theta, float* m1, float* m2, const float* grad, size_t n, float lr, float b1, float b2,
@@ -633,6 +803,7 @@ struct fsp_net {
   float *theta = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr;
   _Float16* wt[NL] = {};
   _Float16* wb[NL] = {};
+  _Float16* wu[NL] = {};  // summed per-parity weights of the upsampling layers (k_conv_up)
   size_t poff[NL] = {};  // offset of layer l's W in theta (b follows)
   int* bad = nullptr;
   int* hbad = nullptr;
@@ -690,6 +861,20 @@ void conv(hipStream_t st, int nx, int ny, int nz, const _Float16* in, const _Flo
   const dim3 grid((unsigned)(nx / Bk::BX), (unsigned)((ny + Bk::BY - 1) / Bk::BY),
                   (unsigned)((nz + Bk::BZ - 1) / Bk::BZ));
   k_conv<CIN, COUT, UP, EPI><<<grid, CTHREADS, 0, st>>>(a);
+}
+
+// Upsampling layer (bias + skip + ReLU) on an output grid nx x ny x nz (nx % 32 == 0,
+// ny and nz even) from the coarse grid of half its size; wu from k_pack_up.
+template <int CIN, int COUT>
+void conv_up(hipStream_t st, int nx, int ny, int nz, const _Float16* in, const _Float16* wu, const float* bias,
+             const _Float16* skip, _Float16* out, const int32_t* gate) {
+  ConvArgs a{};
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  a.cnx = nx / 2; a.cny = ny / 2; a.cnz = nz / 2;
+  a.in = in; a.wt = wu; a.bias = bias; a.skip = skip; a.out = out; a.gate = gate;
+  using Bk = UpBrick<CIN>;
+  const dim3 grid((unsigned)(nx / Bk::BX), (unsigned)((ny + Bk::BY - 1) / Bk::BY), (unsigned)((nz + Bk::BZ - 1) / Bk::BZ));
+  k_conv_up<CIN, COUT><<<grid, CTHREADS, 0, st>>>(a);
 }
 
 // dW of layer l (g: gradient at its pre-activation with channel stride gs)
@@ -759,9 +944,9 @@ void forward(fsp_net* N, const Acts& A) {
   k_pool<2><<<pool_grid(X2, Y2, Z2, 2), dim3(64, 4), 0, st>>>(A.act[3], A.p2, X2, Y2, Z2, gt);
   conv<16, 32, 0, 0>(st, X2, Y2, Z2, A.p2, N->wt[4], B(4), nullptr, A.act[4], nullptr, gt);
   conv<32, 32, 0, 0>(st, X2, Y2, Z2, A.act[4], N->wt[5], B(5), nullptr, A.act[5], nullptr, gt);
-  conv<32, 16, 1, 3>(st, X1, Y1, Z1, A.act[5], N->wt[6], B(6), A.act[3], A.act[6], nullptr, gt);
+  conv_up<32, 16>(st, X1, Y1, Z1, A.act[5], N->wu[6], B(6), A.act[3], A.act[6], gt);
   conv<16, 16, 0, 0>(st, X1, Y1, Z1, A.act[6], N->wt[7], B(7), nullptr, A.act[7], nullptr, gt);
-  conv<16, 8, 1, 3>(st, X0, Y0, Z0, A.act[7], N->wt[8], B(8), A.act[1], A.act[8], nullptr, gt);
+  conv_up<16, 8>(st, X0, Y0, Z0, A.act[7], N->wu[8], B(8), A.act[1], A.act[8], gt);
   conv<8, 8, 0, 0>(st, X0, Y0, Z0, A.act[8], N->wt[9], B(9), nullptr, A.act[9], nullptr, gt);
   conv<8, 1, 0, 1>(st, X0, Y0, Z0, A.act[9], N->wt[10], B(10), nullptr, nullptr, A.y, gt);
 }
@@ -893,6 +1078,9 @@ fsp_status pack_all(fsp_net* N) {
     const LayerDesc& L = LAYERS[l];
     const size_t total = (size_t)npad(L.cout) * kpad(L.cin) + (size_t)npad(L.cin) * kpad(gch(L.cout));
     k_pack<<<blocks_for(total), 256, 0, N->cur>>>(N->theta + N->poff[l], L.cin, L.cout, N->wt[l], l > 0 ? N->wb[l] : nullptr);
+    if (L.up)
+      k_pack_up<<<blocks_for((size_t)64 * npad(L.cout) * L.cin), 256, 0, N->cur>>>(N->theta + N->poff[l], L.cin,
+                                                                                   L.cout, N->wu[l]);
   }
   return hipGetLastError() == hipSuccess ? FSP_OK : FSP_DEVICE;
 }
@@ -1010,7 +1198,8 @@ fsp_status fsp_net_create(int32_t device, fsp_net** out) {
   for (int l = 0; l < NL; ++l) {
     const LayerDesc& L = LAYERS[l];
     if ((st = nalloc(N, &N->wt[l], (size_t)npad(L.cout) * kpad(L.cin))) ||
-        (st = nalloc(N, &N->wb[l], (size_t)npad(L.cin) * kpad(gch(L.cout)))))
+        (st = nalloc(N, &N->wb[l], (size_t)npad(L.cin) * kpad(gch(L.cout)))) ||
+        (L.up && (st = nalloc(N, &N->wu[l], (size_t)8 * npad(L.cout) * 8 * L.cin))))
       return fail(st);
   }
   N->nparam_allocs = N->allocs.size();
