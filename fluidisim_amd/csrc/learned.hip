@@ -74,11 +74,11 @@ size_t param_count() {
 // ---------------------------------------------------------------- convolution
 struct ConvArgs {
   int nx, ny, nz;        // output grid
-  int cnx, cny, cnz;     // input grid when the input is upsampled (else = output grid)
+  int cnx, cny, cnz;     // input grid: the coarse grid of k_conv_up (else = output grid)
   const _Float16* in;    // [cells_in][CIN]
-  const _Float16* wt;    // [npad(COUT)][kpad(CIN)]
+  const _Float16* wt;    // [npad(COUT)][kpad(CIN)]; k_conv_up: [8][npad(COUT)][8 CIN]
   const float* bias;     // [COUT] (forward only)
-  const _Float16* skip;  // [cells][COUT] or null
+  const _Float16* skip;  // [cells][COUT] (k_conv_up)
   _Float16* out;         // [cells][COUT]
   float* yout;           // head: [cells]
   const int32_t* gate;   // return at once while *gate == 0 (null: always run)
@@ -103,7 +103,8 @@ __host__ __device__ constexpr int tap_offset(int tap, int hx, int hy) {
 }
 
 // EPI 0: bias + ReLU -> binary16; 1: bias -> f32 channel 0 (head);
-// 2: plain product -> binary16 (backward data); 3: bias + skip + ReLU.
+// 2: plain product -> binary16 (backward data). The upsampling layers (input
+// from the coarser level, skip added before the ReLU) run k_conv_up.
 //
 // D[co][cell] = W[co][k] X[k][cell] on v_mfma_f32_16x16x32_f16: the weights
 // are the A operand (16 output channels), a 16-cell row of the brick the B
@@ -111,7 +112,7 @@ __host__ __device__ constexpr int tap_offset(int tap, int hx, int hy) {
 // then holds output channels 4g..4g+3 of cell m, written as one 8-byte
 // store. Wavefront w owns rows w*TPW .. w*TPW+TPW-1; for each K chunk the
 // weights are read from LDS once and multiplied into every row's tile.
-template <int CIN, int COUT, int UP, int EPI>
+template <int CIN, int COUT, int EPI>
 __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv(ConvArgs a) {
   using Bk = ConvBrick<CIN, COUT>;
   constexpr int NT = npad(COUT) / 16;
@@ -152,9 +153,7 @@ __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv(ConvArgs 
       const bool in = q < TOTAL && xx >= 0 && xx < a.nx && yy >= 0 && yy < a.ny && zz >= 0 && zz < a.nz;
       buf[r] = hv(0u);
       if (in) {
-        size_t v;
-        if (UP) v = ((size_t)(zz >> 1) * a.cny + (yy >> 1)) * a.cnx + (xx >> 1);
-        else v = ((size_t)zz * a.ny + yy) * a.nx + xx;
+        const size_t v = ((size_t)zz * a.ny + yy) * a.nx + xx;
         buf[r] = *reinterpret_cast<const hv*>(a.in + v * CIN + part * VH);
       }
     }
@@ -176,22 +175,6 @@ __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv(ConvArgs 
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, g = lane >> 4;
-  // skip values of the epilogue (channels 4g..4g+3 of cell m of every row),
-  // loaded now so their latency hides under the products
-  h4 skv[EPI == 3 ? Bk::TPW : 1][NT];
-  if (EPI == 3) {
-#pragma unroll
-    for (int i = 0; i < Bk::TPW; ++i) {
-      const int tile = wave * Bk::TPW + i, y = y0 + tile % Bk::BY, z = z0 + tile / Bk::BY;
-      const size_t cell = ((size_t)z * a.ny + y) * a.nx + x0 + m;
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const int co = t * 16 + 4 * g;
-        skv[i][t] = h4{0, 0, 0, 0};
-        if (y < a.ny && z < a.nz && co < COUT) skv[i][t] = *reinterpret_cast<const h4*>(a.skip + cell * COUT + co);
-      }
-    }
-  }
   f4 acc[Bk::TPW][NT];
 #pragma unroll
   for (int i = 0; i < Bk::TPW; ++i)
@@ -248,7 +231,7 @@ __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv(ConvArgs 
   for (int t = 0; t < NT; ++t) {
     bias[t] = f4{0.f, 0.f, 0.f, 0.f};
     if (EPI == 1 && t == 0) bias[t][0] = a.bias[0];
-    if ((EPI == 0 || EPI == 3) && t * 16 + 4 * g < COUT)
+    if (EPI == 0 && t * 16 + 4 * g < COUT)
       bias[t] = *reinterpret_cast<const f4*>(a.bias + t * 16 + 4 * g);
   }
 #pragma unroll
@@ -272,8 +255,7 @@ __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv(ConvArgs 
       } else {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          float u = val[e] + bias[t][e];
-          if (EPI == 3) u += (float)skv[i][t][e];
+          const float u = val[e] + bias[t][e];
           o[e] = (_Float16)(u > 0.f ? u : 0.f);
         }
       }
@@ -849,18 +831,18 @@ void free_shape(fsp_net* N, size_t keep) {
   }
 }
 
-template <int CIN, int COUT, int UP, int EPI>
+template <int CIN, int COUT, int EPI>
 void conv(hipStream_t st, int nx, int ny, int nz, const _Float16* in, const _Float16* wt, const float* bias,
-          const _Float16* skip, _Float16* out, float* yout, const int32_t* gate = nullptr) {
+          _Float16* out, float* yout, const int32_t* gate = nullptr) {
   ConvArgs a{};
   a.gate = gate;
   a.nx = nx; a.ny = ny; a.nz = nz;
-  a.cnx = UP ? nx / 2 : nx; a.cny = UP ? ny / 2 : ny; a.cnz = UP ? nz / 2 : nz;
-  a.in = in; a.wt = wt; a.bias = bias; a.skip = skip; a.out = out; a.yout = yout;
+  a.cnx = nx; a.cny = ny; a.cnz = nz;
+  a.in = in; a.wt = wt; a.bias = bias; a.out = out; a.yout = yout;
   using Bk = ConvBrick<CIN, COUT>;
   const dim3 grid((unsigned)(nx / Bk::BX), (unsigned)((ny + Bk::BY - 1) / Bk::BY),
                   (unsigned)((nz + Bk::BZ - 1) / Bk::BZ));
-  k_conv<CIN, COUT, UP, EPI><<<grid, CTHREADS, 0, st>>>(a);
+  k_conv<CIN, COUT, EPI><<<grid, CTHREADS, 0, st>>>(a);
 }
 
 // Upsampling layer (bias + skip + ReLU) on an output grid nx x ny x nz (nx % 32 == 0,
@@ -936,19 +918,19 @@ void forward(fsp_net* N, const Acts& A) {
   const int X1 = X0 / 2, Y1 = Y0 / 2, Z1 = Z0 / 2, X2 = X0 / 4, Y2 = Y0 / 4, Z2 = Z0 / 4;
   const size_t V1 = (size_t)X1 * Y1 * Z1, V2 = (size_t)X2 * Y2 * Z2;
   auto B = [&](int l) { return N->theta + N->poff[l] + (size_t)LAYERS[l].cout * 27 * LAYERS[l].cin; };
-  conv<4, 8, 0, 0>(st, X0, Y0, Z0, A.x0, N->wt[0], B(0), nullptr, A.act[0], nullptr, gt);
-  conv<8, 8, 0, 0>(st, X0, Y0, Z0, A.act[0], N->wt[1], B(1), nullptr, A.act[1], nullptr, gt);
+  conv<4, 8, 0>(st, X0, Y0, Z0, A.x0, N->wt[0], B(0), A.act[0], nullptr, gt);
+  conv<8, 8, 0>(st, X0, Y0, Z0, A.act[0], N->wt[1], B(1), A.act[1], nullptr, gt);
   k_pool<1><<<pool_grid(X1, Y1, Z1, 1), dim3(64, 4), 0, st>>>(A.act[1], A.p1, X1, Y1, Z1, gt);
-  conv<8, 16, 0, 0>(st, X1, Y1, Z1, A.p1, N->wt[2], B(2), nullptr, A.act[2], nullptr, gt);
-  conv<16, 16, 0, 0>(st, X1, Y1, Z1, A.act[2], N->wt[3], B(3), nullptr, A.act[3], nullptr, gt);
+  conv<8, 16, 0>(st, X1, Y1, Z1, A.p1, N->wt[2], B(2), A.act[2], nullptr, gt);
+  conv<16, 16, 0>(st, X1, Y1, Z1, A.act[2], N->wt[3], B(3), A.act[3], nullptr, gt);
   k_pool<2><<<pool_grid(X2, Y2, Z2, 2), dim3(64, 4), 0, st>>>(A.act[3], A.p2, X2, Y2, Z2, gt);
-  conv<16, 32, 0, 0>(st, X2, Y2, Z2, A.p2, N->wt[4], B(4), nullptr, A.act[4], nullptr, gt);
-  conv<32, 32, 0, 0>(st, X2, Y2, Z2, A.act[4], N->wt[5], B(5), nullptr, A.act[5], nullptr, gt);
+  conv<16, 32, 0>(st, X2, Y2, Z2, A.p2, N->wt[4], B(4), A.act[4], nullptr, gt);
+  conv<32, 32, 0>(st, X2, Y2, Z2, A.act[4], N->wt[5], B(5), A.act[5], nullptr, gt);
   conv_up<32, 16>(st, X1, Y1, Z1, A.act[5], N->wu[6], B(6), A.act[3], A.act[6], gt);
-  conv<16, 16, 0, 0>(st, X1, Y1, Z1, A.act[6], N->wt[7], B(7), nullptr, A.act[7], nullptr, gt);
+  conv<16, 16, 0>(st, X1, Y1, Z1, A.act[6], N->wt[7], B(7), A.act[7], nullptr, gt);
   conv_up<16, 8>(st, X0, Y0, Z0, A.act[7], N->wu[8], B(8), A.act[1], A.act[8], gt);
-  conv<8, 8, 0, 0>(st, X0, Y0, Z0, A.act[8], N->wt[9], B(9), nullptr, A.act[9], nullptr, gt);
-  conv<8, 1, 0, 1>(st, X0, Y0, Z0, A.act[9], N->wt[10], B(10), nullptr, nullptr, A.y, gt);
+  conv<8, 8, 0>(st, X0, Y0, Z0, A.act[8], N->wt[9], B(9), A.act[9], nullptr, gt);
+  conv<8, 1, 1>(st, X0, Y0, Z0, A.act[9], N->wt[10], B(10), nullptr, A.y, gt);
 }
 
 void forward(fsp_net* N) { forward(N, net_acts(N)); }
@@ -966,38 +948,38 @@ void backward(fsp_net* N) {
   _Float16** g = N->gpre;
   // head (gradient gy: 4 channels, channel 0 used)
   wgrad<8, 1, 0>(N, 10, X0, Y0, Z0, N->gy, N->act[9], 4);
-  conv<4, 8, 0, 2>(st, X0, Y0, Z0, N->gy, N->wb[10], nullptr, nullptr, g[9], nullptr);
+  conv<4, 8, 2>(st, X0, Y0, Z0, N->gy, N->wb[10], nullptr, g[9], nullptr);
   relu_back(N, N->act[9], g[9], V0 * 8);
   wgrad<8, 8, 0>(N, 9, X0, Y0, Z0, g[9], N->act[8]);
-  conv<8, 8, 0, 2>(st, X0, Y0, Z0, g[9], N->wb[9], nullptr, nullptr, g[8], nullptr);
+  conv<8, 8, 2>(st, X0, Y0, Z0, g[9], N->wb[9], nullptr, g[8], nullptr);
   relu_back(N, N->act[8], g[8], V0 * 8);
   wgrad<16, 8, 1>(N, 8, X0, Y0, Z0, g[8], N->act[7]);
-  conv<8, 16, 0, 2>(st, X0, Y0, Z0, g[8], N->wb[8], nullptr, nullptr, N->gf16, nullptr);
+  conv<8, 16, 2>(st, X0, Y0, Z0, g[8], N->wb[8], nullptr, N->gf16, nullptr);
   k_up_back<<<blocks_for(V1 * 16), 256, 0, st>>>(N->gf16, g[7], X1, Y1, Z1, 16);
   relu_back(N, N->act[7], g[7], V1 * 16);
   wgrad<16, 16, 0>(N, 7, X1, Y1, Z1, g[7], N->act[6]);
-  conv<16, 16, 0, 2>(st, X1, Y1, Z1, g[7], N->wb[7], nullptr, nullptr, g[6], nullptr);
+  conv<16, 16, 2>(st, X1, Y1, Z1, g[7], N->wb[7], nullptr, g[6], nullptr);
   relu_back(N, N->act[6], g[6], V1 * 16);
   wgrad<32, 16, 1>(N, 6, X1, Y1, Z1, g[6], N->act[5]);
-  conv<16, 32, 0, 2>(st, X1, Y1, Z1, g[6], N->wb[6], nullptr, nullptr, N->gf32, nullptr);
+  conv<16, 32, 2>(st, X1, Y1, Z1, g[6], N->wb[6], nullptr, N->gf32, nullptr);
   k_up_back<<<blocks_for(V2 * 32), 256, 0, st>>>(N->gf32, g[5], X2, Y2, Z2, 32);
   relu_back(N, N->act[5], g[5], V2 * 32);
   wgrad<32, 32, 0>(N, 5, X2, Y2, Z2, g[5], N->act[4]);
-  conv<32, 32, 0, 2>(st, X2, Y2, Z2, g[5], N->wb[5], nullptr, nullptr, g[4], nullptr);
+  conv<32, 32, 2>(st, X2, Y2, Z2, g[5], N->wb[5], nullptr, g[4], nullptr);
   relu_back(N, N->act[4], g[4], V2 * 32);
   wgrad<16, 32, 0>(N, 4, X2, Y2, Z2, g[4], N->p2);
-  conv<32, 16, 0, 2>(st, X2, Y2, Z2, g[4], N->wb[4], nullptr, nullptr, N->gp2, nullptr);
+  conv<32, 16, 2>(st, X2, Y2, Z2, g[4], N->wb[4], nullptr, N->gp2, nullptr);
   k_pool_back<<<blocks_for(V1 * 16), 256, 0, st>>>(N->gp2, g[6], g[3], X1, Y1, Z1, 16);
   relu_back(N, N->act[3], g[3], V1 * 16);
   wgrad<16, 16, 0>(N, 3, X1, Y1, Z1, g[3], N->act[2]);
-  conv<16, 16, 0, 2>(st, X1, Y1, Z1, g[3], N->wb[3], nullptr, nullptr, g[2], nullptr);
+  conv<16, 16, 2>(st, X1, Y1, Z1, g[3], N->wb[3], nullptr, g[2], nullptr);
   relu_back(N, N->act[2], g[2], V1 * 16);
   wgrad<8, 16, 0>(N, 2, X1, Y1, Z1, g[2], N->p1);
-  conv<16, 8, 0, 2>(st, X1, Y1, Z1, g[2], N->wb[2], nullptr, nullptr, N->gp1, nullptr);
+  conv<16, 8, 2>(st, X1, Y1, Z1, g[2], N->wb[2], nullptr, N->gp1, nullptr);
   k_pool_back<<<blocks_for(V0 * 8), 256, 0, st>>>(N->gp1, g[8], g[1], X0, Y0, Z0, 8);
   relu_back(N, N->act[1], g[1], V0 * 8);
   wgrad<8, 8, 0>(N, 1, X0, Y0, Z0, g[1], N->act[0]);
-  conv<8, 8, 0, 2>(st, X0, Y0, Z0, g[1], N->wb[1], nullptr, nullptr, g[0], nullptr);
+  conv<8, 8, 2>(st, X0, Y0, Z0, g[1], N->wb[1], nullptr, g[0], nullptr);
   relu_back(N, N->act[0], g[0], V0 * 8);
   wgrad<4, 8, 0>(N, 0, X0, Y0, Z0, g[0], N->x0);
 }
