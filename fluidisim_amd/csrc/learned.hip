@@ -62,6 +62,16 @@ constexpr LayerDesc LAYERS[NL] = {
 // K of a layer (27 taps x input channels) padded to the 32 of one product
 __host__ __device__ constexpr int kpad(int cin) { return ((27 * cin + 31) / 32) * 32; }
 __host__ __device__ constexpr int npad(int cout) { return ((cout + 15) / 16) * 16; }
+// Forward layers with 4 or 8 input and at most 8 output channels take two
+// output rows per product: the A operand's rows 0-7 are the weights for row y,
+// rows 8-15 those for row y + 1, and K runs over (dz, ry, dx, ci) of the four
+// input rows y - 1 .. y + 2 the pair reads: 36 x 8 = 288 = 9 products per row
+// pair against 7 per row (8 channels), 144 -> 160 = 5 against 4 (4 channels).
+__host__ __device__ constexpr bool pair_rows(int cin, int cout) { return (cin == 4 || cin == 8) && cout <= 8; }
+// K of a layer's forward weights
+__host__ __device__ constexpr int kfwd(int cin, int cout) {
+  return pair_rows(cin, cout) ? ((36 * cin + 31) / 32) * 32 : kpad(cin);
+}
 // channels of the gradient flowing into a layer's backward-data convolution
 __host__ __device__ constexpr int gch(int cout) { return cout < 4 ? 4 : cout; }
 
@@ -76,7 +86,7 @@ struct ConvArgs {
   int nx, ny, nz;        // output grid
   int cnx, cny, cnz;     // input grid: the coarse grid of k_conv_up (else = output grid)
   const _Float16* in;    // [cells_in][CIN]
-  const _Float16* wt;    // [npad(COUT)][kpad(CIN)]; k_conv_up: [8][npad(COUT)][8 CIN]
+  const _Float16* wt;    // [npad(COUT)][K]; K = kpad(CIN), kfwd for row pairs; k_conv_up: [8][npad(COUT)][8 CIN]
   const float* bias;     // [COUT] (forward only)
   const _Float16* skip;  // [cells][COUT] (k_conv_up)
   _Float16* out;         // [cells][COUT]
@@ -111,12 +121,16 @@ __host__ __device__ constexpr int tap_offset(int tap, int hx, int hy) {
 // operand, K = 32 values of (tap, input channel) per product. Lane (m, g)
 // then holds output channels 4g..4g+3 of cell m, written as one 8-byte
 // store. Wavefront w owns rows w*TPW .. w*TPW+TPW-1; for each K chunk the
-// weights are read from LDS once and multiplied into every row's tile.
+// weights are read from LDS once and multiplied into every row's tile. In
+// the row-pair form (pair_rows) a tile is two rows y, y + 1 (y even) and lane
+// (m, g) holds channels 4(g & 1)..+3 of row y + (g >> 1).
 template <int CIN, int COUT, int EPI>
 __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv(ConvArgs a) {
   using Bk = ConvBrick<CIN, COUT>;
+  constexpr bool PAIR = EPI != 2 && pair_rows(CIN, COUT);
+  constexpr int TPW = PAIR ? Bk::TPW / 2 : Bk::TPW;
   constexpr int NT = npad(COUT) / 16;
-  constexpr int KP = kpad(CIN);
+  constexpr int KP = PAIR ? kfwd(CIN, COUT) : kpad(CIN);
   constexpr int CHUNKS = KP / 32;
   static_assert(COUT % 4 == 0 || (COUT == 1 && EPI == 1), "output channels in groups of 4");
   constexpr int NP = npad(COUT);
@@ -175,16 +189,22 @@ __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv(ConvArgs 
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, g = lane >> 4;
-  f4 acc[Bk::TPW][NT];
+  f4 acc[TPW][NT];
 #pragma unroll
-  for (int i = 0; i < Bk::TPW; ++i)
+  for (int i = 0; i < TPW; ++i)
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[i][t] = f4{0.f, 0.f, 0.f, 0.f};
-  // halo index (halves) of this lane's cell (x = m) in row i of the wavefront, tap (-1,-1,-1)
-  auto row_base = [&](int i) {
-    const int tile = wave * Bk::TPW + i, ly = tile % Bk::BY, lz = tile / Bk::BY;
-    return ((lz * Bk::HY + ly) * Bk::HX + m) * CIN;
+  // local row and plane of tile i of the wavefront (the first row of a pair)
+  auto tile_y = [&](int i) {
+    const int tile = wave * TPW + i;
+    return PAIR ? 2 * (tile % (Bk::BY / 2)) : tile % Bk::BY;
   };
+  auto tile_z = [&](int i) {
+    const int tile = wave * TPW + i;
+    return PAIR ? tile / (Bk::BY / 2) : tile / Bk::BY;
+  };
+  // halo index (halves) of this lane's cell (x = m) in tile i, tap (-1,-1,-1)
+  auto row_base = [&](int i) { return ((tile_z(i) * Bk::HY + tile_y(i)) * Bk::HX + m) * CIN; };
 #pragma unroll
   for (int ch = 0; ch < CHUNKS; ++ch) {
     // this lane group's 8 K values: k = 32 ch + 8 g .. +7
@@ -192,13 +212,37 @@ __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv(ConvArgs 
     h8 w[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) w[t] = *reinterpret_cast<const h8*>(wl + (t * 16 + m) * KPL + kk);
-    if (CIN >= 8) {
+    // halo offset of entry tp = (dz*4 + ry)*3 + dx of a row pair's K (input row ry from y - 1)
+    auto pair_off = [&](int tp) { return (((tp / 12) * Bk::HY + (tp / 3) % 4) * Bk::HX + tp % 3) * CIN; };
+    if (PAIR && CIN == 8) {
+      // 8 channels of one entry
+      const int off = pair_off(kk / CIN);
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) {
+        const h8 xv = *reinterpret_cast<const h8*>(halo + row_base(i) + off);
+        acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(w[0], xv, acc[i][0], 0, 0, 0);
+      }
+    } else if (PAIR) {
+      // CIN == 4: two entries of 4 channels; entries from 36 on are K padding
+      const int tp0 = kk / 4, tp1 = tp0 + 1;
+      const bool live0 = tp0 < 36, live1 = tp1 < 36;
+      const int off0 = live0 ? pair_off(tp0) : 0, off1 = live1 ? pair_off(tp1) : 0;
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) {
+        h4 lo = *reinterpret_cast<const h4*>(halo + row_base(i) + off0);
+        h4 hi = *reinterpret_cast<const h4*>(halo + row_base(i) + off1);
+        if (!live0) lo = h4{0, 0, 0, 0};
+        if (!live1) hi = h4{0, 0, 0, 0};
+        const h8 xv = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+        acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(w[0], xv, acc[i][0], 0, 0, 0);
+      }
+    } else if (CIN >= 8) {
       // one tap, 8 consecutive channels
       const int tap = kk / CIN, ci = kk - tap * CIN;
       const bool live = tap < 27;  // K padding: zero operand
       const int off = (live ? tap_offset(tap, Bk::HX, Bk::HY) : 0) * CIN + ci;
 #pragma unroll
-      for (int i = 0; i < Bk::TPW; ++i) {
+      for (int i = 0; i < TPW; ++i) {
         h8 xv = *reinterpret_cast<const h8*>(halo + row_base(i) + off);
         if (!live) xv = h8(0);
 #pragma unroll
@@ -212,7 +256,7 @@ __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv(ConvArgs 
       const int off0 = (live0 ? tap_offset(tap0, Bk::HX, Bk::HY) : 0) * CIN;
       const int off1 = (live1 ? tap_offset(tap1, Bk::HX, Bk::HY) : 0) * CIN;
 #pragma unroll
-      for (int i = 0; i < Bk::TPW; ++i) {
+      for (int i = 0; i < TPW; ++i) {
         h4 lo = *reinterpret_cast<const h4*>(halo + row_base(i) + off0);
         h4 hi = *reinterpret_cast<const h4*>(halo + row_base(i) + off1);
         if (!live0) lo = h4{0, 0, 0, 0};
@@ -225,28 +269,30 @@ __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv(ConvArgs 
     }
   }
 
-  // epilogue: lane (m, g) holds D[channel t*16 + 4g + v][cell m]
+  // epilogue: lane (m, g) holds D[channel t*16 + 4g + v][cell m]; in the
+  // row-pair form channels 4(g & 1)..+3 of row y + (g >> 1)
+  const int rsel = PAIR ? g >> 1 : 0, cg = PAIR ? (g & 1) : g;
   f4 bias[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     bias[t] = f4{0.f, 0.f, 0.f, 0.f};
     if (EPI == 1 && t == 0) bias[t][0] = a.bias[0];
-    if (EPI == 0 && t * 16 + 4 * g < COUT)
-      bias[t] = *reinterpret_cast<const f4*>(a.bias + t * 16 + 4 * g);
+    if (EPI == 0 && t * 16 + 4 * cg < COUT)
+      bias[t] = *reinterpret_cast<const f4*>(a.bias + t * 16 + 4 * cg);
   }
 #pragma unroll
-  for (int i = 0; i < Bk::TPW; ++i) {
-    const int tile = wave * Bk::TPW + i, y = y0 + tile % Bk::BY, z = z0 + tile / Bk::BY;
-    if (y >= a.ny || z >= a.nz) continue;  // uniform per wavefront
-    const size_t cell = ((size_t)z * a.ny + y) * a.nx + x0 + m;
+  for (int i = 0; i < TPW; ++i) {
+    const int y = y0 + tile_y(i), z = z0 + tile_z(i);
+    if (y >= a.ny || z >= a.nz) continue;  // uniform per wavefront (ny is even: a pair is whole)
+    const size_t cell = ((size_t)z * a.ny + y + rsel) * a.nx + x0 + m;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       const f4 val = acc[i][t];
       if (EPI == 1) {
-        if (g == 0) a.yout[cell] = val[0] + bias[0][0];
+        if (cg == 0) a.yout[cell] = val[0] + bias[0][0];
         continue;
       }
-      const int co = t * 16 + 4 * g;
+      const int co = t * 16 + 4 * cg;
       if (co >= COUT) continue;
       h4 o;
       if (EPI == 2) {
@@ -597,15 +643,24 @@ __global__ void __launch_bounds__(256) k_loss(const float* y, const float* t, co
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0] * inv_n;
 }
 
-// Forward weights (W[co][tap][ci] -> [npad(cout)][kpad(cin)], k = tap*cin+ci)
+// Forward weights (W[co][tap][ci] -> [npad(cout)][kpad(cin)], k = tap*cin+ci;
+// for row pairs [16][kfwd], row h*8 + co, k = ((dz*4 + ry)*3 + dx)*cin + ci
+// holding W[co][dz][ry - h][dx][ci] where 0 <= ry - h <= 2, else 0; zero
+// K padding)
 // and backward-data weights (W'[ci][tap'][c] = W[c][26-tap'][ci], c over
 // gch(cout) channels) in binary16, from the f32 master parameters.
 __global__ void k_pack(const float* theta, int cin, int cout, _Float16* wt, _Float16* wb) {
-  const int KP = kpad(cin), NP = npad(cout);
+  const int KP = kfwd(cin, cout), NP = npad(cout);
   const int gc = gch(cout), KPb = kpad(gc), NPb = npad(cin);
   const int nf = NP * KP, nb = NPb * KPb;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nf + nb; q += gridDim.x * blockDim.x) {
-    if (q < nf) {
+    if (q < nf && pair_rows(cin, cout)) {
+      const int row = q / KP, k = q - row * KP, h = row >> 3, co = row & 7;
+      const int tp = k / cin, ci = k - tp * cin, dz = tp / 12, dy = (tp / 3) % 4 - h, dx = tp % 3;
+      float v = 0.f;
+      if (co < cout && tp < 36 && dy >= 0 && dy <= 2) v = theta[((size_t)co * 27 + dz * 9 + dy * 3 + dx) * cin + ci];
+      wt[q] = (_Float16)v;
+    } else if (q < nf) {
       const int co = q / KP, k = q - co * KP;
       float v = 0.f;
       if (co < cout && k < 27 * cin) v = theta[(size_t)co * 27 * cin + k];
@@ -1058,7 +1113,7 @@ namespace {
 fsp_status pack_all(fsp_net* N) {
   for (int l = 0; l < NL; ++l) {
     const LayerDesc& L = LAYERS[l];
-    const size_t total = (size_t)npad(L.cout) * kpad(L.cin) + (size_t)npad(L.cin) * kpad(gch(L.cout));
+    const size_t total = (size_t)npad(L.cout) * kfwd(L.cin, L.cout) + (size_t)npad(L.cin) * kpad(gch(L.cout));
     k_pack<<<blocks_for(total), 256, 0, N->cur>>>(N->theta + N->poff[l], L.cin, L.cout, N->wt[l], l > 0 ? N->wb[l] : nullptr);
     if (L.up)
       k_pack_up<<<blocks_for((size_t)64 * npad(L.cout) * L.cin), 256, 0, N->cur>>>(N->theta + N->poff[l], L.cin,
@@ -1179,7 +1234,7 @@ fsp_status fsp_net_create(int32_t device, fsp_net** out) {
     return fail(st);
   for (int l = 0; l < NL; ++l) {
     const LayerDesc& L = LAYERS[l];
-    if ((st = nalloc(N, &N->wt[l], (size_t)npad(L.cout) * kpad(L.cin))) ||
+    if ((st = nalloc(N, &N->wt[l], (size_t)npad(L.cout) * kfwd(L.cin, L.cout))) ||
         (st = nalloc(N, &N->wb[l], (size_t)npad(L.cin) * kpad(gch(L.cout)))) ||
         (L.up && (st = nalloc(N, &N->wu[l], (size_t)8 * npad(L.cout) * 8 * L.cin))))
       return fail(st);
