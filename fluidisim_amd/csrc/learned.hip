@@ -86,7 +86,7 @@ struct ConvArgs {
   int nx, ny, nz;        // output grid
   int cnx, cny, cnz;     // input grid: the coarse grid of k_conv_up (else = output grid)
   const _Float16* in;    // [cells_in][CIN]
-  const _Float16* wt;    // [npad(COUT)][K]; K = kpad(CIN), kfwd for row pairs; k_conv_up: [8][npad(COUT)][8 CIN]
+  const _Float16* wt;    // [npad(COUT)][K]; K = kpad(CIN), kfwd for row pairs; k_conv_up: k_pack_up's layout
   const float* bias;     // [COUT] (forward only)
   const _Float16* skip;  // [cells][COUT] (k_conv_up)
   _Float16* out;         // [cells][COUT]
@@ -334,21 +334,36 @@ struct UpBrick {
   static constexpr int TPW = AY * (BZ / 2);
 };
 static_assert(CWAVES == 8, "one wavefront per parity of the upsampling convolution");
+// With at most 8 output channels a product covers both y parities: the A
+// operand's rows 0-7 are parity py = 0, rows 8-15 py = 1, and K runs over
+// (oz, ry, ox, ci) of the three coarse rows cy - 1 + ry the two rows read
+// (py = 0: ry = oy, py = 1: ry = oy + 1): 12 CIN, 6 products per row pair
+// against 4 per row. Wavefront w then takes parities (px, pz) = (w & 1,
+// (w >> 1) & 1) and half w >> 2 of their row pairs.
+__host__ __device__ constexpr bool up_pairs(int cout) { return cout <= 8; }
+// binary16 values of an upsampling layer's packed weights
+__host__ __device__ constexpr int up_weights(int cin, int cout) {
+  return up_pairs(cout) ? 4 * 16 * 12 * cin : 8 * npad(cout) * 8 * cin;
+}
 
 template <int CIN, int COUT>
 __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv_up(ConvArgs a) {
   using Bk = UpBrick<CIN>;
+  constexpr bool PY = up_pairs(COUT);
   constexpr int NT = npad(COUT) / 16;
   constexpr int NP = npad(COUT);
-  constexpr int K = 8 * CIN;
+  constexpr int K = PY ? 12 * CIN : 8 * CIN;
   constexpr int CHUNKS = K / 32;
-  static_assert(CIN % 8 == 0 && K % 32 == 0 && COUT % 4 == 0, "channel groups");
+  constexpr int TPW = PY ? Bk::TPW / 2 : Bk::TPW;
+  static_assert(CIN % 8 == 0 && K % 32 == 0 && COUT % 4 == 0 && (!PY || NT == 1), "channel groups");
   __shared__ __attribute__((aligned(16))) _Float16 halo[Bk::HCELLS * CIN];
   if (a.gate && !*a.gate) return;
   const int x0 = blockIdx.x * Bk::BX, y0 = blockIdx.y * Bk::BY, z0 = blockIdx.z * Bk::BZ;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, g = lane >> 4;
-  const int px = wave & 1, py = (wave >> 1) & 1, pz = wave >> 2;
+  // parities of the wavefront (py from the lane group in the pair form) and its first tile
+  const int px = wave & 1, py = PY ? g >> 1 : (wave >> 1) & 1, pz = PY ? (wave >> 1) & 1 : wave >> 2;
+  const int pq = PY ? (wave & 3) : wave, j0 = PY ? (wave >> 2) * TPW : 0;
 
   // coarse halo (zero outside the coarse grid) -> LDS; origin (x0/2 - 1, y0/2 - 1, z0/2 - 1)
   {
@@ -381,7 +396,7 @@ __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv_up(ConvAr
   for (int ch = 0; ch < CHUNKS; ++ch)
 #pragma unroll
     for (int t = 0; t < NT; ++t)
-      w[ch][t] = *reinterpret_cast<const h8*>(a.wt + (size_t)(wave * NP + t * 16 + m) * K + 32 * ch + 8 * g);
+      w[ch][t] = *reinterpret_cast<const h8*>(a.wt + (size_t)(pq * NP + t * 16 + m) * K + 32 * ch + 8 * g);
   // skip values of the epilogue (8 channels of a cell per item, see below),
   // loaded before the products
   constexpr int CELLS = Bk::BX * Bk::BY * Bk::BZ;
@@ -401,20 +416,23 @@ __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv_up(ConvAr
     skv[r] = h8(0);
     if (olive[r]) skv[r] = *reinterpret_cast<const h8*>(a.skip + ocell[r]);
   }
-  f4 acc[Bk::TPW][NT];
+  f4 acc[TPW][NT];
 #pragma unroll
-  for (int i = 0; i < Bk::TPW; ++i)
+  for (int i = 0; i < TPW; ++i)
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[i][t] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int ch = 0; ch < CHUNKS; ++ch) {
-    // 8 consecutive channels of one coarse tap (ox, oy, oz)
+    // 8 consecutive channels of one coarse tap (ox, oy, oz); in the pair form
+    // of entry (oz, ry, ox), coarse row cy - 1 + ry
     const int kk = 32 * ch + 8 * g;
     const int tap = kk / CIN, ci = kk - tap * CIN;
-    const int ox = tap & 1, oy = (tap >> 1) & 1, oz = tap >> 2;
+    const int ox = tap & 1, oz = PY ? tap / 6 : tap >> 2;
+    const int ry = PY ? (tap >> 1) % 3 : py + ((tap >> 1) & 1);
 #pragma unroll
-    for (int i = 0; i < Bk::TPW; ++i) {
-      const int hx = m + px + ox, hy = i % Bk::AY + py + oy, hz = i / Bk::AY + pz + oz;
+    for (int i = 0; i < TPW; ++i) {
+      const int j = j0 + i;
+      const int hx = m + px + ox, hy = j % Bk::AY + ry, hz = j / Bk::AY + pz + oz;
       const h8 xv = *reinterpret_cast<const h8*>(halo + ((hz * Bk::HY + hy) * Bk::HX + hx) * CIN + ci);
 #pragma unroll
       for (int t = 0; t < NT; ++t)
@@ -424,16 +442,18 @@ __global__ void __launch_bounds__(CTHREADS, FSP_CONV_MIN_WAVES) k_conv_up(ConvAr
 
   // epilogue through LDS: each wavefront stages acc + bias (f32) of its cells,
   // then a thread finishes 8 channels of a cell per item (+ skip, ReLU), so the
-  // skip loads and the output stores are contiguous 16-byte accesses
+  // skip loads and the output stores are contiguous 16-byte accesses. Lane
+  // (m, g) holds channels t*16 + 4g..+3, in the pair form 4(g & 1)..+3 of py.
   __shared__ __attribute__((aligned(16))) float stage[CELLS * COUT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    const int co = t * 16 + 4 * g;
+    const int co = PY ? 4 * (g & 1) : t * 16 + 4 * g;
     if (co >= COUT) continue;
     const f4 bias = *reinterpret_cast<const f4*>(a.bias + co);
 #pragma unroll
-    for (int i = 0; i < Bk::TPW; ++i) {
-      const int lx = 2 * m + px, ly = 2 * (i % Bk::AY) + py, lz = 2 * (i / Bk::AY) + pz;
+    for (int i = 0; i < TPW; ++i) {
+      const int j = j0 + i;
+      const int lx = 2 * m + px, ly = 2 * (j % Bk::AY) + py, lz = 2 * (j / Bk::AY) + pz;
       *reinterpret_cast<f4*>(stage + ((lz * Bk::BY + ly) * Bk::BX + lx) * COUT + co) = acc[i][t] + bias;
     }
   }
@@ -682,14 +702,25 @@ __global__ void k_pack(const float* theta, int cin, int cout, _Float16* wt, _Flo
 // [8 coarse taps x cin], coarse tap (ox, oy, oz) of parity (px, py, pz) the
 // f32 sum of the layer's taps d (index d + 1 per axis) that fall on it:
 // {0} and {1, 2} for p = 0, {0, 1} and {2} for p = 1; rounded once to binary16.
+// With up_pairs: [4 parities (px, pz)][16 rows py*8 + co][12 entries x cin],
+// entry (oz, ry, ox) holding coarse tap oy = ry - py (zero outside 0..1).
 __global__ void k_pack_up(const float* theta, int cin, int cout, _Float16* wu) {
-  const int NP = npad(cout), K = 8 * cin, total = 8 * NP * K;
+  const bool pairs = up_pairs(cout);
+  const int NP = pairs ? 16 : npad(cout), K = (pairs ? 12 : 8) * cin, total = up_weights(cin, cout);
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
-    const int P = q / (NP * K), r = q - P * NP * K, co = r / K, k = r - co * K;
+    const int P = q / (NP * K), r = q - P * NP * K, row = r / K, k = r - row * K;
     const int tap = k / cin, ci = k - tap * cin;
+    int p[3], o[3], co = row;
+    if (pairs) {
+      co = row & 7;
+      p[0] = P & 1; p[1] = row >> 3; p[2] = P >> 1;
+      o[0] = tap & 1; o[1] = (tap >> 1) % 3 - p[1]; o[2] = tap / 6;
+    } else {
+      p[0] = P & 1; p[1] = (P >> 1) & 1; p[2] = P >> 2;
+      o[0] = tap & 1; o[1] = (tap >> 1) & 1; o[2] = tap >> 2;
+    }
     float v = 0.f;
-    if (co < cout) {
-      const int p[3] = {P & 1, (P >> 1) & 1, P >> 2}, o[3] = {tap & 1, (tap >> 1) & 1, tap >> 2};
+    if (co < cout && o[1] >= 0 && o[1] <= 1) {
       int lo[3], hi[3];
       for (int d = 0; d < 3; ++d) {
         lo[d] = p[d] == 0 ? (o[d] == 0 ? 0 : 1) : (o[d] == 0 ? 0 : 2);
@@ -1116,8 +1147,8 @@ fsp_status pack_all(fsp_net* N) {
     const size_t total = (size_t)npad(L.cout) * kfwd(L.cin, L.cout) + (size_t)npad(L.cin) * kpad(gch(L.cout));
     k_pack<<<blocks_for(total), 256, 0, N->cur>>>(N->theta + N->poff[l], L.cin, L.cout, N->wt[l], l > 0 ? N->wb[l] : nullptr);
     if (L.up)
-      k_pack_up<<<blocks_for((size_t)64 * npad(L.cout) * L.cin), 256, 0, N->cur>>>(N->theta + N->poff[l], L.cin,
-                                                                                   L.cout, N->wu[l]);
+      k_pack_up<<<blocks_for((size_t)up_weights(L.cin, L.cout)), 256, 0, N->cur>>>(N->theta + N->poff[l], L.cin,
+                                                                                    L.cout, N->wu[l]);
   }
   return hipGetLastError() == hipSuccess ? FSP_OK : FSP_DEVICE;
 }
@@ -1236,7 +1267,7 @@ fsp_status fsp_net_create(int32_t device, fsp_net** out) {
     const LayerDesc& L = LAYERS[l];
     if ((st = nalloc(N, &N->wt[l], (size_t)npad(L.cout) * kfwd(L.cin, L.cout))) ||
         (st = nalloc(N, &N->wb[l], (size_t)npad(L.cin) * kpad(gch(L.cout)))) ||
-        (L.up && (st = nalloc(N, &N->wu[l], (size_t)8 * npad(L.cout) * 8 * L.cin))))
+        (L.up && (st = nalloc(N, &N->wu[l], (size_t)up_weights(L.cin, L.cout)))))
       return fail(st);
   }
   N->nparam_allocs = N->allocs.size();
