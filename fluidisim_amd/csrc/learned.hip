@@ -7,7 +7,9 @@
 //
 // Activations are channels-last binary16, cells (k*ny+j)*nx+i. Level l of
 // the network is the grid coarsened 2^l times (average pooling down, nearest
-// upsampling folded into the next convolution's loads). Layer weights are
+// upsampling folded into the next convolution: in the forward pass a 2x2x2
+// convolution of the coarse grid per output parity, k_conv_up; in the weight
+// gradient the upsampled loads of k_wgrad). Layer weights are
 // W[co][tap][ci], tap = (dz+1)*9 + (dy+1)*3 + (dx+1), zero padding at the
 // domain boundary.
 //
